@@ -1,0 +1,180 @@
+"""ctypes binding of the gfx950 library (include/dwba_hip.h).
+
+torch is imported before the library is loaded so that the HIP runtime torch
+ships (soname libamdhip64.so.7) is the one the library binds to: device
+pointers and streams created by torch are then valid in the library.
+
+There is no fallback: if the library or a GPU is missing, every entry point
+raises RuntimeError.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch  # noqa: F401  (must precede the CDLL load, see module docstring)
+
+from ._build import lib_path
+
+DWBA_OK = 0
+DWBA_ERR_ARG = 1
+DWBA_ERR_HIP = 2
+DWBA_ERR_NOT_INIT = 3
+DWBA_ERR_WORKSPACE = 4
+
+_lock = threading.Lock()
+_lib = None
+_init_dev = None
+
+c_i32 = ctypes.c_int32
+c_sz = ctypes.c_size_t
+c_vp = ctypes.c_void_p
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+class PlainJob(ctypes.Structure):
+    """dwba_plain_job of include/dwba_hip.h."""
+    _fields_ = [("r", c_vp), ("alpha_dir", c_vp), ("beta_dir", c_vp), ("alpha_ex", c_vp),
+                ("beta_ex", c_vp), ("n_rows", c_i32), ("n_cols", c_i32)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "dwba_version": (c_i32, []),
+    "dwba_init": (c_i32, [c_i32]),
+    "dwba_shutdown": (c_i32, []),
+    "dwba_last_error": (ctypes.c_char_p, []),
+    "dwba_il_plain_f64_workspace_bytes": (c_sz, [c_i32, c_i32, c_i32]),
+    "dwba_il_plain_f64": (c_i32, [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_sz, c_vp]),
+    "dwba_il_plain_f64_batch_workspace_bytes": (c_sz, [c_i32, c_i32p, c_i32p, c_i32]),
+    "dwba_il_plain_f64_batch": (c_i32, [c_i32, c_vp, c_i32p, c_i32p, c_i32, c_vp, c_vp, c_sz, c_vp]),
+}
+
+
+def exported_symbols():
+    """Names of every C-ABI function this binding expects the library to export."""
+    return sorted(_SIGS)
+
+
+def load_library(path: str | None = None):
+    """Load the shared library (no GPU is touched)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        p = path or lib_path()
+        if not os.path.exists(p):
+            raise RuntimeError(f"gfx950 library not built: {p} is missing (run __graft_entry__.build())")
+        lib = ctypes.CDLL(p)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def last_error() -> str:
+    return load_library().dwba_last_error().decode(errors="replace")
+
+
+def check(code: int, what: str):
+    if code == DWBA_OK:
+        return
+    msg = f"{what}: {last_error()}"
+    if code in (DWBA_ERR_ARG, DWBA_ERR_WORKSPACE):
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def device_available() -> bool:
+    try:
+        return torch.cuda.is_available() and torch.cuda.device_count() > 0
+    except Exception:
+        return False
+
+
+def ensure_init(device: int | None = None) -> int:
+    """Initialise the library on `device` (default: torch's current device)."""
+    global _init_dev
+    lib = load_library()
+    if not device_available():
+        raise RuntimeError("no HIP device visible: the gfx950 path has no CPU fallback")
+    dev = torch.cuda.current_device() if device is None else int(device)
+    if _init_dev != dev:
+        check(lib.dwba_init(dev), "dwba_init")
+        _init_dev = dev
+    return dev
+
+
+def stream_handle(stream=None) -> int:
+    s = torch.cuda.current_stream() if stream is None else stream
+    return int(s.cuda_stream)
+
+
+def ptr(t) -> int:
+    return int(t.data_ptr()) if t is not None else 0
+
+
+def _require_f64_cuda(name, t, n):
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise ValueError(f"{name} must be a contiguous float64 CUDA tensor")
+    if t.numel() < n:
+        raise ValueError(f"{name} has {t.numel()} elements, need >= {n}")
+
+
+def il_plain_f64(r, alpha_dir, beta_dir, alpha_ex, beta_ex, n_rows: int, n_cols: int, L_max: int,
+                 out=None, stream=None):
+    """dwba_il_plain_f64 on torch tensors.  Returns a (2, L_max+1) float64 CUDA
+    tensor: row 0 direct, row 1 exchange."""
+    ensure_init()
+    lib = load_library()
+    for nm, t, n in (("r", r, n_cols), ("alpha_dir", alpha_dir, n_rows), ("beta_dir", beta_dir, n_cols),
+                     ("alpha_ex", alpha_ex, n_rows), ("beta_ex", beta_ex, n_cols)):
+        _require_f64_cuda(nm, t, n)
+    ws_bytes = lib.dwba_il_plain_f64_workspace_bytes(n_rows, n_cols, L_max)
+    if ws_bytes == 0:
+        raise ValueError(f"il_plain_f64: invalid shape n_rows={n_rows} n_cols={n_cols} L_max={L_max}")
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=r.device)
+    if out is None:
+        out = torch.empty((2, L_max + 1), dtype=torch.float64, device=r.device)
+    check(lib.dwba_il_plain_f64(ptr(r), n_rows, n_cols, ptr(alpha_dir), ptr(beta_dir), ptr(alpha_ex),
+                                ptr(beta_ex), L_max, ptr(out), ptr(ws), ws_bytes, stream_handle(stream)),
+          "dwba_il_plain_f64")
+    out._dwba_ws = ws  # keep the workspace alive until the tensor is released
+    return out
+
+
+def il_plain_f64_batch(jobs, L_max: int, stream=None):
+    """Batched plain FP64 contraction.  `jobs` is a list of tuples
+    (r, alpha_dir, beta_dir, alpha_ex, beta_ex, n_rows, n_cols) of CUDA tensors.
+    Returns a (n_jobs, 2, L_max+1) float64 CUDA tensor."""
+    ensure_init()
+    lib = load_library()
+    n = len(jobs)
+    if n == 0:
+        raise ValueError("il_plain_f64_batch: no jobs")
+    dev = jobs[0][0].device
+    arr = (PlainJob * n)()
+    nr = (ctypes.c_int32 * n)()
+    nc = (ctypes.c_int32 * n)()
+    for k, (r, ad, bd, ae, be, n_rows, n_cols) in enumerate(jobs):
+        for nm, t, m in (("r", r, n_cols), ("alpha_dir", ad, n_rows), ("beta_dir", bd, n_cols),
+                         ("alpha_ex", ae, n_rows), ("beta_ex", be, n_cols)):
+            _require_f64_cuda(f"job {k} {nm}", t, m)
+        arr[k] = PlainJob(ptr(r), ptr(ad), ptr(bd), ptr(ae), ptr(be), int(n_rows), int(n_cols))
+        nr[k] = int(n_rows)
+        nc[k] = int(n_cols)
+    raw = torch.frombuffer(bytearray(ctypes.string_at(ctypes.addressof(arr), ctypes.sizeof(arr))),
+                           dtype=torch.uint8)
+    d_jobs = raw.to(dev)
+    ws_bytes = lib.dwba_il_plain_f64_batch_workspace_bytes(n, nr, nc, L_max)
+    if ws_bytes == 0:
+        raise ValueError("il_plain_f64_batch: invalid shapes")
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    out = torch.empty((n, 2, L_max + 1), dtype=torch.float64, device=dev)
+    check(lib.dwba_il_plain_f64_batch(n, ptr(d_jobs), nr, nc, L_max, ptr(out), ptr(ws), ws_bytes,
+                                      stream_handle(stream)), "dwba_il_plain_f64_batch")
+    out._dwba_keep = (ws, d_jobs)
+    return out

@@ -1,0 +1,110 @@
+"""Seeded synthetic inputs of the benchmark workloads (host side, numpy).
+
+* grid: r_j = 1e-5 exp(j h), h = ln(1e8)/(N-1) (r_max = 1000 a0), Simpson
+  weights;
+* bound states: u_1s = 2 r e^{-r}, u_2p = r^2 e^{-r/2} / (2 sqrt 6),
+  Slater u = (2 zeta)^{3/2}/sqrt(2) r e^{-zeta r};
+* continuum waves: chi_l(r) = sin(k r - l pi/2 + delta) (1 - e^{-r})^{l+1};
+* direct densities (chi_f chi_i, u_f u_i), exchange (u_f chi_i, chi_f u_i).
+
+The objects returned quack like the reference's BoundOrbital / ContinuumWave
+(bound_states.py:59-89, continuum.py:68-119).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .grid import RadialGrid, exp_grid
+
+
+@dataclass
+class Bound:
+    """BoundOrbital stand-in: u_of_r (+ l, energy_au, n_index)."""
+    u_of_r: np.ndarray
+    l: int = 0
+    energy_au: float = -0.5
+    n_index: int = 1
+
+
+@dataclass
+class Wave:
+    """ContinuumWave stand-in with unit asymptotic amplitude."""
+    l: int
+    k_au: float
+    chi_of_r: np.ndarray
+    phase_shift: float = 0.0
+    r_match: float = 0.0
+    idx_match: int = -1
+    eta: float = 0.0
+    sigma_l: float = 0.0
+
+    @property
+    def u_of_r(self) -> np.ndarray:
+        return self.chi_of_r
+
+
+def u1s(r):
+    return 2.0 * r * np.exp(-r)
+
+
+def u2p(r):
+    return r * r * np.exp(-r / 2) / (2.0 * np.sqrt(6.0))
+
+
+def slater(r, zeta=0.6):
+    return (2 * zeta) ** 1.5 / np.sqrt(2.0) * r * np.exp(-zeta * r)
+
+
+def wave_values(r, k, l, delta):
+    return np.sin(k * r - l * np.pi / 2 + delta) * (1.0 - np.exp(-r)) ** (l + 1)
+
+
+def match_index(r, k, l, tol=1e-12) -> int:
+    """Match point of a synthetic wave: first grid index beyond the turning
+    point (l + 1/2)/k where the envelope deficit 1 - (1-e^{-r})^{l+1} < tol."""
+    r_turn = (l + 0.5) / k
+    deficit = -np.expm1((l + 1) * np.log1p(-np.exp(-r)))
+    ok = np.nonzero((deficit < tol) & (r > r_turn))[0]
+    return int(ok[0]) if len(ok) else len(r) - 1
+
+
+def make_wave(r, k, l, delta, with_match=True) -> Wave:
+    return Wave(l=int(l), k_au=float(k), chi_of_r=wave_values(r, k, l, delta), phase_shift=float(delta),
+                idx_match=match_index(r, k, l) if with_match else -1)
+
+
+@dataclass
+class PairDensities:
+    """alpha/beta-ready densities of one channel pair on the full grid."""
+    rho1_dir: np.ndarray
+    rho2_dir: np.ndarray
+    rho1_ex: np.ndarray
+    rho2_ex: np.ndarray
+
+
+def plain_pair(grid: RadialGrid, seed: int = 0, k_range=(0.5, 8.0), l_max: int = 40):
+    """Single channel pair of the kernel-only workload: H 1s -> 2p bound pair,
+    waves with k ~ U[k_range], delta ~ U[0, pi], l ~ U{0..l_max}.  Returns the
+    Simpson-weighted (alpha, beta) of direct and exchange and the draws."""
+    rng = np.random.default_rng(seed)
+    r, w = grid.r, grid.w_simpson
+    ki, kf = rng.uniform(*k_range, size=2)
+    li, lf = rng.integers(0, l_max + 1, size=2)
+    di, df = rng.uniform(0.0, np.pi, size=2)
+    chi_i = wave_values(r, ki, li, di)
+    chi_f = wave_values(r, kf, lf, df)
+    ui, uf = u1s(r), u2p(r)
+    ad = w * chi_f * chi_i
+    bd = w * uf * ui
+    ae = w * uf * chi_i
+    be = w * chi_f * ui
+    draws = dict(k_i=ki, k_f=kf, l_i=int(li), l_f=int(lf), delta_i=di, delta_f=df)
+    return ad, bd, ae, be, draws
+
+
+def kernel_only_workload(n: int = 15000, seed: int = 0):
+    """Kernel-only single pair: N = 15000 exp grid on [1e-5, 1000]."""
+    g = exp_grid(n, 1e-5, 1e3)
+    return g, plain_pair(g, seed)

@@ -440,3 +440,28 @@ def radial_ME_all_L_reduced(r, w_simpson, w_trapz, V_core, U_i, bound_i, bound_f
         return res, dict(setup=s, alpha_dir=ad, beta_dir=bd, alpha_ex=ae, beta_ex=be,
                          contraction_dir=cd, contraction_ex=ce, tails=tails)
     return res
+
+
+def plain_faithful(r, alpha_dir, beta_dir, alpha_ex, beta_ex, L_values):
+    """Plain contraction the way radial_ME_all_L evaluates it when oscillatory
+    quadrature is off (dwba_matrix_elements.py:1057-1080, :1113-1116,
+    :1243-1245, :1288-1289): full N x N ratio / inv_gtr / log_ratio
+    matrices, then per L one N^2 exp and two GEMV pairs.  Used as the CPU
+    baseline of the kernel-only workload.  Returns {L: (I_dir, I_ex)}."""
+    r = np.asarray(r, dtype=np.float64)
+    r1c, r2r = r[:, None], r[None, :]
+    rl = np.minimum(r1c, r2r)
+    rg = np.maximum(r1c, r2r)
+    ratio = rl / rg
+    del rl
+    inv_gtr = 1.0 / rg
+    del rg
+    log_ratio = np.log(np.minimum(ratio, 1.0 - 1e-12) + 1e-300)
+    del ratio
+    out = {}
+    for L in L_values:
+        K = inv_gtr if L == 0 else inv_gtr * np.exp(L * log_ratio)
+        out[L] = (float(np.dot(alpha_dir, np.dot(K, beta_dir))),
+                  float(np.dot(alpha_ex, np.dot(K, beta_ex))))
+        del K
+    return out

@@ -1,0 +1,206 @@
+"""Pins of the CPU oracle: each check is an independent route to the number
+(library routine, closed form, brute force, mathematical invariant), chosen so
+that a dropped term, wrong sign or index fails it."""
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+from scipy.integrate import simpson
+
+from oracle import brute, grid as OG, quadrature as Q, semisep, tail as T
+from dwba_amd import grid as PG
+
+
+# ---------------------------------------------------------------- grid.py
+@pytest.mark.parametrize("n", [3, 5, 101, 3001])
+def test_simpson_weights_equal_scipy_for_odd_n(n):
+    r = OG.exp_grid(n, 1e-4, 200.0)
+    f = np.sin(3 * r) * np.exp(-r / 7) + r ** 0.5
+    assert abs(np.dot(OG.simpson_weights(r), f) - simpson(f, x=r)) <= 1e-13 * np.sum(np.abs(f) * np.gradient(r))
+
+
+def test_simpson_weights_even_n_trailing_trapezoid():
+    r = OG.exp_grid(10, 1e-3, 5.0)
+    w_even = OG.simpson_weights(r)
+    w_odd = OG.simpson_weights(r[:9])
+    h = r[-1] - r[-2]
+    expect = np.concatenate([w_odd, [0.0]])
+    expect[-2] += 0.5 * h
+    expect[-1] += 0.5 * h
+    assert np.array_equal(w_even, expect)
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 1000, 1001, 15000])
+def test_product_grid_weights_bit_identical_to_oracle(n):
+    r = OG.exp_grid(n, 1e-5, 1e3)
+    assert np.array_equal(PG.exp_grid(n, 1e-5, 1e3).r, r)
+    assert np.array_equal(PG.simpson_weights(r), OG.simpson_weights(r))
+    assert np.array_equal(PG.trapezoid_weights(r), OG.trapz_weights(r))
+    g = PG.make_r_grid(1e-4, 200.0, max(n, 2))
+    r2, wt, ws = OG.make_r_grid(1e-4, 200.0, max(n, 2))
+    assert np.array_equal(g.r, r2) and np.array_equal(g.w_simpson, ws) and np.array_equal(g.w_trapz, wt)
+
+
+# ------------------------------------------------------- CC weights
+def test_cc_module_weights_n5_table():
+    x, w = Q.cc_ref(5)
+    np.testing.assert_allclose(w, [1 / 15, 0.6, 11 / 15, 0.6, 1 / 15], rtol=0, atol=1e-15)
+    assert abs(w.sum() - 31 / 15) < 1e-14          # 2.0667, not 2
+    np.testing.assert_allclose(x, np.cos(np.pi * np.arange(5) / 4), atol=1e-16)
+
+
+def test_cc_nodes_tables_and_exactness():
+    _, w5 = Q.clenshaw_curtis_nodes(5, -1.0, 1.0)
+    np.testing.assert_allclose(w5, [1 / 30, 0.6, 11 / 15, 0.6, 1 / 30], atol=1e-15)
+    _, w7 = Q.cc_ref(7)
+    np.testing.assert_allclose(w7[:4], [0.019047619047619, 0.273015873015873, 0.438095238095238,
+                                        0.539682539682540], atol=1e-12)
+    for n in (3, 4, 6, 7, 9):
+        x, w = Q.clenshaw_curtis_nodes(n, 0.3, 2.1)
+        for deg in range(n - 1):
+            exact = (2.1 ** (deg + 1) - 0.3 ** (deg + 1)) / (deg + 1)
+            assert abs(np.dot(w, x ** deg) - exact) < 1e-12 * max(1, abs(exact)), (n, deg)
+
+
+# ------------------------------------------------------- phase nodes / interp
+def test_phase_nodes_neutral_closed_form():
+    nodes = Q.generate_phase_nodes(0.01, 37.0, 3.3, np.pi / 4)
+    n_exp = int(np.ceil((37.0 - 0.01) / (np.pi / 4 / 3.3))) + 1
+    assert len(nodes) == n_exp
+    assert nodes[0] == 0.01 and nodes[-1] == 37.0
+    np.testing.assert_allclose(np.diff(nodes), (37.0 - 0.01) / (n_exp - 1), rtol=1e-9)
+
+
+def test_phase_nodes_coulomb_stepping_invariant():
+    a, b, k, eta, dphi = 1e-3, 80.0, 1.7, -0.9, np.pi / 8
+    nodes = Q.generate_phase_nodes(a, b, k, dphi, eta)
+    for x0, x1 in zip(nodes[:-2], nodes[1:-1]):
+        dr = min(max(dphi / abs(k + eta / max(x0, 1e-10)), 1e-6), (b - a) / 10)
+        assert abs((x1 - x0) - dr) <= 1e-12 * max(1.0, x1)
+    assert nodes[-1] == b
+
+
+def test_adaptive_phase_increment_cases():
+    r = OG.exp_grid(2000, 1e-5, 100.0)
+    assert Q.adaptive_phase_increment(r, 0.1, 1.5708) == np.pi / 4       # clamp to pi/4
+    max_step = 9.0 * (r[-1] - r[-2])
+    d = Q.adaptive_phase_increment(r, 9.0, 1.5708)
+    assert abs(d - np.pi / 4 * max(0.25, (np.pi / 4) / max_step)) < 1e-15 or d == np.pi / 16
+
+
+def test_log_interp_exact_on_log_linear_functions():
+    r = OG.exp_grid(500, 1e-4, 50.0)
+    y = 3.0 - 2.0 * np.log(r)
+    q = np.random.default_rng(0).uniform(1e-4, 50.0, 200)
+    np.testing.assert_allclose(Q.interp_on_radial_grid(q, r, y), 3.0 - 2.0 * np.log(q), rtol=1e-12)
+
+
+def test_linear_interp_on_non_log_grid():
+    r = np.linspace(0.1, 10.0, 300) ** 1.0
+    r[150:] += np.linspace(0, 3.0, 150)      # not log-spaced
+    assert not Q.is_log_spaced_grid(r)
+    y = 4.0 * r - 1.0
+    q = np.random.default_rng(1).uniform(r[0], r[-1], 100)
+    np.testing.assert_allclose(Q.interp_on_radial_grid(q, r, y), 4.0 * q - 1.0, rtol=1e-12)
+
+
+# ------------------------------------------------------- contraction
+@pytest.mark.parametrize("L", [0, 1, 3, 7])
+def test_semisep_equals_mpmath_brute_force(L):
+    rng = np.random.default_rng(L)
+    r = OG.exp_grid(33, 1e-3, 40.0)
+    a, b = rng.standard_normal(33), rng.standard_normal(33)
+    a[30:] = 0.0
+    got = semisep.il_all_L(r, a, b, L)[L]
+    ref = brute.il_brute(r, a, b, L)
+    assert abs(got - ref) <= 1e-15 * np.sum(np.abs(a)) * np.sum(np.abs(b)) * 1e3
+
+
+def test_semisep_equals_dense_float64_kernel():
+    rng = np.random.default_rng(5)
+    r = OG.exp_grid(300, 1e-5, 1e3)
+    a, b = rng.standard_normal(300), rng.standard_normal(300)
+    ref = semisep.il_all_L(r, a, b, 12)
+    for L in range(13):
+        d = a @ semisep.k_dense(r, L) @ b
+        assert abs(d - ref[L]) <= 1e-12 * semisep.abs_bound_all_L(r, a, b, 12)[L]
+
+
+def test_hydrogenic_slater_integrals():
+    """F0(1s,1s) = 5/8 and G1(1s,2p) = 112/2187 (closed forms)."""
+    g = PG.exp_grid(20001, 1e-6, 80.0)
+    u1 = 2 * g.r * np.exp(-g.r)
+    u2 = g.r ** 2 * np.exp(-g.r / 2) / (2 * np.sqrt(6))
+    d11 = g.w_simpson * u1 * u1
+    d12 = g.w_simpson * u1 * u2
+    assert abs(semisep.il_all_L(g.r, d11, d11, 0)[0] - 0.625) < 1e-6   # kink of K on the diagonal: O(h^2) quadrature error
+    assert abs(semisep.il_all_L(g.r, d12, d12, 1)[1] - 112 / 2187) < 1e-7
+
+
+def test_factorisation_for_disjoint_supports():
+    rng = np.random.default_rng(7)
+    r = OG.exp_grid(400, 1e-4, 100.0)
+    a = np.zeros(400); b = np.zeros(400)
+    a[:150] = rng.standard_normal(150)
+    b[200:] = rng.standard_normal(200)
+    I = semisep.il_all_L(r, a, b, 6)
+    for L in range(7):
+        fact = np.sum(a * r ** L) * np.sum(b * r ** (-(L + 1.0)))
+        assert abs(I[L] - fact) <= 1e-13 * (abs(fact) + 1e-300) * 10
+    # symmetry
+    I2 = semisep.il_all_L(r, b, a, 6)
+    np.testing.assert_allclose(I, I2, rtol=1e-15)
+
+
+# ------------------------------------------------------- tail pieces
+def test_filon_segment_is_textbook_filon_plus_two_documented_defects():
+    """The reference's 3-point Filon rule (oscillatory_integrals.py:1165-1277)
+    = textbook Filon (exact on quadratic f, up to the O(theta^8) truncation of
+    the Taylor coefficients) + H beta (f0 e0 + f2 e2)/2 (the missing 1/2 on
+    the end points) + (1/omega - 1) * (-i H alpha (f2 e2 - f0 e0)) (the extra
+    1/omega on the alpha term), H = half width."""
+    for (a, h, w, p) in [(2.0, 0.02, 4.0, 0.3), (7.0, 0.1, 3.0, -1.0), (1.0, 0.05, 9.0, 2.0)]:
+        nodes = np.array([a, a + h / 2, a + h])
+        for f in (lambda x: 1.0 + 0 * x, lambda x: 1.5 - 0.7 * x + 0.2 * x * x):
+            got = T.filon_segment(f(nodes), nodes, w, p)
+            exact = complex(mp.quad(lambda x: f(x) * mp.expj(w * x + p), [a, a + h]))
+            H = h / 2
+            th = w * H
+            t2 = th * th
+            alpha = 2 * th ** 3 / 45 * (1 - t2 / 7 + t2 ** 2 / 189)
+            beta = 2 / 3 + 2 * t2 / 15 - 4 * t2 ** 2 / 105 + 2 * t2 ** 3 / 567
+            e0, e2 = np.exp(1j * (w * a + p)), np.exp(1j * (w * (a + h) + p))
+            f0, f2 = f(np.array(a)), f(np.array(a + h))
+            defect = H * beta * 0.5 * (f0 * e0 + f2 * e2) + (1 / w - 1) * (-1j * H * alpha * (f2 * e2 - f0 * e0))
+            assert abs(got - (exact + defect)) < 1e-7 * abs(exact), (a, h, w, got, exact + defect)
+
+
+def _cos_tail_exact(omega, phi, n, a):
+    """int_a^inf cos(omega r + phi) r^-n dr = Re[e^{i phi} (-i omega)^(n-1) Gamma(1-n, -i omega a)]."""
+    with mp.workdps(30):
+        z = (-1j * omega) ** (n - 1) * mp.gammainc(1 - n, -1j * omega * a)
+        return float(mp.re(mp.expj(phi) * z))
+
+
+@pytest.mark.parametrize("ki,kf,li,lf,L,rm", [(2.0, 1.3, 2, 3, 2, 300.0), (3.1, 0.9, 0, 5, 1, 150.0),
+                                              (1.2, 1.1, 7, 4, 4, 500.0)])
+def test_analytical_tail_against_exact_integral(ki, kf, li, lf, L, rm):
+    """_analytical_multipole_tail (oscillatory_integrals.py:2064-2184) is the
+    two-term integration-by-parts expansion of
+    1/2 int_{r_m}^inf [cos(A-B) - cos(A+B)] / r^(L+1) dr, except that its
+    leading term carries + sin(phi)/(omega r_m^n) where the expansion has
+    - sin(phi)/(omega r_m^n).  Checked against the exact incomplete-gamma
+    integral: ref = exact + (S_minus - S_plus) + O(r_m^-(n+2))."""
+    di, df = 0.4, 1.1
+    n = L + 1
+    got = T.analytical_multipole_tail(rm, ki, kf, di, df, li, lf, L, bound_overlap=1.0)
+    pa, pb = -li * np.pi / 2 + di, -lf * np.pi / 2 + df
+    km, kp = ki - kf, ki + kf
+    exact = 0.5 * (_cos_tail_exact(km, pa - pb, n, rm) - _cos_tail_exact(kp, pa + pb, n, rm))
+    s_m = np.sin(km * rm + pa - pb) / (km * rm ** n)
+    s_p = np.sin(kp * rm + pa + pb) / (kp * rm ** n)
+    scale = abs(s_m) + abs(s_p)
+    assert abs(got - (exact + s_m - s_p)) < 1.0 * scale * (n + 1) * (n + 2) / (km * km * rm * rm)
+    # and the defect is real: the formula is not the exact tail
+    assert abs(got - exact) > 0.5 * abs(s_m - s_p)
