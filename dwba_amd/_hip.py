@@ -39,6 +39,27 @@ class PlainJob(ctypes.Structure):
                 ("beta_ex", c_vp), ("n_rows", c_i32), ("n_cols", c_i32)]
 
 
+class PrepJob(ctypes.Structure):
+    """dwba_prep_job of include/dwba_hip.h."""
+    _fields_ = [("r", c_vp), ("logr", c_vp), ("w", c_vp), ("u_i", c_vp), ("u_f", c_vp), ("chi_i", c_vp),
+                ("chi_f", c_vp), ("V_diff", c_vp), ("phase_nodes", c_vp), ("alpha_dir", c_vp),
+                ("beta_dir", c_vp), ("alpha_ex", c_vp), ("beta_ex", c_vp), ("scalars", c_vp),
+                ("p_start", ctypes.c_double), ("p_step", ctypes.c_double), ("p_end", ctypes.c_double),
+                ("N", c_i32), ("N1", c_i32), ("n_pnodes", c_i32), ("mode", c_i32), ("log1", c_i32),
+                ("log2", c_i32)]
+
+
+class TailJob(ctypes.Structure):
+    """dwba_tail_job of include/dwba_hip.h."""
+    _fields_ = [("k_a", ctypes.c_double), ("delta_a", ctypes.c_double), ("eta_a", ctypes.c_double),
+                ("sigma_a", ctypes.c_double), ("k_b", ctypes.c_double), ("delta_b", ctypes.c_double),
+                ("eta_b", ctypes.c_double), ("sigma_b", ctypes.c_double), ("l_a", c_i32), ("l_b", c_i32),
+                ("r_m", ctypes.c_double), ("r_max", ctypes.c_double), ("moments", c_vp), ("out", c_vp),
+                ("L_max", c_i32), ("kind", c_i32)]
+
+
+c_dp = ctypes.POINTER(ctypes.c_double)
+
 # name -> (restype, argtypes)
 _SIGS = {
     "dwba_version": (c_i32, []),
@@ -49,7 +70,23 @@ _SIGS = {
     "dwba_il_plain_f64": (c_i32, [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_sz, c_vp]),
     "dwba_il_plain_f64_batch_workspace_bytes": (c_sz, [c_i32, c_i32p, c_i32p, c_i32]),
     "dwba_il_plain_f64_batch": (c_i32, [c_i32, c_vp, c_i32p, c_i32p, c_i32, c_vp, c_vp, c_sz, c_vp]),
+    "dwba_fullsplit_prep_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_dp, c_dp, c_i32, c_vp]),
+    "dwba_moments_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]),
+    "dwba_outer_tail_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, ctypes.c_double, c_dp, c_dp, c_vp]),
 }
+
+
+def struct_array_to_device(structs, device):
+    """Copy a ctypes array of structures into a device uint8 tensor."""
+    raw = ctypes.string_at(ctypes.addressof(structs), ctypes.sizeof(structs))
+    return torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
+
+
+def f64_array(x):
+    """Contiguous host float64 ctypes view (kept alive by the returned tuple)."""
+    import numpy as np
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    return a.ctypes.data_as(c_dp), a
 
 
 def exported_symbols():
@@ -166,9 +203,7 @@ def il_plain_f64_batch(jobs, L_max: int, stream=None):
         arr[k] = PlainJob(ptr(r), ptr(ad), ptr(bd), ptr(ae), ptr(be), int(n_rows), int(n_cols))
         nr[k] = int(n_rows)
         nc[k] = int(n_cols)
-    raw = torch.frombuffer(bytearray(ctypes.string_at(ctypes.addressof(arr), ctypes.sizeof(arr))),
-                           dtype=torch.uint8)
-    d_jobs = raw.to(dev)
+    d_jobs = struct_array_to_device(arr, dev)
     ws_bytes = lib.dwba_il_plain_f64_batch_workspace_bytes(n, nr, nc, L_max)
     if ws_bytes == 0:
         raise ValueError("il_plain_f64_batch: invalid shapes")

@@ -143,19 +143,25 @@ __global__ __launch_bounds__(PTHREADS, (NL > 48 ? 1 : (NL > 16 ? 2 : 3))) void p
         partial[(size_t)b * 2 * NL + x] = ((s_red[0][x] + s_red[1][x]) + s_red[2][x]) + s_red[3][x];
 }
 
-// Sum the partials of each job in block order; out is [job][2][L_max+1].
-__global__ void plain_f64_finalize(const int* __restrict__ block_off, int n_jobs, int NL, int L_max,
-                                   const double* __restrict__ partial, double* __restrict__ out) {
+// Sum the partials of each job: one 256-thread block per (job, output value);
+// thread t adds blocks b0+t, b0+t+256, ... and the block reduces in a fixed
+// tree order (bit-reproducible).  out is [job][2][L_max+1].
+__global__ __launch_bounds__(256) void plain_f64_finalize(const int* __restrict__ block_off, int NL, int L_max,
+                                                          const double* __restrict__ partial,
+                                                          double* __restrict__ out) {
+    __shared__ double s_red[4];
     const int k = blockIdx.x;
-    const int b0 = block_off[k], b1 = block_off[k + 1];
     const int nl = L_max + 1;
-    for (int x = threadIdx.x; x < 2 * nl; x += blockDim.x) {
-        const int fam = x / nl, L = x - fam * nl;
-        const int src = fam * NL + L;
-        double s = 0.0;
-        for (int b = b0; b < b1; ++b) s += partial[(size_t)b * 2 * NL + src];
-        out[(size_t)k * 2 * nl + x] = s;
-    }
+    const int x = blockIdx.y;                 // 0 .. 2*nl-1
+    const int fam = x / nl, L = x - fam * nl;
+    const int src = fam * NL + L;
+    const int b0 = block_off[k], b1 = block_off[k + 1];
+    double s = 0.0;
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += 256) s += partial[(size_t)b * 2 * NL + src];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[(size_t)k * 2 * nl + x] = ((s_red[0] + s_red[1]) + s_red[2]) + s_red[3];
 }
 
 // Per-job block counts -> exclusive prefix sums block_off[0..n_jobs].
@@ -250,7 +256,7 @@ int launch_plain(const PlainPlan& p, int n_jobs, const dwba_plain_job* d_jobs, i
         }
         DWBA_LAUNCH_CHECK("plain_f64_kernel");
     }
-    hipLaunchKernelGGL(plain_f64_finalize, dim3(n_jobs), dim3(128), 0, s, d_off, n_jobs, p.NL, L_max,
+    hipLaunchKernelGGL(plain_f64_finalize, dim3(n_jobs, 2 * (L_max + 1)), dim3(256), 0, s, d_off, p.NL, L_max,
                        d_part, d_out);
     DWBA_LAUNCH_CHECK("plain_f64_finalize");
     return DWBA_OK;

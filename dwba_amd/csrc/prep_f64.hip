@@ -1,0 +1,325 @@
+// FP64 preparation of the alpha / beta vectors of one channel pair so that
+// every quadrature branch of radial_ME_all_L becomes I_L = alpha^T K_L beta
+// (dwba_matrix_elements.py:1030-1042, :1136-1245, :1281-1289 and
+// oscillatory_kernel_integral_2d, oscillatory_integrals.py:2555-2777).
+//
+//  mode PLAIN (no oscillatory quadrature, or k_i + k_f <= k_threshold):
+//     alpha_dir = (w_lim chi_f) chi_i, beta_dir = (w_lim u_f) u_i,
+//     alpha_ex  = (w_lim u_f) chi_i,   beta_ex  = (w_lim chi_f) u_i,
+//     w_lim = w on [0, N1), 0 beyond.
+//  mode FILON: the outer Clenshaw-Curtis sum over phase intervals of
+//     interp(rho1)(x_c) interp(K beta)(x_c) W_c is linear in K beta, hence
+//     alpha_j = sum_c W_c interp(rho1)(x_c) lambda_c(j), lambda_c = the
+//     interpolation weights of node c on r[:N1] (np.interp in log r on
+//     log-spaced grids, :279-302).  beta_dir = (u_f u_i) w on the full grid;
+//     beta_ex scatters W_c interp(chi_f u_i)(x_c) onto the two columns that
+//     bracket x_c with the linear-in-r kernel interpolation weights of
+//     :2729-2742.
+//  mode STANDARD (Filon branch without usable phase intervals, :2596-2613):
+//     alpha_* = rho1 w on [0, N1), beta_* = rho2 w on [0, N).
+//
+// The CC scatter is written as a gather -- grid point i visits the nodes of
+// the phase intervals that overlap [r_{i-1}, r_{i+1}] in a fixed order -- so
+// the result is deterministic without atomics.
+//
+// The same block also reduces the per-pair scalars: the L = 0 correction
+// sums (:1249-1253, :1291-1296) and the multipole moments
+// M_L = sum_j w_j r_j^L u_f u_i (:1167).
+#include "common.hpp"
+#include "prep.hpp"
+
+namespace dwba {
+namespace {
+
+constexpr int PREP_THREADS = 256;
+
+__device__ inline double clean(double v) { return isfinite(v) ? v : 0.0; }
+
+// Phase node m of a job (uniform: numpy linspace; else the stored array).
+__device__ inline double phase_node(const dwba_prep_job& J, int m) {
+    if (J.phase_nodes) return J.phase_nodes[m];
+    if (m == J.n_pnodes - 1) return J.p_end;
+    return __dadd_rn(__dmul_rn((double)m, J.p_step), J.p_start);
+}
+
+// Interval index containing x (clamped to [0, n_int-1]).
+__device__ inline int interval_of(const dwba_prep_job& J, double x, int n_int) {
+    int m;
+    if (!J.phase_nodes) {
+        const double q = (x - J.p_start) / J.p_step;
+        m = q <= 0.0 ? 0 : (q >= (double)n_int ? n_int - 1 : (int)q);
+    } else {
+        int lo = 0, hi = J.n_pnodes - 1;  // largest m with p[m] <= x
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (J.phase_nodes[mid] <= x) lo = mid; else hi = mid - 1;
+        }
+        m = lo;
+    }
+    return m < 0 ? 0 : (m > n_int - 1 ? n_int - 1 : m);
+}
+
+// np.interp cell on a table xp[0..n-1] (ascending): largest j <= n-2 with
+// xp[j] <= X; X is already clipped to [xp[0], xp[n-1]].
+__device__ inline int cell_of(const double* xp, int n, double X) {
+    int lo = 0, hi = n - 2;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (xp[mid] <= X) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Value of np.interp(X, xp, fp) at cell j (slope form of numpy's kernel).
+__device__ inline double interp_at(const double* xp, int n, double X, int j, double f0, double f1,
+                                   double f_last) {
+    if (X >= xp[n - 1]) return f_last;
+    if (X == xp[j]) return f0;
+    const double slope = (f1 - f0) / (xp[j + 1] - xp[j]);
+    return slope * (X - xp[j]) + f0;
+}
+
+struct CCTable {
+    double x[DWBA_MAX_CC];
+    double w[DWBA_MAX_CC];
+    int n;
+};
+
+__global__ __launch_bounds__(PREP_THREADS) void prep_f64_kernel(const dwba_prep_job* __restrict__ jobs,
+                                                                CCTable cc) {
+    const dwba_prep_job J = jobs[blockIdx.x];
+    const int N = J.N, N1 = J.N1;
+    __shared__ double s_red[PREP_THREADS / 64][4];
+
+    // ---------------- L = 0 correction factors ----------------
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    // over the whole grid with w_lim = 0 beyond N1, as the reference's dot
+    // products do (a non-finite density beyond N1 still propagates)
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const double ui = J.u_i[i], uf = J.u_f[i], ci = J.chi_i[i], cf = J.chi_f[i];
+        const double w = i < N1 ? J.w[i] : 0.0;
+        const double vd = J.V_diff ? J.V_diff[i] : 0.0;
+        acc[0] += (w * uf) * ui;              // rho2_dir_w
+        acc[1] += ((w * cf) * ci) * vd;       // rho1_dir_w . V_diff
+        acc[2] += (w * cf) * ui;              // rho2_ex_w
+        acc[3] += ((w * uf) * ci) * vd;       // rho1_ex_w . V_diff
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double sum = wave_sum(acc[q]);
+        if (lane == 0) s_red[wave][q] = sum;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double sum = 0.0;
+        for (int wv = 0; wv < PREP_THREADS / 64; ++wv) sum += s_red[wv][threadIdx.x];
+        J.scalars[threadIdx.x] = sum;
+    }
+
+    // ---------------- alpha / beta ----------------
+    if (J.mode == DWBA_PREP_PLAIN) {
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+            const double wl = i < N1 ? J.w[i] : 0.0;
+            const double ui = J.u_i[i], uf = J.u_f[i], ci = J.chi_i[i], cf = J.chi_f[i];
+            J.alpha_dir[i] = (wl * cf) * ci;
+            J.beta_dir[i] = (wl * uf) * ui;
+            J.alpha_ex[i] = (wl * uf) * ci;
+            J.beta_ex[i] = (wl * cf) * ui;
+        }
+        return;
+    }
+    if (J.mode == DWBA_PREP_ZERO) {
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+            J.alpha_dir[i] = 0.0; J.beta_dir[i] = 0.0; J.alpha_ex[i] = 0.0; J.beta_ex[i] = 0.0;
+        }
+        return;
+    }
+    // beta_dir = (u_f u_i) w on the full grid for both STANDARD and FILON
+    for (int i = threadIdx.x; i < N; i += blockDim.x)
+        J.beta_dir[i] = clean(J.u_f[i] * J.u_i[i]) * J.w[i];
+    if (J.mode == DWBA_PREP_STANDARD) {
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+            const double w = J.w[i];
+            const double wl = i < N1 ? w : 0.0;
+            J.alpha_dir[i] = clean(J.chi_f[i] * J.chi_i[i]) * wl;
+            J.alpha_ex[i] = clean(J.u_f[i] * J.chi_i[i]) * wl;
+            J.beta_ex[i] = clean(J.chi_f[i] * J.u_i[i]) * w;
+        }
+        return;
+    }
+
+    // FILON
+    const int n_int = J.n_pnodes - 1;
+    const double* r = J.r;
+    const double* lr = J.logr;                    // log(max(r, 1e-300))
+    const double* xp1 = J.log1 ? lr : r;          // interpolation table on r[:N1]
+    const double* xp2 = J.log2 ? lr : r;          // interpolation table on r[:N]
+    const double r0 = r[0], r1e = r[N1 - 1], r2e = r[N - 1];
+    const double rho_d_last = clean(J.chi_f[N1 - 1] * J.chi_i[N1 - 1]);
+    const double rho_e_last = clean(J.u_f[N1 - 1] * J.chi_i[N1 - 1]);
+    const double rho2_last = clean(J.chi_f[N - 1] * J.u_i[N - 1]);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        double ad = 0.0, ae = 0.0, be = 0.0;
+        if (i <= N1) {
+            const double lo_x = r[i > 0 ? i - 1 : 0];
+            const double hi_x = r[i + 1 < N ? i + 1 : N - 1];
+            int m0 = interval_of(J, lo_x, n_int) - 1;
+            int m1 = interval_of(J, hi_x, n_int) + 1;
+            if (m0 < 0) m0 = 0;
+            if (m1 > n_int - 1) m1 = n_int - 1;
+            for (int m = m0; m <= m1; ++m) {
+                const double a = phase_node(J, m), b = phase_node(J, m + 1);
+                if (!(b - a > 1e-12)) continue;
+                const double hw = 0.5 * (b - a);
+                for (int k = 0; k < cc.n; ++k) {
+                    const double x = hw * (cc.x[k] + 1.0) + a;
+                    const double W = cc.w[k] * hw;
+                    // ---- alpha: np.interp on r[:N1] ----
+                    if (i < N1) {
+                        const double xc = fmin(fmax(x, r0), r1e);
+                        const double X = J.log1 ? log(fmax(xc, r0 > 1e-300 ? r0 : 1e-300)) : xc;
+                        const int j = cell_of(xp1, N1, X);
+                        double t;
+                        if (X >= xp1[N1 - 1]) t = 1.0;
+                        else if (X == xp1[j]) t = 0.0;
+                        else t = (X - xp1[j]) / (xp1[j + 1] - xp1[j]);
+                        double lam = 0.0;
+                        if (j == i) lam = 1.0 - t;
+                        else if (j + 1 == i) lam = t;
+                        if (lam != 0.0) {
+                            const double fd0 = clean(J.chi_f[j] * J.chi_i[j]);
+                            const double fd1 = clean(J.chi_f[j + 1] * J.chi_i[j + 1]);
+                            const double fe0 = clean(J.u_f[j] * J.chi_i[j]);
+                            const double fe1 = clean(J.u_f[j + 1] * J.chi_i[j + 1]);
+                            ad += (W * interp_at(xp1, N1, X, j, fd0, fd1, rho_d_last)) * lam;
+                            ae += (W * interp_at(xp1, N1, X, j, fe0, fe1, rho_e_last)) * lam;
+                        }
+                    }
+                    // ---- beta_ex: kernel interpolation linear in r on r[:N] ----
+                    {
+                        int ir;  // searchsorted(r, x, 'left') clipped to [1, N-1]
+                        int lo = 0, hi = N;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (r[mid] < x) lo = mid + 1; else hi = mid;
+                        }
+                        ir = lo < 1 ? 1 : (lo > N - 1 ? N - 1 : lo);
+                        const int il = ir - 1;
+                        if (il == i || ir == i) {
+                            const double wr = (x - r[il]) / (r[ir] - r[il] + 1e-30);
+                            const double wl = 1.0 - wr;
+                            const double xc = fmin(fmax(x, r0), r2e);
+                            const double X = J.log2 ? log(fmax(xc, r0 > 1e-300 ? r0 : 1e-300)) : xc;
+                            const int j = cell_of(xp2, N, X);
+                            const double f0 = clean(J.chi_f[j] * J.u_i[j]);
+                            const double f1 = clean(J.chi_f[j + 1] * J.u_i[j + 1]);
+                            const double v = W * interp_at(xp2, N, X, j, f0, f1, rho2_last);
+                            if (il == i) be += v * wl;
+                            if (ir == i) be += v * wr;
+                        }
+                    }
+                }
+            }
+        }
+        J.alpha_dir[i] = ad;
+        J.alpha_ex[i] = ae;
+        J.beta_ex[i] = be;
+    }
+}
+
+// Multipole moments: block (set, L) reduces sum_j ((w_j r_j^L) u_f,j) u_i,j.
+__global__ __launch_bounds__(PREP_THREADS) void moments_f64_kernel(const double* const* __restrict__ R,
+                                                                   const double* const* __restrict__ Wt,
+                                                                   const double* const* __restrict__ UI,
+                                                                   const double* const* __restrict__ UF,
+                                                                   const int* __restrict__ NN, int L_max,
+                                                                   double* __restrict__ out) {
+    __shared__ double s_red[PREP_THREADS / 64];
+    const int set = blockIdx.x, L = blockIdx.y;
+    const double *r = R[set], *w = Wt[set], *ui = UI[set], *uf = UF[set];
+    const int n = NN[set];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double rl = L == 0 ? 1.0 : (L == 1 ? r[i] : (L == 2 ? r[i] * r[i] : pow(r[i], (double)L)));
+        acc += ((w[i] * rl) * uf[i]) * ui[i];
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int k = 0; k < PREP_THREADS / 64; ++k) sum += s_red[k];
+        out[(size_t)set * (L_max + 1) + L] = sum;
+    }
+}
+
+// Coulomb-phase node stepping of generate_phase_nodes (oscillatory_integrals.py:1685-1713):
+// r_{n+1} = r_n + clip(dphi / |k + eta / max(r_n, 1e-10)|, 1e-6, (b - a)/10), capped at b,
+// at most 10001 nodes.  One thread per job; node count written to the job.
+__global__ void coulomb_nodes_kernel(dwba_prep_job* __restrict__ jobs, int n_jobs, const double* __restrict__ par) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_jobs) return;
+    dwba_prep_job& J = jobs[k];
+    if (J.mode != DWBA_PREP_FILON || J.phase_nodes == nullptr) return;
+    const double a = par[4 * k + 0], b = par[4 * k + 1], kt = par[4 * k + 2], eta = par[4 * k + 3];
+    const double dphi = J.p_step;  // phase increment for Coulomb jobs
+    double* nodes = const_cast<double*>(J.phase_nodes);
+    int n = 0;
+    nodes[n++] = a;
+    double r = a;
+    const double cap = (b - a) / 10;
+    while (r < b) {
+        const double pp = kt + eta / fmax(r, 1e-10);
+        double dr = dphi / fabs(pp);
+        dr = fmin(dr, cap);
+        dr = fmax(dr, 1e-6);
+        double rn = r + dr;
+        if (rn >= b) rn = b;
+        nodes[n++] = rn;
+        r = rn;
+        if (n > 10000) break;
+    }
+    J.n_pnodes = n;
+}
+
+}  // namespace
+
+int launch_prep(int n_jobs, dwba_prep_job* d_jobs, const double* d_coulomb_par, int any_coulomb,
+                const double* cc_x, const double* cc_w, int n_cc, hipStream_t s) {
+    CCTable cc{};
+    if (n_cc < 2 || n_cc > DWBA_MAX_CC) return set_error(DWBA_ERR_ARG, "prep: CC node count %d outside [2, %d]", n_cc, DWBA_MAX_CC);
+    for (int k = 0; k < n_cc; ++k) { cc.x[k] = cc_x[k]; cc.w[k] = cc_w[k]; }
+    cc.n = n_cc;
+    if (any_coulomb) {
+        hipLaunchKernelGGL(coulomb_nodes_kernel, dim3((n_jobs + 63) / 64), dim3(64), 0, s, d_jobs, n_jobs, d_coulomb_par);
+        DWBA_LAUNCH_CHECK("coulomb_nodes_kernel");
+    }
+    hipLaunchKernelGGL(prep_f64_kernel, dim3(n_jobs), dim3(PREP_THREADS), 0, s, d_jobs, cc);
+    DWBA_LAUNCH_CHECK("prep_f64_kernel");
+    return DWBA_OK;
+}
+
+}  // namespace dwba
+
+extern "C" int32_t dwba_moments_f64(int32_t n_sets, const double* const* d_r, const double* const* d_w,
+                                    const double* const* d_u_i, const double* const* d_u_f, const int32_t* d_n,
+                                    int32_t L_max, double* d_out, void* stream) {
+    DWBA_REQUIRE(dwba::initialised(), "dwba_moments_f64: call dwba_init first");
+    DWBA_REQUIRE(n_sets > 0 && L_max >= 0 && L_max < 65535 && d_r && d_w && d_u_i && d_u_f && d_n && d_out,
+                 "dwba_moments_f64: bad arguments");
+    hipLaunchKernelGGL(dwba::moments_f64_kernel, dim3(n_sets, L_max + 1), dim3(dwba::PREP_THREADS), 0,
+                       reinterpret_cast<hipStream_t>(stream), d_r, d_w, d_u_i, d_u_f, d_n, L_max, d_out);
+    DWBA_LAUNCH_CHECK("moments_f64_kernel");
+    return DWBA_OK;
+}
+
+extern "C" int32_t dwba_fullsplit_prep_f64(int32_t n_jobs, dwba_prep_job* d_jobs, const double* d_coulomb_par,
+                                           int32_t any_coulomb, const double* h_cc_x, const double* h_cc_w,
+                                           int32_t n_cc, void* stream) {
+    DWBA_REQUIRE(dwba::initialised(), "dwba_fullsplit_prep_f64: call dwba_init first");
+    DWBA_REQUIRE(n_jobs > 0 && d_jobs && h_cc_x && h_cc_w, "dwba_fullsplit_prep_f64: bad arguments");
+    DWBA_REQUIRE(!any_coulomb || d_coulomb_par, "dwba_fullsplit_prep_f64: Coulomb jobs need d_coulomb_par");
+    return dwba::launch_prep(n_jobs, d_jobs, d_coulomb_par, any_coulomb, h_cc_x, h_cc_w, n_cc,
+                             reinterpret_cast<hipStream_t>(stream));
+}

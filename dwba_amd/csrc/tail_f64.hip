@@ -1,0 +1,417 @@
+// FP64 outer tail of the direct integral, one wavefront per (pair, L).
+//
+// Restates, on the GPU, the CPU chain radial_ME_all_L uses for L >= 1
+// excitation tails (dwba_matrix_elements.py:1165-1192, :1216-1241, and the
+// legacy analytic tail :1257-1268):
+//
+//   I_out = 1/2 [Re J(Phi_-) - Re J(Phi_+)] + T(r_max)        (advanced/full_split)
+//   I_out = T(r_match)                                        (legacy)
+//   J(Phi) = int_{r_m}^{r_max} M_L r^-(L+1) exp(i Phi(r)) dr
+//
+// J is the selector compute_outer_integral_oscillatory
+// (oscillatory_integrals.py:1361-1451): Filon with the constant frequency
+// |Phi'(mid)| (filon_oscillatory_integral :1280-1354, _filon_segment_complex
+// :1165-1277, segment cap DWBA_MAX_FILON_SEGMENTS and node up-scaling past
+// DWBA_MAX_EFFECTIVE_DPHI) when |Phi''(mid)| h^2 < 0.1, else Levin collocation
+// on 8 Chebyshev-Lobatto nodes per segment (:1024-1158).  Phases are the
+// sinA*sinB phases of _get_dwba_outer_phase_functions (:350-456); T is
+// _analytical_multipole_tail (:2064-2184).  The reference sums segment
+// contributions with math.fsum; here each lane keeps a double-double
+// (TwoSum) accumulator and the lanes are combined in double-double, which
+// rounds to the same value except in pathological cancellation.
+//
+// Lanes of the wavefront take segments round-robin.
+#include "common.hpp"
+
+namespace dwba {
+namespace {
+
+constexpr double PI = 3.14159265358979323846;
+
+struct DD {
+    double hi, lo;
+};
+
+__device__ inline DD dd_add(DD a, double b) {
+    const double s = a.hi + b;
+    const double bb = s - a.hi;
+    const double e = (a.hi - (s - bb)) + (b - bb);
+    const double lo = a.lo + e;
+    const double hi = s + lo;
+    return {hi, lo - (hi - s)};
+}
+
+__device__ inline DD dd_add(DD a, DD b) {
+    const double s = a.hi + b.hi;
+    const double bb = s - a.hi;
+    const double e = (a.hi - (s - bb)) + (b.hi - bb);
+    const double lo = (a.lo + b.lo) + e;
+    const double hi = s + lo;
+    return {hi, lo - (hi - s)};
+}
+
+__device__ inline DD dd_wave_sum(DD v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        DD o{__shfl_xor(v.hi, m, 64), __shfl_xor(v.lo, m, 64)};
+        v = dd_add(v, o);
+    }
+    return v;
+}
+
+struct cplx {
+    double re, im;
+};
+__device__ inline cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ inline cplx csub(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
+__device__ inline cplx cmul(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ inline cplx cscale(cplx a, double s) { return {a.re * s, a.im * s}; }
+__device__ inline cplx cexpi(double phi) {
+    double s, c;
+    sincos(phi, &s, &c);
+    return {c, s};
+}
+__device__ inline cplx cdiv(cplx a, cplx b) {
+    // Smith's algorithm
+    if (fabs(b.re) >= fabs(b.im)) {
+        const double r = b.im / b.re, d = b.re + b.im * r;
+        return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+    }
+    const double r = b.re / b.im, d = b.re * r + b.im;
+    return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+
+struct Phase {
+    double ka, kb, ea, eb, la, lb;  // l as double
+    bool use_ea, use_eb, use_la, use_lb;
+    double kmin, kplus, cmin, cplus;
+};
+
+__device__ inline double phi(const Phase& P, int sgn, double r) {
+    double res;
+    if (sgn < 0) {
+        res = P.kmin * r + P.cmin;
+        if (P.use_ea) res += P.ea * log(2 * P.ka * r + 1e-30);
+        if (P.use_eb) res -= P.eb * log(2 * P.kb * r + 1e-30);
+        if (P.use_la) res -= (P.la * (P.la + 1)) / (2.0 * P.ka * r);
+        if (P.use_lb) res += (P.lb * (P.lb + 1)) / (2.0 * P.kb * r);
+    } else {
+        res = P.kplus * r + P.cplus;
+        if (P.use_ea) res += P.ea * log(2 * P.ka * r + 1e-30);
+        if (P.use_eb) res += P.eb * log(2 * P.kb * r + 1e-30);
+        if (P.use_la) res -= (P.la * (P.la + 1)) / (2.0 * P.ka * r);
+        if (P.use_lb) res -= (P.lb * (P.lb + 1)) / (2.0 * P.kb * r);
+    }
+    return res;
+}
+
+__device__ inline double dphi(const Phase& P, int sgn, double r) {
+    double res = sgn < 0 ? P.kmin : P.kplus;
+    if (P.use_ea) res += P.ea / r;
+    if (P.use_eb) res = sgn < 0 ? res - P.eb / r : res + P.eb / r;
+    if (P.use_la) res += (P.la * (P.la + 1)) / (2.0 * P.ka * (r * r));
+    if (P.use_lb) {
+        const double t = (P.lb * (P.lb + 1)) / (2.0 * P.kb * (r * r));
+        res = sgn < 0 ? res - t : res + t;
+    }
+    return res;
+}
+
+__device__ inline double d2phi(const Phase& P, int sgn, double r) {
+    double res = 0.0;
+    if (P.use_ea) res -= P.ea / (r * r);
+    if (P.use_eb) res = sgn < 0 ? res + P.eb / (r * r) : res - P.eb / (r * r);
+    if (P.use_la) res -= (P.la * (P.la + 1)) / (P.ka * pow(r, 3.0));
+    if (P.use_lb) {
+        const double t = (P.lb * (P.lb + 1)) / (P.kb * pow(r, 3.0));
+        res = sgn < 0 ? res + t : res - t;
+    }
+    return res;
+}
+
+struct Env {
+    double M;   // moment
+    int n1;     // L + 1
+};
+__device__ inline double envelope(const Env& E, double r) { return r > 1e-6 ? E.M / pow(r, (double)E.n1) : 0.0; }
+
+// scipy.integrate.simpson(y, x) for an odd number of points (pairs of intervals)
+__device__ inline double simpson_odd(const double* f, const double* x, int n) {
+    double s = 0.0;
+    for (int i = 0; i + 2 < n; i += 2) {
+        const double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1];
+        const double hsum = h0 + h1, hprod = h0 * h1;
+        const double h0divh1 = h1 != 0.0 ? h0 / h1 : 0.0;
+        const double t = hsum / 6.0 *
+                         (f[i] * (2.0 - (h0divh1 != 0.0 ? 1.0 / h0divh1 : 0.0)) +
+                          f[i + 1] * (hsum * (hprod != 0.0 ? hsum / hprod : 0.0)) + f[i + 2] * (2.0 - h0divh1));
+        s += t;
+    }
+    return s;
+}
+
+// _filon_segment_complex with 3 nodes
+__device__ inline cplx filon3(const double* f, const double* x, double omega, double p0) {
+    const double a = x[0], b = x[2];
+    const double h = b - a;
+    if (h < 1e-15) return {0.0, 0.0};
+    if (fabs(omega * h) < 1e-3) {
+        const double mid = omega * (a + b) / 2 + p0;
+        return cscale(cexpi(mid), simpson_odd(f, x, 3));
+    }
+    const double th = omega * h / 2;
+    double alpha, beta, gamma;
+    if (fabs(th) < 0.3) {
+        const double t2 = th * th, t3 = th * th * th;
+        alpha = 2 * t3 / 45 * (1 - t2 / 7 + t2 * t2 / 189);
+        beta = 2.0 / 3 + 2 * t2 / 15 - 4 * (t2 * t2) / 105 + 2 * (t2 * t2 * t2) / 567;
+        gamma = 4.0 / 3 - 2 * t2 / 15 + 1 * (t2 * t2) / 210 - (t2 * t2 * t2) / 11340;
+    } else {
+        double st, ct, s2, c2;
+        sincos(th, &st, &ct);
+        sincos(2 * th, &s2, &c2);
+        const double t2 = th * th, t3 = th * th * th;
+        alpha = (t2 + th * s2 / 2 - 2 * (st * st)) / t3;
+        beta = 2 * (th * (1 + c2) - s2) / t3;
+        gamma = 4 * (st - th * ct) / t3;
+    }
+    const cplx e0 = cexpi(omega * x[0] + p0), e1 = cexpi(omega * x[1] + p0), e2 = cexpi(omega * x[2] + p0);
+    const cplx f2e2 = cscale(e2, f[2]), f0e0 = cscale(e0, f[0]);
+    // alpha * (-1j/omega) * (f2 e2 - f0 e0)
+    const cplx d = csub(f2e2, f0e0);
+    const cplx ta = cmul(cplx{0.0, -alpha / omega}, d);
+    const cplx tb = cscale(cadd(f0e0, f2e2), beta);
+    const cplx tg = cscale(e1, gamma * f[1]);
+    return cscale(cadd(cadd(ta, tb), tg), h / 2);
+}
+
+// Filon segment with n (odd, 5..15) equally spaced nodes on [a, b]
+__device__ cplx filon_segment(const Env& E, double a, double b, int n, double omega, double p0) {
+    double x[15], f[15];
+    const double step = (b - a) / (n - 1);
+    for (int k = 0; k < n; ++k) {
+        x[k] = (k == n - 1) ? b : __dadd_rn(__dmul_rn((double)k, step), a);
+        f[k] = envelope(E, x[k]);
+    }
+    const double h = x[n - 1] - x[0];
+    if (h < 1e-15) return {0.0, 0.0};
+    if (fabs(omega * h) < 1e-3) {
+        const double mid = omega * (x[0] + x[n - 1]) / 2 + p0;
+        return cscale(cexpi(mid), simpson_odd(f, x, n));
+    }
+    cplx res{0.0, 0.0};
+    for (int i = 0; i + 2 < n; i += 2) res = cadd(res, filon3(f + i, x + i, omega, p0));
+    return res;
+}
+
+struct LevinTab {
+    double D[8][8];   // Chebyshev differentiation matrix on [-1, 1] (descending Lobatto points)
+    double xr[8];     // ascending Lobatto points
+};
+
+// _levin_segment_complex: solve (D 2/h + i diag(Phi')) u = f, return u_7 e^{i Phi_7} - u_0 e^{i Phi_0}
+__device__ cplx levin_segment(const Env& E, const Phase& P, int sgn, double a, double b, const LevinTab& T) {
+    double x[8];
+    for (int k = 0; k < 8; ++k) x[k] = 0.5 * (b - a) * (T.xr[k] + 1) + a;
+    const double h = x[7] - x[0];
+    if (h < 1e-15) return {0.0, 0.0};
+    const double sc = 2.0 / h;
+    cplx A[8][8];
+    cplx u[8];
+    for (int i = 0; i < 8; ++i) {
+        for (int j = 0; j < 8; ++j) A[i][j] = {T.D[i][j] * sc, 0.0};
+        A[i][i].im = dphi(P, sgn, x[i]);
+        u[i] = {envelope(E, x[i]), 0.0};
+    }
+    // LU with partial pivoting on |re| + |im| (LAPACK izamax convention)
+    for (int c = 0; c < 8; ++c) {
+        int p = c;
+        double best = fabs(A[c][c].re) + fabs(A[c][c].im);
+        for (int i = c + 1; i < 8; ++i) {
+            const double v = fabs(A[i][c].re) + fabs(A[i][c].im);
+            if (v > best) { best = v; p = i; }
+        }
+        if (p != c) {
+            for (int j = 0; j < 8; ++j) { const cplx t = A[c][j]; A[c][j] = A[p][j]; A[p][j] = t; }
+            const cplx t = u[c]; u[c] = u[p]; u[p] = t;
+        }
+        const cplx piv = A[c][c];
+        if (piv.re == 0.0 && piv.im == 0.0) continue;
+        for (int i = c + 1; i < 8; ++i) {
+            const cplx l = cdiv(A[i][c], piv);
+            for (int j = c + 1; j < 8; ++j) A[i][j] = csub(A[i][j], cmul(l, A[c][j]));
+            u[i] = csub(u[i], cmul(l, u[c]));
+        }
+    }
+    for (int i = 7; i >= 0; --i) {
+        cplx s = u[i];
+        for (int j = i + 1; j < 8; ++j) s = csub(s, cmul(A[i][j], u[j]));
+        u[i] = cdiv(s, A[i][i]);
+    }
+    return csub(cmul(u[7], cexpi(phi(P, sgn, x[7]))), cmul(u[0], cexpi(phi(P, sgn, x[0]))));
+}
+
+// non-oscillatory branch (scipy quad in the reference): composite 8-point
+// Gauss-Legendre on 512 panels, lanes over panels
+__device__ cplx gauss_quad(const Env& E, const Phase& P, int sgn, double a, double b, bool const_phase, double p0,
+                           int lane) {
+    const double gx[4] = {0.1834346424956498, 0.5255324099163290, 0.7966664774136267, 0.9602898564975363};
+    const double gw[4] = {0.3626837833783620, 0.3137066458778873, 0.2223810344533745, 0.1012285362903763};
+    const int NP = 512;
+    DD re{0, 0}, im{0, 0};
+    for (int pnl = lane; pnl < NP; pnl += 64) {
+        const double pa = a + (b - a) * pnl / NP, pb = a + (b - a) * (pnl + 1) / NP;
+        const double c = 0.5 * (pa + pb), hw = 0.5 * (pb - pa);
+        for (int q = 0; q < 8; ++q) {
+            const double xx = c + (q < 4 ? -gx[3 - q] : gx[q - 4]) * hw;
+            const double ww = (q < 4 ? gw[3 - q] : gw[q - 4]) * hw;
+            const double ph = const_phase ? p0 : phi(P, sgn, xx);
+            const double fv = envelope(E, xx) * ww;
+            re = dd_add(re, fv * cos(ph));
+            im = dd_add(im, fv * sin(ph));
+        }
+    }
+    re = dd_wave_sum(re);
+    im = dd_wave_sum(im);
+    return {re.hi, im.hi};
+}
+
+struct TailParams {
+    int max_filon_segments;
+    double max_effective_dphi;
+    LevinTab lev;
+};
+
+// compute_outer_integral_oscillatory for one sign, reduced over the wavefront
+__device__ cplx outer_integral(const Env& E, const Phase& P, int sgn, double r_m, double r_max,
+                               const TailParams& TP, int lane) {
+    const double delta_phi = PI / 4;
+    if (r_max <= r_m + 1e-10) return {0.0, 0.0};
+    const double mid = (r_m + r_max) / 2;
+    const double om = fabs(dphi(P, sgn, mid));
+    if (om < 1e-10) return gauss_quad(E, P, sgn, r_m, r_max, false, 0.0, lane);
+    const double h = delta_phi / om;
+    const double lin = fabs(d2phi(P, sgn, mid)) * h * h;
+    DD re{0, 0}, im{0, 0};
+    if (lin < 0.1) {
+        const double p0 = phi(P, sgn, r_m) - om * r_m;
+        // filon_oscillatory_integral
+        const double dr_seg = delta_phi / fabs(om);
+        const double nraw_d = ceil((r_max - r_m) / dr_seg);
+        const long nraw = nraw_d < 1.0 ? 1 : (long)nraw_d;
+        const long nseg = nraw < TP.max_filon_segments ? nraw : TP.max_filon_segments;
+        const double eff = fabs(om) * (r_max - r_m) / (double)(nseg > 1 ? nseg : 1);
+        int neff = 5;
+        if (eff > TP.max_effective_dphi) {
+            const int scale = (int)ceil(eff / TP.max_effective_dphi);
+            neff = 5 + 2 * (scale - 1);
+            if (neff < 5) neff = 5;
+            if (neff > 15) neff = 15;
+        }
+        const double bstep = (r_max - r_m) / (double)nseg;
+        for (long s = lane; s < nseg; s += 64) {
+            const double a = __dadd_rn(__dmul_rn((double)s, bstep), r_m);
+            const double b = (s + 1 == nseg) ? r_max : __dadd_rn(__dmul_rn((double)(s + 1), bstep), r_m);
+            const cplx c = filon_segment(E, a, b, neff, om, p0);
+            re = dd_add(re, c.re);
+            im = dd_add(im, c.im);
+        }
+    } else {
+        const double nseg_d = ceil((r_max - r_m) / h);
+        const long nseg = nseg_d < 1.0 ? 1 : (long)nseg_d;
+        const double bstep = (r_max - r_m) / (double)nseg;
+        for (long s = lane; s < nseg; s += 64) {
+            const double a = __dadd_rn(__dmul_rn((double)s, bstep), r_m);
+            const double b = (s + 1 == nseg) ? r_max : __dadd_rn(__dmul_rn((double)(s + 1), bstep), r_m);
+            const cplx c = levin_segment(E, P, sgn, a, b, TP.lev);
+            re = dd_add(re, c.re);
+            im = dd_add(im, c.im);
+        }
+    }
+    re = dd_wave_sum(re);
+    im = dd_wave_sum(im);
+    return {re.hi + re.lo, im.hi + im.lo};
+}
+
+__device__ double analytic_tail(double r_m, const dwba_tail_job& J, int L, double overlap) {
+    if (r_m < 1e-3 || fabs(overlap) < 1e-12) return 0.0;
+    const double lt_i = fabs(J.eta_a) > 1e-10 ? J.eta_a * log(2 * J.k_a * r_m + 1e-30) : 0.0;
+    const double lt_f = fabs(J.eta_b) > 1e-10 ? J.eta_b * log(2 * J.k_b * r_m + 1e-30) : 0.0;
+    const double kd = J.k_a - J.k_b, ks = J.k_a + J.k_b;
+    const int n = L + 1;
+    double tail = 0.0;
+    const double pic = -J.l_a * PI / 2 + J.sigma_a + J.delta_a + lt_i;
+    const double pfc = -J.l_b * PI / 2 + J.sigma_b + J.delta_b + lt_f;
+    const double pd = kd * r_m + pic - pfc;
+    if (fabs(kd) > 1e-6) {
+        const double t1 = sin(pd) / (kd * pow(r_m, (double)n));
+        const double t1c = n * cos(pd) / ((kd * kd) * pow(r_m, (double)(n + 1)));
+        tail += 0.5 * (t1 + t1c);
+    } else if (n > 1) {
+        tail += 0.5 * cos(pd) / ((n - 1) * pow(r_m, (double)(n - 1)));
+    }
+    const double ps = ks * r_m + pic + pfc;
+    if (ks > 1e-6) {
+        const double t2 = sin(ps) / (ks * pow(r_m, (double)n));
+        const double t2c = n * cos(ps) / ((ks * ks) * pow(r_m, (double)(n + 1)));
+        tail -= 0.5 * (t2 + t2c);
+    }
+    if (!isfinite(tail)) return 0.0;
+    return tail * overlap;
+}
+
+__global__ __launch_bounds__(64) void tail_f64_kernel(const dwba_tail_job* __restrict__ jobs, TailParams TP) {
+    const dwba_tail_job J = jobs[blockIdx.x];
+    const int L = blockIdx.y;
+    if (L > J.L_max) return;
+    const int lane = threadIdx.x;
+    double res = 0.0;
+    const double M = J.moments ? J.moments[L] : 0.0;
+    if (J.kind != DWBA_TAIL_NONE && L >= 1 && fabs(M) > 1e-12) {
+        if (J.kind == DWBA_TAIL_LEGACY) {
+            res = analytic_tail(J.r_m, J, L, M);
+        } else {
+            Phase P;
+            P.ka = J.k_a; P.kb = J.k_b; P.ea = J.eta_a; P.eb = J.eta_b;
+            P.la = (double)J.l_a; P.lb = (double)J.l_b;
+            P.use_ea = fabs(J.eta_a) > 1e-15; P.use_eb = fabs(J.eta_b) > 1e-15;
+            P.use_la = J.l_a > 0; P.use_lb = J.l_b > 0;
+            P.kmin = J.k_a - J.k_b; P.kplus = J.k_a + J.k_b;
+            const double pa = -J.l_a * PI / 2 + J.sigma_a + J.delta_a;
+            const double pb = -J.l_b * PI / 2 + J.sigma_b + J.delta_b;
+            P.cmin = pa - pb; P.cplus = pa + pb;
+            const Env E{M, L + 1};
+            double outer = 0.0;
+            if (!(J.r_max <= J.r_m + 1e-10)) {
+                const cplx im = outer_integral(E, P, -1, J.r_m, J.r_max, TP, lane);
+                const cplx ip = outer_integral(E, P, +1, J.r_m, J.r_max, TP, lane);
+                outer = 0.5 * (im.re - ip.re);
+            }
+            res = outer + analytic_tail(J.r_max, J, L, M);
+        }
+    }
+    if (lane == 0) J.out[L] = res;
+}
+
+}  // namespace
+}  // namespace dwba
+
+extern "C" int32_t dwba_outer_tail_f64(int32_t n_jobs, const dwba_tail_job* d_jobs, int32_t L_max,
+                                       int32_t max_filon_segments, double max_effective_dphi,
+                                       const double* h_levin_D, const double* h_levin_x, void* stream) {
+    DWBA_REQUIRE(dwba::initialised(), "dwba_outer_tail_f64: call dwba_init first");
+    DWBA_REQUIRE(n_jobs > 0 && d_jobs && L_max >= 0 && L_max < 65535 && h_levin_D && h_levin_x,
+                 "dwba_outer_tail_f64: bad arguments");
+    DWBA_REQUIRE(max_filon_segments >= 1 && max_effective_dphi > 0, "dwba_outer_tail_f64: bad Filon limits");
+    dwba::TailParams tp;
+    tp.max_filon_segments = max_filon_segments;
+    tp.max_effective_dphi = max_effective_dphi;
+    for (int i = 0; i < 8; ++i) {
+        tp.lev.xr[i] = h_levin_x[i];
+        for (int j = 0; j < 8; ++j) tp.lev.D[i][j] = h_levin_D[i * 8 + j];
+    }
+    hipLaunchKernelGGL(dwba::tail_f64_kernel, dim3(n_jobs, L_max + 1), dim3(64), 0,
+                       reinterpret_cast<hipStream_t>(stream), d_jobs, tp);
+    DWBA_LAUNCH_CHECK("tail_f64_kernel");
+    return DWBA_OK;
+}

@@ -97,6 +97,105 @@ int32_t dwba_il_plain_f64_batch(int32_t n_jobs, const dwba_plain_job* d_jobs,
                                 int32_t L_max, double* d_out,
                                 void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------ */
+/* Full-Split preparation: alpha / beta vectors of a channel pair.       */
+/* ------------------------------------------------------------------ */
+#define DWBA_MAX_CC 16
+
+enum {
+    DWBA_PREP_PLAIN = 0,     /* oscillatory quadrature off or k_i+k_f <= k_threshold */
+    DWBA_PREP_FILON = 1,     /* Filon / Clenshaw-Curtis direct + exchange            */
+    DWBA_PREP_STANDARD = 2,  /* Filon branch fallback to weighted dot products       */
+    DWBA_PREP_ZERO = 3       /* idx_limit < 2 in the Filon branch: integral is 0     */
+};
+
+/* One channel pair.  Device pointers; outputs are the alpha and beta vectors (N doubles
+ * each) and scalars[4] = {sum rho2_dir_w, rho1_dir_w.V_diff, sum rho2_ex_w,
+ * rho1_ex_w.V_diff} (the L = 0 correction factors).  Phase intervals: either
+ * uniform (phase_nodes == NULL: node m = m*p_step + p_start, the last node
+ * p_end, n_pnodes nodes -- numpy linspace) or an explicit ascending array of
+ * n_pnodes nodes; for Coulomb stepping the array is filled on the device
+ * (see dwba_fullsplit_prep_f64) and p_step holds the phase increment. */
+typedef struct dwba_prep_job {
+    const double* r;          /* grid, N points */
+    const double* logr;       /* log(max(r, 1e-300)), N points */
+    const double* w;          /* Simpson weights, N points */
+    const double* u_i;        /* bound_i.u_of_r */
+    const double* u_f;        /* bound_f.u_of_r (continuum for ionization) */
+    const double* chi_i;      /* cont_i.chi_of_r */
+    const double* chi_f;      /* cont_f.chi_of_r */
+    const double* V_diff;     /* V_core - U_i, or NULL */
+    const double* phase_nodes;
+    double* alpha_dir;
+    double* beta_dir;
+    double* alpha_ex;
+    double* beta_ex;
+    double* scalars;
+    double p_start, p_step, p_end;
+    int32_t N, N1;            /* grid size, split index idx_limit */
+    int32_t n_pnodes;
+    int32_t mode;             /* DWBA_PREP_* */
+    int32_t log1, log2;       /* r[:N1] / r[:N] log-spaced (5e-3 test) */
+} dwba_prep_job;
+
+/* Builds alpha/beta for n_jobs pairs in one launch (one workgroup per pair).
+ * d_jobs: DEVICE array.  Jobs in FILON mode with a non-NULL phase_nodes
+ * buffer (>= 10002 doubles) get their nodes generated on the device by the
+ * Coulomb stepping of generate_phase_nodes when any_coulomb != 0, from
+ * d_coulomb_par[4*k..4*k+3] = {r_start, r_end, k_total, eta_total}.
+ * h_cc_x / h_cc_w: the n_cc reference Clenshaw-Curtis nodes/weights on
+ * [-1, 1] (host).
+ * Replaces oscillatory_kernel_integral_2d's node construction and
+ * interpolations (oscillatory_integrals.py:2578-2644, :2667-2770) and the
+ * density preparation of radial_ME_all_L (dwba_matrix_elements.py:1025-1049). */
+int32_t dwba_fullsplit_prep_f64(int32_t n_jobs, dwba_prep_job* d_jobs, const double* d_coulomb_par,
+                                int32_t any_coulomb, const double* h_cc_x, const double* h_cc_w,
+                                int32_t n_cc, void* stream);
+
+/* Multipole moments M_L = sum_j ((w_j r_j^L) u_f,j) u_i,j for L = 0..L_max of
+ * n_sets density pairs (device pointer arrays d_r/d_w/d_u_i/d_u_f of length
+ * n_sets, extents h_n on the host), written to d_out[set*(L_max+1) + L].
+ * Replaces `np.sum(w * (r ** L) * u_f * u_i)` (dwba_matrix_elements.py:1167,
+ * :1216) and its GPU counterpart (:2150-2170). */
+int32_t dwba_moments_f64(int32_t n_sets, const double* const* d_r, const double* const* d_w,
+                         const double* const* d_u_i, const double* const* d_u_f, const int32_t* d_n,
+                         int32_t L_max, double* d_out, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Outer oscillatory tail of the direct integral (FP64).                 */
+/* ------------------------------------------------------------------ */
+enum {
+    DWBA_TAIL_NONE = 0,    /* no tail (L = 0, ionization, idx_limit >= N-10, ...) */
+    DWBA_TAIL_OUTER = 1,   /* Filon/Levin on [r_m, r_max] + analytic tail at r_max */
+    DWBA_TAIL_LEGACY = 2   /* analytic tail at r_m only (method "legacy")          */
+};
+
+/* One channel pair: incident wave a = (k, l, delta, eta, sigma)_i, scattered
+ * wave b = (...)_f, split radius r_m = r[idx_limit-1], r_max = r[N-1],
+ * moments[L] (device, L = 0..L_max), out[L] (device).  out[0] = 0 and
+ * out[L] = 0 where |moments[L]| <= 1e-12. */
+typedef struct dwba_tail_job {
+    double k_a, delta_a, eta_a, sigma_a;
+    double k_b, delta_b, eta_b, sigma_b;
+    int32_t l_a, l_b;
+    double r_m, r_max;
+    const double* moments;
+    double* out;
+    int32_t L_max;
+    int32_t kind;             /* DWBA_TAIL_* */
+} dwba_tail_job;
+
+/* One wavefront per (pair, L) in one launch.  max_filon_segments /
+ * max_effective_dphi are DWBA_MAX_FILON_SEGMENTS / DWBA_MAX_EFFECTIVE_DPHI;
+ * h_levin_D (8x8 row-major) and h_levin_x (8) are the Chebyshev
+ * differentiation matrix and the ascending Lobatto points of the Levin rule.
+ * Replaces dwba_outer_integral_1d + _analytical_multipole_tail
+ * (oscillatory_integrals.py:459-535, :1024-1451, :2064-2184) as called at
+ * dwba_matrix_elements.py:1165-1192 and :1257-1268. */
+int32_t dwba_outer_tail_f64(int32_t n_jobs, const dwba_tail_job* d_jobs, int32_t L_max,
+                            int32_t max_filon_segments, double max_effective_dphi,
+                            const double* h_levin_D, const double* h_levin_x, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -254,6 +254,17 @@ class _Setup:
                                            sigma_i=self.sigma_i, sigma_f=self.sigma_f)
         return out
 
+    def direct_tail_noise(self, L) -> float:
+        """Conditioning noise of direct_tail (see tail.outer_integral_noise)."""
+        if not (L >= 1 and self.idx_limit < self.N - 10 and self.is_excitation):
+            return 0.0
+        m = self.moment(L)
+        if abs(m) <= 1e-12:
+            return 0.0
+        return T.outer_integral_noise(T.multipole_envelope(m, L), self.k_i, self.l_i, self.delta_i, self.eta_i,
+                                      self.sigma_i, self.k_f, self.l_f, self.delta_f, self.eta_f, self.sigma_f,
+                                      self.r_match, self.r[-1])
+
     def legacy_tail(self, L) -> float:
         """Legacy-method analytical tail at r_match (dwba_matrix_elements.py:1257-1268)."""
         if not (L >= 1 and self.use_osc and self.idx_limit < self.N - 10 and self.is_excitation):
@@ -421,10 +432,13 @@ def radial_ME_all_L_reduced(r, w_simpson, w_trapz, V_core, U_i, bound_i, bound_f
     ce = semisep.il_all_L(s.r, ae, be, s.L_max)
     I_dir, I_ex = {}, {}
     tails = np.zeros(s.L_max + 1)
+    noise = np.zeros(s.L_max + 1)
     for L in range(s.L_max + 1):
         t = 0.0
         if s.oscillatory and s.method != "legacy":
             t += s.direct_tail(L)
+            if return_parts:
+                noise[L] = s.direct_tail_noise(L)
         if s.method == "legacy":
             t += s.legacy_tail(L)
         tails[L] = t
@@ -438,7 +452,7 @@ def radial_ME_all_L_reduced(r, w_simpson, w_trapz, V_core, U_i, bound_i, bound_f
     res = Result(I_dir, I_ex)
     if return_parts:
         return res, dict(setup=s, alpha_dir=ad, beta_dir=bd, alpha_ex=ae, beta_ex=be,
-                         contraction_dir=cd, contraction_ex=ce, tails=tails)
+                         contraction_dir=cd, contraction_ex=ce, tails=tails, tail_noise=noise)
     return res
 
 

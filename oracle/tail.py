@@ -328,3 +328,36 @@ def analytical_multipole_tail(r_m, k_i, k_f, delta_i, delta_f, l_i, l_f, L,
     if not np.isfinite(tail):
         return 0.0
     return float(tail * bound_overlap)
+
+
+def outer_integral_noise(envelope, k_a, l_a, delta_a, eta_a, sigma_a, k_b, l_b, delta_b, eta_b, sigma_b,
+                         r_m, r_max, delta_phi=np.pi / 4) -> float:
+    """Rounding-noise bound of dwba_outer_integral: 0.5 * sum over the Levin
+    segments of both phases of cond(A_s) * 2^-53 * |J_s|.  The Levin
+    collocation systems become ill-conditioned when |Phi'| is small (the
+    Chebyshev matrix has the constants in its null space), so two correct
+    solvers (LAPACK zgesv and any other LU) legitimately differ by this much;
+    Filon segments contribute nothing at this level."""
+    if r_max <= r_m + 1e-10:
+        return 0.0
+    pf = PhaseFunctions(k_a, l_a, delta_a, eta_a, sigma_a, k_b, l_b, delta_b, eta_b, sigma_b)
+    tot = 0.0
+    x_ref = np.cos(np.pi * np.arange(8) / 7)[::-1]
+    Dref = chebyshev_diff_matrix(8)
+    for sg in (-1, 1):
+        mid = (r_m + r_max) / 2
+        om = abs(pf.dphi(sg, mid))
+        if om < 1e-10:
+            continue
+        h = delta_phi / om
+        if abs(pf.d2phi(sg, mid)) * h * h < 0.1:
+            continue
+        n_seg = max(1, int(np.ceil((r_max - r_m) / h)))
+        bounds = np.linspace(r_m, r_max, n_seg + 1)
+        for i in range(n_seg):
+            nodes = 0.5 * (bounds[i + 1] - bounds[i]) * (x_ref + 1) + bounds[i]
+            A = Dref * (2.0 / (nodes[-1] - nodes[0])) + 1j * np.diag(pf.dphi(sg, nodes))
+            fv = np.array([envelope(x) for x in nodes], dtype=float)
+            J = levin_segment(fv, nodes, pf.phi(sg, nodes), pf.dphi(sg, nodes))
+            tot += np.linalg.cond(A) * 2.0 ** -53 * (abs(J) + np.max(np.abs(fv)) * (nodes[-1] - nodes[0]))
+    return 0.5 * tot

@@ -1,0 +1,128 @@
+"""GPU parity of the drop-in radial_ME_all_L_gpu (prep + contraction + tail
+kernels) against the oracle's restatement of the reference CPU path.
+
+Bar (FP64): |I_gpu - I_ref| <= 1e-10 |I_ref| + 1e-13 * M, where
+M = |alpha|^T K |beta| + |tail| is the cancellation-free magnitude of the same
+integral (it only matters for integrals cancelled far below their terms),
+plus 10x the oracle's estimate of the reference's own rounding noise in the
+Levin branch of the outer tail (cond(A_s) 2^-53 |J_s| per segment): those
+8x8 collocation systems reach condition numbers ~1e8 when |Phi'| is small.
+"""
+import numpy as np
+import pytest
+
+from oracle import grid as OG, radial as R, semisep
+from dwba_amd import grid as PG, synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(seed, n=1200, r_max=100.0, eta=0.0, ionization=False, with_match=True, k_scale=1.0):
+    rng = np.random.default_rng(seed)
+    g = PG.make_r_grid(1e-4, r_max, n)
+    r = g.r
+    ki, kf = rng.uniform(0.6, 3.0, 2) * k_scale
+    li, lf = rng.integers(0, 12, 2)
+    ci = S.make_wave(r, ki, li, rng.uniform(0, np.pi), with_match)
+    cf = S.make_wave(r, kf, lf, rng.uniform(0, np.pi), with_match)
+    if eta:
+        ci.eta, cf.eta = eta / ki, eta / kf
+        ci.sigma_l, cf.sigma_l = 0.3, -0.2
+    bi = S.Bound(S.u1s(r))
+    bf = S.make_wave(r, rng.uniform(0.3, 1.5), int(rng.integers(0, 6)), 0.4, with_match) if ionization \
+        else S.Bound(S.u2p(r), l=1)
+    V = -1.0 / r
+    Ui = -np.exp(-2 * r) * (1 + 1 / r)
+    Uf = Ui * 0.9
+    return g, V, Ui, Uf, bi, bf, ci, cf
+
+
+def _compare(args, kw, L_max, faithful=True, rtol=1e-10, atol_rel=1e-13):
+    from dwba_amd.dwba_matrix_elements import radial_ME_all_L_gpu
+    g, V, Ui, Uf, bi, bf, ci, cf = args
+    got = radial_ME_all_L_gpu(g, V, Ui, bi, bf, ci, cf, L_max, U_f_array=Uf, **kw)
+    okw = {k: v for k, v in kw.items() if k in ("use_oscillatory_quadrature", "oscillatory_method", "CC_nodes",
+                                                "phase_increment", "min_grid_fraction", "k_threshold")}
+    red, parts = R.radial_ME_all_L_reduced(g.r, g.w_simpson, g.w_trapz, V, Ui, bi, bf, ci, cf, L_max,
+                                           U_f_array=Uf, return_parts=True, **okw)
+    ref = R.radial_ME_all_L_faithful(g.r, g.w_simpson, g.w_trapz, V, Ui, bi, bf, ci, cf, L_max,
+                                     U_f_array=Uf, **okw) if faithful else red
+    r = g.r
+    mag_d = semisep.il_all_L(r, np.abs(parts["alpha_dir"]), np.abs(parts["beta_dir"]), L_max)
+    mag_e = semisep.il_all_L(r, np.abs(parts["alpha_ex"]), np.abs(parts["beta_ex"]), L_max)
+    for L in range(L_max + 1):
+        for fam, a, b, mag in (("dir", got.I_L_direct, ref.I_L_direct, mag_d[L] + abs(parts["tails"][L])),
+                               ("ex", got.I_L_exchange, ref.I_L_exchange, mag_e[L])):
+            if np.isnan(b[L]):
+                assert np.isnan(a[L]), (fam, L, a[L])
+                continue
+            tol = rtol * abs(b[L]) + atol_rel * mag + (10.0 * parts["tail_noise"][L] if fam == "dir" else 0.0)
+            assert abs(a[L] - b[L]) <= tol, (fam, L, a[L], b[L], abs(a[L] - b[L]) / max(abs(b[L]), 1e-300))
+    return got, ref
+
+
+@pytest.mark.parametrize("method", ["advanced", "full_split", "legacy"])
+@pytest.mark.parametrize("cc", [5, 7])
+def test_filon_excitation(method, cc):
+    _compare(_case(1), dict(oscillatory_method=method, CC_nodes=cc), 4)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_filon_excitation_seeds_L12(seed):
+    _compare(_case(10 + seed, n=1500), dict(oscillatory_method="full_split"), 12)
+
+
+def test_plain_branch_below_k_threshold():
+    _compare(_case(2), dict(k_threshold=50.0), 6)
+
+
+def test_plain_branch_quadrature_off():
+    _compare(_case(3), dict(use_oscillatory_quadrature=False), 6)
+
+
+def test_coulomb_phase_nodes():
+    _compare(_case(4, eta=-1.0), dict(oscillatory_method="advanced"), 5)
+
+
+def test_ionization_no_tail_and_l0_correction():
+    got, ref = _compare(_case(5, ionization=True), dict(oscillatory_method="full_split"), 5)
+    assert got.I_L_direct[0] != 0.0
+
+
+def test_no_match_point_full_grid():
+    _compare(_case(6, with_match=False), dict(), 4)
+
+
+def test_high_k_many_intervals():
+    _compare(_case(7, n=2000, k_scale=3.0), dict(CC_nodes=7), 6)
+
+
+def test_large_grid_against_reduced_route():
+    _compare(_case(8, n=8001, r_max=400.0), dict(oscillatory_method="full_split"), 20, faithful=False)
+
+
+def test_gpu_cache_same_results():
+    from dwba_amd.dwba_matrix_elements import GPUCache, radial_ME_all_L_gpu
+    g, V, Ui, Uf, bi, bf, ci, cf = _case(9)
+    a = radial_ME_all_L_gpu(g, V, Ui, bi, bf, ci, cf, 6, U_f_array=Uf)
+    cache = GPUCache.from_grid(g)
+    b = radial_ME_all_L_gpu(g, V, Ui, bi, bf, ci, cf, 6, U_f_array=Uf, gpu_cache=cache)
+    cache.clear()
+    assert a.I_L_direct == b.I_L_direct and a.I_L_exchange == b.I_L_exchange
+
+
+def test_errors_like_reference():
+    from dwba_amd.dwba_matrix_elements import radial_ME_all_L_gpu, check_cupy_runtime, HAS_CUPY
+    assert HAS_CUPY and check_cupy_runtime()
+    g, V, Ui, Uf, bi, bf, ci, cf = _case(9)
+    with pytest.raises(ValueError):
+        radial_ME_all_L_gpu(g, V, Ui, bi, bf, ci, cf, -1)
+    with pytest.raises(ValueError):
+        radial_ME_all_L_gpu(g, V[:-5], Ui, bi, bf, ci, cf, 3)
+
+
+def test_nan_density_sanitised_in_filon_branch():
+    g, V, Ui, Uf, bi, bf, ci, cf = _case(11)
+    ci.chi_of_r = ci.chi_of_r.copy()
+    ci.chi_of_r[500] = np.nan
+    _compare((g, V, Ui, Uf, bi, bf, ci, cf), dict(), 3)

@@ -1,0 +1,90 @@
+"""Host planner (product) against the oracle's restatement: split index,
+phase increments, phase-interval layout, reference tables (no GPU)."""
+import numpy as np
+import pytest
+
+from oracle import grid as OG, quadrature as Q, radial as R, tail as T
+from dwba_amd import planner, synthetic as S
+from dwba_amd.planner import GridStats
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 7, 9, 15])
+def test_cc_reference_tables_bit_identical(n):
+    x, w = planner.cc_reference(n)
+    xo, wo = Q.cc_ref(n)
+    assert np.array_equal(x, xo) and np.array_equal(w, wo)
+
+
+def test_levin_tables_bit_identical():
+    D, x = planner.levin_tables(8)
+    assert np.array_equal(D, T.chebyshev_diff_matrix(8))
+    assert np.array_equal(x, np.cos(np.pi * np.arange(8) / 7)[::-1])
+
+
+def _waves(r, rng, with_match=True):
+    ki, kf = rng.uniform(0.3, 4.0, 2)
+    li, lf = rng.integers(0, 25, 2)
+    return (S.make_wave(r, ki, li, rng.uniform(0, np.pi), with_match),
+            S.make_wave(r, kf, lf, rng.uniform(0, np.pi), with_match))
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_split_index_matches_oracle(seed):
+    rng = np.random.default_rng(seed)
+    g = OG.make_r_grid(1e-4, 200.0, 3000)
+    r, wt = g[0], g[1]
+    ci, cf = _waves(r, rng, with_match=seed % 3 != 0)
+    ui, uf = S.u1s(r), S.u2p(r) if seed % 2 else S.slater(r, 0.3)
+    Ui = -np.exp(-r) / r * (1 + r)
+    Uf = Ui * 0.7 if seed % 4 else None
+    gs = GridStats.of(r, wt)
+    got = planner.split_index(gs, Ui, Uf, ui, uf, ci.k_au, cf.k_au, ci.l, cf.l, ci.idx_match, cf.idx_match, 0.1)
+    ref = R.compute_idx_limit(r, wt, Ui, Uf, ui, uf, ci.k_au, cf.k_au, ci.l, cf.l, ci.idx_match, cf.idx_match, 0.1)
+    assert got == ref
+
+
+@pytest.mark.parametrize("k,eta,n1", [(0.7, 0.0, 2500), (9.0, 0.0, 3000), (3.0, -1.5, 2000), (12.0, 0.8, 2999)])
+def test_phase_increment_and_layout_match_oracle(k, eta, n1):
+    r, wt, _ = OG.make_r_grid(1e-4, 200.0, 3000)
+    gs = GridStats.of(r, wt)
+    for req in (1.5708, 0.3, 0.05):
+        got = planner.phase_increment(gs, n1, k, req, eta)
+        ref = Q.adaptive_phase_increment(r[:n1], k, req, eta)
+        assert got == ref
+    lay = planner.filon_layout(gs, n1, k, 1.5708, eta)
+    dphi = Q.adaptive_phase_increment(r[:n1], k, 1.5708, eta)
+    ref_nodes = Q.generate_phase_nodes(r[0], r[n1 - 1], k, dphi, eta)
+    if eta == 0.0:
+        assert lay.mode == planner.PREP_FILON and not lay.coulomb
+        assert lay.n_pnodes == len(ref_nodes)
+        m = np.arange(lay.n_pnodes, dtype=np.float64)
+        nodes = m * lay.p_step + lay.p_start
+        nodes[-1] = lay.p_end
+        assert np.array_equal(nodes, ref_nodes)
+    else:
+        assert lay.mode == planner.PREP_FILON and lay.coulomb and lay.p_step == dphi
+
+
+def test_log_spacing_flags():
+    r, wt, _ = OG.make_r_grid(1e-4, 200.0, 3000)
+    gs = GridStats.of(r, wt)
+    for n in (2, 3, 10, 3000):
+        assert gs.log_spaced(n) == Q.is_log_spaced_grid(r[:n])
+    lin = np.linspace(0.01, 10, 500)
+    gl = GridStats.of(lin, OG.trapz_weights(lin))
+    assert gl.log_spaced(500) == Q.is_log_spaced_grid(lin) == False
+
+
+def test_filon_layout_degenerate_cases():
+    r, wt, _ = OG.make_r_grid(1e-4, 200.0, 3000)
+    gs = GridStats.of(r, wt)
+    assert planner.filon_layout(gs, 1, 3.0, 1.5708, 0.0).mode == planner.PREP_ZERO
+    assert planner.filon_layout(gs, 100, 1e-8, 1.5708, 0.0).mode == planner.PREP_STANDARD
+
+
+def test_filon_limits_env(monkeypatch):
+    monkeypatch.setenv("DWBA_MAX_FILON_SEGMENTS", "50")
+    monkeypatch.setenv("DWBA_MAX_EFFECTIVE_DPHI", "0.01")
+    seg, d = planner.filon_limits()
+    assert seg == 128 and d == np.pi / 16
+    assert T.filon_max_segments() == seg and T.filon_max_effective_dphi() == d
