@@ -73,6 +73,12 @@ _SIGS = {
     "dwba_fullsplit_prep_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_dp, c_dp, c_i32, c_vp]),
     "dwba_moments_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]),
     "dwba_outer_tail_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, ctypes.c_double, c_dp, c_dp, c_vp]),
+    "dwba_cast_f64_f32": (c_i32, [c_vp, c_vp, ctypes.c_int64, c_vp]),
+    "dwba_xgemm_partial_doubles": (c_sz, [c_i32, c_i32, c_i32]),
+    "dwba_xgemm_f32": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp, c_i32, c_vp, c_i32, c_vp]),
+    "dwba_dgemv_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_vp]),
+    "dwba_ddot_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
+    "dwba_batch_finalize": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
 }
 
 

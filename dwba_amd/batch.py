@@ -1,0 +1,523 @@
+"""Batched scan engine: every (energy, channel pair) of a scan in one pass.
+
+Each energy may have its own grid (LOCAL grids, driver.py / DW_main.py:1621-
+1748).  For a batch the reference calls radial_ME_all_L(_gpu) once per pair
+(driver.py:907-1029, ionization.py:526-608); here the whole scan is planned
+once on the host and then runs as a fixed sequence of launches on one
+stream, which can be captured into a HIP graph and replayed:
+
+  1  dwba_fullsplit_prep_f64  alpha/beta of every pair (one workgroup each),
+                              written straight into per-energy stacked matrices
+  2  dwba_moments_f64         M_L per distinct bound-density pair
+  3  dwba_outer_tail_f64      FP64 tails, one wavefront per (pair, L)
+  4  direct family            shared beta -> dwba_dgemv_f64 + dwba_ddot_f64 (FP64);
+                              distinct betas (ionization) -> stacked GEMM
+  5  exchange family          precision "fp32": dwba_cast_f64_f32 + dwba_xgemm_f32
+                              (MFMA); precision "fp64": dwba_il_plain_f64_batch
+  6  dwba_batch_finalize      tails and L=0 corrections, in place
+
+Pairs in the oscillatory-off branch (PLAIN mode) always take the FP64 plain
+kernel for both families.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import planner
+from .planner import GridStats, PREP_PLAIN, PREP_FILON, PREP_STANDARD, PREP_ZERO
+from .dwba_matrix_elements import (PairPlan, RadialDWBAIntegrals, TAIL_NONE, TAIL_OUTER, TAIL_LEGACY, plan_pair,
+                                   _fit)
+
+XROWS, XCOLS, XK = 256, 128, 32
+
+
+class XgemmEnergy(ctypes.Structure):
+    """dwba_xgemm_energy of include/dwba_hip.h."""
+    _fields_ = [("logr", ctypes.c_void_p), ("B", ctypes.c_void_p), ("A", ctypes.c_void_p),
+                ("col_extent", ctypes.c_void_p), ("row_blocks", ctypes.c_void_p), ("part_off", ctypes.c_int64),
+                ("N", ctypes.c_int32), ("P", ctypes.c_int32), ("ld", ctypes.c_int32), ("n_cb", ctypes.c_int32),
+                ("n_rb", ctypes.c_int32), ("pair_base", ctypes.c_int32), ("out_slot", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
+
+
+class DgemvEnergy(ctypes.Structure):
+    _fields_ = [("r", ctypes.c_void_p), ("beta", ctypes.c_void_p), ("t", ctypes.c_void_p),
+                ("N", ctypes.c_int32), ("rows", ctypes.c_int32), ("ld", ctypes.c_int32), ("L_max", ctypes.c_int32)]
+
+
+class DdotPair(ctypes.Structure):
+    _fields_ = [("alpha", ctypes.c_void_p), ("t", ctypes.c_void_p), ("n", ctypes.c_int32), ("ld", ctypes.c_int32),
+                ("out_index", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+class FinalizePair(ctypes.Structure):
+    _fields_ = [("scalars", ctypes.c_void_p), ("tails", ctypes.c_void_p), ("out_index", ctypes.c_int32),
+                ("oscillatory", ctypes.c_int32), ("tail_kind", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+@dataclass
+class EnergyInput:
+    """One energy point of a scan: its grid, potentials and channel pairs.
+
+    pairs: list of (cont_i, cont_f) or (cont_i, cont_f, bound_f) -- the third
+    entry overrides bound_f per pair (ionization: the ejected wave)."""
+    grid: object
+    V_core: np.ndarray
+    U_i: np.ndarray
+    bound_i: object
+    bound_f: object
+    pairs: list
+    U_f: Optional[np.ndarray] = None
+
+
+def _round_up(x, m):
+    return (x + m - 1) // m * m
+
+
+@dataclass
+class _EnergyState:
+    N: int
+    ld: int
+    gs: GridStats
+    r_d: object
+    logr_d: object
+    w_d: object
+    plans: list = field(default_factory=list)
+    order: list = field(default_factory=list)        # batch pair ids of this energy in column order
+
+
+class ScanBatch:
+    """Plans and runs all I_L of a scan.  precision: "fp32" (exchange on
+    FP32 MFMA) or "fp64" (everything through the FP64 plain kernel)."""
+
+    def __init__(self, energies: Sequence[EnergyInput], L_max: int, precision: str = "fp32",
+                 use_oscillatory_quadrature=True, oscillatory_method="advanced", CC_nodes=5,
+                 phase_increment=1.5708, min_grid_fraction=0.10, k_threshold=0.5):
+        if precision not in ("fp32", "fp64"):
+            raise ValueError(f"precision must be 'fp32' or 'fp64', got {precision!r}")
+        if L_max < 0 or L_max > 63:
+            raise ValueError("L_max must be in [0, 63]")
+        import torch
+        from . import _hip
+        self.torch, self.hip = torch, _hip
+        self.lib = _hip.load_library()
+        self.dev = torch.device("cuda", _hip.ensure_init())
+        self.L_max = int(L_max)
+        self.precision = precision
+        self.CC_nodes = int(CC_nodes)
+        self._keep = []
+        t0 = time.perf_counter()
+        self._plan(energies, use_oscillatory_quadrature, oscillatory_method, phase_increment,
+                   min_grid_fraction, k_threshold)
+        self.plan_seconds = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        self._build()
+        torch.cuda.synchronize()
+        self.build_seconds = time.perf_counter() - t0
+        self.graph = None
+
+    # ------------------------------------------------------------ planning
+    def _plan(self, energies, use_osc, method, phase_inc, mgf, k_thr):
+        torch = self.torch
+        T = lambda a: torch.as_tensor(a, device=self.dev)
+        self.E = []
+        self.pairs = []        # (energy index, pair index in energy, PairPlan, inputs)
+        self._uploads = {}
+        self._bound_objs = [en.bound_i for en in energies]
+        for ei, en in enumerate(energies):
+            r = np.ascontiguousarray(en.grid.r, dtype=np.float64)
+            N = len(r)
+            wt = en.grid.w_trapz if hasattr(en.grid, "w_trapz") else np.gradient(r)
+            gs = GridStats.of(r, wt)
+            ld = _round_up(N, XK)
+            st = _EnergyState(N=N, ld=ld, gs=gs, r_d=T(r), logr_d=T(np.log(np.maximum(r, 1e-300))),
+                              w_d=T(np.ascontiguousarray(en.grid.w_simpson, dtype=np.float64)))
+            V = _fit("V_core", en.V_core, N)
+            Ui = _fit("U_i", en.U_i, N)
+            Uf = _fit("U_f", en.U_f, N) if en.U_f is not None else None
+            st.vdiff_d = T(V - Ui)
+            st.ui = _fit("bound_i.u", en.bound_i.u_of_r, N)
+            ext = {}
+            for pi, pr in enumerate(en.pairs):
+                ci, cf = pr[0], pr[1]
+                bf = pr[2] if len(pr) > 2 else en.bound_f
+                uf = _fit("bound_f.u", bf.u_of_r, N)
+                is_exc = not hasattr(bf, "k_au")
+                plan = plan_pair(gs, Ui, Uf, st.ui, uf, ci, cf, self.L_max, use_osc, method, phase_inc, mgf,
+                                 k_thr, is_exc, ext)
+                self.pairs.append((ei, pi, plan, ci, cf, bf))
+            self.E.append(st)
+
+    def _dev_array(self, obj, attr, N):
+        """Upload a wave / bound density once per object."""
+        key = (id(obj), attr, N)
+        hit = self._uploads.get(key)
+        if hit is not None and hit[0] is obj:
+            return hit[1]
+        t = self.torch.as_tensor(_fit(attr, getattr(obj, attr), N), device=self.dev)
+        self._uploads[key] = (obj, t)
+        return t
+
+    # ------------------------------------------------------------ device setup
+    def _build(self):
+        torch, hip, lib = self.torch, self.hip, self.lib
+        dev, Lm = self.dev, self.L_max
+        nl = Lm + 1
+        f64 = torch.float64
+        n_pairs = len(self.pairs)
+        # global output order: PLAIN pairs first (FP64 plain batch), then per
+        # energy in column order (pairs sorted by exchange support, descending)
+        if self.precision == "fp64":
+            plain_ids = []
+            order = list(range(n_pairs))
+            for ei, st in enumerate(self.E):
+                st.order = [k for k in order if self.pairs[k][0] == ei]
+        else:
+            plain_ids = [k for k, p in enumerate(self.pairs) if p[2].mode == PREP_PLAIN]
+            for ei, st in enumerate(self.E):
+                ids = [k for k, p in enumerate(self.pairs) if p[0] == ei and p[2].mode != PREP_PLAIN]
+                ids.sort(key=lambda k: -self._ex_support(self.pairs[k][2]))
+                st.order = ids
+            order = plain_ids + [k for st in self.E for k in st.order]
+        self.out_index = np.empty(n_pairs, dtype=np.int64)
+        self.out_index[order] = np.arange(n_pairs)
+        self.n_pairs = n_pairs
+        self.out = torch.zeros((n_pairs, 2, nl), dtype=f64, device=dev)
+
+        # per-energy stacked alpha/beta (pairs in column order), zero padded
+        cols = {}
+        for ei, st in enumerate(self.E):
+            st.members = st.order
+            P = len(st.members)
+            st.ad = torch.zeros((max(P, 1), st.ld), dtype=f64, device=dev)
+            st.bd = torch.zeros((max(P, 1), st.ld), dtype=f64, device=dev)
+            st.ae = torch.zeros((max(P, 1), st.ld), dtype=f64, device=dev)
+            st.be = torch.zeros((max(P, 1), st.ld), dtype=f64, device=dev)
+            for c, k in enumerate(st.members):
+                cols[k] = (ei, c)
+        # plain-mode pairs get private vectors
+        self._plain_vec = {}
+        for k in plain_ids:
+            ei = self.pairs[k][0]
+            if self.precision != "fp64":
+                self._plain_vec[k] = torch.zeros((4, self.E[ei].ld), dtype=f64, device=dev)
+
+        self.scalars = torch.zeros((n_pairs, 4), dtype=f64, device=dev)
+        self.tails = torch.zeros((n_pairs, nl), dtype=f64, device=dev)
+
+        # ---- prep jobs
+        PJ = hip.PrepJob * n_pairs
+        prep = PJ()
+        coul_par = np.zeros((n_pairs, 4))
+        any_coul = False
+        node_bufs = []
+        self.mom_sets = {}   # (energy, id(u_f)) -> index
+        for k, (ei, pi, plan, ci, cf, bf) in enumerate(self.pairs):
+            st = self.E[ei]
+            ui_d = self._dev_array(self._bound_i(ei), "u_of_r", st.N)
+            uf_d = self._dev_array(bf, "u_of_r", st.N)
+            ci_d = self._dev_array(ci, "chi_of_r", st.N)
+            cf_d = self._dev_array(cf, "chi_of_r", st.N)
+            if k in self._plain_vec:
+                v = self._plain_vec[k]
+                outs = (v[0], v[1], v[2], v[3])
+            else:
+                e2, c = cols[k]
+                outs = (st.ad[c], st.bd[c], st.ae[c], st.be[c])
+            lay = plan.layout
+            nodes = None
+            if plan.mode == PREP_FILON and lay.coulomb:
+                nodes = torch.empty(10002, dtype=f64, device=dev)
+                node_bufs.append(nodes)
+                coul_par[k] = (lay.p_start, lay.p_end, lay.k_total, lay.eta_total)
+                any_coul = True
+            prep[k] = hip.PrepJob(hip.ptr(st.r_d), hip.ptr(st.logr_d), hip.ptr(st.w_d), hip.ptr(ui_d),
+                                  hip.ptr(uf_d), hip.ptr(ci_d), hip.ptr(cf_d), hip.ptr(st.vdiff_d), hip.ptr(nodes),
+                                  hip.ptr(outs[0]), hip.ptr(outs[1]), hip.ptr(outs[2]), hip.ptr(outs[3]),
+                                  hip.ptr(self.scalars[k]), lay.p_start, lay.p_step, lay.p_end, st.N, plan.N1,
+                                  lay.n_pnodes, plan.mode, plan.log1, plan.log2)
+        self.prep_d = hip.struct_array_to_device(prep, dev)
+        self.coul_d = torch.as_tensor(coul_par.ravel(), device=dev) if any_coul else None
+        self.any_coul = any_coul
+        self._keep += node_bufs
+        cx, cw = planner.cc_reference(self.CC_nodes)
+        self.cc = (np.ascontiguousarray(cx), np.ascontiguousarray(cw))
+
+        # ---- moments and tails
+        tail_ids = [k for k, p in enumerate(self.pairs) if p[2].tail_kind != TAIL_NONE]
+        self.n_tail = len(tail_ids)
+        if tail_ids:
+            sets, set_of = [], {}
+            for k in tail_ids:
+                ei, _, plan, ci, cf, bf = self.pairs[k]
+                key = (ei, id(bf))
+                if key not in set_of:
+                    set_of[key] = len(sets)
+                    sets.append((ei, bf))
+            ns = len(sets)
+            self.moments = torch.zeros((ns, nl), dtype=f64, device=dev)
+            ptr_tab = np.zeros((4, ns), dtype=np.int64)
+            n_tab = np.zeros(ns, dtype=np.int32)
+            for s_i, (ei, bf) in enumerate(sets):
+                st = self.E[ei]
+                ptr_tab[:, s_i] = (hip.ptr(st.r_d), hip.ptr(st.w_d),
+                                   hip.ptr(self._dev_array(self._bound_i(ei), "u_of_r", st.N)),
+                                   hip.ptr(self._dev_array(bf, "u_of_r", st.N)))
+                n_tab[s_i] = st.N
+            self.mom_ptr = torch.as_tensor(ptr_tab, device=dev)
+            self.mom_n = torch.as_tensor(n_tab, device=dev)
+            self.n_mom_sets = ns
+            TJ = hip.TailJob * len(tail_ids)
+            tj = TJ()
+            for q, k in enumerate(tail_ids):
+                ei, _, plan, ci, cf, bf = self.pairs[k]
+                s_i = set_of[(ei, id(bf))]
+                tj[q] = hip.TailJob(plan.k_i, plan.delta_i, plan.eta_i, plan.sigma_i, plan.k_f, plan.delta_f,
+                                    plan.eta_f, plan.sigma_f, plan.l_i, plan.l_f, plan.r_match, plan.r_max,
+                                    hip.ptr(self.moments[s_i]), hip.ptr(self.tails[k]), Lm, plan.tail_kind)
+            self.tail_d = hip.struct_array_to_device(tj, dev)
+            D, x = planner.levin_tables(8)
+            self.levin = (np.ascontiguousarray(D.ravel()), np.ascontiguousarray(x))
+
+        # ---- finalize table
+        FJ = FinalizePair * n_pairs
+        fj = FJ()
+        for k, (ei, pi, plan, ci, cf, bf) in enumerate(self.pairs):
+            fj[k] = FinalizePair(hip.ptr(self.scalars[k]), hip.ptr(self.tails[k]), int(self.out_index[k]),
+                                 int(plan.oscillatory), plan.tail_kind, 0)
+        self.fin_d = hip.struct_array_to_device(fj, dev)
+
+        # ---- contraction tables
+        if self.precision == "fp64":
+            self._build_fp64(order)
+        else:
+            self._build_fp32(plain_ids)
+
+    def _bound_i(self, ei):
+        return self._bound_objs[ei]
+
+    def _ex_support(self, plan: PairPlan) -> int:
+        if plan.mode == PREP_FILON:
+            return min(plan.N1 + 1, plan.N)
+        if plan.mode == PREP_STANDARD:
+            return plan.N
+        return 0
+
+    def _build_fp64(self, order):
+        hip = self.hip
+        jobs, nr, nc = [], [], []
+        arr = (hip.PlainJob * self.n_pairs)()
+        nra = (ctypes.c_int32 * self.n_pairs)()
+        nca = (ctypes.c_int32 * self.n_pairs)()
+        pos = {k: c for st in self.E for c, k in enumerate(st.members)}
+        for g, k in enumerate(order):
+            ei, pi, plan, ci, cf, bf = self.pairs[k]
+            st = self.E[ei]
+            c = pos[k]
+            arr[g] = hip.PlainJob(hip.ptr(st.r_d), hip.ptr(st.ad[c]), hip.ptr(st.bd[c]), hip.ptr(st.ae[c]),
+                                  hip.ptr(st.be[c]), plan.rows, plan.cols)
+            nra[g], nca[g] = plan.rows, plan.cols
+        self.plain_d = hip.struct_array_to_device(arr, self.dev)
+        self.plain_n = (nra, nca)
+        self.plain_ws_bytes = self.lib.dwba_il_plain_f64_batch_workspace_bytes(self.n_pairs, nra, nca, self.L_max)
+        self.plain_ws = self.torch.empty(max(self.plain_ws_bytes, 1), dtype=self.torch.uint8, device=self.dev)
+        self.plain_count = self.n_pairs
+
+    def _build_fp32(self, plain_ids):
+        torch, hip, lib = self.torch, self.hip, self.lib
+        dev, Lm = self.dev, self.L_max
+        nl = Lm + 1
+        f64 = torch.float64
+        # plain-mode pairs: FP64 plain batch writing out[0:len(plain_ids)]
+        self.plain_count = len(plain_ids)
+        if plain_ids:
+            arr = (hip.PlainJob * len(plain_ids))()
+            nra = (ctypes.c_int32 * len(plain_ids))()
+            nca = (ctypes.c_int32 * len(plain_ids))()
+            for g, k in enumerate(plain_ids):
+                ei, pi, plan, ci, cf, bf = self.pairs[k]
+                v = self._plain_vec[k]
+                arr[g] = hip.PlainJob(hip.ptr(self.E[ei].r_d), hip.ptr(v[0]), hip.ptr(v[1]), hip.ptr(v[2]),
+                                      hip.ptr(v[3]), plan.rows, plan.cols)
+                nra[g], nca[g] = plan.rows, plan.cols
+            self.plain_d = hip.struct_array_to_device(arr, dev)
+            self.plain_n = (nra, nca)
+            self.plain_ws_bytes = lib.dwba_il_plain_f64_batch_workspace_bytes(len(plain_ids), nra, nca, Lm)
+            self.plain_ws = torch.empty(self.plain_ws_bytes, dtype=torch.uint8, device=dev)
+        # direct family: shared beta per energy -> dgemv + ddot; else stacked GEMM
+        dg, dd, xg_fams = [], [], []
+        max_rows = 0
+        self.t_bufs = []
+        for ei, st in enumerate(self.E):
+            if not st.order:
+                continue
+            plans = [self.pairs[k][2] for k in st.order]
+            bfs = {id(self.pairs[k][5]) for k in st.order}
+            shared = len(bfs) == 1
+            rows_d = max(p.N1 if p.mode in (PREP_FILON, PREP_STANDARD) else 0 for p in plans)
+            base = int(self.out_index[st.order[0]])
+            if shared:
+                if rows_d > 0:
+                    src = next(c for c, p in enumerate(plans) if p.mode in (PREP_FILON, PREP_STANDARD))
+                    t = torch.zeros((nl, st.ld), dtype=f64, device=dev)
+                    self.t_bufs.append(t)
+                    dg.append(DgemvEnergy(hip.ptr(st.r_d), hip.ptr(st.bd[src]), hip.ptr(t), st.N, rows_d, st.ld, Lm))
+                    max_rows = max(max_rows, rows_d)
+                    for c, k in enumerate(st.order):
+                        p = self.pairs[k][2]
+                        if p.mode in (PREP_FILON, PREP_STANDARD):
+                            dd.append(DdotPair(hip.ptr(st.ad[c]), hip.ptr(t), p.N1, st.ld, base + c, 0))
+            else:
+                sup_d = [p.N if p.mode in (PREP_FILON, PREP_STANDARD) else 0 for p in plans]
+                rows = [p.N1 if p.mode in (PREP_FILON, PREP_STANDARD) else 0 for p in plans]
+                xg_fams.append((ei, st.ad, st.bd, rows, sup_d, base, 0))
+            sup_e = [self._ex_support(p) for p in plans]
+            rows_e = [p.N1 if p.mode in (PREP_FILON, PREP_STANDARD) else 0 for p in plans]
+            xg_fams.append((ei, st.ae, st.be, rows_e, sup_e, base, 1))
+        self.dgemv = None
+        if dg:
+            arr = (DgemvEnergy * len(dg))(*dg)
+            self.dgemv = (hip.struct_array_to_device(arr, dev), len(dg), max_rows)
+        self.ddot = None
+        if dd:
+            arr = (DdotPair * len(dd))(*dd)
+            self.ddot = (hip.struct_array_to_device(arr, dev), len(dd))
+        # stacked GEMM descriptors and work list
+        descs, items, costs = [], [], []
+        part_off = 0
+        self.b32 = []
+        max_cb = 1
+        for fi, (ei, Amat, Bmat, rows, sup, base, slot) in enumerate(xg_fams):
+            st = self.E[ei]
+            P = len(rows)
+            n_cb = (P + XCOLS - 1) // XCOLS
+            col_ext = np.zeros(n_cb, dtype=np.int32)
+            row_blk = np.zeros(n_cb, dtype=np.int32)
+            for cb in range(n_cb):
+                sl = slice(cb * XCOLS, min(P, (cb + 1) * XCOLS))
+                col_ext[cb] = max(sup[sl]) if sup[sl] else 0
+                row_blk[cb] = (max(rows[sl]) + XROWS - 1) // XROWS if rows[sl] else 0
+            n_rb = int(row_blk.max()) if n_cb else 0
+            self._check_factor_range(st, n_rb)
+            Bf = torch.zeros((max(P, 1), st.ld), dtype=torch.float32, device=dev)
+            self.b32.append((Bmat, Bf))
+            ce_d = torch.as_tensor(col_ext, device=dev)
+            rb_d = torch.as_tensor(row_blk, device=dev)
+            self._keep += [ce_d, rb_d]
+            descs.append(XgemmEnergy(hip.ptr(st.logr_d), hip.ptr(Bf), hip.ptr(Amat), hip.ptr(ce_d), hip.ptr(rb_d),
+                                     part_off, st.N, P, st.ld, n_cb, n_rb, base, slot, 0))
+            part_off += int(lib.dwba_xgemm_partial_doubles(n_rb, n_cb, Lm))
+            max_cb = max(max_cb, n_cb)
+            for cb in range(n_cb):
+                ncol = min(XCOLS, P - cb * XCOLS)
+                for rb in range(int(row_blk[cb])):
+                    cost = min(XROWS, st.N - rb * XROWS) * int(col_ext[cb]) * ((ncol + 31) // 32)
+                    for L in range(nl):
+                        items.append((fi, L, cb, rb))
+                        costs.append(cost)
+        self.xg_count = len(descs)
+        self.xg_max_cb = max_cb
+        if descs:
+            arr = (XgemmEnergy * len(descs))(*descs)
+            self.xg_d = hip.struct_array_to_device(arr, dev)
+            order = np.argsort(-np.asarray(costs, dtype=np.int64), kind="stable")
+            it = np.asarray(items, dtype=np.int32).reshape(-1, 4)[order] if items else np.zeros((0, 4), np.int32)
+            self.items_d = torch.as_tensor(np.ascontiguousarray(it), device=dev)
+            self.n_items = len(items)
+            self.partial = torch.zeros(max(part_off, 1), dtype=f64, device=dev)
+            self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def _check_factor_range(self, st, n_rb):
+        lr = np.log(st.gs.r)
+        for rb in range(n_rb):
+            a = rb * XROWS
+            b = min(st.N, a + XROWS) - 1
+            if a < st.N and self.L_max * (lr[b] - lr[a]) > 80.0:
+                raise ValueError("grid too coarse for the FP32 factorised kernel (row block spans "
+                                 f"r in [{st.gs.r[a]:.3g}, {st.gs.r[b]:.3g}]); use precision='fp64'")
+
+    # ------------------------------------------------------------ launch
+    def launch(self, stream=None):
+        """Enqueue the whole scan on `stream` (torch's current by default)."""
+        hip, lib = self.hip, self.lib
+        sh = hip.stream_handle(stream)
+        Lm = self.L_max
+        cxp, _ = hip.f64_array(self.cc[0])
+        cwp, _ = hip.f64_array(self.cc[1])
+        hip.check(lib.dwba_fullsplit_prep_f64(self.n_pairs, hip.ptr(self.prep_d), hip.ptr(self.coul_d),
+                                              int(self.any_coul), cxp, cwp, self.CC_nodes, sh),
+                  "dwba_fullsplit_prep_f64")
+        if self.n_tail:
+            hip.check(lib.dwba_moments_f64(self.n_mom_sets, hip.ptr(self.mom_ptr[0]), hip.ptr(self.mom_ptr[1]),
+                                           hip.ptr(self.mom_ptr[2]), hip.ptr(self.mom_ptr[3]), hip.ptr(self.mom_n),
+                                           Lm, hip.ptr(self.moments), sh), "dwba_moments_f64")
+            seg, dmax = planner.filon_limits()
+            Dp, _ = hip.f64_array(self.levin[0])
+            Xp, _ = hip.f64_array(self.levin[1])
+            hip.check(lib.dwba_outer_tail_f64(self.n_tail, hip.ptr(self.tail_d), Lm, seg, dmax, Dp, Xp, sh),
+                      "dwba_outer_tail_f64")
+        if self.plain_count:
+            nra, nca = self.plain_n
+            hip.check(lib.dwba_il_plain_f64_batch(self.plain_count, hip.ptr(self.plain_d), nra, nca, Lm,
+                                                  hip.ptr(self.out), hip.ptr(self.plain_ws), self.plain_ws_bytes, sh),
+                      "dwba_il_plain_f64_batch")
+        if self.precision == "fp32":
+            if self.dgemv is not None:
+                d, n, mr = self.dgemv
+                hip.check(lib.dwba_dgemv_f64(n, hip.ptr(d), mr, Lm, sh), "dwba_dgemv_f64")
+            if self.ddot is not None:
+                d, n = self.ddot
+                hip.check(lib.dwba_ddot_f64(n, hip.ptr(d), Lm, hip.ptr(self.out), sh), "dwba_ddot_f64")
+            if self.xg_count:
+                for Bm, Bf in self.b32:
+                    hip.check(lib.dwba_cast_f64_f32(hip.ptr(Bm), hip.ptr(Bf), Bm.numel(), sh), "dwba_cast_f64_f32")
+                hip.check(lib.dwba_xgemm_f32(self.xg_count, hip.ptr(self.xg_d), self.n_items, hip.ptr(self.items_d),
+                                             Lm, hip.ptr(self.partial), hip.ptr(self.counter), self.xg_max_cb,
+                                             hip.ptr(self.out), 0, sh), "dwba_xgemm_f32")
+        hip.check(lib.dwba_batch_finalize(self.n_pairs, hip.ptr(self.fin_d), Lm, hip.ptr(self.out), sh),
+                  "dwba_batch_finalize")
+
+    def capture(self):
+        """Capture launch() into a HIP graph (replayed by run())."""
+        torch = self.torch
+        s = torch.cuda.Stream(device=self.dev)
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.launch(s)          # warm-up outside capture
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            self.launch(s)
+        self.graph = g
+        return g
+
+    def run(self):
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self.launch()
+
+    def results(self):
+        """(n_pairs, 2, L_max+1) numpy array in the order the pairs were given."""
+        host = self.out.cpu().numpy()
+        return host[self.out_index]
+
+    def integrals(self) -> List[List[RadialDWBAIntegrals]]:
+        res = self.results()
+        out, k = [], 0
+        counts = {}
+        for (ei, pi, *_rest) in self.pairs:
+            counts[ei] = counts.get(ei, 0) + 1
+        for ei in range(len(self.E)):
+            row = []
+            for _ in range(counts.get(ei, 0)):
+                row.append(RadialDWBAIntegrals({L: float(res[k, 0, L]) for L in range(self.L_max + 1)},
+                                               {L: float(res[k, 1, L]) for L in range(self.L_max + 1)}))
+                k += 1
+            out.append(row)
+        return out

@@ -1,0 +1,178 @@
+// Direct family of a batch, FP64:  t_L = K_L beta_dir  (one shared beta per
+// energy: u_f u_i w on the full grid, dwba_matrix_elements.py:1136-1160,
+// oscillatory_integrals.py:2571), then I_p(L) = alpha_dir,p^T t_L for every
+// pair of the energy.  The work formula W = 2 N1 N2 min(D1, D2) + 2 N1 P with
+// D2 = 1 makes this a matrix-vector product, so it runs on the FP64 VALUs in
+// every precision mode; only the stacked exchange GEMM uses the matrix cores.
+//
+// Also the per-pair assembly of a batch (dwba_batch_finalize): tails and
+// L = 0 corrections added in the order of radial_ME_all_L
+// (dwba_matrix_elements.py:1112-1298).
+#include "common.hpp"
+#include "batch.hpp"
+
+namespace dwba {
+namespace {
+
+constexpr double KCLAMP = 1.0 - 1e-12;
+constexpr int GT = 256;       // rows per workgroup (one per thread)
+constexpr int GCH = 256;      // columns staged per LDS round
+
+template <int NL>
+__global__ __launch_bounds__(GT, 2) void dgemv_kernel(const dwba_dgemv_energy* __restrict__ en) {
+    __shared__ double s_r[GCH], s_ir[GCH], s_b[GCH];
+    const dwba_dgemv_energy E = en[blockIdx.y];
+    const int row0 = blockIdx.x * GT;
+    if (row0 >= E.rows) return;
+    const int i = row0 + threadIdx.x;
+    const bool valid = i < E.rows;
+    const double ri = valid ? E.r[i] : 1.0;
+    const double iri = 1.0 / ri;
+    double acc[NL];
+#pragma unroll
+    for (int L = 0; L < NL; ++L) acc[L] = 0.0;
+    for (int j0 = 0; j0 < E.N; j0 += GCH) {
+        __syncthreads();
+        {
+            const int j = j0 + threadIdx.x;
+            const bool in = j < E.N;
+            const double rj = in ? E.r[j] : 1.0;
+            s_r[threadIdx.x] = rj;
+            s_ir[threadIdx.x] = 1.0 / rj;
+            s_b[threadIdx.x] = in ? E.beta[j] : 0.0;
+        }
+        __syncthreads();
+        const int jn = min(GCH, E.N - j0);
+#pragma unroll 1
+        for (int c = 0; c < jn; ++c) {
+            const int j = j0 + c;
+            const double b = s_b[c];
+            double K, q;
+            if (j > i) { K = s_ir[c]; q = fmin(ri * s_ir[c], KCLAMP); }
+            else if (j < i) { K = iri; q = fmin(s_r[c] * iri, KCLAMP); }
+            else { K = iri; q = KCLAMP; }
+#pragma unroll
+            for (int L = 0; L < NL; ++L) {
+                acc[L] = fma(b, K, acc[L]);
+                asm volatile("" : "+v"(K), "+v"(acc[L]));
+                K *= q;
+            }
+        }
+    }
+    if (valid) {
+#pragma unroll
+        for (int L = 0; L < NL; ++L)
+            if (L <= E.L_max) E.t[(size_t)L * E.ld + i] = acc[L];
+    }
+}
+
+// I_p(L) = sum_i alpha_p[i] t_L[i], one workgroup per pair.
+template <int NL>
+__global__ __launch_bounds__(256) void ddot_kernel(const dwba_ddot_pair* __restrict__ pairs, int L_max,
+                                                   double* __restrict__ out) {
+    __shared__ double s_red[4][NL];
+    const dwba_ddot_pair P = pairs[blockIdx.x];
+    double acc[NL];
+#pragma unroll
+    for (int L = 0; L < NL; ++L) acc[L] = 0.0;
+    for (int i = threadIdx.x; i < P.n; i += 256) {
+        const double a = P.alpha[i];
+        if (a != 0.0) {
+#pragma unroll
+            for (int L = 0; L < NL; ++L)
+                if (L <= L_max) acc[L] = fma(a, P.t[(size_t)L * P.ld + i], acc[L]);
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int L = 0; L < NL; ++L) {
+        const double v = wave_sum(acc[L]);
+        if (lane == 0) s_red[wave][L] = v;
+    }
+    __syncthreads();
+    for (int L = threadIdx.x; L <= L_max; L += 256)
+        out[(size_t)P.out_index * 2 * (L_max + 1) + L] = ((s_red[0][L] + s_red[1][L]) + s_red[2][L]) + s_red[3][L];
+}
+
+// Per-pair assembly (see dwba_batch_finalize in the header).
+__global__ void finalize_kernel(const dwba_finalize_pair* __restrict__ pairs, int n_pairs, int L_max,
+                                double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_pairs) return;
+    const dwba_finalize_pair P = pairs[k];
+    const int nl = L_max + 1;
+    double* o = out + (size_t)P.out_index * 2 * nl;
+    const double* sc = P.scalars;
+    for (int L = 0; L <= L_max; ++L) {
+        double d = o[L], e = o[nl + L];
+        if (P.oscillatory) {
+            if (!isfinite(d)) d = 0.0;
+            if (!isfinite(e)) e = 0.0;
+            if (P.tail_kind == DWBA_TAIL_OUTER) d = d + P.tails[L];
+        }
+        if (L == 0) {
+            if (fabs(sc[0]) > 1e-12) d += sc[1] * sc[0];
+            if (fabs(sc[2]) > 1e-12) e += sc[3] * sc[2];
+        }
+        if (P.tail_kind == DWBA_TAIL_LEGACY && L >= 1) d += P.tails[L];
+        o[L] = d;
+        o[nl + L] = e;
+    }
+}
+
+int nl_bucket_g(int L_max) {
+    static const int buckets[] = {1, 2, 4, 8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64};
+    for (int b : buckets)
+        if (L_max + 1 <= b) return b;
+    return -1;
+}
+
+}  // namespace
+}  // namespace dwba
+
+using namespace dwba;
+
+extern "C" {
+
+int32_t dwba_dgemv_f64(int32_t n_energies, const dwba_dgemv_energy* d_energies, int32_t max_rows,
+                       int32_t L_max, void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_dgemv_f64: call dwba_init first");
+    DWBA_REQUIRE(n_energies > 0 && d_energies && max_rows > 0 && L_max >= 0 && L_max <= 63,
+                 "dwba_dgemv_f64: bad arguments");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    dim3 grid((max_rows + GT - 1) / GT, n_energies);
+    switch (nl_bucket_g(L_max)) {
+#define C(n) case n: hipLaunchKernelGGL(dgemv_kernel<n>, grid, dim3(GT), 0, s, d_energies); break;
+        C(1) C(2) C(4) C(8) C(12) C(16) C(20) C(24) C(28) C(32) C(40) C(48) C(56) C(64)
+#undef C
+        default: return set_error(DWBA_ERR_ARG, "dwba_dgemv_f64: L_max %d", L_max);
+    }
+    DWBA_LAUNCH_CHECK("dgemv_kernel");
+    return DWBA_OK;
+}
+
+int32_t dwba_ddot_f64(int32_t n_pairs, const dwba_ddot_pair* d_pairs, int32_t L_max, double* d_out, void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_ddot_f64: call dwba_init first");
+    DWBA_REQUIRE(n_pairs > 0 && d_pairs && d_out && L_max >= 0 && L_max <= 63, "dwba_ddot_f64: bad arguments");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    switch (nl_bucket_g(L_max)) {
+#define C(n) case n: hipLaunchKernelGGL(ddot_kernel<n>, dim3(n_pairs), dim3(256), 0, s, d_pairs, L_max, d_out); break;
+        C(1) C(2) C(4) C(8) C(12) C(16) C(20) C(24) C(28) C(32) C(40) C(48) C(56) C(64)
+#undef C
+        default: return set_error(DWBA_ERR_ARG, "dwba_ddot_f64: L_max %d", L_max);
+    }
+    DWBA_LAUNCH_CHECK("ddot_kernel");
+    return DWBA_OK;
+}
+
+int32_t dwba_batch_finalize(int32_t n_pairs, const dwba_finalize_pair* d_pairs, int32_t L_max, double* d_out,
+                            void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_batch_finalize: call dwba_init first");
+    DWBA_REQUIRE(n_pairs > 0 && d_pairs && d_out && L_max >= 0, "dwba_batch_finalize: bad arguments");
+    hipLaunchKernelGGL(finalize_kernel, dim3((n_pairs + 255) / 256), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), d_pairs, n_pairs, L_max, d_out);
+    DWBA_LAUNCH_CHECK("finalize_kernel");
+    return DWBA_OK;
+}
+
+}  // extern "C"

@@ -108,3 +108,64 @@ def kernel_only_workload(n: int = 15000, seed: int = 0):
     """Kernel-only single pair: N = 15000 exp grid on [1e-5, 1000]."""
     g = exp_grid(n, 1e-5, 1e3)
     return g, plain_pair(g, seed)
+
+
+def h_potentials(r):
+    """V_core = -1/r (Z = 1) and the static distorting potential of H(1s),
+    U = -(1 + 1/r) e^{-2r} (V_core + Hartree of the 1s density)."""
+    V = -1.0 / r
+    U = -(1.0 + 1.0 / r) * np.exp(-2.0 * r)
+    return V, U
+
+
+def excitation_scan(n_energies=64, n_pairs=400, n_range=(4000, 15000), l_max=40, k_range=(0.5, 8.0),
+                    seed=0, with_match=True):
+    """Batched precision-sweep workload: per energy its own exp grid
+    N ~ U{n_range} on [1e-5, 1000] a0, (k_i, k_f) ~ U[k_range] per energy,
+    n_pairs channel pairs with (l_i, l_f) ~ U{0..l_max}, delta ~ U[0, pi]
+    per wave; bound pair u_1s -> u_2p; waves carry match points."""
+    from .batch import EnergyInput
+    rng = np.random.default_rng(seed)
+    out = []
+    for e in range(n_energies):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        g = exp_grid(n, 1e-5, 1e3)
+        r = g.r
+        ki, kf = rng.uniform(*k_range, size=2)
+        V, U = h_potentials(r)
+        pairs = []
+        for _ in range(n_pairs):
+            li, lf = rng.integers(0, l_max + 1, size=2)
+            di, df = rng.uniform(0.0, np.pi, size=2)
+            pairs.append((make_wave(r, ki, li, di, with_match), make_wave(r, kf, lf, df, with_match)))
+        out.append(EnergyInput(grid=g, V_core=V, U_i=U, bound_i=Bound(u1s(r), l=0, energy_au=-0.5),
+                               bound_f=Bound(u2p(r), l=1, energy_au=-0.125), pairs=pairs, U_f=U))
+    return out
+
+
+def ionization_scan(n_points=48, n_triples=400, n_range=(6000, 15000), l_max=30, seed=0, k_i=2.0):
+    """Ionization-style workload: Slater bound state (zeta = 0.6) and three
+    continuum waves (incident k_i, ejected k_e ~ U[0.2, 4], scattered k_f with
+    k_f^2 = k_i^2 - k_e^2 - 2*0.18) per ejected-energy point on its own grid
+    N ~ U{n_range}; n_triples (l_i, l_f, l_ej) ~ U{0..l_max} per point, drawn
+    with l_i + l_f + l_ej even.  The ejected wave takes the place of u_f."""
+    from .batch import EnergyInput
+    rng = np.random.default_rng(seed)
+    out = []
+    for e in range(n_points):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        g = exp_grid(n, 1e-5, 1e3)
+        r = g.r
+        ke = rng.uniform(0.2, 4.0)
+        kf = np.sqrt(max(k_i ** 2 - ke ** 2 - 0.36, 0.04))
+        V, U = h_potentials(r)
+        pairs = []
+        while len(pairs) < n_triples:
+            li, lf, le = rng.integers(0, l_max + 1, size=3)
+            if (li + lf + le) % 2:
+                continue
+            di, df, de = rng.uniform(0.0, np.pi, size=3)
+            pairs.append((make_wave(r, k_i, li, di), make_wave(r, kf, lf, df), make_wave(r, ke, le, de)))
+        out.append(EnergyInput(grid=g, V_core=V, U_i=U, bound_i=Bound(slater(r, 0.6), l=0, energy_au=-0.18),
+                               bound_f=pairs[0][2], pairs=pairs, U_f=U))
+    return out

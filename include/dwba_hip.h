@@ -196,6 +196,80 @@ int32_t dwba_outer_tail_f64(int32_t n_jobs, const dwba_tail_job* d_jobs, int32_t
                             int32_t max_filon_segments, double max_effective_dphi,
                             const double* h_levin_D, const double* h_levin_x, void* stream);
 
+/* ------------------------------------------------------------------ */
+/* Batched scan: all channel pairs of all energies in one launch each.    */
+/* ------------------------------------------------------------------ */
+/* Exchange family on MFMA.  One descriptor per energy; pairs are columns.
+ * B: beta_ex of pair p at B + p*ld (FP32, zero beyond its support and for
+ * ld > j >= N); A: alpha_ex (FP64, same layout, zero beyond N1_p).  Pairs
+ * are grouped in column blocks of 128 (ideally sorted by support);
+ * col_extent[cb] is the largest beta support in block cb and row_blocks[cb]
+ * the number of 256-row blocks covering the largest alpha support.  Partial
+ * sums of (L, cb, rb) live at partial + part_off + ((L*n_cb + cb)*n_rb + rb)*128. */
+typedef struct dwba_xgemm_energy {
+    const double* logr;
+    const float* B;
+    const double* A;
+    const int32_t* col_extent;
+    const int32_t* row_blocks;
+    int64_t part_off;
+    int32_t N, P, ld, n_cb, n_rb;
+    int32_t pair_base;        /* index of this energy's first pair in d_out */
+    int32_t out_slot;         /* 0: direct, 1: exchange half of d_out */
+    int32_t pad_;
+} dwba_xgemm_energy;
+
+/* y[i] = (float)x[i], i < n (rounding of the beta columns for the FP32 MFMA path). */
+int32_t dwba_cast_f64_f32(const double* d_x, float* d_y, int64_t n, void* stream);
+
+/* Doubles of partial storage one energy needs. */
+size_t dwba_xgemm_partial_doubles(int32_t n_rb, int32_t n_cb, int32_t L_max);
+
+/* d_items: n_items work items (int32 x4: energy, L, cb, rb) in the order they
+ * should start (longest first); d_counter: one int32 of scratch (reset by the
+ * call with a kernel, so the call can be captured in a HIP graph).  Writes
+ * d_out[(pair_base+p)*2*(L_max+1) + out_slot*(L_max+1) + L] for every pair p.
+ * n_workgroups <= 0 picks one persistent workgroup per CU. */
+int32_t dwba_xgemm_f32(int32_t n_energies, const dwba_xgemm_energy* d_energies, int32_t n_items,
+                       const int32_t* d_items, int32_t L_max, double* d_partial, int32_t* d_counter,
+                       int32_t max_cb, double* d_out, int32_t n_workgroups, void* stream);
+
+/* Direct family (FP64 VALU): t[L*ld + i] = sum_j K_L(i, j) beta[j] for
+ * i < rows, j < N, L = 0..L_max, for each energy (one shared beta per
+ * energy).  max_rows = max over energies of rows. */
+typedef struct dwba_dgemv_energy {
+    const double* r;
+    const double* beta;
+    double* t;
+    int32_t N, rows, ld, L_max;
+} dwba_dgemv_energy;
+int32_t dwba_dgemv_f64(int32_t n_energies, const dwba_dgemv_energy* d_energies, int32_t max_rows,
+                       int32_t L_max, void* stream);
+
+/* d_out[out_index*2*(L_max+1) + L] = sum_{i<n} alpha[i] t[L*ld + i], one
+ * workgroup per pair. */
+typedef struct dwba_ddot_pair {
+    const double* alpha;
+    const double* t;
+    int32_t n, ld, out_index, pad_;
+} dwba_ddot_pair;
+int32_t dwba_ddot_f64(int32_t n_pairs, const dwba_ddot_pair* d_pairs, int32_t L_max, double* d_out,
+                      void* stream);
+
+/* Per-pair assembly of I_L in place in d_out (layout [pair][2][L_max+1]):
+ * contractions zeroed when non-finite in the oscillatory branch, tail added
+ * (DWBA_TAIL_OUTER before, DWBA_TAIL_LEGACY after the L = 0 correction),
+ * L = 0 correction scalars[1]*scalars[0] / scalars[3]*scalars[2] when
+ * |scalars[0|2]| > 1e-12 -- the order of radial_ME_all_L
+ * (dwba_matrix_elements.py:1112-1298). */
+typedef struct dwba_finalize_pair {
+    const double* scalars;    /* 4 values from dwba_fullsplit_prep_f64 */
+    const double* tails;      /* L_max+1 values from dwba_outer_tail_f64 (or NULL if none) */
+    int32_t out_index, oscillatory, tail_kind, pad_;
+} dwba_finalize_pair;
+int32_t dwba_batch_finalize(int32_t n_pairs, const dwba_finalize_pair* d_pairs, int32_t L_max, double* d_out,
+                            void* stream);
+
 #ifdef __cplusplus
 }
 #endif

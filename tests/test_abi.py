@@ -37,3 +37,39 @@ def test_version_and_errors_without_gpu():
     assert lib.dwba_il_plain_f64_workspace_bytes(100, 100, 30) > 0
     assert lib.dwba_il_plain_f64_workspace_bytes(101, 100, 30) == 0   # n_rows > n_cols
     assert lib.dwba_il_plain_f64_workspace_bytes(10, 10, 64) == 0     # L_max too large
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors of the C structs have the header's size and field offsets."""
+    import shutil
+    import subprocess
+    from dwba_amd import batch
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        import pytest
+        pytest.skip("gcc not available")
+    structs = {
+        "dwba_plain_job": _hip.PlainJob, "dwba_prep_job": _hip.PrepJob, "dwba_tail_job": _hip.TailJob,
+        "dwba_xgemm_energy": batch.XgemmEnergy, "dwba_dgemv_energy": batch.DgemvEnergy,
+        "dwba_ddot_pair": batch.DdotPair, "dwba_finalize_pair": batch.FinalizePair,
+    }
+    src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
+    for cname, cls in structs.items():
+        src.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            src.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    src.append("return 0;}")
+    c = tmp_path / "probe.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "probe"
+    subprocess.run([gcc, str(c), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line.strip():
+            a, b, v = line.split()
+            got[(a, b)] = int(v)
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)

@@ -1,0 +1,98 @@
+"""GPU checks of the batched scan engine (ScanBatch).
+
+* precision "fp64": every pair must equal the single-pair drop-in
+  (itself checked against the oracle) to 1e-12 relative.
+* precision "fp32": direct integrals stay FP64 (GEMV on the VALUs) and must
+  match to 1e-10; exchange integrals come from the FP32 MFMA stacked GEMM and
+  must satisfy |dI| <= 1e-6 * sum_ij |alpha_i| K_ij |beta_j| (the
+  cancellation-aware bound: FP32 rounding of K and beta gives ~1e-8 of it).
+* a captured HIP graph replays to the eager result, several times, with
+  eager work and synchronisations in between.
+"""
+import numpy as np
+import pytest
+
+from oracle import radial as R, semisep
+from dwba_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_scan(seed=0, n_e=3, n_p=21, n_range=(1500, 3000), ionization=False):
+    if ionization:
+        return S.ionization_scan(n_points=n_e, n_triples=n_p, n_range=n_range, l_max=12, seed=seed)
+    return S.excitation_scan(n_energies=n_e, n_pairs=n_p, n_range=n_range, l_max=15, k_range=(0.5, 4.0), seed=seed)
+
+
+def _dropin(en, pr, L_max, **kw):
+    from dwba_amd.dwba_matrix_elements import radial_ME_all_L_gpu
+    bf = pr[2] if len(pr) > 2 else en.bound_f
+    return radial_ME_all_L_gpu(en.grid, en.V_core, en.U_i, en.bound_i, bf, pr[0], pr[1], L_max, U_f_array=en.U_f, **kw)
+
+
+@pytest.mark.parametrize("ionization", [False, True])
+def test_fp64_batch_equals_dropin(ionization):
+    from dwba_amd.batch import ScanBatch
+    scan = _small_scan(1, ionization=ionization)
+    b = ScanBatch(scan, 8, precision="fp64", oscillatory_method="full_split")
+    b.run()
+    res = b.integrals()
+    for ei, en in enumerate(scan):
+        for pi, pr in enumerate(en.pairs):
+            ref = _dropin(en, pr, 8, oscillatory_method="full_split")
+            # same kernels, different block partition of the contraction:
+            # rounding-level differences relative to the largest |I_L| of the family
+            scale_d = max(abs(v) for v in ref.I_L_direct.values())
+            scale_e = max(abs(v) for v in ref.I_L_exchange.values())
+            for L in range(9):
+                for a, r_, sc in ((res[ei][pi].I_L_direct[L], ref.I_L_direct[L], scale_d),
+                                  (res[ei][pi].I_L_exchange[L], ref.I_L_exchange[L], scale_e)):
+                    assert abs(a - r_) <= 1e-12 * abs(r_) + 1e-13 * sc, (ei, pi, L, a, r_)
+
+
+@pytest.mark.parametrize("ionization", [False, True])
+def test_fp32_batch_against_oracle(ionization):
+    from dwba_amd.batch import ScanBatch
+    scan = _small_scan(2, n_e=2, n_p=40, ionization=ionization)
+    L_max = 10
+    b = ScanBatch(scan, L_max, precision="fp32", oscillatory_method="advanced")
+    b.run()
+    res = b.results()
+    k = 0
+    worst = 0.0
+    for en in scan:
+        g = en.grid
+        for pr in en.pairs:
+            bf = pr[2] if len(pr) > 2 else en.bound_f
+            red, parts = R.radial_ME_all_L_reduced(g.r, g.w_simpson, g.w_trapz, en.V_core, en.U_i, en.bound_i, bf,
+                                                   pr[0], pr[1], L_max, U_f_array=en.U_f, return_parts=True)
+            mag_e = semisep.abs_bound_all_L(g.r, parts["alpha_ex"], parts["beta_ex"], L_max)
+            mag_d = semisep.abs_bound_all_L(g.r, parts["alpha_dir"], parts["beta_dir"], L_max)
+            for L in range(L_max + 1):
+                d_ref, e_ref = red.I_L_direct[L], red.I_L_exchange[L]
+                if ionization:   # direct family also through the FP32 stacked GEMM
+                    assert abs(res[k, 0, L] - d_ref) <= 1e-6 * mag_d[L] + 10 * parts["tail_noise"][L]
+                else:
+                    assert abs(res[k, 0, L] - d_ref) <= 1e-10 * abs(d_ref) + 1e-13 * mag_d[L] + 10 * parts["tail_noise"][L]
+                err = abs(res[k, 1, L] - e_ref) / mag_e[L]
+                worst = max(worst, err)
+                assert err <= 1e-6, (k, L, res[k, 1, L], e_ref, err)
+            k += 1
+    print("worst exchange |dI|/sum|aKb| =", worst)
+
+
+def test_graph_replay_matches_eager():
+    import torch
+    from dwba_amd.batch import ScanBatch
+    scan = _small_scan(3, n_e=2, n_p=30)
+    b = ScanBatch(scan, 6, precision="fp32")
+    b.launch()
+    eager = b.results().copy()
+    b.capture()
+    for rep in range(3):
+        b.out.zero_()
+        torch.cuda.synchronize()
+        b.run()
+        torch.cuda.synchronize()
+        assert np.array_equal(b.results(), eager), rep
+        _ = (torch.ones(1000, device="cuda") * rep).sum().item()   # eager work between replays
