@@ -442,16 +442,30 @@ class ScanBatch:
                                  f"r in [{st.gs.r[a]:.3g}, {st.gs.r[b]:.3g}]); use precision='fp64'")
 
     # ------------------------------------------------------------ launch
-    def launch(self, stream=None):
-        """Enqueue the whole scan on `stream` (torch's current by default)."""
+    def launch(self, stream=None, events=None):
+        """Enqueue the whole scan on `stream` (torch's current by default).
+        `events`: optional dict; HIP events are recorded around each stage
+        under keys (stage, 'start'|'end') for timing on the launch stream."""
         hip, lib = self.hip, self.lib
         sh = hip.stream_handle(stream)
+        torch = self.torch
+        strm = stream if stream is not None else torch.cuda.current_stream()
+
+        def mark(name, which):
+            if events is not None:
+                ev = torch.cuda.Event(enable_timing=True)
+                ev.record(strm)
+                events[(name, which)] = ev
+        mark("total", "start")
+        mark("prep", "start")
         Lm = self.L_max
         cxp, _ = hip.f64_array(self.cc[0])
         cwp, _ = hip.f64_array(self.cc[1])
         hip.check(lib.dwba_fullsplit_prep_f64(self.n_pairs, hip.ptr(self.prep_d), hip.ptr(self.coul_d),
                                               int(self.any_coul), cxp, cwp, self.CC_nodes, sh),
                   "dwba_fullsplit_prep_f64")
+        mark("prep", "end")
+        mark("tail", "start")
         if self.n_tail:
             hip.check(lib.dwba_moments_f64(self.n_mom_sets, hip.ptr(self.mom_ptr[0]), hip.ptr(self.mom_ptr[1]),
                                            hip.ptr(self.mom_ptr[2]), hip.ptr(self.mom_ptr[3]), hip.ptr(self.mom_n),
@@ -461,26 +475,101 @@ class ScanBatch:
             Xp, _ = hip.f64_array(self.levin[1])
             hip.check(lib.dwba_outer_tail_f64(self.n_tail, hip.ptr(self.tail_d), Lm, seg, dmax, Dp, Xp, sh),
                       "dwba_outer_tail_f64")
+        mark("tail", "end")
+        mark("plain", "start")
         if self.plain_count:
             nra, nca = self.plain_n
             hip.check(lib.dwba_il_plain_f64_batch(self.plain_count, hip.ptr(self.plain_d), nra, nca, Lm,
                                                   hip.ptr(self.out), hip.ptr(self.plain_ws), self.plain_ws_bytes, sh),
                       "dwba_il_plain_f64_batch")
+        mark("plain", "end")
         if self.precision == "fp32":
+            mark("direct", "start")
             if self.dgemv is not None:
                 d, n, mr = self.dgemv
                 hip.check(lib.dwba_dgemv_f64(n, hip.ptr(d), mr, Lm, sh), "dwba_dgemv_f64")
             if self.ddot is not None:
                 d, n = self.ddot
                 hip.check(lib.dwba_ddot_f64(n, hip.ptr(d), Lm, hip.ptr(self.out), sh), "dwba_ddot_f64")
+            mark("direct", "end")
             if self.xg_count:
                 for Bm, Bf in self.b32:
                     hip.check(lib.dwba_cast_f64_f32(hip.ptr(Bm), hip.ptr(Bf), Bm.numel(), sh), "dwba_cast_f64_f32")
+                mark("xgemm", "start")
                 hip.check(lib.dwba_xgemm_f32(self.xg_count, hip.ptr(self.xg_d), self.n_items, hip.ptr(self.items_d),
                                              Lm, hip.ptr(self.partial), hip.ptr(self.counter), self.xg_max_cb,
                                              hip.ptr(self.out), 0, sh), "dwba_xgemm_f32")
+                mark("xgemm", "end")
         hip.check(lib.dwba_batch_finalize(self.n_pairs, hip.ptr(self.fin_d), Lm, hip.ptr(self.out), sh),
                   "dwba_batch_finalize")
+        mark("total", "end")
+
+    def stage_ms(self, events):
+        """Stage durations (ms) from the events recorded by launch(events=...)."""
+        out = {}
+        for (name, which), ev in events.items():
+            if which == "start" and (name, "end") in events:
+                out[name] = ev.elapsed_time(events[(name, "end")])
+        return out
+
+    def magnitudes(self):
+        """sum_ij |alpha_i| K_L(i,j) |beta_j| per (pair, family, L) in FP64: the
+        cancellation-free size of each contraction, used to state errors of
+        reduced-precision modes relative to the terms that are summed.  Needs
+        the stacked alpha/beta of a run (call after run()).  Returns an array
+        in the order the pairs were given."""
+        torch, hip, lib = self.torch, self.hip, self.lib
+        n = self.n_pairs
+        arr = (hip.PlainJob * n)()
+        nra = (ctypes.c_int32 * n)()
+        nca = (ctypes.c_int32 * n)()
+        keep = []
+        pos = {k: c for st in self.E for c, k in enumerate(st.members)}
+        for k, (ei, pi, plan, ci, cf, bf) in enumerate(self.pairs):
+            st = self.E[ei]
+            if k in pos:
+                c = pos[k]
+                vec = [st.ad[c], st.bd[c], st.ae[c], st.be[c]]
+            else:
+                vec = list(self._plain_vec[k])
+            av = [v.abs() for v in vec]
+            keep += av
+            g = int(self.out_index[k])
+            arr[g] = hip.PlainJob(hip.ptr(st.r_d), hip.ptr(av[0]), hip.ptr(av[1]), hip.ptr(av[2]), hip.ptr(av[3]),
+                                  plan.rows, plan.cols)
+            nra[g], nca[g] = plan.rows, plan.cols
+        d_jobs = hip.struct_array_to_device(arr, self.dev)
+        wsb = lib.dwba_il_plain_f64_batch_workspace_bytes(n, nra, nca, self.L_max)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=self.dev)
+        mag = torch.zeros((n, 2, self.L_max + 1), dtype=torch.float64, device=self.dev)
+        hip.check(lib.dwba_il_plain_f64_batch(n, hip.ptr(d_jobs), nra, nca, self.L_max, hip.ptr(mag), hip.ptr(ws),
+                                              wsb, hip.stream_handle()), "dwba_il_plain_f64_batch")
+        return mag.cpu().numpy()[self.out_index]
+
+    def work_flop(self):
+        """Algorithmic work W = 2 N1 N2 min(D1, D2) + 2 N1 P per (energy, L,
+        family), summed: exchange columns are distinct (D = P), so each pair
+        contributes 2 N1_p S_p + 2 N1_p with S_p its beta support; the direct
+        family with one shared beta contributes 2 R N + 2 sum_p N1_p per energy
+        (R = largest N1).  Returns (W_direct, W_exchange) per launch."""
+        nl = self.L_max + 1
+        wd = we = 0.0
+        for ei, st in enumerate(self.E):
+            plans = [p[2] for p in self.pairs if p[0] == ei]
+            shared = len({id(p[5]) for p in self.pairs if p[0] == ei}) == 1
+            act = [p for p in plans if p.mode in (PREP_FILON, PREP_STANDARD)]
+            if act:
+                R = max(p.N1 for p in act)
+                if shared:
+                    wd += nl * (2.0 * R * st.N + 2.0 * sum(p.N1 for p in act))
+                else:
+                    wd += nl * sum(2.0 * p.N1 * st.N + 2.0 * p.N1 for p in act)
+                we += nl * sum(2.0 * p.N1 * self._ex_support(p) + 2.0 * p.N1 for p in act)
+            for p in plans:
+                if p.mode == PREP_PLAIN:
+                    wd += nl * (2.0 * p.N1 * p.N1 + 2.0 * p.N1)
+                    we += nl * (2.0 * p.N1 * p.N1 + 2.0 * p.N1)
+        return wd, we
 
     def capture(self):
         """Capture launch() into a HIP graph (replayed by run())."""

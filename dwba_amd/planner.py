@@ -211,10 +211,17 @@ def split_index(gs: GridStats, U_i, U_f, u_i, u_f, k_i, k_f, l_i, l_f, idx_match
             return max(ui, uf) + cent / (2.0 * r[t] ** 2)
 
         if veff(max(0, idx - 1)) / kin > 0.03:
-            for t in range(idx, min(idx + 500, n)):
-                if veff(t) / kin < 0.03:
-                    idx = t + 1
-                    break
+            # first t in [idx, idx+500) with V_eff(t)/E < 0.03 (vectorised scan)
+            t1 = min(idx + 500, n)
+            if t1 > idx:
+                rr = r[idx:t1]
+                ui = np.abs(np.asarray(U_i[idx:t1], dtype=np.float64))
+                v = np.maximum(ui, np.abs(np.asarray(U_f[idx:t1], dtype=np.float64))) if U_f is not None \
+                    else np.maximum(ui, 0.0)
+                v = v + cent / (2.0 * rr ** 2)
+                hit = np.nonzero(v / kin < 0.03)[0]
+                if len(hit):
+                    idx = idx + int(hit[0]) + 1
     return int(idx)
 
 

@@ -57,17 +57,50 @@ def slater(r, zeta=0.6):
     return (2 * zeta) ** 1.5 / np.sqrt(2.0) * r * np.exp(-zeta * r)
 
 
+_WENV: dict = {}
+
+
+def _wave_envelope(r, l):
+    """(1 - e^{-r})^{l+1}, cached per (grid, l)."""
+    key = (id(r), len(r), float(r[0]), float(r[-1]), int(l))
+    hit = _WENV.get(key)
+    if hit is not None and hit[0] is r:
+        return hit[1]
+    v = (1.0 - np.exp(-r)) ** (l + 1)
+    if len(_WENV) > 8192:
+        _WENV.clear()
+    _WENV[key] = (r, v)
+    return v
+
+
 def wave_values(r, k, l, delta):
-    return np.sin(k * r - l * np.pi / 2 + delta) * (1.0 - np.exp(-r)) ** (l + 1)
+    return np.sin(k * r - l * np.pi / 2 + delta) * _wave_envelope(r, l)
+
+
+_ENV_CACHE: dict = {}
+
+
+def _envelope_index(r, l, tol):
+    """First index from which 1 - (1-e^{-r})^{l+1} < tol (the deficit decreases with r)."""
+    key = (id(r), len(r), float(r[0]), float(r[-1]), int(l), tol)
+    hit = _ENV_CACHE.get(key)
+    if hit is not None and hit[0] is r:
+        return hit[1]
+    deficit = -np.expm1((l + 1) * np.log1p(-np.exp(-r)))
+    ok = np.nonzero(deficit < tol)[0]
+    v = int(ok[0]) if len(ok) else len(r)
+    if len(_ENV_CACHE) > 4096:
+        _ENV_CACHE.clear()
+    _ENV_CACHE[key] = (r, v)
+    return v
 
 
 def match_index(r, k, l, tol=1e-12) -> int:
     """Match point of a synthetic wave: first grid index beyond the turning
     point (l + 1/2)/k where the envelope deficit 1 - (1-e^{-r})^{l+1} < tol."""
     r_turn = (l + 0.5) / k
-    deficit = -np.expm1((l + 1) * np.log1p(-np.exp(-r)))
-    ok = np.nonzero((deficit < tol) & (r > r_turn))[0]
-    return int(ok[0]) if len(ok) else len(r) - 1
+    i = max(_envelope_index(r, l, tol), int(np.searchsorted(r, r_turn, side="right")))
+    return i if i < len(r) else len(r) - 1
 
 
 def make_wave(r, k, l, delta, with_match=True) -> Wave:
