@@ -215,6 +215,27 @@ def bench_batched(args):
                               "kernel": "xgemm_kernel<0> (v_mfma_f32_32x32x2_f32)",
                               "work_flop_per_launch": we, "kernel_ms": t_x}
 
+    resbf = None
+    if not args.no_bf16:
+        del b32
+        bbf = ScanBatch(scan, L_max, precision="bf16x3", oscillatory_method="full_split")
+        bbf.launch()
+        evb = {}
+        bbf.launch(events=evb)
+        torch.cuda.synchronize()
+        st_bf = bbf.stage_ms(evb)
+        if not args.no_graph:
+            bbf.capture()
+        for _ in range(args.warmup):
+            bbf.run()
+        msbf = _events_ms(bbf.run, args.steps, stream)
+        resbf = bbf.results()
+        print(f"[bench] bf16x3: {msbf:.1f} ms/step, stages {st_bf}", file=sys.stderr, flush=True)
+        result["bf16x3"] = {"value": n_int / (msbf * 1e-3), "unit": "integrals/s", "ms_per_step": msbf,
+                            "stage_ms": st_bf,
+                            "xgemm_tflops_effective": (we / (st_bf["xgemm"] * 1e-3) / 1e12) if st_bf.get("xgemm") else None,
+                            "peak_effective_tflops": 2500.0 / 3}
+        del bbf
     if not args.no_fp64:
         b64 = ScanBatch(scan, L_max, precision="fp64", oscillatory_method="full_split")
         b64.run()
@@ -230,6 +251,18 @@ def bench_batched(args):
         rel = d / np.maximum(np.abs(res64), 1e-300)
         canc = d / np.maximum(mag, 1e-300)
         ex_rel, ex_canc = rel[:, 1, :], canc[:, 1, :]
+        if resbf is not None:
+            dbf = np.abs(resbf - res64)
+            rb = dbf / np.maximum(np.abs(res64), 1e-300)
+            cb = dbf / np.maximum(mag, 1e-300)
+            result["bf16x3"]["speedup_vs_fp64_plain"] = ms64 / result["bf16x3"]["ms_per_step"]
+            result["bf16x3"]["rel_err_vs_fp64"] = {
+                "exchange_max_abs_over_sum_abs_terms": float(np.max(cb[:, 1, :])),
+                "exchange_max_abs_over_sum_abs_terms_per_L": [float(x) for x in np.max(cb[:, 1, :], axis=0)],
+                "exchange_median_rel": float(np.median(rb[:, 1, :])),
+                "exchange_frac_rel_le_1e-6": float(np.mean(rb[:, 1, :] <= 1e-6)),
+                "direct_max_abs_over_sum_abs_terms": float(np.max(cb[:, 0, :])),
+            }
         result["rel_err_vs_fp64"] = {
             "exchange_max_abs_over_sum_abs_terms": float(np.max(ex_canc)),
             "exchange_max_abs_over_sum_abs_terms_per_L": [float(x) for x in np.max(ex_canc, axis=0)],
@@ -237,6 +270,7 @@ def bench_batched(args):
             "exchange_frac_rel_le_1e-6": float(np.mean(ex_rel <= 1e-6)),
             "exchange_max_rel": float(np.max(ex_rel)),
             "direct_max_rel": float(np.max(rel[:, 0, :])),
+            "direct_max_abs_over_sum_abs_terms": float(np.max(canc[:, 0, :])),
             "note": "|dI| / sum_ij |a_i| K_ij |b_j| is the cancellation-aware error; exchange integrals cancelled "
                     "far below their terms cannot reach 1e-6 relative in FP32 operands",
         }
@@ -285,6 +319,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=400)
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

@@ -72,10 +72,13 @@ _SIGS = {
     "dwba_il_plain_f64_batch": (c_i32, [c_i32, c_vp, c_i32p, c_i32p, c_i32, c_vp, c_vp, c_sz, c_vp]),
     "dwba_fullsplit_prep_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_dp, c_dp, c_i32, c_vp]),
     "dwba_moments_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]),
-    "dwba_outer_tail_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, ctypes.c_double, c_dp, c_dp, c_vp]),
+    "dwba_outer_tail_workspace_bytes": (c_sz, [c_i32, c_i32]),
+    "dwba_outer_tail_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, ctypes.c_double, c_dp, c_dp, c_vp, c_sz, c_vp]),
     "dwba_cast_f64_f32": (c_i32, [c_vp, c_vp, ctypes.c_int64, c_vp]),
     "dwba_xgemm_partial_doubles": (c_sz, [c_i32, c_i32, c_i32]),
     "dwba_xgemm_f32": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp, c_i32, c_vp, c_i32, c_vp]),
+    "dwba_xgemm_bf16x3": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp, c_i32, c_vp, c_i32, c_vp]),
+    "dwba_split_bf16": (c_i32, [c_vp, c_vp, c_vp, ctypes.c_int64, c_vp]),
     "dwba_dgemv_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_vp]),
     "dwba_ddot_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
     "dwba_batch_finalize": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),

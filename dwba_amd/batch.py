@@ -38,7 +38,8 @@ XROWS, XCOLS, XK = 256, 128, 32
 
 class XgemmEnergy(ctypes.Structure):
     """dwba_xgemm_energy of include/dwba_hip.h."""
-    _fields_ = [("logr", ctypes.c_void_p), ("B", ctypes.c_void_p), ("A", ctypes.c_void_p),
+    _fields_ = [("logr", ctypes.c_void_p), ("B", ctypes.c_void_p), ("Bh", ctypes.c_void_p), ("Bl", ctypes.c_void_p),
+                ("A", ctypes.c_void_p),
                 ("col_extent", ctypes.c_void_p), ("row_blocks", ctypes.c_void_p), ("part_off", ctypes.c_int64),
                 ("N", ctypes.c_int32), ("P", ctypes.c_int32), ("ld", ctypes.c_int32), ("n_cb", ctypes.c_int32),
                 ("n_rb", ctypes.c_int32), ("pair_base", ctypes.c_int32), ("out_slot", ctypes.c_int32),
@@ -92,14 +93,15 @@ class _EnergyState:
 
 
 class ScanBatch:
-    """Plans and runs all I_L of a scan.  precision: "fp32" (exchange on
-    FP32 MFMA) or "fp64" (everything through the FP64 plain kernel)."""
+    """Plans and runs all I_L of a scan.  precision: "fp32" (stacked
+    contractions on FP32 MFMA), "bf16x3" (split-BF16x3 MFMA with FP32
+    accumulation) or "fp64" (everything through the FP64 plain kernel)."""
 
     def __init__(self, energies: Sequence[EnergyInput], L_max: int, precision: str = "fp32",
                  use_oscillatory_quadrature=True, oscillatory_method="advanced", CC_nodes=5,
                  phase_increment=1.5708, min_grid_fraction=0.10, k_threshold=0.5):
-        if precision not in ("fp32", "fp64"):
-            raise ValueError(f"precision must be 'fp32' or 'fp64', got {precision!r}")
+        if precision not in ("fp32", "bf16x3", "fp64"):
+            raise ValueError(f"precision must be 'fp32', 'bf16x3' or 'fp64', got {precision!r}")
         if L_max < 0 or L_max > 63:
             raise ValueError("L_max must be in [0, 63]")
         import torch
@@ -281,6 +283,8 @@ class ScanBatch:
                                     plan.eta_f, plan.sigma_f, plan.l_i, plan.l_f, plan.r_match, plan.r_max,
                                     hip.ptr(self.moments[s_i]), hip.ptr(self.tails[k]), Lm, plan.tail_kind)
             self.tail_d = hip.struct_array_to_device(tj, dev)
+            self.tail_ws_bytes = lib.dwba_outer_tail_workspace_bytes(len(tail_ids), Lm)
+            self.tail_ws = torch.empty(self.tail_ws_bytes, dtype=torch.uint8, device=dev)
             D, x = planner.levin_tables(8)
             self.levin = (np.ascontiguousarray(D.ravel()), np.ascontiguousarray(x))
 
@@ -296,7 +300,7 @@ class ScanBatch:
         if self.precision == "fp64":
             self._build_fp64(order)
         else:
-            self._build_fp32(plain_ids)
+            self._build_mfma(plain_ids)
 
     def _bound_i(self, ei):
         return self._bound_objs[ei]
@@ -328,7 +332,7 @@ class ScanBatch:
         self.plain_ws = self.torch.empty(max(self.plain_ws_bytes, 1), dtype=self.torch.uint8, device=self.dev)
         self.plain_count = self.n_pairs
 
-    def _build_fp32(self, plain_ids):
+    def _build_mfma(self, plain_ids):
         torch, hip, lib = self.torch, self.hip, self.lib
         dev, Lm = self.dev, self.L_max
         nl = Lm + 1
@@ -404,13 +408,20 @@ class ScanBatch:
                 row_blk[cb] = (max(rows[sl]) + XROWS - 1) // XROWS if rows[sl] else 0
             n_rb = int(row_blk.max()) if n_cb else 0
             self._check_factor_range(st, n_rb)
-            Bf = torch.zeros((max(P, 1), st.ld), dtype=torch.float32, device=dev)
-            self.b32.append((Bmat, Bf))
+            if self.precision == "fp32":
+                Bf = torch.zeros((max(P, 1), st.ld), dtype=torch.float32, device=dev)
+                Bh = Bl = None
+                self.b32.append((Bmat, Bf))
+            else:
+                Bf = None
+                Bh = torch.zeros((max(P, 1), st.ld), dtype=torch.int16, device=dev)
+                Bl = torch.zeros((max(P, 1), st.ld), dtype=torch.int16, device=dev)
+                self.b32.append((Bmat, (Bh, Bl)))
             ce_d = torch.as_tensor(col_ext, device=dev)
             rb_d = torch.as_tensor(row_blk, device=dev)
             self._keep += [ce_d, rb_d]
-            descs.append(XgemmEnergy(hip.ptr(st.logr_d), hip.ptr(Bf), hip.ptr(Amat), hip.ptr(ce_d), hip.ptr(rb_d),
-                                     part_off, st.N, P, st.ld, n_cb, n_rb, base, slot, 0))
+            descs.append(XgemmEnergy(hip.ptr(st.logr_d), hip.ptr(Bf), hip.ptr(Bh), hip.ptr(Bl), hip.ptr(Amat),
+                                     hip.ptr(ce_d), hip.ptr(rb_d), part_off, st.N, P, st.ld, n_cb, n_rb, base, slot, 0))
             part_off += int(lib.dwba_xgemm_partial_doubles(n_rb, n_cb, Lm))
             max_cb = max(max_cb, n_cb)
             for cb in range(n_cb):
@@ -473,8 +484,8 @@ class ScanBatch:
             seg, dmax = planner.filon_limits()
             Dp, _ = hip.f64_array(self.levin[0])
             Xp, _ = hip.f64_array(self.levin[1])
-            hip.check(lib.dwba_outer_tail_f64(self.n_tail, hip.ptr(self.tail_d), Lm, seg, dmax, Dp, Xp, sh),
-                      "dwba_outer_tail_f64")
+            hip.check(lib.dwba_outer_tail_f64(self.n_tail, hip.ptr(self.tail_d), Lm, seg, dmax, Dp, Xp,
+                                              hip.ptr(self.tail_ws), self.tail_ws_bytes, sh), "dwba_outer_tail_f64")
         mark("tail", "end")
         mark("plain", "start")
         if self.plain_count:
@@ -483,7 +494,7 @@ class ScanBatch:
                                                   hip.ptr(self.out), hip.ptr(self.plain_ws), self.plain_ws_bytes, sh),
                       "dwba_il_plain_f64_batch")
         mark("plain", "end")
-        if self.precision == "fp32":
+        if self.precision in ("fp32", "bf16x3"):
             mark("direct", "start")
             if self.dgemv is not None:
                 d, n, mr = self.dgemv
@@ -493,12 +504,18 @@ class ScanBatch:
                 hip.check(lib.dwba_ddot_f64(n, hip.ptr(d), Lm, hip.ptr(self.out), sh), "dwba_ddot_f64")
             mark("direct", "end")
             if self.xg_count:
-                for Bm, Bf in self.b32:
-                    hip.check(lib.dwba_cast_f64_f32(hip.ptr(Bm), hip.ptr(Bf), Bm.numel(), sh), "dwba_cast_f64_f32")
+                for Bm, Bx in self.b32:
+                    if self.precision == "fp32":
+                        hip.check(lib.dwba_cast_f64_f32(hip.ptr(Bm), hip.ptr(Bx), Bm.numel(), sh),
+                                  "dwba_cast_f64_f32")
+                    else:
+                        hip.check(lib.dwba_split_bf16(hip.ptr(Bm), hip.ptr(Bx[0]), hip.ptr(Bx[1]), Bm.numel(), sh),
+                                  "dwba_split_bf16")
                 mark("xgemm", "start")
-                hip.check(lib.dwba_xgemm_f32(self.xg_count, hip.ptr(self.xg_d), self.n_items, hip.ptr(self.items_d),
-                                             Lm, hip.ptr(self.partial), hip.ptr(self.counter), self.xg_max_cb,
-                                             hip.ptr(self.out), 0, sh), "dwba_xgemm_f32")
+                fn = lib.dwba_xgemm_f32 if self.precision == "fp32" else lib.dwba_xgemm_bf16x3
+                hip.check(fn(self.xg_count, hip.ptr(self.xg_d), self.n_items, hip.ptr(self.items_d),
+                             Lm, hip.ptr(self.partial), hip.ptr(self.counter), self.xg_max_cb,
+                             hip.ptr(self.out), 0, sh), "dwba_xgemm")
                 mark("xgemm", "end")
         hip.check(lib.dwba_batch_finalize(self.n_pairs, hip.ptr(self.fin_d), Lm, hip.ptr(self.out), sh),
                   "dwba_batch_finalize")

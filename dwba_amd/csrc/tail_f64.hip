@@ -360,49 +360,239 @@ __device__ double analytic_tail(double r_m, const dwba_tail_job& J, int L, doubl
     return tail * overlap;
 }
 
-__global__ __launch_bounds__(64) void tail_f64_kernel(const dwba_tail_job* __restrict__ jobs, TailParams TP) {
-    const dwba_tail_job J = jobs[blockIdx.x];
-    const int L = blockIdx.y;
-    if (L > J.L_max) return;
-    const int lane = threadIdx.x;
-    double res = 0.0;
-    const double M = J.moments ? J.moments[L] : 0.0;
-    if (J.kind != DWBA_TAIL_NONE && L >= 1 && fabs(M) > 1e-12) {
-        if (J.kind == DWBA_TAIL_LEGACY) {
-            res = analytic_tail(J.r_m, J, L, M);
+// Filon segments of one (pair, sign), all L: lanes take segments
+// round-robin; NE (odd, 5..15) nodes per segment are register resident.
+template <int NL, int NE>
+__device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, double om, double p0, double r_m, double r_max,
+                                            double bstep, long nseg, int lane, DD* acc) {
+    constexpr int NT = (NE - 1) / 2;
+    const int Lm = J.L_max < NL - 1 ? J.L_max : NL - 1;
+    const double inv_om = -1.0 / om;
+    for (long sg = lane; sg < nseg; sg += 64) {
+        const double a = __dadd_rn(__dmul_rn((double)sg, bstep), r_m);
+        const double b = (sg + 1 == nseg) ? r_max : __dadd_rn(__dmul_rn((double)(sg + 1), bstep), r_m);
+        double x[NE], c[NE], s[NE], ix[NE], px[NE];
+        const double step = (b - a) / (NE - 1);
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            x[k] = (k == NE - 1) ? b : __dadd_rn(__dmul_rn((double)k, step), a);
+            ix[k] = x[k] > 1e-6 ? 1.0 / x[k] : 0.0;
+            px[k] = ix[k] * ix[k];           // x^-(L+1) at L = 1
+        }
+        const double hseg = x[NE - 1] - x[0];
+        if (hseg < 1e-15) continue;
+        const bool small = fabs(om * hseg) < 1e-3;
+        double cA[NT], cB[NT], cG[NT], Hh[NT];
+        bool tsmall[NT];
+        double cmid = 0.0;
+        if (small) {
+            cmid = cos(om * (x[0] + x[NE - 1]) / 2 + p0);
         } else {
-            Phase P;
-            P.ka = J.k_a; P.kb = J.k_b; P.ea = J.eta_a; P.eb = J.eta_b;
-            P.la = (double)J.l_a; P.lb = (double)J.l_b;
-            P.use_ea = fabs(J.eta_a) > 1e-15; P.use_eb = fabs(J.eta_b) > 1e-15;
-            P.use_la = J.l_a > 0; P.use_lb = J.l_b > 0;
-            P.kmin = J.k_a - J.k_b; P.kplus = J.k_a + J.k_b;
-            const double pa = -J.l_a * PI / 2 + J.sigma_a + J.delta_a;
-            const double pb = -J.l_b * PI / 2 + J.sigma_b + J.delta_b;
-            P.cmin = pa - pb; P.cplus = pa + pb;
-            const Env E{M, L + 1};
-            double outer = 0.0;
-            if (!(J.r_max <= J.r_m + 1e-10)) {
-                const cplx im = outer_integral(E, P, -1, J.r_m, J.r_max, TP, lane);
-                const cplx ip = outer_integral(E, P, +1, J.r_m, J.r_max, TP, lane);
-                outer = 0.5 * (im.re - ip.re);
+#pragma unroll
+            for (int k = 0; k < NE; ++k) sincos(om * x[k] + p0, &s[k], &c[k]);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const double h3 = x[2 * t + 2] - x[2 * t];
+                Hh[t] = h3 / 2;
+                tsmall[t] = fabs(om * h3) < 1e-3 || h3 < 1e-15;
+                const double th = om * h3 / 2;
+                double al, be, ga;
+                if (fabs(th) < 0.3) {
+                    const double t2 = th * th, t3 = th * th * th;
+                    al = 2 * t3 / 45 * (1 - t2 / 7 + t2 * t2 / 189);
+                    be = 2.0 / 3 + 2 * t2 / 15 - 4 * (t2 * t2) / 105 + 2 * (t2 * t2 * t2) / 567;
+                    ga = 4.0 / 3 - 2 * t2 / 15 + 1 * (t2 * t2) / 210 - (t2 * t2 * t2) / 11340;
+                } else {
+                    double st, ct, s2, c2;
+                    sincos(th, &st, &ct);
+                    sincos(2 * th, &s2, &c2);
+                    const double t2 = th * th, t3 = th * th * th;
+                    al = (t2 + th * s2 / 2 - 2 * (st * st)) / t3;
+                    be = 2 * (th * (1 + c2) - s2) / t3;
+                    ga = 4 * (st - th * ct) / t3;
+                }
+                cA[t] = al * inv_om;     // Re[alpha (-i/omega) D] = -(cA) Im D
+                cB[t] = be;
+                cG[t] = ga;
             }
-            res = outer + analytic_tail(J.r_max, J, L, M);
+        }
+#pragma unroll
+        for (int L = 1; L < NL; ++L) {
+            const double M = (L <= Lm) ? J.moments[L] : 0.0;
+            if (fabs(M) > 1e-12) {
+                double f[NE];
+#pragma unroll
+                for (int k = 0; k < NE; ++k) f[k] = x[k] > 1e-6 ? M * px[k] : 0.0;
+                double re = 0.0;
+                if (small) {
+                    re = simpson_odd(f, x, NE) * cmid;
+                } else {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        const int i0 = 2 * t;
+                        double v;
+                        if (tsmall[t]) {
+                            const double h3 = x[i0 + 2] - x[i0];
+                            v = h3 < 1e-15 ? 0.0
+                                           : simpson_odd(f + i0, x + i0, 3) * cos(om * (x[i0] + x[i0 + 2]) / 2 + p0);
+                        } else {
+                            const double dim = f[i0 + 2] * s[i0 + 2] - f[i0] * s[i0];
+                            const double bre = f[i0] * c[i0] + f[i0 + 2] * c[i0 + 2];
+                            v = Hh[t] * ((-cA[t] * dim + cB[t] * bre) + cG[t] * f[i0 + 1] * c[i0 + 1]);
+                        }
+                        re += v;
+                    }
+                }
+                acc[L] = dd_add(acc[L], re);
+            }
+#pragma unroll
+            for (int k = 0; k < NE; ++k) px[k] *= ix[k];
         }
     }
-    if (lane == 0) J.out[L] = res;
+}
+
+// ---------------------------------------------------------------------------
+// All-L form: the phase functions do not depend on L (only the envelope
+// M_L / r^(L+1) does), so one wavefront per (pair, sign) evaluates the nodes,
+// the exponentials and the Filon coefficients once per segment and sweeps all
+// L with x^-(L+1) built by repeated multiplication.  Only Re J is needed
+// (dwba_outer_integral_1d keeps the real parts), so each lane keeps per-L
+// double-double sums of the real parts.  Levin / non-oscillatory signs fall
+// back to the per-L routines above.
+template <int NL>
+__global__ __launch_bounds__(64) void tail_allL_kernel(const dwba_tail_job* __restrict__ jobs, TailParams TP,
+                                                       double* __restrict__ jre) {
+    const int job = blockIdx.x, sgn = blockIdx.y == 0 ? -1 : 1;
+    const dwba_tail_job J = jobs[job];
+    const int lane = threadIdx.x;
+    const int Lm = J.L_max < NL - 1 ? J.L_max : NL - 1;
+    double* out = jre + ((size_t)job * 2 + blockIdx.y) * NL;
+    bool any = false;
+    for (int L = 1; L <= Lm; ++L) any |= (J.moments && fabs(J.moments[L]) > 1e-12);
+    if (J.kind != DWBA_TAIL_OUTER || !any || J.r_max <= J.r_m + 1e-10) {
+        for (int L = lane; L < NL; L += 64) out[L] = 0.0;
+        return;
+    }
+    Phase P;
+    P.ka = J.k_a; P.kb = J.k_b; P.ea = J.eta_a; P.eb = J.eta_b;
+    P.la = (double)J.l_a; P.lb = (double)J.l_b;
+    P.use_ea = fabs(J.eta_a) > 1e-15; P.use_eb = fabs(J.eta_b) > 1e-15;
+    P.use_la = J.l_a > 0; P.use_lb = J.l_b > 0;
+    P.kmin = J.k_a - J.k_b; P.kplus = J.k_a + J.k_b;
+    const double pa = -J.l_a * PI / 2 + J.sigma_a + J.delta_a;
+    const double pb = -J.l_b * PI / 2 + J.sigma_b + J.delta_b;
+    P.cmin = pa - pb; P.cplus = pa + pb;
+    const double r_m = J.r_m, r_max = J.r_max;
+    const double delta_phi = PI / 4;
+    const double mid = (r_m + r_max) / 2;
+    const double om = fabs(dphi(P, sgn, mid));
+    bool filon = false;
+    if (!(om < 1e-10)) {
+        const double h = delta_phi / om;
+        filon = fabs(d2phi(P, sgn, mid)) * h * h < 0.1;
+    }
+    if (!filon) {
+        // Levin or non-oscillatory: per-L evaluation
+        for (int L = 1; L <= Lm; ++L) {
+            const double M = J.moments[L];
+            double v = 0.0;
+            if (fabs(M) > 1e-12) {
+                const Env E{M, L + 1};
+                v = outer_integral(E, P, sgn, r_m, r_max, TP, lane).re;
+            }
+            if (lane == 0) out[L] = v;
+        }
+        if (lane == 0) out[0] = 0.0;
+        for (int L = Lm + 1 + lane; L < NL; L += 64) out[L] = 0.0;
+        return;
+    }
+    DD acc[NL];
+#pragma unroll
+    for (int L = 0; L < NL; ++L) acc[L] = DD{0.0, 0.0};
+
+    const double p0 = phi(P, sgn, r_m) - om * r_m;
+    const double dr_seg = delta_phi / fabs(om);
+    const double nraw_d = ceil((r_max - r_m) / dr_seg);
+    const long nraw = nraw_d < 1.0 ? 1 : (long)nraw_d;
+    const long nseg = nraw < TP.max_filon_segments ? nraw : TP.max_filon_segments;
+    const double eff = fabs(om) * (r_max - r_m) / (double)(nseg > 1 ? nseg : 1);
+    int neff = 5;
+    if (eff > TP.max_effective_dphi) {
+        const int scale = (int)ceil(eff / TP.max_effective_dphi);
+        neff = 5 + 2 * (scale - 1);
+        if (neff < 5) neff = 5;
+        if (neff > 15) neff = 15;
+    }
+    const double bstep = (r_max - r_m) / (double)nseg;
+    switch (neff) {
+        case 5: filon_sweep<NL, 5>(J, om, p0, r_m, r_max, bstep, nseg, lane, acc); break;
+        case 7: filon_sweep<NL, 7>(J, om, p0, r_m, r_max, bstep, nseg, lane, acc); break;
+        case 9: filon_sweep<NL, 9>(J, om, p0, r_m, r_max, bstep, nseg, lane, acc); break;
+        case 11: filon_sweep<NL, 11>(J, om, p0, r_m, r_max, bstep, nseg, lane, acc); break;
+        case 13: filon_sweep<NL, 13>(J, om, p0, r_m, r_max, bstep, nseg, lane, acc); break;
+        default: filon_sweep<NL, 15>(J, om, p0, r_m, r_max, bstep, nseg, lane, acc); break;
+    }
+#pragma unroll
+    for (int L = 0; L < NL; ++L) {
+        const DD t = dd_wave_sum(acc[L]);
+        if (lane == 0) out[L] = (L >= 1 && L <= Lm) ? t.hi + t.lo : 0.0;
+    }
+}
+
+// tail[L] = 1/2 (Re J_- - Re J_+) + analytic tail at r_max  (or the legacy analytic tail)
+__global__ void tail_combine_kernel(const dwba_tail_job* __restrict__ jobs, int n_jobs, int NL,
+                                    const double* __restrict__ jre) {
+    const int job = blockIdx.x;
+    const dwba_tail_job J = jobs[job];
+    for (int L = threadIdx.x; L <= J.L_max; L += blockDim.x) {
+        double res = 0.0;
+        const double M = J.moments ? J.moments[L] : 0.0;
+        if (J.kind != DWBA_TAIL_NONE && L >= 1 && fabs(M) > 1e-12) {
+            if (J.kind == DWBA_TAIL_LEGACY) {
+                res = analytic_tail(J.r_m, J, L, M);
+            } else {
+                double outer = 0.0;
+                if (!(J.r_max <= J.r_m + 1e-10) && L < NL) {
+                    const double jm = jre[((size_t)job * 2 + 0) * NL + L];
+                    const double jp = jre[((size_t)job * 2 + 1) * NL + L];
+                    outer = 0.5 * (jm - jp);
+                }
+                res = outer + analytic_tail(J.r_max, J, L, M);
+            }
+        }
+        J.out[L] = res;
+    }
+}
+
+int tail_nl_bucket(int L_max) {
+    static const int buckets[] = {2, 4, 8, 16, 24, 32, 48, 64};
+    for (int b : buckets)
+        if (L_max + 1 <= b) return b;
+    return -1;
 }
 
 }  // namespace
 }  // namespace dwba
 
+extern "C" size_t dwba_outer_tail_workspace_bytes(int32_t n_jobs, int32_t L_max) {
+    const int nl = dwba::tail_nl_bucket(L_max);
+    if (n_jobs <= 0 || nl < 0) return 0;
+    return (size_t)n_jobs * 2 * nl * sizeof(double);
+}
+
 extern "C" int32_t dwba_outer_tail_f64(int32_t n_jobs, const dwba_tail_job* d_jobs, int32_t L_max,
                                        int32_t max_filon_segments, double max_effective_dphi,
-                                       const double* h_levin_D, const double* h_levin_x, void* stream) {
+                                       const double* h_levin_D, const double* h_levin_x, void* d_workspace,
+                                       size_t workspace_bytes, void* stream) {
     DWBA_REQUIRE(dwba::initialised(), "dwba_outer_tail_f64: call dwba_init first");
-    DWBA_REQUIRE(n_jobs > 0 && d_jobs && L_max >= 0 && L_max < 65535 && h_levin_D && h_levin_x,
+    DWBA_REQUIRE(n_jobs > 0 && d_jobs && L_max >= 0 && L_max <= 63 && h_levin_D && h_levin_x,
                  "dwba_outer_tail_f64: bad arguments");
     DWBA_REQUIRE(max_filon_segments >= 1 && max_effective_dphi > 0, "dwba_outer_tail_f64: bad Filon limits");
+    const int nl = dwba::tail_nl_bucket(L_max);
+    const size_t need = (size_t)n_jobs * 2 * nl * sizeof(double);
+    if (!d_workspace || workspace_bytes < need)
+        return dwba::set_error(DWBA_ERR_WORKSPACE, "dwba_outer_tail_f64: workspace %zu < %zu bytes",
+                               workspace_bytes, need);
     dwba::TailParams tp;
     tp.max_filon_segments = max_filon_segments;
     tp.max_effective_dphi = max_effective_dphi;
@@ -410,8 +600,16 @@ extern "C" int32_t dwba_outer_tail_f64(int32_t n_jobs, const dwba_tail_job* d_jo
         tp.lev.xr[i] = h_levin_x[i];
         for (int j = 0; j < 8; ++j) tp.lev.D[i][j] = h_levin_D[i * 8 + j];
     }
-    hipLaunchKernelGGL(dwba::tail_f64_kernel, dim3(n_jobs, L_max + 1), dim3(64), 0,
-                       reinterpret_cast<hipStream_t>(stream), d_jobs, tp);
-    DWBA_LAUNCH_CHECK("tail_f64_kernel");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    double* jre = reinterpret_cast<double*>(d_workspace);
+    switch (nl) {
+#define C(n) case n: hipLaunchKernelGGL(dwba::tail_allL_kernel<n>, dim3(n_jobs, 2), dim3(64), 0, s, d_jobs, tp, jre); break;
+        C(2) C(4) C(8) C(16) C(24) C(32) C(48) C(64)
+#undef C
+        default: return dwba::set_error(DWBA_ERR_ARG, "dwba_outer_tail_f64: L_max %d", L_max);
+    }
+    DWBA_LAUNCH_CHECK("tail_allL_kernel");
+    hipLaunchKernelGGL(dwba::tail_combine_kernel, dim3(n_jobs), dim3(64), 0, s, d_jobs, n_jobs, nl, jre);
+    DWBA_LAUNCH_CHECK("tail_combine_kernel");
     return DWBA_OK;
 }

@@ -182,10 +182,8 @@ __global__ __launch_bounds__(XT, 1) void xgemm_kernel(const dwba_xgemm_energy* _
                     }
                 }
             }
-            if (c + 1 < nchunk) {
-                __syncthreads();   // everyone done reading buf^1 from iteration c-1
-                store_chunk(c + 1, buf ^ 1);
-            }
+            // buf^1 was last read in iteration c-1, which ended with a barrier
+            if (c + 1 < nchunk) store_chunk(c + 1, buf ^ 1);
             __syncthreads();
         }
 
@@ -220,6 +218,206 @@ __global__ __launch_bounds__(XT, 1) void xgemm_kernel(const dwba_xgemm_energy* _
             const size_t off = E.part_off + ((size_t)(I.L * E.n_cb + I.cb) * E.n_rb + I.rb) * XCOLS;
             partial[off + tid] = v;
         }
+    }
+}
+
+constexpr int HPITCH = 40;       // LDS row pitch (bf16 elements) of the split-beta tiles
+
+// Split-BF16x3 variant: K and beta are each split into bf16 hi + lo parts and
+// T += Khi*Bhi + Khi*Blo + Klo*Bhi on v_mfma_f32_32x32x16_bf16 (FP32
+// accumulation).  Same tiling, work list and epilogue as the FP32 kernel.
+__global__ __launch_bounds__(XT, 1) void xgemm_bf16x3_kernel(const dwba_xgemm_energy* __restrict__ en,
+                                                             const int4* __restrict__ items, int n_items,
+                                                             int* __restrict__ counter,
+                                                             double* __restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) unsigned short s_bh[2][XCOLS * HPITCH];
+    __shared__ __attribute__((aligned(16))) unsigned short s_bl[2][XCOLS * HPITCH];
+    __shared__ __attribute__((aligned(16))) float s_fu[2][XK], s_fv[2][XK];
+    __shared__ double s_red[4][XCOLS];
+    __shared__ int s_item;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int half = lane >> 5, l32 = lane & 31;
+
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= n_items) break;
+        const int4 w4 = items[it];
+        const Item I{w4.x, w4.y, w4.z, w4.w};
+        const dwba_xgemm_energy E = en[I.e];
+        const int L = I.L;
+        const int row0 = I.rb * XROWS;
+        const int p0 = I.cb * XCOLS;
+        const int jmax = E.col_extent[I.cb];
+        const int nchunk = (jmax + XK - 1) / XK;
+        const int ncb_act = (min(XCOLS, E.P - p0) + 31) / 32;
+        const double c0 = E.logr[row0];
+
+        float ru[2], rv[2];
+        int rowi[2];
+#pragma unroll
+        for (int rg = 0; rg < 2; ++rg) {
+            const int i = row0 + wave * 64 + rg * 32 + l32;
+            rowi[rg] = i;
+            if (i < E.N) {
+                const double lr = E.logr[i];
+                ru[rg] = fac_u(lr, c0, L);
+                rv[rg] = fac_v(lr, c0, L);
+            } else {
+                ru[rg] = 0.0f;
+                rv[rg] = 0.0f;
+            }
+        }
+
+        v16f acc[2][4];
+#pragma unroll
+        for (int rg = 0; rg < 2; ++rg)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc[rg][cb][q] = 0.0f;
+
+        // chunk staging: 128 pairs x 32 points of hi and lo (64 B each per pair)
+        v4f stage[4];
+        auto load_chunk = [&](int c) {
+            const int jc = c * XK;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int idx = tid + XT * q;          // 0..1023: [hi|lo][col][part of 4]
+                const int lohi = idx >> 9;
+                const int col = (idx >> 2) & 127, part = idx & 3;
+                const int p = p0 + col;
+                const unsigned short* src = lohi ? E.Bl : E.Bh;
+                if (p < E.P)
+                    stage[q] = *reinterpret_cast<const v4f*>(src + (size_t)p * E.ld + jc + 8 * part);
+                else
+                    stage[q] = v4f{0.f, 0.f, 0.f, 0.f};
+            }
+        };
+        auto store_chunk = [&](int c, int buf) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int idx = tid + XT * q;
+                const int lohi = idx >> 9;
+                const int col = (idx >> 2) & 127, part = idx & 3;
+                unsigned short* dst = lohi ? s_bl[buf] : s_bh[buf];
+                *reinterpret_cast<v4f*>(dst + col * HPITCH + 8 * part) = stage[q];
+            }
+            if (tid < 2 * XK) {
+                const int jj = tid & (XK - 1);
+                const int j = c * XK + jj;
+                const bool is_u = tid < XK;
+                float f = 0.0f;
+                if (j < E.N) {
+                    const double lr = E.logr[j];
+                    if (is_u && j < row0 + XROWS) f = fac_u(lr, c0, L);
+                    if (!is_u && j >= row0) f = fac_v(lr, c0, L);
+                }
+                if (is_u) s_fu[buf][jj] = f; else s_fv[buf][jj] = f;
+            }
+        };
+
+        if (nchunk > 0) {
+            load_chunk(0);
+            store_chunk(0, 0);
+        }
+        __syncthreads();
+        for (int c = 0; c < nchunk; ++c) {
+            const int buf = c & 1;
+            if (c + 1 < nchunk) load_chunk(c + 1);
+            const int jc = c * XK;
+            const bool left = jc + XK <= row0;
+            const bool right = jc >= row0 + XROWS;
+#pragma unroll
+            for (int st = 0; st < 2; ++st) {            // two k-steps of 16
+                const int kb = 16 * st + 8 * half;      // this lane's 8 columns in the chunk
+                const v4f fu0 = *reinterpret_cast<const v4f*>(&s_fu[buf][kb]);
+                const v4f fu1 = *reinterpret_cast<const v4f*>(&s_fu[buf][kb + 4]);
+                const v4f fv0 = *reinterpret_cast<const v4f*>(&s_fv[buf][kb]);
+                const v4f fv1 = *reinterpret_cast<const v4f*>(&s_fv[buf][kb + 4]);
+                v8bf ah[2], al[2];
+#pragma unroll
+                for (int rg = 0; rg < 2; ++rg) {
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) {
+                        const float fu = t < 4 ? fu0[t] : fu1[t - 4];
+                        const float fv = t < 4 ? fv0[t] : fv1[t - 4];
+                        float kv;
+                        if (right) kv = ru[rg] * fv;
+                        else if (left) kv = fu * rv[rg];
+                        else {
+                            const int j = jc + kb + t, i = rowi[rg];
+                            kv = j > i ? ru[rg] * fv : (j < i ? fu * rv[rg] : ru[rg] * rv[rg]);
+                        }
+                        const __bf16 h = (__bf16)kv;
+                        ah[rg][t] = h;
+                        al[rg][t] = (__bf16)(kv - (float)h);
+                    }
+                }
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb) {
+                    if (cb < ncb_act) {
+                        const int off = (cb * 32 + l32) * HPITCH + kb;
+                        const v8bf bh = *reinterpret_cast<const v8bf*>(&s_bh[buf][off]);
+                        const v8bf bl = *reinterpret_cast<const v8bf*>(&s_bl[buf][off]);
+#pragma unroll
+                        for (int rg = 0; rg < 2; ++rg) {
+                            acc[rg][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[rg], bh, acc[rg][cb], 0, 0, 0);
+                            acc[rg][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[rg], bl, acc[rg][cb], 0, 0, 0);
+                            acc[rg][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[rg], bh, acc[rg][cb], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+            if (c + 1 < nchunk) store_chunk(c + 1, buf ^ 1);
+            __syncthreads();
+        }
+
+        double part[4];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            double sacc = 0.0;
+            const int p = p0 + cb * 32 + l32;
+            if (cb < ncb_act && p < E.P) {
+                const double* ap = E.A + (size_t)p * E.ld;
+#pragma unroll
+                for (int rg = 0; rg < 2; ++rg) {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        const int m = 8 * (q >> 2) + 4 * half + (q & 3);
+                        const int i = row0 + wave * 64 + rg * 32 + m;
+                        if (i < E.N) sacc = fma(ap[i], (double)acc[rg][cb][q], sacc);
+                    }
+                }
+            }
+            sacc += __shfl_xor(sacc, 32, 64);
+            part[cb] = sacc;
+        }
+        if (half == 0) {
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) s_red[wave][cb * 32 + l32] = part[cb];
+        }
+        __syncthreads();
+        if (tid < XCOLS) {
+            const double v = ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
+            const size_t off = E.part_off + ((size_t)(I.L * E.n_cb + I.cb) * E.n_rb + I.rb) * XCOLS;
+            partial[off + tid] = v;
+        }
+    }
+}
+
+// hi = bf16(x), lo = bf16(x - hi) (round to nearest even), from FP64.
+__global__ void split_bf16_kernel(const double* __restrict__ x, unsigned short* __restrict__ hi,
+                                  unsigned short* __restrict__ lo, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = x[i];
+        const __bf16 h = (__bf16)(float)v;
+        const __bf16 l = (__bf16)(float)(v - (double)(float)h);
+        hi[i] = __builtin_bit_cast(unsigned short, h);
+        lo[i] = __builtin_bit_cast(unsigned short, l);
     }
 }
 
@@ -264,14 +462,25 @@ int32_t dwba_cast_f64_f32(const double* d_x, float* d_y, int64_t n, void* stream
     return DWBA_OK;
 }
 
+int32_t dwba_split_bf16(const double* d_x, uint16_t* d_hi, uint16_t* d_lo, int64_t n, void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_split_bf16: call dwba_init first");
+    DWBA_REQUIRE(n >= 0 && (n == 0 || (d_x && d_hi && d_lo)), "dwba_split_bf16: bad arguments");
+    if (n == 0) return DWBA_OK;
+    const int64_t nb = std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)nb), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       d_x, reinterpret_cast<unsigned short*>(d_hi), reinterpret_cast<unsigned short*>(d_lo), n);
+    DWBA_LAUNCH_CHECK("split_bf16_kernel");
+    return DWBA_OK;
+}
+
 size_t dwba_xgemm_partial_doubles(int32_t n_rb, int32_t n_cb, int32_t L_max) {
     return (size_t)(L_max + 1) * n_cb * n_rb * XCOLS;
 }
 
-int32_t dwba_xgemm_f32(int32_t n_energies, const dwba_xgemm_energy* d_energies, int32_t n_items,
-                       const int32_t* d_items, int32_t L_max, double* d_partial, int32_t* d_counter,
-                       int32_t max_cb, double* d_out, int32_t n_workgroups, void* stream) {
-    DWBA_REQUIRE(initialised(), "dwba_xgemm_f32: call dwba_init first");
+static int32_t xgemm_launch(int mode, int32_t n_energies, const dwba_xgemm_energy* d_energies, int32_t n_items,
+                            const int32_t* d_items, int32_t L_max, double* d_partial, int32_t* d_counter,
+                            int32_t max_cb, double* d_out, int32_t n_workgroups, void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_xgemm: call dwba_init first");
     DWBA_REQUIRE(n_energies > 0 && d_energies && d_partial && d_counter && d_out && L_max >= 0 && max_cb > 0,
                  "dwba_xgemm_f32: bad arguments");
     DWBA_REQUIRE(n_items >= 0 && (n_items == 0 || d_items), "dwba_xgemm_f32: bad work list");
@@ -281,14 +490,32 @@ int32_t dwba_xgemm_f32(int32_t n_energies, const dwba_xgemm_energy* d_energies, 
     if (n_items > 0) {
         int nwg = n_workgroups > 0 ? n_workgroups : device_cu_count();
         if (nwg > n_items) nwg = n_items;
-        hipLaunchKernelGGL(xgemm_kernel<0>, dim3(nwg), dim3(XT), 0, s, d_energies,
-                           reinterpret_cast<const int4*>(d_items), n_items, d_counter, d_partial);
+        if (mode == 0)
+            hipLaunchKernelGGL(xgemm_kernel<0>, dim3(nwg), dim3(XT), 0, s, d_energies,
+                               reinterpret_cast<const int4*>(d_items), n_items, d_counter, d_partial);
+        else
+            hipLaunchKernelGGL(xgemm_bf16x3_kernel, dim3(nwg), dim3(XT), 0, s, d_energies,
+                               reinterpret_cast<const int4*>(d_items), n_items, d_counter, d_partial);
         DWBA_LAUNCH_CHECK("xgemm_kernel");
     }
     hipLaunchKernelGGL(xgemm_reduce, dim3(n_energies, L_max + 1, max_cb), dim3(XCOLS), 0, s, d_energies,
                        d_partial, L_max, d_out);
     DWBA_LAUNCH_CHECK("xgemm_reduce");
     return DWBA_OK;
+}
+
+int32_t dwba_xgemm_f32(int32_t n_energies, const dwba_xgemm_energy* d_energies, int32_t n_items,
+                       const int32_t* d_items, int32_t L_max, double* d_partial, int32_t* d_counter,
+                       int32_t max_cb, double* d_out, int32_t n_workgroups, void* stream) {
+    return xgemm_launch(0, n_energies, d_energies, n_items, d_items, L_max, d_partial, d_counter, max_cb, d_out,
+                        n_workgroups, stream);
+}
+
+int32_t dwba_xgemm_bf16x3(int32_t n_energies, const dwba_xgemm_energy* d_energies, int32_t n_items,
+                          const int32_t* d_items, int32_t L_max, double* d_partial, int32_t* d_counter,
+                          int32_t max_cb, double* d_out, int32_t n_workgroups, void* stream) {
+    return xgemm_launch(1, n_energies, d_energies, n_items, d_items, L_max, d_partial, d_counter, max_cb, d_out,
+                        n_workgroups, stream);
 }
 
 }  // extern "C"

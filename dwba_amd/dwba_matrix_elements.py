@@ -289,9 +289,11 @@ class _PairRunner:
             seg, dmax = planner.filon_limits()
             LD, LDa = hip.f64_array(self.levin_D.ravel())
             LX, LXa = hip.f64_array(self.levin_x)
-            hip.check(lib.dwba_outer_tail_f64(1, hip.ptr(tj_d), L_max, seg, dmax, LD, LX, sh),
+            twb = lib.dwba_outer_tail_workspace_bytes(1, L_max)
+            tws = torch.empty(twb, dtype=torch.uint8, device=dev)
+            hip.check(lib.dwba_outer_tail_f64(1, hip.ptr(tj_d), L_max, seg, dmax, LD, LX, hip.ptr(tws), twb, sh),
                       "dwba_outer_tail_f64")
-            keep += [mom, ptrs, n_d, tj_d, LDa, LXa]
+            keep += [mom, ptrs, n_d, tj_d, LDa, LXa, tws]
         if plan.rows > 0:
             out = res[:2 * nl].view(2, nl)
             ws_bytes = lib.dwba_il_plain_f64_workspace_bytes(plan.rows, plan.cols, L_max)

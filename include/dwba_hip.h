@@ -185,16 +185,21 @@ typedef struct dwba_tail_job {
     int32_t kind;             /* DWBA_TAIL_* */
 } dwba_tail_job;
 
-/* One wavefront per (pair, L) in one launch.  max_filon_segments /
- * max_effective_dphi are DWBA_MAX_FILON_SEGMENTS / DWBA_MAX_EFFECTIVE_DPHI;
- * h_levin_D (8x8 row-major) and h_levin_x (8) are the Chebyshev
- * differentiation matrix and the ascending Lobatto points of the Levin rule.
+/* One wavefront per (pair, sign of the sinA*sinB split) in one launch; each
+ * wavefront evaluates the phases and Filon coefficients once per segment and
+ * sweeps all L.  max_filon_segments / max_effective_dphi are
+ * DWBA_MAX_FILON_SEGMENTS / DWBA_MAX_EFFECTIVE_DPHI; h_levin_D (8x8
+ * row-major) and h_levin_x (8) are the Chebyshev differentiation matrix and
+ * the ascending Lobatto points of the Levin rule.  L_max in [0, 63];
+ * workspace >= dwba_outer_tail_workspace_bytes(n_jobs, L_max).
  * Replaces dwba_outer_integral_1d + _analytical_multipole_tail
  * (oscillatory_integrals.py:459-535, :1024-1451, :2064-2184) as called at
  * dwba_matrix_elements.py:1165-1192 and :1257-1268. */
+size_t dwba_outer_tail_workspace_bytes(int32_t n_jobs, int32_t L_max);
 int32_t dwba_outer_tail_f64(int32_t n_jobs, const dwba_tail_job* d_jobs, int32_t L_max,
                             int32_t max_filon_segments, double max_effective_dphi,
-                            const double* h_levin_D, const double* h_levin_x, void* stream);
+                            const double* h_levin_D, const double* h_levin_x, void* d_workspace,
+                            size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Batched scan: all channel pairs of all energies in one launch each.    */
@@ -208,7 +213,9 @@ int32_t dwba_outer_tail_f64(int32_t n_jobs, const dwba_tail_job* d_jobs, int32_t
  * sums of (L, cb, rb) live at partial + part_off + ((L*n_cb + cb)*n_rb + rb)*128. */
 typedef struct dwba_xgemm_energy {
     const double* logr;
-    const float* B;
+    const float* B;           /* FP32 beta (dwba_xgemm_f32) */
+    const uint16_t* Bh;       /* bf16 hi / lo parts of beta (dwba_xgemm_bf16x3) */
+    const uint16_t* Bl;
     const double* A;
     const int32_t* col_extent;
     const int32_t* row_blocks;
@@ -218,6 +225,17 @@ typedef struct dwba_xgemm_energy {
     int32_t out_slot;         /* 0: direct, 1: exchange half of d_out */
     int32_t pad_;
 } dwba_xgemm_energy;
+
+/* hi[i] = bf16(x[i]), lo[i] = bf16(x[i] - hi[i]) (round to nearest even):
+ * the two-term bf16 split of the beta columns for dwba_xgemm_bf16x3. */
+int32_t dwba_split_bf16(const double* d_x, uint16_t* d_hi, uint16_t* d_lo, int64_t n, void* stream);
+
+/* Split-BF16x3 variant of dwba_xgemm_f32: K and beta split into bf16 hi + lo,
+ * T += Khi Bhi + Khi Blo + Klo Bhi on v_mfma_f32_32x32x16_bf16, FP32
+ * accumulation, FP64 epilogue.  Reads E.Bh / E.Bl instead of E.B. */
+int32_t dwba_xgemm_bf16x3(int32_t n_energies, const dwba_xgemm_energy* d_energies, int32_t n_items,
+                          const int32_t* d_items, int32_t L_max, double* d_partial, int32_t* d_counter,
+                          int32_t max_cb, double* d_out, int32_t n_workgroups, void* stream);
 
 /* y[i] = (float)x[i], i < n (rounding of the beta columns for the FP32 MFMA path). */
 int32_t dwba_cast_f64_f32(const double* d_x, float* d_y, int64_t n, void* stream);

@@ -50,12 +50,15 @@ def test_fp64_batch_equals_dropin(ionization):
                     assert abs(a - r_) <= 1e-12 * abs(r_) + 1e-13 * sc, (ei, pi, L, a, r_)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
 @pytest.mark.parametrize("ionization", [False, True])
-def test_fp32_batch_against_oracle(ionization):
+def test_mfma_batch_against_oracle(ionization, precision):
+    """Exchange (and, for ionization, direct) through the stacked MFMA GEMM:
+    |dI| <= 1e-6 sum|a K b| (FP32 ~1e-8 of it, split-BF16x3 ~5e-7)."""
     from dwba_amd.batch import ScanBatch
     scan = _small_scan(2, n_e=2, n_p=40, ionization=ionization)
     L_max = 10
-    b = ScanBatch(scan, L_max, precision="fp32", oscillatory_method="advanced")
+    b = ScanBatch(scan, L_max, precision=precision, oscillatory_method="advanced")
     b.run()
     res = b.results()
     k = 0
@@ -78,7 +81,7 @@ def test_fp32_batch_against_oracle(ionization):
                 worst = max(worst, err)
                 assert err <= 1e-6, (k, L, res[k, 1, L], e_ref, err)
             k += 1
-    print("worst exchange |dI|/sum|aKb| =", worst)
+    print(precision, "worst exchange |dI|/sum|aKb| =", worst)
 
 
 def test_graph_replay_matches_eager():

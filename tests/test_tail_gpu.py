@@ -37,7 +37,10 @@ def _gpu_tail(params, moments, kind=1):
     seg, dmax = planner.filon_limits()
     Dp, Da = _hip.f64_array(D.ravel())
     Xp, Xa = _hip.f64_array(x)
-    _hip.check(lib.dwba_outer_tail_f64(1, _hip.ptr(tj_d), L_max, seg, dmax, Dp, Xp, _hip.stream_handle()), "tail")
+    wsb = lib.dwba_outer_tail_workspace_bytes(1, L_max)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    _hip.check(lib.dwba_outer_tail_f64(1, _hip.ptr(tj_d), L_max, seg, dmax, Dp, Xp, _hip.ptr(ws), wsb,
+                                       _hip.stream_handle()), "tail")
     return out.cpu().numpy()
 
 
@@ -88,13 +91,14 @@ def test_outer_tail_matches_oracle(name):
     p = dict(CASES[name], da=0.3, db=0.7, sa=0.1 if CASES[name]["ea"] else 0.0, sb=-0.2 if CASES[name]["eb"] else 0.0)
     if name == "levin":
         assert _branch(p["ka"], p["la"], p["kb"], p["lb"], p["ea"], p["eb"], p["rm"], p["rmax"], -1) == "levin"
-    moments = np.array([0.0, 0.8, -0.5, 0.3, 1e-13, 0.05])
+    moments = np.array([0.0, 0.8, -0.5, 0.3, 1e-13, 0.05] + [0.02 * (-1) ** k for k in range(20)])
     got = _gpu_tail(p, moments)
     ref = _oracle_tail(p, moments)
     scale = np.max(np.abs(ref)) + 1e-300
     tol = 1e-10 * np.abs(ref) + 1e-13 * scale + 10.0 * _oracle_noise(p, moments)
     assert np.all(np.abs(got - ref) <= tol), (got, ref, np.abs(got - ref) / tol)
     assert got[0] == 0.0 and got[4] == 0.0
+    assert len(got) == 26
 
 
 def test_legacy_analytic_tail():
