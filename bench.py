@@ -126,6 +126,49 @@ def bench_kernel_only(args):
     return result
 
 
+def bench_backward(args):
+    """Gradient workload: N = 15000, 32 pairs, L = 0..20; d/d rho_a and d/d rho_b
+    of sum_{p,L} c_L I_L(p) by the backward kernels (FP64 and FP32)."""
+    import torch
+    from dwba_amd import synthetic as S
+    from dwba_amd.autograd import density_gradients
+    N, P, L_max = args.n, 32, 20
+    grid, ra, rb, c = S.backward_workload(N, P, L_max, seed=args.seed)
+    dev = torch.device("cuda:0")
+    t = lambda x: torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)
+    r_d, w_d, ra_d, rb_d, c_d = map(t, (grid.r, grid.w_simpson, ra, rb, c))
+    stream = torch.cuda.current_stream()
+    # algorithmic work per element and pair: Horner for Keff (L_max FMAs + 1 mul), two products
+    W = float(P) * N * N * (2 * L_max + 1 + 4)
+    legs = {}
+    for prec in ("fp64", "fp32"):
+        fn = lambda: density_gradients(ra_d, rb_d, r_d, w_d, c_d, L_max, prec)
+        for _ in range(args.warmup):
+            fn()
+        legs[prec] = _events_ms(fn, args.steps, stream)
+    from oracle import gradient as OG
+    ga, gb = density_gradients(ra_d, rb_d, r_d, w_d, c_d, L_max, "fp32")
+    sub = slice(0, 4)   # error check on 4 pairs (oracle cost)
+    ref_a, _ = OG.density_gradients(grid.r, grid.w_simpson, ra[sub], rb[sub], c[sub], L_max)
+    mag_a, _ = OG.density_gradients(grid.r, grid.w_simpson, ra[sub], rb[sub], c[sub], L_max, absolute=True)
+    err32 = float(np.max(np.abs(ga[sub].cpu().numpy() - ref_a) / np.maximum(mag_a, 1e-300)))
+    ms = legs["fp32"]
+    peak_fp32_valu = 157.3e12
+    return {
+        "metric": "gradient of sum_L c_L I_L w.r.t. rho_a and rho_b, pairs/s", "value": P / (ms * 1e-3),
+        "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "backward check: exp grid N=15000 on [1e-5,1000] a0, 32 pairs, L=0..20, "
+                               "c_L ~ N(0,1)", "N": N, "pairs": P, "L_max": L_max},
+        "roofline": {"bound": "valu", "achieved": W / (ms * 1e-3) / 1e12, "peak": peak_fp32_valu / 1e12,
+                     "unit": "TFLOP/s", "frac": W / (ms * 1e-3) / peak_fp32_valu, "traffic": None,
+                     "kernel": "il_backward_kernel<21,float>", "work_flop_per_launch": W},
+        "fp64": {"ms": legs["fp64"], "tflops": W / (legs["fp64"] * 1e-3) / 1e12,
+                 "frac_of_fp64_peak": W / (legs["fp64"] * 1e-3) / PEAK_FP64},
+        "fp32_max_err_over_magnitude": err32,
+    }
+
+
 def cpu_baseline_kernel_only(g, ad, bd, ae, be, L_max, sample_L=(0, 1, 2, 3)):
     """Oracle's faithful restatement of the reference's plain branch (full
     N x N ratio/inv_gtr/log_ratio, per-L N^2 exp + GEMVs), timed on a sample
@@ -314,7 +357,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=15000)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", default="batched", choices=["batched", "kernel_only"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "kernel_only", "backward"])
     ap.add_argument("--energies", type=int, default=64)
     ap.add_argument("--pairs", type=int, default=400)
     ap.add_argument("--fp64-steps", type=int, default=1)
@@ -324,10 +367,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: the I_L path is built and timed on a single MI355X; "
-              "multi-GPU sharding is described in DESIGN.md, not run.", file=sys.stderr)
+        print("bench.py measures one GPU only: the I_L path runs and is timed on a single MI355X; "
+              "there is no multi-GPU mode to launch.", file=sys.stderr)
         sys.exit(2)
-    result = bench_batched(args) if args.workload == "batched" else bench_kernel_only(args)
+    result = {"batched": bench_batched, "kernel_only": bench_kernel_only, "backward": bench_backward}[args.workload](args)
     print(json.dumps(result), flush=True)
 
 

@@ -82,6 +82,9 @@ _SIGS = {
     "dwba_dgemv_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_vp]),
     "dwba_ddot_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
     "dwba_batch_finalize": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
+    "dwba_il_backward_f64": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
+    "dwba_il_backward_f32": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
+    "dwba_weight_rows": (c_i32, [c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_i32, c_vp]),
 }
 
 

@@ -202,3 +202,29 @@ def ionization_scan(n_points=48, n_triples=400, n_range=(6000, 15000), l_max=30,
         out.append(EnergyInput(grid=g, V_core=V, U_i=U, bound_i=Bound(slater(r, 0.6), l=0, energy_au=-0.18),
                                bound_f=pairs[0][2], pairs=pairs, U_f=U))
     return out
+
+
+def backward_workload(n: int = 15000, n_pairs: int = 32, L_max: int = 20, seed: int = 0):
+    """Gradient workload: N-point exp grid on [1e-5, 1000] a0 and n_pairs
+    channel pairs as in the kernel-only workload (k ~ U[0.5, 8], delta ~
+    U[0, pi], l ~ U{0..40}); even pairs are direct (rho_a = u_1s u_2p,
+    rho_b = chi_f chi_i), odd pairs exchange (rho_a = u_2p chi_i,
+    rho_b = chi_f u_1s).  Loss weights c_L ~ N(0, 1) seeded, shared by all
+    pairs.  Returns (grid, rho_a (P, N), rho_b (P, N), g (P, L_max+1))."""
+    g = exp_grid(n, 1e-5, 1e3)
+    r = g.r
+    rng = np.random.default_rng(seed)
+    ui, uf = u1s(r), u2p(r)
+    ra = np.empty((n_pairs, n))
+    rb = np.empty((n_pairs, n))
+    for p in range(n_pairs):
+        ki, kf = rng.uniform(0.5, 8.0, size=2)
+        li, lf = rng.integers(0, 41, size=2)
+        di, df = rng.uniform(0.0, np.pi, size=2)
+        ci, cf = wave_values(r, ki, li, di), wave_values(r, kf, lf, df)
+        if p % 2 == 0:
+            ra[p], rb[p] = ui * uf, ci * cf
+        else:
+            ra[p], rb[p] = uf * ci, cf * ui
+    c = rng.standard_normal(L_max + 1)
+    return g, ra, rb, np.tile(c, (n_pairs, 1))

@@ -288,6 +288,33 @@ typedef struct dwba_finalize_pair {
 int32_t dwba_batch_finalize(int32_t n_pairs, const dwba_finalize_pair* d_pairs, int32_t L_max, double* d_out,
                             void* stream);
 
+/* ---- differentiation of the plain contraction -------------------------
+ * Forward quantity (the plain branch of radial_ME_all_L,
+ * dwba_matrix_elements.py:1243-1245 and :1288-1289, for P pairs sharing one
+ * grid): I_L(p) = sum_ij w_i rho_a[p][i] K_L(i,j) w_j rho_b[p][j].
+ * The reference has no gradient; these entry points give, for the loss
+ * sum_{p,L} g[p][L] I_L(p),
+ *   grad_a[p][i] = w_i sum_j Keff_p(i,j) w_j rho_b[p][j]
+ *   grad_b[p][i] = w_i sum_j Keff_p(i,j) w_j rho_a[p][j],   Keff_p = sum_L g[p][L] K_L,
+ * regenerating Keff element by element (nothing of size N x N is stored).
+ * rho_a, rho_b, grad_a, grad_b: [n_pairs][ld] doubles (rows i >= n of the
+ * outputs are written as 0); g: [n_pairs][L_max+1]; either output may be
+ * NULL.  _f64 evaluates Keff and the products in FP64; _f32 in FP32 with
+ * each 256-column chunk summed in FP32 and the chunks in FP64. */
+int32_t dwba_il_backward_f64(const double* d_r, const double* d_w, int32_t n, int32_t n_pairs, int32_t ld,
+                             const double* d_rho_a, const double* d_rho_b, const double* d_g, int32_t L_max,
+                             double* d_grad_a, double* d_grad_b, void* stream);
+int32_t dwba_il_backward_f32(const double* d_r, const double* d_w, int32_t n, int32_t n_pairs, int32_t ld,
+                             const double* d_rho_a, const double* d_rho_b, const double* d_g, int32_t L_max,
+                             double* d_grad_a, double* d_grad_b, void* stream);
+
+/* d_dst[k*dst_ld + i] = d_w[i] * d_src[k*src_ld + i] for i < n, 0 for
+ * n <= i < dst_ld, k < n_rows (the quadrature-weighted densities
+ * alpha = w rho_a, beta = w rho_b of the plain branch,
+ * dwba_matrix_elements.py:1243-1245). */
+int32_t dwba_weight_rows(const double* d_w, int32_t n, int32_t n_rows, const double* d_src, int32_t src_ld,
+                         double* d_dst, int32_t dst_ld, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
