@@ -70,6 +70,8 @@ _SIGS = {
     "dwba_il_plain_f64": (c_i32, [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_sz, c_vp]),
     "dwba_il_plain_f64_batch_workspace_bytes": (c_sz, [c_i32, c_i32p, c_i32p, c_i32]),
     "dwba_il_plain_f64_batch": (c_i32, [c_i32, c_vp, c_i32p, c_i32p, c_i32, c_vp, c_vp, c_sz, c_vp]),
+    "dwba_il_semisep_f64_workspace_bytes": (c_sz, [c_i32, c_i32p, c_i32p, c_i32, c_i32]),
+    "dwba_il_semisep_f64": (c_i32, [c_i32, c_vp, c_i32p, c_i32p, c_i32, c_i32, c_vp, c_vp, c_sz, c_vp]),
     "dwba_fullsplit_prep_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_dp, c_dp, c_i32, c_vp]),
     "dwba_moments_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]),
     "dwba_outer_tail_workspace_bytes": (c_sz, [c_i32, c_i32]),
@@ -192,6 +194,37 @@ def il_plain_f64(r, alpha_dir, beta_dir, alpha_ex, beta_ex, n_rows: int, n_cols:
                                 ptr(beta_ex), L_max, ptr(out), ptr(ws), ws_bytes, stream_handle(stream)),
           "dwba_il_plain_f64")
     out._dwba_ws = ws  # keep the workspace alive until the tensor is released
+    return out
+
+
+def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None):
+    """dwba_il_semisep_f64 on the same job tuples as il_plain_f64_batch
+    (n_rows <= n_cols).  Returns a (n_jobs, 2, L_max+1) float64 CUDA tensor."""
+    ensure_init()
+    lib = load_library()
+    n = len(jobs)
+    if n == 0:
+        raise ValueError("il_semisep_f64_batch: no jobs")
+    dev = jobs[0][0].device
+    arr = (PlainJob * n)()
+    nr = (ctypes.c_int32 * n)()
+    nc = (ctypes.c_int32 * n)()
+    for k, (r, ad, bd, ae, be, n_rows, n_cols) in enumerate(jobs):
+        for nm, t, m in (("r", r, n_cols), ("alpha_dir", ad, n_rows), ("beta_dir", bd, n_cols),
+                         ("alpha_ex", ae, n_rows), ("beta_ex", be, n_cols)):
+            _require_f64_cuda(f"job {k} {nm}", t, m)
+        arr[k] = PlainJob(ptr(r), ptr(ad), ptr(bd), ptr(ae), ptr(be), int(n_rows), int(n_cols))
+        nr[k] = int(n_rows)
+        nc[k] = int(n_cols)
+    d_jobs = struct_array_to_device(arr, dev)
+    ws_bytes = lib.dwba_il_semisep_f64_workspace_bytes(n, nr, nc, L_max, seg)
+    if ws_bytes == 0:
+        raise ValueError("il_semisep_f64_batch: invalid shapes (need 0 <= n_rows <= n_cols, 0 <= L_max <= 63)")
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    out = torch.empty((n, 2, L_max + 1), dtype=torch.float64, device=dev)
+    check(lib.dwba_il_semisep_f64(n, ptr(d_jobs), nr, nc, L_max, seg, ptr(out), ptr(ws), ws_bytes,
+                                  stream_handle(stream)), "dwba_il_semisep_f64")
+    out._dwba_keep = (ws, d_jobs)
     return out
 
 

@@ -13,7 +13,9 @@ stream, which can be captured into a HIP graph and replayed:
   4  direct family            shared beta -> dwba_dgemv_f64 + dwba_ddot_f64 (FP64);
                               distinct betas (ionization) -> stacked GEMM
   5  exchange family          precision "fp32": dwba_cast_f64_f32 + dwba_xgemm_f32
-                              (MFMA); precision "fp64": dwba_il_plain_f64_batch
+                              (MFMA); precision "fp64": both families through
+                              dwba_il_semisep_f64 (O(N) per L, default) or the
+                              dense dwba_il_plain_f64_batch
   6  dwba_batch_finalize      tails and L=0 corrections, in place
 
 Pairs in the oscillatory-off branch (PLAIN mode) always take the FP64 plain
@@ -93,15 +95,21 @@ class _EnergyState:
 
 
 class ScanBatch:
-    """Plans and runs all I_L of a scan.  precision: "fp32" (stacked
-    contractions on FP32 MFMA), "bf16x3" (split-BF16x3 MFMA with FP32
-    accumulation) or "fp64" (everything through the FP64 plain kernel)."""
+    """Plans and runs all I_L of a scan.  precision: "fp64" (every
+    contraction in FP64), "fp32" (stacked contractions on FP32 MFMA) or
+    "bf16x3" (split-BF16x3 MFMA with FP32 accumulation).  contraction (FP64
+    only): "semiseparable" (O(N) per L, dwba_il_semisep_f64) or "dense"
+    (O(N^2) per L, dwba_il_plain_f64_batch); the MFMA modes are dense."""
 
-    def __init__(self, energies: Sequence[EnergyInput], L_max: int, precision: str = "fp32",
+    def __init__(self, energies: Sequence[EnergyInput], L_max: int, precision: str = "fp64",
                  use_oscillatory_quadrature=True, oscillatory_method="advanced", CC_nodes=5,
-                 phase_increment=1.5708, min_grid_fraction=0.10, k_threshold=0.5):
+                 phase_increment=1.5708, min_grid_fraction=0.10, k_threshold=0.5,
+                 contraction: str = "semiseparable"):
         if precision not in ("fp32", "bf16x3", "fp64"):
             raise ValueError(f"precision must be 'fp32', 'bf16x3' or 'fp64', got {precision!r}")
+        if contraction not in ("semiseparable", "dense"):
+            raise ValueError(f"contraction must be 'semiseparable' or 'dense', got {contraction!r}")
+        self.contraction = contraction if precision == "fp64" else "dense"
         if L_max < 0 or L_max > 63:
             raise ValueError("L_max must be in [0, 63]")
         import torch
@@ -328,7 +336,11 @@ class ScanBatch:
             nra[g], nca[g] = plan.rows, plan.cols
         self.plain_d = hip.struct_array_to_device(arr, self.dev)
         self.plain_n = (nra, nca)
-        self.plain_ws_bytes = self.lib.dwba_il_plain_f64_batch_workspace_bytes(self.n_pairs, nra, nca, self.L_max)
+        if self.contraction == "semiseparable":
+            self.plain_ws_bytes = self.lib.dwba_il_semisep_f64_workspace_bytes(self.n_pairs, nra, nca, self.L_max, 0)
+        else:
+            self.plain_ws_bytes = self.lib.dwba_il_plain_f64_batch_workspace_bytes(self.n_pairs, nra, nca,
+                                                                                   self.L_max)
         self.plain_ws = self.torch.empty(max(self.plain_ws_bytes, 1), dtype=self.torch.uint8, device=self.dev)
         self.plain_count = self.n_pairs
 
@@ -490,9 +502,14 @@ class ScanBatch:
         mark("plain", "start")
         if self.plain_count:
             nra, nca = self.plain_n
-            hip.check(lib.dwba_il_plain_f64_batch(self.plain_count, hip.ptr(self.plain_d), nra, nca, Lm,
+            if self.contraction == "semiseparable":
+                hip.check(lib.dwba_il_semisep_f64(self.plain_count, hip.ptr(self.plain_d), nra, nca, Lm, 0,
                                                   hip.ptr(self.out), hip.ptr(self.plain_ws), self.plain_ws_bytes, sh),
-                      "dwba_il_plain_f64_batch")
+                          "dwba_il_semisep_f64")
+            else:
+                hip.check(lib.dwba_il_plain_f64_batch(self.plain_count, hip.ptr(self.plain_d), nra, nca, Lm,
+                                                      hip.ptr(self.out), hip.ptr(self.plain_ws), self.plain_ws_bytes,
+                                                      sh), "dwba_il_plain_f64_batch")
         mark("plain", "end")
         if self.precision in ("fp32", "bf16x3"):
             mark("direct", "start")
@@ -556,11 +573,12 @@ class ScanBatch:
                                   plan.rows, plan.cols)
             nra[g], nca[g] = plan.rows, plan.cols
         d_jobs = hip.struct_array_to_device(arr, self.dev)
-        wsb = lib.dwba_il_plain_f64_batch_workspace_bytes(n, nra, nca, self.L_max)
+        # all terms are non-negative: the O(N) form has no cancellation to lose accuracy to
+        wsb = lib.dwba_il_semisep_f64_workspace_bytes(n, nra, nca, self.L_max, 0)
         ws = torch.empty(wsb, dtype=torch.uint8, device=self.dev)
         mag = torch.zeros((n, 2, self.L_max + 1), dtype=torch.float64, device=self.dev)
-        hip.check(lib.dwba_il_plain_f64_batch(n, hip.ptr(d_jobs), nra, nca, self.L_max, hip.ptr(mag), hip.ptr(ws),
-                                              wsb, hip.stream_handle()), "dwba_il_plain_f64_batch")
+        hip.check(lib.dwba_il_semisep_f64(n, hip.ptr(d_jobs), nra, nca, self.L_max, 0, hip.ptr(mag), hip.ptr(ws),
+                                          wsb, hip.stream_handle()), "dwba_il_semisep_f64")
         return mag.cpu().numpy()[self.out_index]
 
     def work_flop(self):

@@ -1,0 +1,275 @@
+// il_semisep_f64: I_L = alpha^T K_L beta for all L, direct + exchange, FP64,
+// in O(N) work per (pair, L) instead of O(N^2).
+//
+// Same jobs, output layout and meaning as dwba_il_plain_f64_batch (the
+// per-L kernel build + GEMVs of radial_ME_all_L, dwba_matrix_elements.py:
+// 1057-1080, :1113-1116, :1243-1245, :1281-1289); only the order of the FP64
+// arithmetic differs.  K_L is semiseparable on an increasing grid:
+//     K_L(i, j) = (r_j / r_i)^L / r_i  (j < i),   (r_i / r_j)^L / r_j  (j > i),
+//     K_L(i, i) = (1 - 1e-12)^L / r_i          (the reference's clamp),
+// so with the forward recurrences  (q_i = r_i / r_{i+1} < 1)
+//     A_{i+1} = q_i^L (A_i + beta_i),      A_i  = sum_{j<i} (r_j / r_i)^L beta_j
+//     Z_{i+1} = q_i^L (Z_i + alpha_i),     Z_i  = sum_{j<i} (r_j / r_i)^L alpha_j
+// the whole double sum is one pass over the grid:
+//     I_L = sum_i (1 / r_i) (alpha_i A_i + beta_i Z_i + d_L alpha_i beta_i).
+// Every multiplier is below one, so rounding errors are damped, not amplified.
+//
+// Parallel form: a task is one wavefront over a contiguous range of the grid
+// of one (job, family); each lane runs the recurrences from zero over its own
+// contiguous sub-range and also accumulates
+//     Ca = sum alpha_i / r_i * P_i,  Cb = sum beta_i / r_i * P_i,  P_i = (r_a / r_i)^L
+// (r_a: first point of the lane), which is what a carried-in A or Z adds to the
+// lane's sum.  The lanes are joined by an in-wave scan of the affine maps
+// (M, A) -> (M1 M2, A1 M2 + A2), M = (r_a / r_b)^L; tasks of one job are joined
+// in order by a second kernel.  L is processed in blocks of LB values held in
+// registers (6 * LB doubles per lane); each block re-reads the lane's range.
+#include "common.hpp"
+
+namespace dwba {
+namespace {
+
+constexpr int ST = 256;          // threads per workgroup (4 tasks)
+constexpr int NSUM = 6;          // per (task, L): total, Ca, Cb, A_out, Z_out, M
+
+__device__ inline int find_job(const int* __restrict__ off, int n_jobs, int t) {
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ inline double pow_int(double x, int e) {
+    double r = 1.0;
+    while (e > 0) {
+        if (e & 1) r *= x;
+        x *= x;
+        e >>= 1;
+    }
+    return r;
+}
+
+// Tasks of job k: 2 families x W_k range tasks, W_k = ceil(n_cols / (64 * seg)).
+__device__ inline int tasks_of(int n_cols, int seg) {
+    if (n_cols <= 0) return 0;
+    const int span = 64 * seg;
+    return 2 * ((n_cols + span - 1) / span);
+}
+
+template <int LB>
+__global__ __launch_bounds__(ST) void semisep_kernel(const dwba_plain_job* __restrict__ jobs,
+                                                     const int* __restrict__ task_off, int n_jobs, int seg,
+                                                     int L_max, double* __restrict__ summ) {
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * (ST / 64) + (threadIdx.x >> 6);
+    if (t >= task_off[n_jobs]) return;
+    const int k = find_job(task_off, n_jobs, t);
+    const dwba_plain_job J = jobs[k];
+    const int local = t - task_off[k];
+    const int w_per_fam = tasks_of(J.n_cols, seg) / 2;
+    const int fam = local / w_per_fam;
+    const int wt = local - fam * w_per_fam;
+    const double* __restrict__ al = fam ? J.alpha_ex : J.alpha_dir;
+    const double* __restrict__ be = fam ? J.beta_ex : J.beta_dir;
+    const int n = J.n_cols;
+    const int nr = J.n_rows;
+    const int span = 64 * seg;
+    const int e0 = wt * span;
+    const int e1 = min(n, e0 + span);
+    const int sl = (e1 - e0 + 63) / 64;
+    const int a = min(e1, e0 + lane * sl);
+    const int b = min(e1, a + sl);
+    const int NL = L_max + 1;
+    const double ldl = log(1.0 - 1e-12);
+    double* __restrict__ out = summ + (size_t)t * NL * NSUM;
+
+    for (int L0 = 0; L0 < NL; L0 += LB) {
+        double A[LB], Z[LB], acc[LB], Ca[LB], Cb[LB], P[LB], dL[LB];
+#pragma unroll
+        for (int l = 0; l < LB; ++l) {
+            A[l] = 0.0; Z[l] = 0.0; acc[l] = 0.0; Ca[l] = 0.0; Cb[l] = 0.0; P[l] = 1.0;
+            const int L = L0 + l;
+            dL[l] = L == 0 ? 1.0 : exp((double)L * ldl);
+        }
+        double r_next = a < n ? J.r[a] : 1.0;
+        double ir_next = 1.0 / r_next;
+#pragma unroll 1
+        for (int i = a; i < b; ++i) {
+            const double ri = r_next, iri = ir_next;
+            r_next = i + 1 < n ? J.r[i + 1] : ri;
+            ir_next = 1.0 / r_next;
+            const double q = ri * ir_next;
+            const double av = i < nr ? al[i] : 0.0;
+            const double bv = be[i];
+            const double aw = av * iri, bw = bv * iri, ab = av * bw;
+            double qL = pow_int(q, L0);
+#pragma unroll
+            for (int l = 0; l < LB; ++l) {
+                acc[l] = fma(aw, A[l], acc[l]);
+                acc[l] = fma(bw, Z[l], acc[l]);
+                acc[l] = fma(dL[l], ab, acc[l]);
+                Ca[l] = fma(aw, P[l], Ca[l]);
+                Cb[l] = fma(bw, P[l], Cb[l]);
+                A[l] = qL * (A[l] + bv);
+                Z[l] = qL * (Z[l] + av);
+                P[l] *= qL;
+                qL *= q;
+            }
+        }
+        // join the lanes: inclusive scan of (M, A, Z) over lanes, M = P at lane end
+#pragma unroll
+        for (int l = 0; l < LB; ++l) {
+            if (L0 + l >= NL) break;
+            double M = P[l], Ai = A[l], Zi = Z[l];
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const double Mp = __shfl_up(M, d, 64);
+                const double Ap = __shfl_up(Ai, d, 64);
+                const double Zp = __shfl_up(Zi, d, 64);
+                if (lane >= d) {
+                    Ai = fma(Ap, M, Ai);
+                    Zi = fma(Zp, M, Zi);
+                    M = Mp * M;
+                }
+            }
+            // exclusive prefix: carry into this lane and the factor from the task start
+            double Ain = __shfl_up(Ai, 1, 64), Zin = __shfl_up(Zi, 1, 64), Mpre = __shfl_up(M, 1, 64);
+            if (lane == 0) { Ain = 0.0; Zin = 0.0; Mpre = 1.0; }
+            const double tot = wave_sum(acc[l] + Ain * Ca[l] + Zin * Cb[l]);
+            const double ca = wave_sum(Mpre * Ca[l]);
+            const double cb = wave_sum(Mpre * Cb[l]);
+            const double A_out = __shfl(Ai, 63, 64), Z_out = __shfl(Zi, 63, 64), M_out = __shfl(M, 63, 64);
+            if (lane == 0) {
+                double* o = out + (size_t)(L0 + l) * NSUM;
+                o[0] = tot; o[1] = ca; o[2] = cb; o[3] = A_out; o[4] = Z_out; o[5] = M_out;
+            }
+        }
+    }
+}
+
+// Join the tasks of each (job, family) in grid order: one thread per L.
+__global__ void semisep_join(const dwba_plain_job* __restrict__ jobs, const int* __restrict__ task_off, int seg,
+                             int L_max, const double* __restrict__ summ, double* __restrict__ out) {
+    const int k = blockIdx.x, fam = blockIdx.y, L = threadIdx.x;
+    const int NL = L_max + 1;
+    if (L >= NL) return;
+    const int w = tasks_of(jobs[k].n_cols, seg) / 2;
+    double tot = 0.0, Ain = 0.0, Zin = 0.0;
+    if (jobs[k].n_rows > 0) {
+        for (int x = 0; x < w; ++x) {
+            const double* s = summ + ((size_t)(task_off[k] + fam * w + x) * NL + L) * NSUM;
+            tot += s[0] + Ain * s[1] + Zin * s[2];
+            Ain = fma(Ain, s[5], s[3]);
+            Zin = fma(Zin, s[5], s[4]);
+        }
+    }
+    out[(size_t)k * 2 * NL + fam * NL + L] = tot;
+}
+
+__global__ void semisep_plan(const dwba_plain_job* __restrict__ jobs, int n_jobs, int seg, int* __restrict__ off) {
+    __shared__ int s_tot[1024];
+    const int nt = blockDim.x;
+    const int per = (n_jobs + nt - 1) / nt;
+    const int k0 = threadIdx.x * per;
+    const int k1 = min(n_jobs, k0 + per);
+    int sum = 0;
+    for (int k = k0; k < k1; ++k) sum += tasks_of(jobs[k].n_cols, seg);
+    s_tot[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int x = 0; x < nt; ++x) { const int v = s_tot[x]; s_tot[x] = run; run += v; }
+        off[n_jobs] = run;
+    }
+    __syncthreads();
+    int run = s_tot[threadIdx.x];
+    for (int k = k0; k < k1; ++k) {
+        off[k] = run;
+        run += tasks_of(jobs[k].n_cols, seg);
+    }
+}
+
+int host_tasks(int32_t n_jobs, const int32_t* h_n_cols, int seg) {
+    int64_t tot = 0;
+    for (int k = 0; k < n_jobs; ++k) {
+        const int n = h_n_cols[k];
+        if (n > 0) tot += 2 * ((n + 64 * seg - 1) / (64 * seg));
+    }
+    return tot > INT32_MAX ? -1 : (int)tot;
+}
+
+// Lane range length: enough tasks to give every SIMD several waves, no
+// shorter than needed to keep tasks below 2^31.
+int pick_seg(int32_t n_jobs, const int32_t* h_n_cols, int seg_hint) {
+    if (seg_hint > 0) return std::min(seg_hint, 1 << 16);
+    int64_t elems = 0;
+    for (int k = 0; k < n_jobs; ++k) elems += 2 * (int64_t)std::max(0, h_n_cols[k]);
+    const int64_t target_waves = (int64_t)std::max(1, device_cu_count()) * 16;
+    int64_t seg = elems / (64 * target_waves);
+    return (int)std::min<int64_t>(std::max<int64_t>(seg, 1), 1 << 16);
+}
+
+struct SemisepLayout {
+    int seg, n_tasks;
+    size_t off_bytes, summ_bytes, total;
+};
+
+bool semisep_layout(int32_t n_jobs, const int32_t* h_n_rows, const int32_t* h_n_cols, int32_t L_max, int seg_hint,
+                    SemisepLayout* lay) {
+    if (n_jobs <= 0 || !h_n_rows || !h_n_cols || L_max < 0 || L_max > 63) return false;
+    for (int k = 0; k < n_jobs; ++k)
+        if (h_n_cols[k] < 0 || h_n_rows[k] < 0 || h_n_rows[k] > h_n_cols[k]) return false;
+    lay->seg = pick_seg(n_jobs, h_n_cols, seg_hint);
+    lay->n_tasks = host_tasks(n_jobs, h_n_cols, lay->seg);
+    if (lay->n_tasks < 0) return false;
+    lay->off_bytes = align_up(sizeof(int) * (size_t)(n_jobs + 1));
+    lay->summ_bytes = align_up(sizeof(double) * (size_t)std::max(1, lay->n_tasks) * (L_max + 1) * NSUM);
+    lay->total = lay->off_bytes + lay->summ_bytes;
+    return true;
+}
+
+}  // namespace
+}  // namespace dwba
+
+using namespace dwba;
+
+extern "C" {
+
+size_t dwba_il_semisep_f64_workspace_bytes(int32_t n_jobs, const int32_t* h_n_rows, const int32_t* h_n_cols,
+                                           int32_t L_max, int32_t seg) {
+    SemisepLayout lay;
+    if (!semisep_layout(n_jobs, h_n_rows, h_n_cols, L_max, seg, &lay)) return 0;
+    return lay.total;
+}
+
+int32_t dwba_il_semisep_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const int32_t* h_n_rows,
+                            const int32_t* h_n_cols, int32_t L_max, int32_t seg, double* d_out, void* d_workspace,
+                            size_t workspace_bytes, void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_il_semisep_f64: call dwba_init first");
+    DWBA_REQUIRE(d_jobs && d_out, "dwba_il_semisep_f64: null pointer");
+    SemisepLayout lay;
+    DWBA_REQUIRE(semisep_layout(n_jobs, h_n_rows, h_n_cols, L_max, seg, &lay),
+                 "dwba_il_semisep_f64: invalid jobs (n_jobs=%d, L_max=%d; need 0 <= n_rows <= n_cols)", n_jobs, L_max);
+    if (workspace_bytes < lay.total)
+        return set_error(DWBA_ERR_WORKSPACE, "dwba_il_semisep_f64: workspace %zu < %zu bytes", workspace_bytes,
+                         lay.total);
+    DWBA_REQUIRE(d_workspace, "dwba_il_semisep_f64: null workspace");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    char* ws = static_cast<char*>(d_workspace);
+    int* d_off = reinterpret_cast<int*>(ws);
+    double* d_summ = reinterpret_cast<double*>(ws + lay.off_bytes);
+    hipLaunchKernelGGL(semisep_plan, dim3(1), dim3(256), 0, s, d_jobs, n_jobs, lay.seg, d_off);
+    DWBA_LAUNCH_CHECK("semisep_plan");
+    if (lay.n_tasks > 0) {
+        const int blocks = (lay.n_tasks + ST / 64 - 1) / (ST / 64);
+        hipLaunchKernelGGL(semisep_kernel<8>, dim3(blocks), dim3(ST), 0, s, d_jobs, d_off, n_jobs, lay.seg, L_max,
+                           d_summ);
+        DWBA_LAUNCH_CHECK("semisep_kernel");
+    }
+    hipLaunchKernelGGL(semisep_join, dim3(n_jobs, 2), dim3(64), 0, s, d_jobs, d_off, lay.seg, L_max, d_summ, d_out);
+    DWBA_LAUNCH_CHECK("semisep_join");
+    return DWBA_OK;
+}
+
+}  // extern "C"

@@ -15,7 +15,9 @@ places; the CPU values are the ones reproduced):
   host   split index, branch and phase-interval decisions (planner.py)
   GPU    dwba_fullsplit_prep_f64   alpha/beta of direct and exchange, L=0 factors
          dwba_moments_f64          M_L = sum w r^L u_f u_i
-         dwba_il_plain_f64         alpha^T K_L beta for all L, K_L never stored
+         dwba_il_semisep_f64       alpha^T K_L beta for all L in O(N) per L (default), or
+         dwba_il_plain_f64         the dense O(N^2) form; K_L is never stored
+                                   (set_contraction() / DWBA_CONTRACTION=dense|semiseparable)
          dwba_outer_tail_f64       Filon/Levin outer tail + analytic tail
   host   I_L = contraction + tail (+ L=0 correction), one device-to-host copy.
 
@@ -24,7 +26,9 @@ raises RuntimeError (HAS_CUPY / check_cupy_runtime() report it up front).
 """
 from __future__ import annotations
 
+import ctypes
 import logging
+import os
 from dataclasses import dataclass, field
 from typing import Dict, Literal, Optional
 
@@ -36,6 +40,25 @@ from .planner import GridStats, PREP_PLAIN, PREP_FILON, PREP_STANDARD, PREP_ZERO
 logger = logging.getLogger(__name__)
 
 OscillatoryMethod = Literal["legacy", "advanced", "full_split"]
+
+_CONTRACTIONS = ("semiseparable", "dense")
+_contraction = os.environ.get("DWBA_CONTRACTION", "semiseparable")
+if _contraction not in _CONTRACTIONS:
+    raise ValueError(f"DWBA_CONTRACTION must be one of {_CONTRACTIONS}, got {_contraction!r}")
+
+
+def set_contraction(name: str) -> None:
+    """Select how alpha^T K_L beta is evaluated: "semiseparable" (O(N) per L,
+    the default) or "dense" (O(N^2) per L).  Both are FP64 and agree to
+    rounding; the choice is process-wide."""
+    global _contraction
+    if name not in _CONTRACTIONS:
+        raise ValueError(f"contraction must be one of {_CONTRACTIONS}, got {name!r}")
+    _contraction = name
+
+
+def get_contraction() -> str:
+    return _contraction
 
 TAIL_NONE, TAIL_OUTER, TAIL_LEGACY = 0, 1, 2
 
@@ -296,12 +319,25 @@ class _PairRunner:
             keep += [mom, ptrs, n_d, tj_d, LDa, LXa, tws]
         if plan.rows > 0:
             out = res[:2 * nl].view(2, nl)
-            ws_bytes = lib.dwba_il_plain_f64_workspace_bytes(plan.rows, plan.cols, L_max)
-            ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-            hip.check(lib.dwba_il_plain_f64(hip.ptr(r_d), plan.rows, plan.cols, hip.ptr(vec[0]), hip.ptr(vec[1]),
-                                            hip.ptr(vec[2]), hip.ptr(vec[3]), L_max, hip.ptr(out), hip.ptr(ws),
-                                            ws_bytes, sh), "dwba_il_plain_f64")
-            keep.append(ws)
+            if _contraction == "semiseparable":
+                pj = (hip.PlainJob * 1)()
+                pj[0] = hip.PlainJob(hip.ptr(r_d), hip.ptr(vec[0]), hip.ptr(vec[1]), hip.ptr(vec[2]),
+                                     hip.ptr(vec[3]), plan.rows, plan.cols)
+                pj_d = hip.struct_array_to_device(pj, dev)
+                nr = (ctypes.c_int32 * 1)(plan.rows)
+                nc = (ctypes.c_int32 * 1)(plan.cols)
+                ws_bytes = lib.dwba_il_semisep_f64_workspace_bytes(1, nr, nc, L_max, 0)
+                ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+                hip.check(lib.dwba_il_semisep_f64(1, hip.ptr(pj_d), nr, nc, L_max, 0, hip.ptr(out), hip.ptr(ws),
+                                                  ws_bytes, sh), "dwba_il_semisep_f64")
+                keep += [pj_d, ws]
+            else:
+                ws_bytes = lib.dwba_il_plain_f64_workspace_bytes(plan.rows, plan.cols, L_max)
+                ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+                hip.check(lib.dwba_il_plain_f64(hip.ptr(r_d), plan.rows, plan.cols, hip.ptr(vec[0]),
+                                                hip.ptr(vec[1]), hip.ptr(vec[2]), hip.ptr(vec[3]), L_max,
+                                                hip.ptr(out), hip.ptr(ws), ws_bytes, sh), "dwba_il_plain_f64")
+                keep.append(ws)
         host = res.cpu().numpy()          # the one device-to-host copy (synchronises)
         del keep
         return (host[:2 * nl].reshape(2, nl), host[2 * nl:2 * nl + 4], host[2 * nl + 4:])

@@ -97,6 +97,22 @@ int32_t dwba_il_plain_f64_batch(int32_t n_jobs, const dwba_plain_job* d_jobs,
                                 int32_t L_max, double* d_out,
                                 void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* O(N)-per-L form of dwba_il_plain_f64_batch: same jobs, same output layout,
+ * same FP64 values up to rounding order.  K_L is semiseparable on an
+ * increasing grid (K_L(i,j) = (r_j/r_i)^L / r_i for j < i), so
+ *   I_L = sum_i (alpha_i A_i + beta_i Z_i + (1-1e-12)^L alpha_i beta_i) / r_i,
+ *   A_{i+1} = (r_i/r_{i+1})^L (A_i + beta_i),  Z_{i+1} = (r_i/r_{i+1})^L (Z_i + alpha_i),
+ * evaluated as wavefront-parallel affine scans.  Replaces the same reference
+ * code as dwba_il_plain_f64 (dwba_matrix_elements.py:1057-1080, :1113-1116,
+ * :1243-1245, :1281-1289).  Requires 0 <= n_rows <= n_cols and d_r strictly
+ * increasing and > 0.  seg: grid points per lane (0 = chosen from the work
+ * size); the workspace size depends on it. */
+size_t dwba_il_semisep_f64_workspace_bytes(int32_t n_jobs, const int32_t* h_n_rows, const int32_t* h_n_cols,
+                                           int32_t L_max, int32_t seg);
+int32_t dwba_il_semisep_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const int32_t* h_n_rows,
+                            const int32_t* h_n_cols, int32_t L_max, int32_t seg, double* d_out,
+                            void* d_workspace, size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* Full-Split preparation: alpha / beta vectors of a channel pair.       */
 /* ------------------------------------------------------------------ */

@@ -38,9 +38,17 @@ def _case(seed, n=1200, r_max=100.0, eta=0.0, ionization=False, with_match=True,
 
 
 def _compare(args, kw, L_max, faithful=True, rtol=1e-10, atol_rel=1e-13):
-    from dwba_amd.dwba_matrix_elements import radial_ME_all_L_gpu
+    """Both contraction algorithms (semiseparable and dense) against the oracle."""
+    from dwba_amd import dwba_matrix_elements as ME
     g, V, Ui, Uf, bi, bf, ci, cf = args
-    got = radial_ME_all_L_gpu(g, V, Ui, bi, bf, ci, cf, L_max, U_f_array=Uf, **kw)
+    prev = ME.get_contraction()
+    try:
+        gots = []
+        for alg in ("semiseparable", "dense"):
+            ME.set_contraction(alg)
+            gots.append(ME.radial_ME_all_L_gpu(g, V, Ui, bi, bf, ci, cf, L_max, U_f_array=Uf, **kw))
+    finally:
+        ME.set_contraction(prev)
     okw = {k: v for k, v in kw.items() if k in ("use_oscillatory_quadrature", "oscillatory_method", "CC_nodes",
                                                 "phase_increment", "min_grid_fraction", "k_threshold")}
     red, parts = R.radial_ME_all_L_reduced(g.r, g.w_simpson, g.w_trapz, V, Ui, bi, bf, ci, cf, L_max,
@@ -50,15 +58,17 @@ def _compare(args, kw, L_max, faithful=True, rtol=1e-10, atol_rel=1e-13):
     r = g.r
     mag_d = semisep.il_all_L(r, np.abs(parts["alpha_dir"]), np.abs(parts["beta_dir"]), L_max)
     mag_e = semisep.il_all_L(r, np.abs(parts["alpha_ex"]), np.abs(parts["beta_ex"]), L_max)
-    for L in range(L_max + 1):
-        for fam, a, b, mag in (("dir", got.I_L_direct, ref.I_L_direct, mag_d[L] + abs(parts["tails"][L])),
-                               ("ex", got.I_L_exchange, ref.I_L_exchange, mag_e[L])):
-            if np.isnan(b[L]):
-                assert np.isnan(a[L]), (fam, L, a[L])
-                continue
-            tol = rtol * abs(b[L]) + atol_rel * mag + (10.0 * parts["tail_noise"][L] if fam == "dir" else 0.0)
-            assert abs(a[L] - b[L]) <= tol, (fam, L, a[L], b[L], abs(a[L] - b[L]) / max(abs(b[L]), 1e-300))
-    return got, ref
+    for alg, got in zip(("semiseparable", "dense"), gots):
+        for L in range(L_max + 1):
+            for fam, a, b, mag in (("dir", got.I_L_direct, ref.I_L_direct, mag_d[L] + abs(parts["tails"][L])),
+                                   ("ex", got.I_L_exchange, ref.I_L_exchange, mag_e[L])):
+                if np.isnan(b[L]):
+                    assert np.isnan(a[L]), (alg, fam, L, a[L])
+                    continue
+                tol = rtol * abs(b[L]) + atol_rel * mag + (10.0 * parts["tail_noise"][L] if fam == "dir" else 0.0)
+                assert abs(a[L] - b[L]) <= tol, (alg, fam, L, a[L], b[L],
+                                                 abs(a[L] - b[L]) / max(abs(b[L]), 1e-300))
+    return gots[0], ref
 
 
 @pytest.mark.parametrize("method", ["advanced", "full_split", "legacy"])
