@@ -7,10 +7,14 @@ N ~ U{4000..15000}, 400 channel pairs per energy (l_i, l_f ~ U{0..40}, per
 energy k_i, k_f ~ U[0.5, 8], per wave delta ~ U[0, pi]), H 1s -> 2p bound
 pair, Full-Split quadrature (Filon/Clenshaw-Curtis alpha/beta, FP64 outer
 tails), L = 0..30, direct + exchange: 64*400*31*2 = 1,587,200 integrals per
-step.  `value` is the FP32-MFMA throughput; the FP64 plain path is timed on
-the same batch and the FP32 errors are stated against it.
+step.  `value` is the throughput of the FP64 path whose contractions use the
+O(N) semiseparable kernel; the FP32-MFMA, split-BF16x3 and dense FP64 paths
+are timed on the same batch and their errors stated against the FP64 result
+(per L, per family, relative to |I| and to sum |alpha_i| K_ij |beta_j|).
 
-Workload "kernel_only": one channel pair, N = 15000, L = 0..30, plain FP64.
+Workload "kernel_only": one channel pair, N = 15000, L = 0..30, FP64 (dense
+plain kernel and semiseparable kernel).
+Workload "backward": gradients of sum_L c_L I_L, N = 15000, 32 pairs, L = 0..20.
 
 A step is one pass of the hot path over the batch with inputs resident in
 HBM (HIP-graph replay of prep, tails, contractions and assembly); host
@@ -100,12 +104,32 @@ def bench_kernel_only(args):
     W = plain_work_flops(N, N, L_max + 1)
     res = out.cpu().numpy()
 
+    # the same pair through the O(N) semiseparable kernel
+    import ctypes
+    pj = (_hip.PlainJob * 1)(_hip.PlainJob(P(r_d), P(ad_d), P(bd_d), P(ae_d), P(be_d), N, N))
+    pj_d = _hip.struct_array_to_device(pj, dev)
+    nr = (ctypes.c_int32 * 1)(N)
+    ss_bytes = lib.dwba_il_semisep_f64_workspace_bytes(1, nr, nr, L_max, 0)
+    ss_ws = torch.empty(ss_bytes, dtype=torch.uint8, device=dev)
+    out_ss = torch.empty((1, 2, L_max + 1), dtype=torch.float64, device=dev)
+
+    def step_ss():
+        _hip.check(lib.dwba_il_semisep_f64(1, P(pj_d), nr, nr, L_max, 0, P(out_ss), P(ss_ws), ss_bytes, sh),
+                   "dwba_il_semisep_f64")
+
+    for _ in range(args.warmup + 1):
+        step_ss()
+    ms_ss = _events_ms(step_ss, args.steps, stream)
+    res_ss = out_ss.cpu().numpy()[0]
+
     # parity against the oracle's exact O(N) long-double contraction
     from oracle import semisep
     ref_d = semisep.il_all_L(g.r, ad, bd, L_max)
     ref_e = semisep.il_all_L(g.r, ae, be, L_max)
     rel = max(float(np.max(np.abs(res[0] - ref_d) / np.abs(ref_d))),
               float(np.max(np.abs(res[1] - ref_e) / np.abs(ref_e))))
+    rel_ss = max(float(np.max(np.abs(res_ss[0] - ref_d) / np.abs(ref_d))),
+                 float(np.max(np.abs(res_ss[1] - ref_e) / np.abs(ref_e))))
 
     result = {
         "metric": METRIC, "value": value, "unit": "integrals/s", "n_gpus": 1, "steps": args.steps,
@@ -120,6 +144,9 @@ def bench_kernel_only(args):
                      "note": "FP64 VALU path; MI355X FP64 vector peak = FP64 matrix peak = 78.6 TF"},
         "rel_err_vs_fp64_oracle": rel,
         "wall_s_timed": t_wall,
+        "semiseparable": {"value": n_int / (ms_ss * 1e-3), "unit": "integrals/s", "ms_per_step": ms_ss,
+                          "rel_err_vs_fp64_oracle": rel_ss, "kernel": "semisep_kernel<8> (dwba_il_semisep_f64)",
+                          "note": "O(N) per L; one pair fills few waves, so this time is launch latency"},
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline_kernel_only(g, ad, bd, ae, be, L_max)
@@ -203,10 +230,43 @@ def _events_ms(fn, steps, stream):
     return e0.elapsed_time(e1) / steps
 
 
+def _scan_leg(scan, L_max, precision, contraction, steps, warmup, graph, stream):
+    """Plan/build one ScanBatch, time `steps` graph replays after `warmup`,
+    and take per-stage times from one eager launch with events."""
+    import torch
+    from dwba_amd.batch import ScanBatch
+    b = ScanBatch(scan, L_max, precision=precision, oscillatory_method="full_split", contraction=contraction)
+    b.launch()
+    ev = {}
+    b.launch(events=ev)
+    torch.cuda.synchronize()
+    stages = b.stage_ms(ev)
+    if graph:
+        b.capture()
+    for _ in range(warmup):
+        b.run()
+    ms = _events_ms(b.run, steps, stream)
+    return b, ms, stages, b.results()
+
+
+def _err_stats(res, ref, mag):
+    d = np.abs(res - ref)
+    rel = d / np.maximum(np.abs(ref), 1e-300)
+    canc = d / np.maximum(mag, 1e-300)
+    return {
+        "exchange_max_abs_over_sum_abs_terms": float(np.max(canc[:, 1, :])),
+        "exchange_max_abs_over_sum_abs_terms_per_L": [float(x) for x in np.max(canc[:, 1, :], axis=0)],
+        "direct_max_abs_over_sum_abs_terms": float(np.max(canc[:, 0, :])),
+        "direct_max_abs_over_sum_abs_terms_per_L": [float(x) for x in np.max(canc[:, 0, :], axis=0)],
+        "exchange_median_rel": float(np.median(rel[:, 1, :])),
+        "exchange_frac_rel_le_1e-6": float(np.mean(rel[:, 1, :] <= 1e-6)),
+        "direct_max_rel": float(np.max(rel[:, 0, :])),
+    }
+
+
 def bench_batched(args):
     import torch
     from dwba_amd import synthetic as S
-    from dwba_amd.batch import ScanBatch
 
     L_max = 30
     t0 = time.perf_counter()
@@ -216,108 +276,80 @@ def bench_batched(args):
     n_pairs = sum(len(e.pairs) for e in scan)
     n_int = n_pairs * (L_max + 1) * 2
     stream = torch.cuda.current_stream()
-    print(f"[bench] scan generated: {len(scan)} energies, {n_pairs} pairs ({t_gen:.1f} s)", file=sys.stderr, flush=True)
+    graph = not args.no_graph
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
+    log(f"scan generated: {len(scan)} energies, {n_pairs} pairs ({t_gen:.1f} s)")
 
-    b32 = ScanBatch(scan, L_max, precision="fp32", oscillatory_method="full_split")
-    print(f"[bench] fp32 batch planned {b32.plan_seconds:.1f} s, built {b32.build_seconds:.1f} s",
-          file=sys.stderr, flush=True)
-    # stage times from one eager pass (events on the launch stream)
-    b32.launch()
-    ev = {}
-    b32.launch(events=ev)
-    torch.cuda.synchronize()
-    stages = b32.stage_ms(ev)
-    if not args.no_graph:
-        b32.capture()
-    for _ in range(args.warmup):
-        b32.run()
-    ms32 = _events_ms(b32.run, args.steps, stream)
-    res32 = b32.results()
-    print(f"[bench] fp32: {ms32:.1f} ms/step, stages {stages}", file=sys.stderr, flush=True)
-    wd, we = b32.work_flop()
-
+    # the product path: every contraction in FP64 by the O(N) semiseparable kernel
+    b64s, ms64s, st64s, res64s = _scan_leg(scan, L_max, "fp64", "semiseparable", args.steps, args.warmup, graph,
+                                           stream)
+    log(f"fp64 semiseparable: {ms64s:.2f} ms/step, stages {st64s}")
+    mag = b64s.magnitudes()
+    sizes = {"alpha_beta_bytes": 0}
+    for (ei, _, plan, *_rest) in b64s.pairs:
+        # alpha (rows) and beta (cols) of both families, FP64
+        sizes["alpha_beta_bytes"] += 2 * 8 * (plan.rows + plan.cols)
+    del b64s
     result = {
-        "metric": METRIC, "value": n_int / (ms32 * 1e-3), "unit": "integrals/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms32, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "metric": METRIC, "value": n_int / (ms64s * 1e-3), "unit": "integrals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms64s, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "batched precision sweep: 64 energies x 400 channel pairs, per-energy exp grid "
                                "N~U{4000..15000} on [1e-5,1000] a0, L=0..30, direct+exchange, Full-Split "
-                               "(FP64 alpha/beta prep and tails); exchange contraction on FP32 MFMA, direct "
-                               "(one shared beta per energy) FP64 GEMV",
+                               "(FP64 alpha/beta prep and tails); value = FP64 path with the O(N) "
+                               "semiseparable contraction; FP32-MFMA, split-BF16x3 and dense FP64 legs below",
                    "energies": len(scan), "pairs_per_energy": args.pairs, "L_max": L_max,
-                   "integrals_per_step": n_int, "seed": args.seed, "graph": not args.no_graph},
-        "stage_ms": stages,
-        "work_flop": {"direct": wd, "exchange": we},
-        "host_plan_s": b32.plan_seconds, "host_build_s": b32.build_seconds,
+                   "integrals_per_step": n_int, "seed": args.seed, "graph": graph},
+        "stage_ms": st64s,
+        "host_note": "host planning and uploads precede the timed region",
     }
-    t_x = stages.get("xgemm")
-    if t_x:
-        ach = we / (t_x * 1e-3)
-        result["roofline"] = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK_FP32_MFMA / 1e12,
-                              "unit": "TFLOP/s", "frac": ach / PEAK_FP32_MFMA, "traffic": None,
-                              "kernel": "xgemm_kernel<0> (v_mfma_f32_32x32x2_f32)",
-                              "work_flop_per_launch": we, "kernel_ms": t_x}
+    t_c = st64s.get("plain")
+    if t_c:
+        byts = sizes["alpha_beta_bytes"]
+        result["roofline"] = {"bound": "hbm", "achieved": byts / (t_c * 1e-3) / 1e9, "peak": 8000.0,
+                              "unit": "GB/s", "frac": byts / (t_c * 1e-3) / 8e12, "traffic": None,
+                              "kernel": "semisep_kernel<8> + semisep_join (dwba_il_semisep_f64)",
+                              "algorithmic_bytes_per_launch": byts, "kernel_ms": t_c,
+                              "note": "algorithmic bytes = alpha (n_rows) + beta (n_cols) of both families, FP64, "
+                                      "read once; the contraction is O(N) per (pair, L)"}
 
-    resbf = None
+    b32, ms32, st32, res32 = _scan_leg(scan, L_max, "fp32", "dense", args.steps, args.warmup, graph, stream)
+    wd, we = b32.work_flop()
+    del b32
+    log(f"fp32 MFMA: {ms32:.1f} ms/step, stages {st32}")
+    leg32 = {"value": n_int / (ms32 * 1e-3), "unit": "integrals/s", "ms_per_step": ms32, "stage_ms": st32,
+             "work_flop": {"direct": wd, "exchange": we},
+             "rel_err_vs_fp64": _err_stats(res32, res64s, mag)}
+    if st32.get("xgemm"):
+        ach = we / (st32["xgemm"] * 1e-3)
+        leg32["roofline"] = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK_FP32_MFMA / 1e12,
+                             "unit": "TFLOP/s", "frac": ach / PEAK_FP32_MFMA,
+                             "kernel": "xgemm_kernel<0> (v_mfma_f32_32x32x2_f32)", "kernel_ms": st32["xgemm"]}
+    result["fp32_mfma"] = leg32
+    del res32
+
     if not args.no_bf16:
-        del b32
-        bbf = ScanBatch(scan, L_max, precision="bf16x3", oscillatory_method="full_split")
-        bbf.launch()
-        evb = {}
-        bbf.launch(events=evb)
-        torch.cuda.synchronize()
-        st_bf = bbf.stage_ms(evb)
-        if not args.no_graph:
-            bbf.capture()
-        for _ in range(args.warmup):
-            bbf.run()
-        msbf = _events_ms(bbf.run, args.steps, stream)
-        resbf = bbf.results()
-        print(f"[bench] bf16x3: {msbf:.1f} ms/step, stages {st_bf}", file=sys.stderr, flush=True)
-        result["bf16x3"] = {"value": n_int / (msbf * 1e-3), "unit": "integrals/s", "ms_per_step": msbf,
-                            "stage_ms": st_bf,
-                            "xgemm_tflops_effective": (we / (st_bf["xgemm"] * 1e-3) / 1e12) if st_bf.get("xgemm") else None,
-                            "peak_effective_tflops": 2500.0 / 3}
+        bbf, msbf, stbf, resbf = _scan_leg(scan, L_max, "bf16x3", "dense", args.steps, args.warmup, graph, stream)
         del bbf
+        log(f"bf16x3: {msbf:.1f} ms/step, stages {stbf}")
+        result["bf16x3"] = {"value": n_int / (msbf * 1e-3), "unit": "integrals/s", "ms_per_step": msbf,
+                            "stage_ms": stbf,
+                            "xgemm_tflops_effective": (we / (stbf["xgemm"] * 1e-3) / 1e12) if stbf.get("xgemm")
+                            else None, "peak_effective_tflops": 2500.0 / 3,
+                            "rel_err_vs_fp64": _err_stats(resbf, res64s, mag)}
+        del resbf
     if not args.no_fp64:
-        b64 = ScanBatch(scan, L_max, precision="fp64", oscillatory_method="full_split")
-        b64.run()
-        ms64 = _events_ms(b64.run, args.fp64_steps, stream)
-        res64 = b64.results()
-        print(f"[bench] fp64 plain: {ms64:.1f} ms/step", file=sys.stderr, flush=True)
-        mag = b64.magnitudes()
-        result["fp64_plain"] = {"value": n_int / (ms64 * 1e-3), "unit": "integrals/s", "ms_per_step": ms64,
-                                "kernel": "plain_f64_kernel (batched, FP64 VALU)",
-                                "achieved_tflops": (wd + we) / (ms64 * 1e-3) / 1e12}
-        result["speedup_fp32_mfma_vs_fp64_plain"] = ms64 / ms32
-        d = np.abs(res32 - res64)
-        rel = d / np.maximum(np.abs(res64), 1e-300)
-        canc = d / np.maximum(mag, 1e-300)
-        ex_rel, ex_canc = rel[:, 1, :], canc[:, 1, :]
-        if resbf is not None:
-            dbf = np.abs(resbf - res64)
-            rb = dbf / np.maximum(np.abs(res64), 1e-300)
-            cb = dbf / np.maximum(mag, 1e-300)
-            result["bf16x3"]["speedup_vs_fp64_plain"] = ms64 / result["bf16x3"]["ms_per_step"]
-            result["bf16x3"]["rel_err_vs_fp64"] = {
-                "exchange_max_abs_over_sum_abs_terms": float(np.max(cb[:, 1, :])),
-                "exchange_max_abs_over_sum_abs_terms_per_L": [float(x) for x in np.max(cb[:, 1, :], axis=0)],
-                "exchange_median_rel": float(np.median(rb[:, 1, :])),
-                "exchange_frac_rel_le_1e-6": float(np.mean(rb[:, 1, :] <= 1e-6)),
-                "direct_max_abs_over_sum_abs_terms": float(np.max(cb[:, 0, :])),
-            }
-        result["rel_err_vs_fp64"] = {
-            "exchange_max_abs_over_sum_abs_terms": float(np.max(ex_canc)),
-            "exchange_max_abs_over_sum_abs_terms_per_L": [float(x) for x in np.max(ex_canc, axis=0)],
-            "exchange_median_rel": float(np.median(ex_rel)),
-            "exchange_frac_rel_le_1e-6": float(np.mean(ex_rel <= 1e-6)),
-            "exchange_max_rel": float(np.max(ex_rel)),
-            "direct_max_rel": float(np.max(rel[:, 0, :])),
-            "direct_max_abs_over_sum_abs_terms": float(np.max(canc[:, 0, :])),
-            "note": "|dI| / sum_ij |a_i| K_ij |b_j| is the cancellation-aware error; exchange integrals cancelled "
-                    "far below their terms cannot reach 1e-6 relative in FP32 operands",
-        }
-        del b64
+        bd, msd, std, resd = _scan_leg(scan, L_max, "fp64", "dense", args.fp64_steps, 0, graph, stream)
+        del bd
+        log(f"fp64 dense plain: {msd:.1f} ms/step")
+        result["fp64_dense"] = {"value": n_int / (msd * 1e-3), "unit": "integrals/s", "ms_per_step": msd,
+                                "stage_ms": std, "kernel": "plain_f64_kernel (batched, FP64 VALU)",
+                                "achieved_tflops": (wd + we) / (std.get("plain", msd) * 1e-3) / 1e12,
+                                "max_abs_diff_vs_semiseparable_over_sum_abs_terms":
+                                    float(np.max(np.abs(resd - res64s) / np.maximum(mag, 1e-300)))}
+        result["speedups_vs_fp64_dense"] = {"fp64_semiseparable": msd / ms64s, "fp32_mfma": msd / ms32}
+        if "bf16x3" in result:
+            result["speedups_vs_fp64_dense"]["bf16x3"] = msd / result["bf16x3"]["ms_per_step"]
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline_batched(scan, L_max)
     return result
