@@ -79,11 +79,157 @@ __device__ inline double interp_at(const double* xp, int n, double X, int j, dou
     return slope * (X - xp[j]) + f0;
 }
 
+// Largest j in [0, n-2] with xp[j] <= X, X in [xp[0], xp[n-1]] (np.interp's
+// cell).  The start is guessed from a uniform table and corrected by a short
+// walk; a long walk (strongly non-uniform table) falls back to bisection.
+__device__ inline int cell_fast(const double* xp, int n, double X) {
+    if (n <= 2) return 0;
+    const double x0 = xp[0], x1 = xp[n - 1];
+    double f = (X - x0) / (x1 - x0) * (double)(n - 1);
+    int j = f <= 0.0 ? 0 : (f >= (double)(n - 2) ? n - 2 : (int)f);
+    for (int s = 0; s < 16; ++s) {
+        if (xp[j] > X && j > 0) { --j; continue; }
+        if (j < n - 2 && xp[j + 1] <= X) { ++j; continue; }
+        return j;
+    }
+    return cell_of(xp, n, X);
+}
+
+// numpy searchsorted(r, x, 'left') clipped to [1, n-1], guessed in log r.
+__device__ inline int right_col(const double* r, const double* lr, int n, double x) {
+    int lo;   // number of r[j] < x
+    if (!(r[0] < x)) lo = 0;
+    else if (r[n - 1] < x) lo = n;
+    else {
+        // largest j with r[j] < x, in [0, n-2]
+        const double f = (log(x) - lr[0]) / (lr[n - 1] - lr[0]) * (double)(n - 1);
+        int j = f <= 0.0 ? 0 : (f >= (double)(n - 2) ? n - 2 : (int)f);
+        int s = 0;
+        for (; s < 16; ++s) {
+            if (!(r[j] < x) && j > 0) { --j; continue; }
+            if (j < n - 2 && r[j + 1] < x) { ++j; continue; }
+            break;
+        }
+        if (s == 16) {
+            int a = 0, b = n;
+            while (a < b) {
+                const int mid = (a + b) >> 1;
+                if (r[mid] < x) a = mid + 1; else b = mid;
+            }
+            j = a - 1;
+        }
+        lo = j + 1;
+    }
+    return lo < 1 ? 1 : (lo > n - 1 ? n - 1 : lo);
+}
+
 struct CCTable {
     double x[DWBA_MAX_CC];
     double w[DWBA_MAX_CC];
     int n;
 };
+
+constexpr int CH_NODES = 512;    // CC nodes per LDS round
+
+// FILON mode.  Every CC node c (interval m, node k, nodes ascending in x) is
+// evaluated once per pair: its alpha cell ja (np.interp on r[:N1]) with the
+// weights W f1(x) (1-t), W f1(x) t for direct and exchange, and its beta
+// columns il = ir - 1 (searchsorted, linear-in-r kernel interpolation) with
+// W f2(x) wl, W f2(x) wr.  Cells are non-decreasing along the node list, so
+// a grid point finds the nodes that touch it by bisection in LDS and adds
+// them in node order: deterministic, no atomics.
+__device__ void filon_alpha_beta(const dwba_prep_job& J, const CCTable& cc) {
+    __shared__ int s_ja[CH_NODES], s_jb[CH_NODES];
+    __shared__ double s_ad0[CH_NODES], s_ad1[CH_NODES], s_ae0[CH_NODES], s_ae1[CH_NODES];
+    __shared__ double s_b0[CH_NODES], s_b1[CH_NODES];
+    const int N = J.N, N1 = J.N1;
+    const int n_int = J.n_pnodes - 1;
+    const int ncc = cc.n;
+    const double* r = J.r;
+    const double* lr = J.logr;                    // log(max(r, 1e-300))
+    const double* xp1 = J.log1 ? lr : r;          // interpolation table on r[:N1]
+    const double* xp2 = J.log2 ? lr : r;          // interpolation table on r[:N]
+    const double r0 = r[0], r1e = r[N1 - 1], r2e = r[N - 1];
+    const double rlog0 = r0 > 1e-300 ? r0 : 1e-300;
+    const double rho_d_last = clean(J.chi_f[N1 - 1] * J.chi_i[N1 - 1]);
+    const double rho_e_last = clean(J.u_f[N1 - 1] * J.chi_i[N1 - 1]);
+    const double rho2_last = clean(J.chi_f[N - 1] * J.u_i[N - 1]);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        J.alpha_dir[i] = 0.0;
+        J.alpha_ex[i] = 0.0;
+        J.beta_ex[i] = 0.0;
+    }
+    const int per = CH_NODES / ncc;               // intervals per round
+    for (int m0 = 0; m0 < n_int; m0 += per) {
+        const int m_end = min(n_int, m0 + per);
+        const int nq = (m_end - m0) * ncc;
+        __syncthreads();                          // previous round's gathers are done
+        for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+            const int m = m0 + q / ncc, k = q - (q / ncc) * ncc;
+            const double a = phase_node(J, m), b = phase_node(J, m + 1);
+            const double hw = 0.5 * (b - a);
+            // clamped into [a, b] so that x never decreases along the node list
+            const double x = fmin(fmax(hw * (cc.x[k] + 1.0) + a, a), b);
+            const double W = (b - a > 1e-12) ? cc.w[k] * hw : 0.0;
+            // alpha: np.interp of rho1 on r[:N1]
+            const double xc = fmin(fmax(x, r0), r1e);
+            const double X = J.log1 ? log(fmax(xc, rlog0)) : xc;
+            const int j = cell_fast(xp1, N1, X);
+            double t;
+            if (X >= xp1[N1 - 1]) t = 1.0;
+            else if (X == xp1[j]) t = 0.0;
+            else t = (X - xp1[j]) / (xp1[j + 1] - xp1[j]);
+            const double fd0 = clean(J.chi_f[j] * J.chi_i[j]), fd1 = clean(J.chi_f[j + 1] * J.chi_i[j + 1]);
+            const double fe0 = clean(J.u_f[j] * J.chi_i[j]), fe1 = clean(J.u_f[j + 1] * J.chi_i[j + 1]);
+            const double vd = W * interp_at(xp1, N1, X, j, fd0, fd1, rho_d_last);
+            const double ve = W * interp_at(xp1, N1, X, j, fe0, fe1, rho_e_last);
+            s_ja[q] = j;
+            s_ad0[q] = vd * (1.0 - t);
+            s_ad1[q] = vd * t;
+            s_ae0[q] = ve * (1.0 - t);
+            s_ae1[q] = ve * t;
+            // beta_ex: kernel interpolation linear in r between columns il, ir
+            const int ir = right_col(r, lr, N, x);
+            const int il = ir - 1;
+            const double wr = (x - r[il]) / (r[ir] - r[il] + 1e-30);
+            const double wl = 1.0 - wr;
+            const double xc2 = fmin(fmax(x, r0), r2e);
+            const double X2 = J.log2 ? log(fmax(xc2, rlog0)) : xc2;
+            const int j2 = cell_fast(xp2, N, X2);
+            const double f0 = clean(J.chi_f[j2] * J.u_i[j2]), f1 = clean(J.chi_f[j2 + 1] * J.u_i[j2 + 1]);
+            const double v = W * interp_at(xp2, N, X2, j2, f0, f1, rho2_last);
+            s_jb[q] = il;
+            s_b0[q] = v * wl;
+            s_b1[q] = v * wr;
+        }
+        __syncthreads();
+        // gather onto the grid points this round touches
+        const int ja_lo = s_ja[0], ja_hi = s_ja[nq - 1] + 1;
+        const int jb_lo = s_jb[0], jb_hi = s_jb[nq - 1] + 1;
+        for (int i = ja_lo + (int)threadIdx.x; i <= ja_hi; i += blockDim.x) {
+            // nodes with ja == i-1 contribute their t-part, nodes with ja == i their (1-t)-part
+            int lo = 0, hi = nq;                  // first q with ja >= i-1
+            while (lo < hi) { const int mid = (lo + hi) >> 1; if (s_ja[mid] < i - 1) lo = mid + 1; else hi = mid; }
+            double sd = 0.0, se = 0.0;
+            int q = lo;
+            for (; q < nq && s_ja[q] == i - 1; ++q) { sd += s_ad1[q]; se += s_ae1[q]; }
+            for (; q < nq && s_ja[q] == i; ++q) { sd += s_ad0[q]; se += s_ae0[q]; }
+            if (i < N1 && (sd != 0.0 || se != 0.0)) {
+                J.alpha_dir[i] += sd;
+                J.alpha_ex[i] += se;
+            }
+        }
+        for (int i = jb_lo + (int)threadIdx.x; i <= jb_hi; i += blockDim.x) {
+            int lo = 0, hi = nq;
+            while (lo < hi) { const int mid = (lo + hi) >> 1; if (s_jb[mid] < i - 1) lo = mid + 1; else hi = mid; }
+            double sb = 0.0;
+            int q = lo;
+            for (; q < nq && s_jb[q] == i - 1; ++q) sb += s_b1[q];
+            for (; q < nq && s_jb[q] == i; ++q) sb += s_b0[q];
+            if (i < N && sb != 0.0) J.beta_ex[i] += sb;
+        }
+    }
+}
 
 __global__ __launch_bounds__(PREP_THREADS) void prep_f64_kernel(const dwba_prep_job* __restrict__ jobs,
                                                                 CCTable cc) {
@@ -150,82 +296,7 @@ __global__ __launch_bounds__(PREP_THREADS) void prep_f64_kernel(const dwba_prep_
     }
 
     // FILON
-    const int n_int = J.n_pnodes - 1;
-    const double* r = J.r;
-    const double* lr = J.logr;                    // log(max(r, 1e-300))
-    const double* xp1 = J.log1 ? lr : r;          // interpolation table on r[:N1]
-    const double* xp2 = J.log2 ? lr : r;          // interpolation table on r[:N]
-    const double r0 = r[0], r1e = r[N1 - 1], r2e = r[N - 1];
-    const double rho_d_last = clean(J.chi_f[N1 - 1] * J.chi_i[N1 - 1]);
-    const double rho_e_last = clean(J.u_f[N1 - 1] * J.chi_i[N1 - 1]);
-    const double rho2_last = clean(J.chi_f[N - 1] * J.u_i[N - 1]);
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        double ad = 0.0, ae = 0.0, be = 0.0;
-        if (i <= N1) {
-            const double lo_x = r[i > 0 ? i - 1 : 0];
-            const double hi_x = r[i + 1 < N ? i + 1 : N - 1];
-            int m0 = interval_of(J, lo_x, n_int) - 1;
-            int m1 = interval_of(J, hi_x, n_int) + 1;
-            if (m0 < 0) m0 = 0;
-            if (m1 > n_int - 1) m1 = n_int - 1;
-            for (int m = m0; m <= m1; ++m) {
-                const double a = phase_node(J, m), b = phase_node(J, m + 1);
-                if (!(b - a > 1e-12)) continue;
-                const double hw = 0.5 * (b - a);
-                for (int k = 0; k < cc.n; ++k) {
-                    const double x = hw * (cc.x[k] + 1.0) + a;
-                    const double W = cc.w[k] * hw;
-                    // ---- alpha: np.interp on r[:N1] ----
-                    if (i < N1) {
-                        const double xc = fmin(fmax(x, r0), r1e);
-                        const double X = J.log1 ? log(fmax(xc, r0 > 1e-300 ? r0 : 1e-300)) : xc;
-                        const int j = cell_of(xp1, N1, X);
-                        double t;
-                        if (X >= xp1[N1 - 1]) t = 1.0;
-                        else if (X == xp1[j]) t = 0.0;
-                        else t = (X - xp1[j]) / (xp1[j + 1] - xp1[j]);
-                        double lam = 0.0;
-                        if (j == i) lam = 1.0 - t;
-                        else if (j + 1 == i) lam = t;
-                        if (lam != 0.0) {
-                            const double fd0 = clean(J.chi_f[j] * J.chi_i[j]);
-                            const double fd1 = clean(J.chi_f[j + 1] * J.chi_i[j + 1]);
-                            const double fe0 = clean(J.u_f[j] * J.chi_i[j]);
-                            const double fe1 = clean(J.u_f[j + 1] * J.chi_i[j + 1]);
-                            ad += (W * interp_at(xp1, N1, X, j, fd0, fd1, rho_d_last)) * lam;
-                            ae += (W * interp_at(xp1, N1, X, j, fe0, fe1, rho_e_last)) * lam;
-                        }
-                    }
-                    // ---- beta_ex: kernel interpolation linear in r on r[:N] ----
-                    {
-                        int ir;  // searchsorted(r, x, 'left') clipped to [1, N-1]
-                        int lo = 0, hi = N;
-                        while (lo < hi) {
-                            const int mid = (lo + hi) >> 1;
-                            if (r[mid] < x) lo = mid + 1; else hi = mid;
-                        }
-                        ir = lo < 1 ? 1 : (lo > N - 1 ? N - 1 : lo);
-                        const int il = ir - 1;
-                        if (il == i || ir == i) {
-                            const double wr = (x - r[il]) / (r[ir] - r[il] + 1e-30);
-                            const double wl = 1.0 - wr;
-                            const double xc = fmin(fmax(x, r0), r2e);
-                            const double X = J.log2 ? log(fmax(xc, r0 > 1e-300 ? r0 : 1e-300)) : xc;
-                            const int j = cell_of(xp2, N, X);
-                            const double f0 = clean(J.chi_f[j] * J.u_i[j]);
-                            const double f1 = clean(J.chi_f[j + 1] * J.u_i[j + 1]);
-                            const double v = W * interp_at(xp2, N, X, j, f0, f1, rho2_last);
-                            if (il == i) be += v * wl;
-                            if (ir == i) be += v * wr;
-                        }
-                    }
-                }
-            }
-        }
-        J.alpha_dir[i] = ad;
-        J.alpha_ex[i] = ae;
-        J.beta_ex[i] = be;
-    }
+    filon_alpha_beta(J, cc);
 }
 
 // Multipole moments: block (set, L) reduces sum_j ((w_j r_j^L) u_f,j) u_i,j.
@@ -289,7 +360,11 @@ int launch_prep(int n_jobs, dwba_prep_job* d_jobs, const double* d_coulomb_par, 
                 const double* cc_x, const double* cc_w, int n_cc, hipStream_t s) {
     CCTable cc{};
     if (n_cc < 2 || n_cc > DWBA_MAX_CC) return set_error(DWBA_ERR_ARG, "prep: CC node count %d outside [2, %d]", n_cc, DWBA_MAX_CC);
-    for (int k = 0; k < n_cc; ++k) { cc.x[k] = cc_x[k]; cc.w[k] = cc_w[k]; }
+    // nodes ascending in x (the FILON gather relies on it; the sum order is free)
+    int ord[DWBA_MAX_CC];
+    for (int k = 0; k < n_cc; ++k) ord[k] = k;
+    std::sort(ord, ord + n_cc, [&](int p, int q) { return cc_x[p] < cc_x[q]; });
+    for (int k = 0; k < n_cc; ++k) { cc.x[k] = cc_x[ord[k]]; cc.w[k] = cc_w[ord[k]]; }
     cc.n = n_cc;
     if (any_coulomb) {
         hipLaunchKernelGGL(coulomb_nodes_kernel, dim3((n_jobs + 63) / 64), dim3(64), 0, s, d_jobs, n_jobs, d_coulomb_par);
