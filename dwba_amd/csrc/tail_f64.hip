@@ -360,41 +360,57 @@ __device__ double analytic_tail(double r_m, const dwba_tail_job& J, int L, doubl
     return tail * overlap;
 }
 
-// Filon segments of one (pair, sign), all L: lanes take segments
-// round-robin; NE (odd, 5..15) nodes per segment are register resident.
-template <int NL, int NE>
-__device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, double om, double p0, double r_m, double r_max,
-                                            double bstep, long nseg, int lane, DD* acc) {
-    constexpr int NT = (NE - 1) / 2;
+// Filon segments of one (pair, sign), all L.  Lanes take segments
+// round-robin; a segment of neff (odd) equally spaced nodes is walked triple
+// by triple (the recursion of _filon_segment_complex), so only three nodes
+// are live at a time whatever neff is.  Per triple the exponentials and the
+// Filon coefficients are formed once and all L are swept with x^-(L+1) built
+// by repeated multiplication; each triple's real part is added to a per-L
+// double-double sum.
+template <int NL>
+__device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, double om, double p0, double r_m,
+                                            double r_max, double bstep, long nseg, int lane, DD* acc) {
     const int Lm = J.L_max < NL - 1 ? J.L_max : NL - 1;
+    const int nt = (neff - 1) / 2;
     const double inv_om = -1.0 / om;
     for (long sg = lane; sg < nseg; sg += 64) {
         const double a = __dadd_rn(__dmul_rn((double)sg, bstep), r_m);
         const double b = (sg + 1 == nseg) ? r_max : __dadd_rn(__dmul_rn((double)(sg + 1), bstep), r_m);
-        double x[NE], c[NE], s[NE], ix[NE], px[NE];
-        const double step = (b - a) / (NE - 1);
-#pragma unroll
-        for (int k = 0; k < NE; ++k) {
-            x[k] = (k == NE - 1) ? b : __dadd_rn(__dmul_rn((double)k, step), a);
-            ix[k] = x[k] > 1e-6 ? 1.0 / x[k] : 0.0;
-            px[k] = ix[k] * ix[k];           // x^-(L+1) at L = 1
-        }
-        const double hseg = x[NE - 1] - x[0];
+        const double step = (b - a) / (neff - 1);
+        const double hseg = b - a;
         if (hseg < 1e-15) continue;
         const bool small = fabs(om * hseg) < 1e-3;
-        double cA[NT], cB[NT], cG[NT], Hh[NT];
-        bool tsmall[NT];
-        double cmid = 0.0;
-        if (small) {
-            cmid = cos(om * (x[0] + x[NE - 1]) / 2 + p0);
-        } else {
-#pragma unroll
-            for (int k = 0; k < NE; ++k) sincos(om * x[k] + p0, &s[k], &c[k]);
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const double h3 = x[2 * t + 2] - x[2 * t];
-                Hh[t] = h3 / 2;
-                tsmall[t] = fabs(om * h3) < 1e-3 || h3 < 1e-15;
+        const double cmid = small ? cos(om * (a + b) / 2 + p0) : 0.0;
+        double x0 = a;
+        double ix0 = x0 > 1e-6 ? 1.0 / x0 : 0.0;
+        double s0 = 0.0, c0 = 0.0;
+        if (!small) sincos(om * x0 + p0, &s0, &c0);
+#pragma unroll 1
+        for (int t = 0; t < nt; ++t) {
+            const int k1 = 2 * t + 1, k2 = 2 * t + 2;
+            const double x1 = __dadd_rn(__dmul_rn((double)k1, step), a);
+            const double x2 = (k2 == neff - 1) ? b : __dadd_rn(__dmul_rn((double)k2, step), a);
+            const double ix1 = x1 > 1e-6 ? 1.0 / x1 : 0.0;
+            const double ix2 = x2 > 1e-6 ? 1.0 / x2 : 0.0;
+            const double h3 = x2 - x0;
+            // per-triple weights: re = wa f0 + wm f1 + wb f2 for every L
+            double wa, wm, wb;
+            if (small || fabs(om * h3) < 1e-3 || h3 < 1e-15) {
+                // simpson_odd on the triple times cos of the mid phase (of the
+                // segment when the whole segment is non-oscillatory)
+                const double wscale = small ? cmid : (h3 < 1e-15 ? 0.0 : cos(om * (x0 + x2) / 2 + p0));
+                const double h0 = x1 - x0, h1 = x2 - x1;
+                const double hsum = h0 + h1, hprod = h0 * h1;
+                const double h0divh1 = h1 != 0.0 ? h0 / h1 : 0.0;
+                const double q = hsum / 6.0;
+                wa = q * (2.0 - (h0divh1 != 0.0 ? 1.0 / h0divh1 : 0.0)) * wscale;
+                wm = q * (hsum * (hprod != 0.0 ? hsum / hprod : 0.0)) * wscale;
+                wb = q * (2.0 - h0divh1) * wscale;
+                if (!small) sincos(om * x2 + p0, &s0, &c0);   // node x2 starts the next triple
+            } else {
+                double s1, c1, s2, c2;
+                sincos(om * x1 + p0, &s1, &c1);
+                sincos(om * x2 + p0, &s2, &c2);
                 const double th = om * h3 / 2;
                 double al, be, ga;
                 if (fabs(th) < 0.3) {
@@ -403,77 +419,61 @@ __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, double om, d
                     be = 2.0 / 3 + 2 * t2 / 15 - 4 * (t2 * t2) / 105 + 2 * (t2 * t2 * t2) / 567;
                     ga = 4.0 / 3 - 2 * t2 / 15 + 1 * (t2 * t2) / 210 - (t2 * t2 * t2) / 11340;
                 } else {
-                    double st, ct, s2, c2;
+                    double st, ct, sd, cd;
                     sincos(th, &st, &ct);
-                    sincos(2 * th, &s2, &c2);
+                    sincos(2 * th, &sd, &cd);
                     const double t2 = th * th, t3 = th * th * th;
-                    al = (t2 + th * s2 / 2 - 2 * (st * st)) / t3;
-                    be = 2 * (th * (1 + c2) - s2) / t3;
+                    al = (t2 + th * sd / 2 - 2 * (st * st)) / t3;
+                    be = 2 * (th * (1 + cd) - sd) / t3;
                     ga = 4 * (st - th * ct) / t3;
                 }
-                cA[t] = al * inv_om;     // Re[alpha (-i/omega) D] = -(cA) Im D
-                cB[t] = be;
-                cG[t] = ga;
+                // Re[(h/2)(al (-i/om)(f2 e2 - f0 e0) + be (f0 e0 + f2 e2) + ga f1 e1)]
+                const double hh = h3 / 2, cA = al * inv_om;
+                wa = hh * (cA * s0 + be * c0);
+                wb = hh * (-cA * s2 + be * c2);
+                wm = hh * (ga * c1);
+                s0 = s2; c0 = c2;
             }
-        }
+            double p0x = ix0 * ix0, p1x = ix1 * ix1, p2x = ix2 * ix2;   // x^-(L+1) at L = 1
 #pragma unroll
-        for (int L = 1; L < NL; ++L) {
-            const double M = (L <= Lm) ? J.moments[L] : 0.0;
-            if (fabs(M) > 1e-12) {
-                double f[NE];
-#pragma unroll
-                for (int k = 0; k < NE; ++k) f[k] = x[k] > 1e-6 ? M * px[k] : 0.0;
-                double re = 0.0;
-                if (small) {
-                    re = simpson_odd(f, x, NE) * cmid;
-                } else {
-#pragma unroll
-                    for (int t = 0; t < NT; ++t) {
-                        const int i0 = 2 * t;
-                        double v;
-                        if (tsmall[t]) {
-                            const double h3 = x[i0 + 2] - x[i0];
-                            v = h3 < 1e-15 ? 0.0
-                                           : simpson_odd(f + i0, x + i0, 3) * cos(om * (x[i0] + x[i0 + 2]) / 2 + p0);
-                        } else {
-                            const double dim = f[i0 + 2] * s[i0 + 2] - f[i0] * s[i0];
-                            const double bre = f[i0] * c[i0] + f[i0 + 2] * c[i0 + 2];
-                            v = Hh[t] * ((-cA[t] * dim + cB[t] * bre) + cG[t] * f[i0 + 1] * c[i0 + 1]);
-                        }
-                        re += v;
+            for (int L = 1; L < NL; ++L) {
+                if (L <= Lm) {
+                    const double M = J.moments[L];
+                    if (fabs(M) > 1e-12) {
+                        const double f0 = x0 > 1e-6 ? M * p0x : 0.0;
+                        const double f1 = x1 > 1e-6 ? M * p1x : 0.0;
+                        const double f2 = x2 > 1e-6 ? M * p2x : 0.0;
+                        acc[L] = dd_add(acc[L], (wa * f0 + wm * f1) + wb * f2);
                     }
                 }
-                acc[L] = dd_add(acc[L], re);
+                p0x *= ix0;
+                p1x *= ix1;
+                p2x *= ix2;
             }
-#pragma unroll
-            for (int k = 0; k < NE; ++k) px[k] *= ix[k];
+            x0 = x2;
+            ix0 = ix2;
         }
     }
 }
 
-// ---------------------------------------------------------------------------
-// All-L form: the phase functions do not depend on L (only the envelope
-// M_L / r^(L+1) does), so one wavefront per (pair, sign) evaluates the nodes,
-// the exponentials and the Filon coefficients once per segment and sweeps all
-// L with x^-(L+1) built by repeated multiplication.  Only Re J is needed
-// (dwba_outer_integral_1d keeps the real parts), so each lane keeps per-L
-// double-double sums of the real parts.  Levin / non-oscillatory signs fall
-// back to the per-L routines above.
-template <int NL>
-__global__ __launch_bounds__(64) void tail_allL_kernel(const dwba_tail_job* __restrict__ jobs, TailParams TP,
-                                                       double* __restrict__ jre) {
-    const int job = blockIdx.x, sgn = blockIdx.y == 0 ? -1 : 1;
-    const dwba_tail_job J = jobs[job];
-    const int lane = threadIdx.x;
+// Phases and the Filon / Levin choice of compute_outer_integral_oscillatory
+// for one (job, sign).
+struct TailSetup {
+    Phase P;
+    double om;
+    int kind;     // 0: nothing to do (zeros), 1: Filon, 2: Levin or non-oscillatory
+};
+
+__device__ inline TailSetup tail_setup(const dwba_tail_job& J, int sgn, int NL) {
+    TailSetup T{};
     const int Lm = J.L_max < NL - 1 ? J.L_max : NL - 1;
-    double* out = jre + ((size_t)job * 2 + blockIdx.y) * NL;
     bool any = false;
     for (int L = 1; L <= Lm; ++L) any |= (J.moments && fabs(J.moments[L]) > 1e-12);
     if (J.kind != DWBA_TAIL_OUTER || !any || J.r_max <= J.r_m + 1e-10) {
-        for (int L = lane; L < NL; L += 64) out[L] = 0.0;
-        return;
+        T.kind = 0;
+        return T;
     }
-    Phase P;
+    Phase& P = T.P;
     P.ka = J.k_a; P.kb = J.k_b; P.ea = J.eta_a; P.eb = J.eta_b;
     P.la = (double)J.l_a; P.lb = (double)J.l_b;
     P.use_ea = fabs(J.eta_a) > 1e-15; P.use_eb = fabs(J.eta_b) > 1e-15;
@@ -482,35 +482,44 @@ __global__ __launch_bounds__(64) void tail_allL_kernel(const dwba_tail_job* __re
     const double pa = -J.l_a * PI / 2 + J.sigma_a + J.delta_a;
     const double pb = -J.l_b * PI / 2 + J.sigma_b + J.delta_b;
     P.cmin = pa - pb; P.cplus = pa + pb;
-    const double r_m = J.r_m, r_max = J.r_max;
-    const double delta_phi = PI / 4;
-    const double mid = (r_m + r_max) / 2;
-    const double om = fabs(dphi(P, sgn, mid));
+    const double mid = (J.r_m + J.r_max) / 2;
+    T.om = fabs(dphi(P, sgn, mid));
     bool filon = false;
-    if (!(om < 1e-10)) {
-        const double h = delta_phi / om;
+    if (!(T.om < 1e-10)) {
+        const double h = (PI / 4) / T.om;
         filon = fabs(d2phi(P, sgn, mid)) * h * h < 0.1;
     }
-    if (!filon) {
-        // Levin or non-oscillatory: per-L evaluation
-        for (int L = 1; L <= Lm; ++L) {
-            const double M = J.moments[L];
-            double v = 0.0;
-            if (fabs(M) > 1e-12) {
-                const Env E{M, L + 1};
-                v = outer_integral(E, P, sgn, r_m, r_max, TP, lane).re;
-            }
-            if (lane == 0) out[L] = v;
-        }
-        if (lane == 0) out[0] = 0.0;
-        for (int L = Lm + 1 + lane; L < NL; L += 64) out[L] = 0.0;
+    T.kind = filon ? 1 : 2;
+    return T;
+}
+
+// ---------------------------------------------------------------------------
+// Filon signs, all L: the phase functions do not depend on L (only the
+// envelope M_L / r^(L+1) does), so one wavefront per (pair, sign) evaluates
+// nodes, exponentials and Filon coefficients once and sweeps all L.  Only
+// Re J is needed (dwba_outer_integral_1d keeps the real parts).  Writes the
+// zero rows too; Levin / non-oscillatory signs are left to tail_levin_kernel.
+template <int NL>
+__global__ __launch_bounds__(64) void tail_filon_kernel(const dwba_tail_job* __restrict__ jobs, TailParams TP,
+                                                        double* __restrict__ jre) {
+    const int job = blockIdx.x, sgn = blockIdx.y == 0 ? -1 : 1;
+    const dwba_tail_job J = jobs[job];
+    const int lane = threadIdx.x;
+    const int Lm = J.L_max < NL - 1 ? J.L_max : NL - 1;
+    double* out = jre + ((size_t)job * 2 + blockIdx.y) * NL;
+    const TailSetup T = tail_setup(J, sgn, NL);
+    if (T.kind == 0) {
+        for (int L = lane; L < NL; L += 64) out[L] = 0.0;
         return;
     }
+    if (T.kind != 1) return;
+    const double om = T.om;
+    const double r_m = J.r_m, r_max = J.r_max;
+    const double delta_phi = PI / 4;
     DD acc[NL];
 #pragma unroll
     for (int L = 0; L < NL; ++L) acc[L] = DD{0.0, 0.0};
-
-    const double p0 = phi(P, sgn, r_m) - om * r_m;
+    const double p0 = phi(T.P, sgn, r_m) - om * r_m;
     const double dr_seg = delta_phi / fabs(om);
     const double nraw_d = ceil((r_max - r_m) / dr_seg);
     const long nraw = nraw_d < 1.0 ? 1 : (long)nraw_d;
@@ -524,19 +533,35 @@ __global__ __launch_bounds__(64) void tail_allL_kernel(const dwba_tail_job* __re
         if (neff > 15) neff = 15;
     }
     const double bstep = (r_max - r_m) / (double)nseg;
-    switch (neff) {
-        case 5: filon_sweep<NL, 5>(J, om, p0, r_m, r_max, bstep, nseg, lane, acc); break;
-        case 7: filon_sweep<NL, 7>(J, om, p0, r_m, r_max, bstep, nseg, lane, acc); break;
-        case 9: filon_sweep<NL, 9>(J, om, p0, r_m, r_max, bstep, nseg, lane, acc); break;
-        case 11: filon_sweep<NL, 11>(J, om, p0, r_m, r_max, bstep, nseg, lane, acc); break;
-        case 13: filon_sweep<NL, 13>(J, om, p0, r_m, r_max, bstep, nseg, lane, acc); break;
-        default: filon_sweep<NL, 15>(J, om, p0, r_m, r_max, bstep, nseg, lane, acc); break;
-    }
+    filon_sweep<NL>(J, neff, om, p0, r_m, r_max, bstep, nseg, lane, acc);
 #pragma unroll
     for (int L = 0; L < NL; ++L) {
         const DD t = dd_wave_sum(acc[L]);
         if (lane == 0) out[L] = (L >= 1 && L <= Lm) ? t.hi + t.lo : 0.0;
     }
+}
+
+// Levin and non-oscillatory signs, per L (outer_integral above).
+__global__ __launch_bounds__(64) void tail_levin_kernel(const dwba_tail_job* __restrict__ jobs, TailParams TP,
+                                                        int NL, double* __restrict__ jre) {
+    const int job = blockIdx.x, sgn = blockIdx.y == 0 ? -1 : 1;
+    const dwba_tail_job J = jobs[job];
+    const int lane = threadIdx.x;
+    const TailSetup T = tail_setup(J, sgn, NL);
+    if (T.kind != 2) return;
+    const int Lm = J.L_max < NL - 1 ? J.L_max : NL - 1;
+    double* out = jre + ((size_t)job * 2 + blockIdx.y) * NL;
+    for (int L = 1; L <= Lm; ++L) {
+        const double M = J.moments[L];
+        double v = 0.0;
+        if (fabs(M) > 1e-12) {
+            const Env E{M, L + 1};
+            v = outer_integral(E, T.P, sgn, J.r_m, J.r_max, TP, lane).re;
+        }
+        if (lane == 0) out[L] = v;
+    }
+    if (lane == 0) out[0] = 0.0;
+    for (int L = Lm + 1 + lane; L < NL; L += 64) out[L] = 0.0;
 }
 
 // tail[L] = 1/2 (Re J_- - Re J_+) + analytic tail at r_max  (or the legacy analytic tail)
@@ -603,12 +628,14 @@ extern "C" int32_t dwba_outer_tail_f64(int32_t n_jobs, const dwba_tail_job* d_jo
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     double* jre = reinterpret_cast<double*>(d_workspace);
     switch (nl) {
-#define C(n) case n: hipLaunchKernelGGL(dwba::tail_allL_kernel<n>, dim3(n_jobs, 2), dim3(64), 0, s, d_jobs, tp, jre); break;
+#define C(n) case n: hipLaunchKernelGGL(dwba::tail_filon_kernel<n>, dim3(n_jobs, 2), dim3(64), 0, s, d_jobs, tp, jre); break;
         C(2) C(4) C(8) C(16) C(24) C(32) C(48) C(64)
 #undef C
         default: return dwba::set_error(DWBA_ERR_ARG, "dwba_outer_tail_f64: L_max %d", L_max);
     }
-    DWBA_LAUNCH_CHECK("tail_allL_kernel");
+    DWBA_LAUNCH_CHECK("tail_filon_kernel");
+    hipLaunchKernelGGL(dwba::tail_levin_kernel, dim3(n_jobs, 2), dim3(64), 0, s, d_jobs, tp, nl, jre);
+    DWBA_LAUNCH_CHECK("tail_levin_kernel");
     hipLaunchKernelGGL(dwba::tail_combine_kernel, dim3(n_jobs), dim3(64), 0, s, d_jobs, n_jobs, nl, jre);
     DWBA_LAUNCH_CHECK("tail_combine_kernel");
     return DWBA_OK;
