@@ -30,6 +30,7 @@ namespace {
 
 constexpr int ST = 256;          // threads per workgroup (4 tasks)
 constexpr int NSUM = 6;          // per (task, L): total, Ca, Cb, A_out, Z_out, M
+constexpr int GU = 8;            // grid points per load group
 
 __device__ inline int find_job(const int* __restrict__ off, int n_jobs, int t) {
     int lo = 0, hi = n_jobs - 1;
@@ -40,13 +41,16 @@ __device__ inline int find_job(const int* __restrict__ off, int n_jobs, int t) {
     return lo;
 }
 
-__device__ inline double pow_int(double x, int e) {
-    double r = 1.0;
-    while (e > 0) {
-        if (e & 1) r *= x;
-        x *= x;
-        e >>= 1;
-    }
+// q^L0 for L0 a multiple of LB (LB a power of two): LB-th power by squaring,
+// then L0/LB - 1 multiplications; every trip count is uniform (no selects).
+template <int LB>
+__device__ inline double pow_block(double q, int L0) {
+    if (L0 == 0) return 1.0;
+    double qb = q;
+#pragma unroll
+    for (int m = 1; m < LB; m <<= 1) qb *= qb;
+    double r = qb;
+    for (int k = LB; k < L0; k += LB) r *= qb;
     return r;
 }
 
@@ -57,7 +61,75 @@ __device__ inline int tasks_of(int n_cols, int seg) {
     return 2 * ((n_cols + span - 1) / span);
 }
 
-template <int LB>
+// One grid point of the recurrences for LB multipoles L0..L0+LB-1.
+// HAS_A: alpha_i may be non-zero (i < n_rows); beyond the rows only the beta
+// recurrence and the decay of Z remain.
+template <int LB, bool HAS_A>
+__device__ __forceinline__ void ss_point(double ri, double r_next, double a_i, double b_i, int L0, const double* dL,
+                                         double* A, double* Z, double* acc, double* Ca, double* Cb, double* P,
+                                         double& ir_cur) {
+    const double iri = ir_cur;
+    const double ir_next = 1.0 / r_next;
+    const double q = ri * ir_next;
+    const double bw = b_i * iri;
+    double qL = pow_block<LB>(q, L0);
+    if (HAS_A) {
+        const double aw = a_i * iri, ab = a_i * bw;
+#pragma unroll
+        for (int l = 0; l < LB; ++l) {
+            acc[l] = fma(aw, A[l], acc[l]);
+            acc[l] = fma(bw, Z[l], acc[l]);
+            acc[l] = fma(dL[l], ab, acc[l]);
+            Ca[l] = fma(aw, P[l], Ca[l]);
+            Cb[l] = fma(bw, P[l], Cb[l]);
+            A[l] = qL * (A[l] + b_i);
+            Z[l] = qL * (Z[l] + a_i);
+            P[l] *= qL;
+            qL *= q;
+        }
+    } else {
+#pragma unroll
+        for (int l = 0; l < LB; ++l) {
+            acc[l] = fma(bw, Z[l], acc[l]);
+            Cb[l] = fma(bw, P[l], Cb[l]);
+            A[l] = qL * (A[l] + b_i);
+            Z[l] *= qL;
+            P[l] *= qL;
+            qL *= q;
+        }
+    }
+    ir_cur = ir_next;
+}
+
+// Points [i0, i1) of one lane: groups of GU with all loads issued first,
+// then single points.
+template <int LB, bool HAS_A>
+__device__ __forceinline__ void ss_range(const double* __restrict__ r, const double* __restrict__ al,
+                                         const double* __restrict__ be, int n, int i0, int i1, int L0,
+                                         const double* dL, double* A, double* Z, double* acc, double* Ca, double* Cb,
+                                         double* P, double& ir_cur) {
+    int i = i0;
+#pragma unroll 1
+    for (; i + GU <= i1; i += GU) {
+        double rr[GU + 1], av[GU], bv[GU];
+        rr[0] = r[i];
+#pragma unroll
+        for (int u = 0; u < GU; ++u) {
+            rr[u + 1] = r[min(i + u + 1, n - 1)];
+            av[u] = HAS_A ? al[i + u] : 0.0;
+            bv[u] = be[i + u];
+        }
+#pragma unroll
+        for (int u = 0; u < GU; ++u)
+            ss_point<LB, HAS_A>(rr[u], rr[u + 1], av[u], bv[u], L0, dL, A, Z, acc, Ca, Cb, P, ir_cur);
+    }
+#pragma unroll 1
+    for (; i < i1; ++i)
+        ss_point<LB, HAS_A>(r[i], r[min(i + 1, n - 1)], HAS_A ? al[i] : 0.0, be[i], L0, dL, A, Z, acc, Ca, Cb, P,
+                            ir_cur);
+}
+
+template <int LB, int NB>
 __global__ __launch_bounds__(ST) void semisep_kernel(const dwba_plain_job* __restrict__ jobs,
                                                      const int* __restrict__ task_off, int n_jobs, int seg,
                                                      int L_max, double* __restrict__ summ) {
@@ -70,6 +142,7 @@ __global__ __launch_bounds__(ST) void semisep_kernel(const dwba_plain_job* __res
     const int w_per_fam = tasks_of(J.n_cols, seg) / 2;
     const int fam = local / w_per_fam;
     const int wt = local - fam * w_per_fam;
+    if (J.n_rows <= 0) return;            // the join writes 0 for such jobs
     const double* __restrict__ al = fam ? J.alpha_ex : J.alpha_dir;
     const double* __restrict__ be = fam ? J.beta_ex : J.beta_dir;
     const int n = J.n_cols;
@@ -84,7 +157,9 @@ __global__ __launch_bounds__(ST) void semisep_kernel(const dwba_plain_job* __res
     const double ldl = log(1.0 - 1e-12);
     double* __restrict__ out = summ + (size_t)t * NL * NSUM;
 
-    for (int L0 = 0; L0 < NL; L0 += LB) {
+#pragma unroll
+    for (int blk = 0; blk < NB; ++blk) {
+        const int L0 = blk * LB;             // a constant after unrolling: q^L0 is straight-line code
         double A[LB], Z[LB], acc[LB], Ca[LB], Cb[LB], P[LB], dL[LB];
 #pragma unroll
         for (int l = 0; l < LB; ++l) {
@@ -92,31 +167,10 @@ __global__ __launch_bounds__(ST) void semisep_kernel(const dwba_plain_job* __res
             const int L = L0 + l;
             dL[l] = L == 0 ? 1.0 : exp((double)L * ldl);
         }
-        double r_next = a < n ? J.r[a] : 1.0;
-        double ir_next = 1.0 / r_next;
-#pragma unroll 1
-        for (int i = a; i < b; ++i) {
-            const double ri = r_next, iri = ir_next;
-            r_next = i + 1 < n ? J.r[i + 1] : ri;
-            ir_next = 1.0 / r_next;
-            const double q = ri * ir_next;
-            const double av = i < nr ? al[i] : 0.0;
-            const double bv = be[i];
-            const double aw = av * iri, bw = bv * iri, ab = av * bw;
-            double qL = pow_int(q, L0);
-#pragma unroll
-            for (int l = 0; l < LB; ++l) {
-                acc[l] = fma(aw, A[l], acc[l]);
-                acc[l] = fma(bw, Z[l], acc[l]);
-                acc[l] = fma(dL[l], ab, acc[l]);
-                Ca[l] = fma(aw, P[l], Ca[l]);
-                Cb[l] = fma(bw, P[l], Cb[l]);
-                A[l] = qL * (A[l] + bv);
-                Z[l] = qL * (Z[l] + av);
-                P[l] *= qL;
-                qL *= q;
-            }
-        }
+        double ir_cur = a < n ? 1.0 / J.r[a] : 1.0;
+        const int am = min(b, max(a, nr));               // [a, am): alpha may be non-zero
+        ss_range<LB, true>(J.r, al, be, n, a, am, L0, dL, A, Z, acc, Ca, Cb, P, ir_cur);
+        ss_range<LB, false>(J.r, al, be, n, am, b, L0, dL, A, Z, acc, Ca, Cb, P, ir_cur);
         // join the lanes: inclusive scan of (M, A, Z) over lanes, M = P at lane end
 #pragma unroll
         for (int l = 0; l < LB; ++l) {
@@ -263,8 +317,13 @@ int32_t dwba_il_semisep_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const 
     DWBA_LAUNCH_CHECK("semisep_plan");
     if (lay.n_tasks > 0) {
         const int blocks = (lay.n_tasks + ST / 64 - 1) / (ST / 64);
-        hipLaunchKernelGGL(semisep_kernel<8>, dim3(blocks), dim3(ST), 0, s, d_jobs, d_off, n_jobs, lay.seg, L_max,
-                           d_summ);
+        switch ((L_max + 8) / 8) {
+#define C(nb) case nb: hipLaunchKernelGGL((semisep_kernel<8, nb>), dim3(blocks), dim3(ST), 0, s, d_jobs, d_off, \
+                                          n_jobs, lay.seg, L_max, d_summ); break;
+            C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8)
+#undef C
+            default: return set_error(DWBA_ERR_ARG, "dwba_il_semisep_f64: L_max %d", L_max);
+        }
         DWBA_LAUNCH_CHECK("semisep_kernel");
     }
     hipLaunchKernelGGL(semisep_join, dim3(n_jobs, 2), dim3(64), 0, s, d_jobs, d_off, lay.seg, L_max, d_summ, d_out);
