@@ -17,8 +17,9 @@
 // sinA*sinB phases of _get_dwba_outer_phase_functions (:350-456); T is
 // _analytical_multipole_tail (:2064-2184).  The reference sums segment
 // contributions with math.fsum; here each lane keeps a double-double
-// (TwoSum) accumulator and the lanes are combined in double-double, which
-// rounds to the same value except in pathological cancellation.
+// (TwoSum) accumulator (Levin path) or an FP64 partial sum (Filon path), and
+// the lanes are combined in double-double; the result agrees with math.fsum
+// to ~1e-15 of the segment size.
 //
 // Lanes of the wavefront take segments round-robin.
 #include "common.hpp"
@@ -365,11 +366,13 @@ __device__ double analytic_tail(double r_m, const dwba_tail_job& J, int L, doubl
 // by triple (the recursion of _filon_segment_complex), so only three nodes
 // are live at a time whatever neff is.  Per triple the exponentials and the
 // Filon coefficients are formed once and all L are swept with x^-(L+1) built
-// by repeated multiplication; each triple's real part is added to a per-L
-// double-double sum.
+// by repeated multiplication.  Each lane adds its triples' real parts to
+// per-L FP64 partial sums (a few hundred terms of similar size); the lanes are
+// combined in double-double.  The reference sums segments with math.fsum;
+// the two agree to ~1e-15 of the segment size.
 template <int NL>
 __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, double om, double p0, double r_m,
-                                            double r_max, double bstep, long nseg, int lane, DD* acc) {
+                                            double r_max, double bstep, long nseg, int lane, double* acc) {
     const int Lm = J.L_max < NL - 1 ? J.L_max : NL - 1;
     const int nt = (neff - 1) / 2;
     const double inv_om = -1.0 / om;
@@ -443,7 +446,7 @@ __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, do
                         const double f0 = x0 > 1e-6 ? M * p0x : 0.0;
                         const double f1 = x1 > 1e-6 ? M * p1x : 0.0;
                         const double f2 = x2 > 1e-6 ? M * p2x : 0.0;
-                        acc[L] = dd_add(acc[L], (wa * f0 + wm * f1) + wb * f2);
+                        acc[L] += (wa * f0 + wm * f1) + wb * f2;
                     }
                 }
                 p0x *= ix0;
@@ -516,9 +519,9 @@ __global__ __launch_bounds__(64) void tail_filon_kernel(const dwba_tail_job* __r
     const double om = T.om;
     const double r_m = J.r_m, r_max = J.r_max;
     const double delta_phi = PI / 4;
-    DD acc[NL];
+    double acc[NL];
 #pragma unroll
-    for (int L = 0; L < NL; ++L) acc[L] = DD{0.0, 0.0};
+    for (int L = 0; L < NL; ++L) acc[L] = 0.0;
     const double p0 = phi(T.P, sgn, r_m) - om * r_m;
     const double dr_seg = delta_phi / fabs(om);
     const double nraw_d = ceil((r_max - r_m) / dr_seg);
@@ -536,7 +539,7 @@ __global__ __launch_bounds__(64) void tail_filon_kernel(const dwba_tail_job* __r
     filon_sweep<NL>(J, neff, om, p0, r_m, r_max, bstep, nseg, lane, acc);
 #pragma unroll
     for (int L = 0; L < NL; ++L) {
-        const DD t = dd_wave_sum(acc[L]);
+        const DD t = dd_wave_sum(DD{acc[L], 0.0});
         if (lane == 0) out[L] = (L >= 1 && L <= Lm) ? t.hi + t.lo : 0.0;
     }
 }
