@@ -15,6 +15,8 @@ are timed on the same batch and their errors stated against the FP64 result
 Workload "kernel_only": one channel pair, N = 15000, L = 0..30, FP64 (dense
 plain kernel and semiseparable kernel).
 Workload "backward": gradients of sum_L c_L I_L, N = 15000, 32 pairs, L = 0..20.
+Workload "ionization": 48 ejected-energy points x 400 (l_i, l_f, l_ej) triples,
+Slater bound state x 3 continuum waves, N ~ U{6000..15000}, L = 0..30.
 
 A step is one pass of the hot path over the batch with inputs resident in
 HBM (HIP-graph replay of prep, tails, contractions and assembly); host
@@ -264,14 +266,18 @@ def _err_stats(res, ref, mag):
     }
 
 
-def bench_batched(args):
+def bench_batched(args, ionization=False):
     import torch
     from dwba_amd import synthetic as S
 
     L_max = 30
     t0 = time.perf_counter()
-    scan = S.excitation_scan(n_energies=args.energies, n_pairs=args.pairs, n_range=(4000, 15000), l_max=40,
-                             k_range=(0.5, 8.0), seed=args.seed)
+    if ionization:
+        scan = S.ionization_scan(n_points=args.energies, n_triples=args.pairs, n_range=(6000, 15000), l_max=30,
+                                 seed=args.seed)
+    else:
+        scan = S.excitation_scan(n_energies=args.energies, n_pairs=args.pairs, n_range=(4000, 15000), l_max=40,
+                                 k_range=(0.5, 8.0), seed=args.seed)
     t_gen = time.perf_counter() - t0
     n_pairs = sum(len(e.pairs) for e in scan)
     n_int = n_pairs * (L_max + 1) * 2
@@ -294,10 +300,15 @@ def bench_batched(args):
         "metric": METRIC, "value": n_int / (ms64s * 1e-3), "unit": "integrals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms64s, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "batched precision sweep: 64 energies x 400 channel pairs, per-energy exp grid "
-                               "N~U{4000..15000} on [1e-5,1000] a0, L=0..30, direct+exchange, Full-Split "
-                               "(FP64 alpha/beta prep and tails); value = FP64 path with the O(N) "
-                               "semiseparable contraction; FP32-MFMA, split-BF16x3 and dense FP64 legs below",
+        "config": {"workload": (f"ionization-style batch: {len(scan)} ejected-energy points (k_e ~ U[0.2,4]) x "
+                                f"{args.pairs} (l_i, l_f, l_ej <= 30) triples, Slater bound state (zeta 0.6) x 3 "
+                                "continuum waves, per-point exp grid N~U{6000..15000}, L=0..30, direct+exchange"
+                                if ionization else
+                                "batched precision sweep: 64 energies x 400 channel pairs, per-energy exp grid "
+                                "N~U{4000..15000} on [1e-5,1000] a0, L=0..30, direct+exchange, Full-Split "
+                                "(FP64 alpha/beta prep and tails)")
+                               + "; value = FP64 path with the O(N) semiseparable contraction; FP32-MFMA, "
+                                 "split-BF16x3 and dense FP64 legs below",
                    "energies": len(scan), "pairs_per_energy": args.pairs, "L_max": L_max,
                    "integrals_per_step": n_int, "seed": args.seed, "graph": graph},
         "stage_ms": st64s,
@@ -389,7 +400,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=15000)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", default="batched", choices=["batched", "kernel_only", "backward"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "ionization", "kernel_only", "backward"])
     ap.add_argument("--energies", type=int, default=64)
     ap.add_argument("--pairs", type=int, default=400)
     ap.add_argument("--fp64-steps", type=int, default=1)
@@ -402,7 +413,10 @@ def main():
         print("bench.py measures one GPU only: the I_L path runs and is timed on a single MI355X; "
               "there is no multi-GPU mode to launch.", file=sys.stderr)
         sys.exit(2)
-    result = {"batched": bench_batched, "kernel_only": bench_kernel_only, "backward": bench_backward}[args.workload](args)
+    if args.workload == "ionization" and args.energies == 64:
+        args.energies = 48          # the ionization workload has 48 ejected-energy points
+    result = {"batched": bench_batched, "ionization": lambda a: bench_batched(a, ionization=True),
+              "kernel_only": bench_kernel_only, "backward": bench_backward}[args.workload](args)
     print(json.dumps(result), flush=True)
 
 

@@ -46,6 +46,47 @@ struct Item {
 __device__ inline float fac_u(double lr, double c, int L) { return (float)exp((double)L * (lr - c)); }
 __device__ inline float fac_v(double lr, double c, int L) { return (float)exp((double)L * (c - lr) - lr); }
 
+constexpr int XFOLD = 8;         // chunks (256 grid points) per FP32 accumulation run
+
+// Two-level FP32 accumulation: the MFMA accumulators T run over XFOLD chunks
+// and are then folded into a second set T2 (T2 += T, T = 0), so no FP32 sum
+// is longer than 256 grid points plus the (short) sum of the folds; the FP32
+// rounding error of a sum grows with its length.
+__device__ __forceinline__ void fold_acc(v16f (&acc)[2][4], v16f (&acc2)[2][4]) {
+#pragma unroll
+    for (int rg = 0; rg < 2; ++rg)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            acc2[rg][cb] += acc[rg][cb];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[rg][cb][q] = 0.0f;
+        }
+}
+
+// part[cb] = sum over this lane's rows of alpha_p[i] (T2[i, p] + T[i, p]), FP64.
+__device__ __forceinline__ void contract_acc(const v16f (&acc)[2][4], const v16f (&acc2)[2][4], double (&part)[4],
+                                             const dwba_xgemm_energy& E, int p0, int row0, int wave, int half,
+                                             int l32, int ncb_act) {
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        double sacc = 0.0;
+        const int p = p0 + cb * 32 + l32;
+        if (cb < ncb_act && p < E.P) {
+            const double* ap = E.A + (size_t)p * E.ld;
+#pragma unroll
+            for (int rg = 0; rg < 2; ++rg) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int m = 8 * (q >> 2) + 4 * half + (q & 3);
+                    const int i = row0 + wave * 64 + rg * 32 + m;
+                    if (i < E.N) sacc = fma(ap[i], (double)acc2[rg][cb][q] + (double)acc[rg][cb][q], sacc);
+                }
+            }
+        }
+        part[cb] = sacc;
+    }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(XT, 1) void xgemm_kernel(const dwba_xgemm_energy* __restrict__ en,
                                                       const int4* __restrict__ items, int n_items,
@@ -99,6 +140,14 @@ __global__ __launch_bounds__(XT, 1) void xgemm_kernel(const dwba_xgemm_energy* _
             for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
                 for (int q = 0; q < 16; ++q) acc[rg][cb][q] = 0.0f;
+        v16f acc2[2][4];
+#pragma unroll
+        for (int rg = 0; rg < 2; ++rg)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc2[rg][cb][q] = 0.0f;
+        double part[4];
 
         // ---- staging of chunk c: beta tile (128 pairs x 32 points) + column factors
         v4f stage[4];
@@ -182,32 +231,16 @@ __global__ __launch_bounds__(XT, 1) void xgemm_kernel(const dwba_xgemm_energy* _
                     }
                 }
             }
+            if ((c & (XFOLD - 1)) == XFOLD - 1 && c + 1 < nchunk) fold_acc(acc, acc2);
             // buf^1 was last read in iteration c-1, which ended with a barrier
             if (c + 1 < nchunk) store_chunk(c + 1, buf ^ 1);
             __syncthreads();
         }
 
         // ---- epilogue: partial_p = sum over this block's rows of alpha_p[i] T[i, p]  (FP64)
-        double part[4];
+        contract_acc(acc, acc2, part, E, p0, row0, wave, half, l32, ncb_act);
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-            double sacc = 0.0;
-            const int p = p0 + cb * 32 + l32;
-            if (cb < ncb_act && p < E.P) {
-                const double* ap = E.A + (size_t)p * E.ld;
-#pragma unroll
-                for (int rg = 0; rg < 2; ++rg) {
-#pragma unroll
-                    for (int q = 0; q < 16; ++q) {
-                        const int m = 8 * (q >> 2) + 4 * half + (q & 3);
-                        const int i = row0 + wave * 64 + rg * 32 + m;
-                        if (i < E.N) sacc = fma(ap[i], (double)acc[rg][cb][q], sacc);
-                    }
-                }
-            }
-            sacc += __shfl_xor(sacc, 32, 64);
-            part[cb] = sacc;
-        }
+        for (int cb = 0; cb < 4; ++cb) part[cb] += __shfl_xor(part[cb], 32, 64);
         if (half == 0) {
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb) s_red[wave][cb * 32 + l32] = part[cb];
@@ -279,6 +312,14 @@ __global__ __launch_bounds__(XT, 1) void xgemm_bf16x3_kernel(const dwba_xgemm_en
             for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
                 for (int q = 0; q < 16; ++q) acc[rg][cb][q] = 0.0f;
+        v16f acc2[2][4];
+#pragma unroll
+        for (int rg = 0; rg < 2; ++rg)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc2[rg][cb][q] = 0.0f;
+        double part[4];
 
         // chunk staging: 128 pairs x 32 points of hi and lo (64 B each per pair)
         v4f stage[4];
@@ -372,30 +413,14 @@ __global__ __launch_bounds__(XT, 1) void xgemm_bf16x3_kernel(const dwba_xgemm_en
                     }
                 }
             }
+            if ((c & (XFOLD - 1)) == XFOLD - 1 && c + 1 < nchunk) fold_acc(acc, acc2);
             if (c + 1 < nchunk) store_chunk(c + 1, buf ^ 1);
             __syncthreads();
         }
 
-        double part[4];
+        contract_acc(acc, acc2, part, E, p0, row0, wave, half, l32, ncb_act);
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-            double sacc = 0.0;
-            const int p = p0 + cb * 32 + l32;
-            if (cb < ncb_act && p < E.P) {
-                const double* ap = E.A + (size_t)p * E.ld;
-#pragma unroll
-                for (int rg = 0; rg < 2; ++rg) {
-#pragma unroll
-                    for (int q = 0; q < 16; ++q) {
-                        const int m = 8 * (q >> 2) + 4 * half + (q & 3);
-                        const int i = row0 + wave * 64 + rg * 32 + m;
-                        if (i < E.N) sacc = fma(ap[i], (double)acc[rg][cb][q], sacc);
-                    }
-                }
-            }
-            sacc += __shfl_xor(sacc, 32, 64);
-            part[cb] = sacc;
-        }
+        for (int cb = 0; cb < 4; ++cb) part[cb] += __shfl_xor(part[cb], 32, 64);
         if (half == 0) {
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb) s_red[wave][cb * 32 + l32] = part[cb];
