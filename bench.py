@@ -116,7 +116,7 @@ def bench_kernel_only(args):
     out_ss = torch.empty((1, 2, L_max + 1), dtype=torch.float64, device=dev)
 
     def step_ss():
-        _hip.check(lib.dwba_il_semisep_f64(1, P(pj_d), nr, nr, L_max, 0, P(out_ss), P(ss_ws), ss_bytes, sh),
+        _hip.check(lib.dwba_il_semisep_f64(1, P(pj_d), nr, nr, L_max, 0, 0, P(out_ss), P(ss_ws), ss_bytes, sh),
                    "dwba_il_semisep_f64")
 
     for _ in range(args.warmup + 1):
@@ -232,12 +232,15 @@ def _events_ms(fn, steps, stream):
     return e0.elapsed_time(e1) / steps
 
 
-def _scan_leg(scan, L_max, precision, contraction, steps, warmup, graph, stream):
-    """Plan/build one ScanBatch, time `steps` graph replays after `warmup`,
-    and take per-stage times from one eager launch with events."""
+def _scan_leg(scan, L_max, precision, contraction, steps, warmup, graph, stream, tune=True):
+    """Plan/build one ScanBatch (autotuned when it has tunable kernels), time
+    `steps` graph replays after `warmup`, and take per-stage times from one
+    eager launch with events."""
     import torch
     from dwba_amd.batch import ScanBatch
     b = ScanBatch(scan, L_max, precision=precision, oscillatory_method="full_split", contraction=contraction)
+    if tune:
+        b.tune()
     b.launch()
     ev = {}
     b.launch(events=ev)
@@ -288,8 +291,9 @@ def bench_batched(args, ionization=False):
 
     # the product path: every contraction in FP64 by the O(N) semiseparable kernel
     b64s, ms64s, st64s, res64s = _scan_leg(scan, L_max, "fp64", "semiseparable", args.steps, args.warmup, graph,
-                                           stream)
-    log(f"fp64 semiseparable: {ms64s:.2f} ms/step, stages {st64s}")
+                                           stream, tune=not args.no_tune)
+    tuned = getattr(b64s, "tuning", None)
+    log(f"fp64 semiseparable: {ms64s:.2f} ms/step, stages {st64s}, tuned {tuned.params if tuned else None}")
     mag = b64s.magnitudes()
     sizes = {"alpha_beta_bytes": 0}
     for (ei, _, plan, *_rest) in b64s.pairs:
@@ -312,7 +316,9 @@ def bench_batched(args, ionization=False):
                    "energies": len(scan), "pairs_per_energy": args.pairs, "L_max": L_max,
                    "integrals_per_step": n_int, "seed": args.seed, "graph": graph},
         "stage_ms": st64s,
-        "host_note": "host planning and uploads precede the timed region",
+        "autotune": ({"params": tuned.params, "ms": tuned.ms, "cached": tuned.cached, "candidates": tuned.timings}
+                     if tuned else None),
+        "host_note": "host planning, uploads and autotuning precede the timed region",
     }
     t_c = st64s.get("plain")
     if t_c:
@@ -408,6 +414,7 @@ def main():
     ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="library defaults instead of autotuned launch parameters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the I_L path runs and is timed on a single MI355X; "

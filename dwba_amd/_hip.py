@@ -71,7 +71,7 @@ _SIGS = {
     "dwba_il_plain_f64_batch_workspace_bytes": (c_sz, [c_i32, c_i32p, c_i32p, c_i32]),
     "dwba_il_plain_f64_batch": (c_i32, [c_i32, c_vp, c_i32p, c_i32p, c_i32, c_vp, c_vp, c_sz, c_vp]),
     "dwba_il_semisep_f64_workspace_bytes": (c_sz, [c_i32, c_i32p, c_i32p, c_i32, c_i32]),
-    "dwba_il_semisep_f64": (c_i32, [c_i32, c_vp, c_i32p, c_i32p, c_i32, c_i32, c_vp, c_vp, c_sz, c_vp]),
+    "dwba_il_semisep_f64": (c_i32, [c_i32, c_vp, c_i32p, c_i32p, c_i32, c_i32, c_i32, c_vp, c_vp, c_sz, c_vp]),
     "dwba_fullsplit_prep_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_dp, c_dp, c_i32, c_vp]),
     "dwba_moments_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]),
     "dwba_outer_tail_workspace_bytes": (c_sz, [c_i32, c_i32]),
@@ -197,7 +197,7 @@ def il_plain_f64(r, alpha_dir, beta_dir, alpha_ex, beta_ex, n_rows: int, n_cols:
     return out
 
 
-def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None):
+def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None, lb: int = 0):
     """dwba_il_semisep_f64 on the same job tuples as il_plain_f64_batch
     (n_rows <= n_cols).  Returns a (n_jobs, 2, L_max+1) float64 CUDA tensor."""
     ensure_init()
@@ -222,7 +222,7 @@ def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None):
         raise ValueError("il_semisep_f64_batch: invalid shapes (need 0 <= n_rows <= n_cols, 0 <= L_max <= 63)")
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     out = torch.empty((n, 2, L_max + 1), dtype=torch.float64, device=dev)
-    check(lib.dwba_il_semisep_f64(n, ptr(d_jobs), nr, nc, L_max, seg, ptr(out), ptr(ws), ws_bytes,
+    check(lib.dwba_il_semisep_f64(n, ptr(d_jobs), nr, nc, L_max, seg, lb, ptr(out), ptr(ws), ws_bytes,
                                   stream_handle(stream)), "dwba_il_semisep_f64")
     out._dwba_keep = (ws, d_jobs)
     return out

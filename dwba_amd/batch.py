@@ -337,6 +337,7 @@ class ScanBatch:
         self.plain_d = hip.struct_array_to_device(arr, self.dev)
         self.plain_n = (nra, nca)
         if self.contraction == "semiseparable":
+            self.semisep_config = (0, 0)      # (seg, lb): library defaults until tuned
             self.plain_ws_bytes = self.lib.dwba_il_semisep_f64_workspace_bytes(self.n_pairs, nra, nca, self.L_max, 0)
         else:
             self.plain_ws_bytes = self.lib.dwba_il_plain_f64_batch_workspace_bytes(self.n_pairs, nra, nca,
@@ -503,7 +504,8 @@ class ScanBatch:
         if self.plain_count:
             nra, nca = self.plain_n
             if self.contraction == "semiseparable":
-                hip.check(lib.dwba_il_semisep_f64(self.plain_count, hip.ptr(self.plain_d), nra, nca, Lm, 0,
+                seg, lb = self.semisep_config
+                hip.check(lib.dwba_il_semisep_f64(self.plain_count, hip.ptr(self.plain_d), nra, nca, Lm, seg, lb,
                                                   hip.ptr(self.out), hip.ptr(self.plain_ws), self.plain_ws_bytes, sh),
                           "dwba_il_semisep_f64")
             else:
@@ -577,8 +579,8 @@ class ScanBatch:
         wsb = lib.dwba_il_semisep_f64_workspace_bytes(n, nra, nca, self.L_max, 0)
         ws = torch.empty(wsb, dtype=torch.uint8, device=self.dev)
         mag = torch.zeros((n, 2, self.L_max + 1), dtype=torch.float64, device=self.dev)
-        hip.check(lib.dwba_il_semisep_f64(n, hip.ptr(d_jobs), nra, nca, self.L_max, 0, hip.ptr(mag), hip.ptr(ws),
-                                          wsb, hip.stream_handle()), "dwba_il_semisep_f64")
+        hip.check(lib.dwba_il_semisep_f64(n, hip.ptr(d_jobs), nra, nca, self.L_max, 0, 0, hip.ptr(mag),
+                                          hip.ptr(ws), wsb, hip.stream_handle()), "dwba_il_semisep_f64")
         return mag.cpu().numpy()[self.out_index]
 
     def work_flop(self):
@@ -605,6 +607,47 @@ class ScanBatch:
                     wd += nl * (2.0 * p.N1 * p.N1 + 2.0 * p.N1)
                     we += nl * (2.0 * p.N1 * p.N1 + 2.0 * p.N1)
         return wd, we
+
+    def tune(self, tuner=None, steps: int = 3):
+        """Pick the semiseparable kernel's (seg, lb) for this batch with the
+        autotuner (cached per device and workload signature) and use them from
+        now on; a captured graph is captured again.  Returns the autotune
+        Choice, or None when the batch has no semiseparable contraction."""
+        if self.precision != "fp64" or self.contraction != "semiseparable" or not self.plain_count:
+            return None
+        from . import autotune as AT
+        torch, hip, lib = self.torch, self.hip, self.lib
+        tuner = tuner or AT.default_tuner()
+        nra, nca = self.plain_n
+        Lm = self.L_max
+        key = AT.signature("semisep_f64", torch.cuda.get_device_name(self.dev), jobs=AT.bucket(self.plain_count),
+                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm)
+        self.launch()                       # alpha/beta in place for the contraction
+        torch.cuda.synchronize()
+        out = torch.empty_like(self.out)
+        wss = {}
+
+        def run(c):
+            seg = int(c["seg"])
+            if seg not in wss:
+                nb = lib.dwba_il_semisep_f64_workspace_bytes(self.plain_count, nra, nca, Lm, seg)
+                wss[seg] = (torch.empty(max(nb, 1), dtype=torch.uint8, device=self.dev), nb)
+            ws, nb = wss[seg]
+            hip.check(lib.dwba_il_semisep_f64(self.plain_count, hip.ptr(self.plain_d), nra, nca, Lm, seg,
+                                              int(c["lb"]), hip.ptr(out), hip.ptr(ws), nb, hip.stream_handle()),
+                      "dwba_il_semisep_f64")
+
+        choice = tuner.tune(key, AT.semisep_candidates(), run, steps=steps)
+        del wss
+        seg, lb = int(choice.params["seg"]), int(choice.params["lb"])
+        self.semisep_config = (seg, lb)
+        self.plain_ws_bytes = lib.dwba_il_semisep_f64_workspace_bytes(self.plain_count, nra, nca, Lm, seg)
+        self.plain_ws = torch.empty(max(self.plain_ws_bytes, 1), dtype=torch.uint8, device=self.dev)
+        self.tuning = choice
+        if self.graph is not None:
+            self.graph = None
+            self.capture()
+        return choice
 
     def capture(self):
         """Capture launch() into a HIP graph (replayed by run())."""

@@ -298,10 +298,12 @@ size_t dwba_il_semisep_f64_workspace_bytes(int32_t n_jobs, const int32_t* h_n_ro
 }
 
 int32_t dwba_il_semisep_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const int32_t* h_n_rows,
-                            const int32_t* h_n_cols, int32_t L_max, int32_t seg, double* d_out, void* d_workspace,
-                            size_t workspace_bytes, void* stream) {
+                            const int32_t* h_n_cols, int32_t L_max, int32_t seg, int32_t lb, double* d_out,
+                            void* d_workspace, size_t workspace_bytes, void* stream) {
     DWBA_REQUIRE(initialised(), "dwba_il_semisep_f64: call dwba_init first");
     DWBA_REQUIRE(d_jobs && d_out, "dwba_il_semisep_f64: null pointer");
+    if (lb == 0) lb = 8;
+    DWBA_REQUIRE(lb == 4 || lb == 8, "dwba_il_semisep_f64: lb must be 0, 4 or 8 (got %d)", lb);
     SemisepLayout lay;
     DWBA_REQUIRE(semisep_layout(n_jobs, h_n_rows, h_n_cols, L_max, seg, &lay),
                  "dwba_il_semisep_f64: invalid jobs (n_jobs=%d, L_max=%d; need 0 <= n_rows <= n_cols)", n_jobs, L_max);
@@ -317,10 +319,14 @@ int32_t dwba_il_semisep_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const 
     DWBA_LAUNCH_CHECK("semisep_plan");
     if (lay.n_tasks > 0) {
         const int blocks = (lay.n_tasks + ST / 64 - 1) / (ST / 64);
-        switch ((L_max + 8) / 8) {
-#define C(nb) case nb: hipLaunchKernelGGL((semisep_kernel<8, nb>), dim3(blocks), dim3(ST), 0, s, d_jobs, d_off, \
-                                          n_jobs, lay.seg, L_max, d_summ); break;
-            C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8)
+        const int nb = (L_max + lb) / lb;     // L-blocks of lb multipoles
+        const int key = lb * 100 + nb;
+        switch (key) {
+#define C(LBV, NBV) case LBV * 100 + NBV: hipLaunchKernelGGL((semisep_kernel<LBV, NBV>), dim3(blocks), dim3(ST), 0, s, \
+                                                   d_jobs, d_off, n_jobs, lay.seg, L_max, d_summ); break;
+            C(8, 1) C(8, 2) C(8, 3) C(8, 4) C(8, 5) C(8, 6) C(8, 7) C(8, 8)
+            C(4, 1) C(4, 2) C(4, 3) C(4, 4) C(4, 5) C(4, 6) C(4, 7) C(4, 8)
+            C(4, 9) C(4, 10) C(4, 11) C(4, 12) C(4, 13) C(4, 14) C(4, 15) C(4, 16)
 #undef C
             default: return set_error(DWBA_ERR_ARG, "dwba_il_semisep_f64: L_max %d", L_max);
         }

@@ -328,7 +328,7 @@ class _PairRunner:
                 nc = (ctypes.c_int32 * 1)(plan.cols)
                 ws_bytes = lib.dwba_il_semisep_f64_workspace_bytes(1, nr, nc, L_max, 0)
                 ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-                hip.check(lib.dwba_il_semisep_f64(1, hip.ptr(pj_d), nr, nc, L_max, 0, hip.ptr(out), hip.ptr(ws),
+                hip.check(lib.dwba_il_semisep_f64(1, hip.ptr(pj_d), nr, nc, L_max, 0, 0, hip.ptr(out), hip.ptr(ws),
                                                   ws_bytes, sh), "dwba_il_semisep_f64")
                 keep += [pj_d, ws]
             else:

@@ -105,12 +105,13 @@ int32_t dwba_il_plain_f64_batch(int32_t n_jobs, const dwba_plain_job* d_jobs,
  * evaluated as wavefront-parallel affine scans.  Replaces the same reference
  * code as dwba_il_plain_f64 (dwba_matrix_elements.py:1057-1080, :1113-1116,
  * :1243-1245, :1281-1289).  Requires 0 <= n_rows <= n_cols and d_r strictly
- * increasing and > 0.  seg: grid points per lane (0 = chosen from the work
- * size); the workspace size depends on it. */
+ * increasing and > 0.  Tuning (results agree to rounding for every choice):
+ * seg = grid points per lane (0 = chosen from the work size; the workspace
+ * size depends on it), lb = multipoles per register block (4 or 8, 0 = 8). */
 size_t dwba_il_semisep_f64_workspace_bytes(int32_t n_jobs, const int32_t* h_n_rows, const int32_t* h_n_cols,
                                            int32_t L_max, int32_t seg);
 int32_t dwba_il_semisep_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const int32_t* h_n_rows,
-                            const int32_t* h_n_cols, int32_t L_max, int32_t seg, double* d_out,
+                            const int32_t* h_n_cols, int32_t L_max, int32_t seg, int32_t lb, double* d_out,
                             void* d_workspace, size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------ */

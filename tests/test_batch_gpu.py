@@ -99,3 +99,30 @@ def test_graph_replay_matches_eager():
         torch.cuda.synchronize()
         assert np.array_equal(b.results(), eager), rep
         _ = (torch.ones(1000, device="cuda") * rep).sum().item()   # eager work between replays
+
+
+def test_autotuned_batch_matches_default(tmp_path, monkeypatch):
+    """Autotuning only reorders FP64 operations: the tuned batch equals the
+    library-default one to rounding, and a second batch of the same
+    signature takes the cached choice without timing."""
+    from dwba_amd import autotune as AT
+    from dwba_amd.batch import ScanBatch
+    monkeypatch.setenv("DWBA_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    tuner = AT.Autotuner()
+    scan = _small_scan(4)
+    ref = ScanBatch(scan, 10, precision="fp64", oscillatory_method="full_split")
+    ref.run()
+    r0 = ref.results()
+    b = ScanBatch(scan, 10, precision="fp64", oscillatory_method="full_split")
+    b.capture()
+    ch = b.tune(tuner)
+    print(f"\n[autotune] choice {ch.params} ({ch.ms:.3f} ms) of {len(ch.timings)} candidates", flush=True)
+    assert not ch.cached and len(ch.timings) == len(AT.semisep_candidates())
+    for _ in range(2):
+        b.run()
+    r1 = b.results()
+    mag = ref.magnitudes()
+    assert np.all(np.abs(r1 - r0) <= 1e-12 * np.abs(r0) + 1e-14 * mag)
+    b2 = ScanBatch(scan, 10, precision="fp64", oscillatory_method="full_split")
+    ch2 = b2.tune(AT.Autotuner())
+    assert ch2.cached and ch2.params == ch.params
