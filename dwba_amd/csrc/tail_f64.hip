@@ -366,14 +366,14 @@ __device__ double analytic_tail(double r_m, const dwba_tail_job& J, int L, doubl
 // by triple (the recursion of _filon_segment_complex), so only three nodes
 // are live at a time whatever neff is.  Per triple the exponentials and the
 // Filon coefficients are formed once and all L are swept with x^-(L+1) built
-// by repeated multiplication.  Each lane adds its triples' real parts to
-// per-L FP64 partial sums (a few hundred terms of similar size); the lanes are
-// combined in double-double.  The reference sums segments with math.fsum;
-// the two agree to ~1e-15 of the segment size.
+// by repeated multiplication.  The envelope is M_L x^-(L+1); M_L multiplies
+// the finished sum.  Each lane adds its triples' real parts to per-L FP64
+// partial sums (a few hundred terms of similar size); the lanes are combined
+// in double-double.  The reference sums segments with math.fsum; the two
+// agree to ~1e-15 of the segment size.
 template <int NL>
 __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, double om, double p0, double r_m,
                                             double r_max, double bstep, long nseg, int lane, double* acc) {
-    const int Lm = J.L_max < NL - 1 ? J.L_max : NL - 1;
     const int nt = (neff - 1) / 2;
     const double inv_om = -1.0 / om;
     for (long sg = lane; sg < nseg; sg += 64) {
@@ -425,10 +425,10 @@ __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, do
                     double st, ct, sd, cd;
                     sincos(th, &st, &ct);
                     sincos(2 * th, &sd, &cd);
-                    const double t2 = th * th, t3 = th * th * th;
-                    al = (t2 + th * sd / 2 - 2 * (st * st)) / t3;
-                    be = 2 * (th * (1 + cd) - sd) / t3;
-                    ga = 4 * (st - th * ct) / t3;
+                    const double t2 = th * th, it3 = 1.0 / (th * th * th);
+                    al = (t2 + th * sd / 2 - 2 * (st * st)) * it3;
+                    be = 2 * (th * (1 + cd) - sd) * it3;
+                    ga = 4 * (st - th * ct) * it3;
                 }
                 // Re[(h/2)(al (-i/om)(f2 e2 - f0 e0) + be (f0 e0 + f2 e2) + ga f1 e1)]
                 const double hh = h3 / 2, cA = al * inv_om;
@@ -437,18 +437,14 @@ __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, do
                 wm = hh * (ga * c1);
                 s0 = s2; c0 = c2;
             }
-            double p0x = ix0 * ix0, p1x = ix1 * ix1, p2x = ix2 * ix2;   // x^-(L+1) at L = 1
+            // x^-(L+1) at L = 1 (ix = 0 where the envelope is cut off, x <= 1e-6);
+            // M_L multiplies the whole sum at the end
+            double p0x = ix0 * ix0, p1x = ix1 * ix1, p2x = ix2 * ix2;
 #pragma unroll
             for (int L = 1; L < NL; ++L) {
-                if (L <= Lm) {
-                    const double M = J.moments[L];
-                    if (fabs(M) > 1e-12) {
-                        const double f0 = x0 > 1e-6 ? M * p0x : 0.0;
-                        const double f1 = x1 > 1e-6 ? M * p1x : 0.0;
-                        const double f2 = x2 > 1e-6 ? M * p2x : 0.0;
-                        acc[L] += (wa * f0 + wm * f1) + wb * f2;
-                    }
-                }
+                acc[L] = fma(wa, p0x, acc[L]);
+                acc[L] = fma(wm, p1x, acc[L]);
+                acc[L] = fma(wb, p2x, acc[L]);
                 p0x *= ix0;
                 p1x *= ix1;
                 p2x *= ix2;
@@ -540,7 +536,8 @@ __global__ __launch_bounds__(64) void tail_filon_kernel(const dwba_tail_job* __r
 #pragma unroll
     for (int L = 0; L < NL; ++L) {
         const DD t = dd_wave_sum(DD{acc[L], 0.0});
-        if (lane == 0) out[L] = (L >= 1 && L <= Lm) ? t.hi + t.lo : 0.0;
+        const double M = (L >= 1 && L <= Lm) ? J.moments[L] : 0.0;
+        if (lane == 0) out[L] = fabs(M) > 1e-12 ? M * (t.hi + t.lo) : 0.0;
     }
 }
 
