@@ -327,14 +327,58 @@ class ScanBatch:
         nra = (ctypes.c_int32 * self.n_pairs)()
         nca = (ctypes.c_int32 * self.n_pairs)()
         pos = {k: c for st in self.E for c, k in enumerate(st.members)}
+        # semiseparable mode: an energy whose pairs share one bound_f has one
+        # direct beta (u_f u_i w) for its Filon/standard pairs, so their direct
+        # family is t_L = K_L beta once per energy (O(N)) and a dot per pair
+        shared_direct = set()
+        dg, dd, dd_counts = [], [], []
+        max_rows = 0
+        self.t_bufs = []
+        if self.contraction == "semiseparable":
+            for ei, st in enumerate(self.E):
+                ks = [k for k in st.members if self.pairs[k][2].mode in (PREP_FILON, PREP_STANDARD)]
+                if not ks or len({id(self.pairs[k][5]) for k in st.members}) != 1:
+                    continue
+                rows_d = max(self.pairs[k][2].N1 for k in ks)
+                dd_counts.append(len(ks))
+                t = self.torch.zeros((self.L_max + 1, st.ld), dtype=self.torch.float64, device=self.dev)
+                self.t_bufs.append(t)
+                dg.append(DgemvEnergy(hip.ptr(st.r_d), hip.ptr(st.bd[pos[ks[0]]]), hip.ptr(t), st.N, rows_d, st.ld,
+                                      self.L_max))
+                max_rows = max(max_rows, rows_d)
+                for k in ks:
+                    shared_direct.add(k)
+                    dd.append(DdotPair(hip.ptr(st.ad[pos[k]]), hip.ptr(t), self.pairs[k][2].N1, st.ld,
+                                       int(self.out_index[k]), 0))
+        self.ss_gemv = None
+        if dg:
+            # tiles of <= 64 consecutive pairs of one energy for the tiled dot
+            items = []
+            first = 0
+            for cnt in dd_counts:
+                for q in range(0, cnt, 64):
+                    items.append((first + q, min(64, cnt - q)))
+                first += cnt
+            items_d = self.torch.as_tensor(np.asarray(items, dtype=np.int32).reshape(-1, 2), device=self.dev)
+            self._keep.append(items_d)
+            self.ss_gemv = (hip.struct_array_to_device((DgemvEnergy * len(dg))(*dg), self.dev), len(dg),
+                            hip.struct_array_to_device((DdotPair * len(dd))(*dd), self.dev), len(dd),
+                            items_d, len(items))
+        arr_split = (hip.PlainJob * self.n_pairs)()
         for g, k in enumerate(order):
             ei, pi, plan, ci, cf, bf = self.pairs[k]
             st = self.E[ei]
             c = pos[k]
             arr[g] = hip.PlainJob(hip.ptr(st.r_d), hip.ptr(st.ad[c]), hip.ptr(st.bd[c]), hip.ptr(st.ae[c]),
                                   hip.ptr(st.be[c]), plan.rows, plan.cols)
+            skip_dir = k in shared_direct      # direct family through the shared-beta GEMV instead
+            arr_split[g] = hip.PlainJob(hip.ptr(st.r_d), 0 if skip_dir else hip.ptr(st.ad[c]),
+                                        0 if skip_dir else hip.ptr(st.bd[c]), hip.ptr(st.ae[c]), hip.ptr(st.be[c]),
+                                        plan.rows, plan.cols)
             nra[g], nca[g] = plan.rows, plan.cols
         self.plain_d = hip.struct_array_to_device(arr, self.dev)
+        self.plain_d_split = hip.struct_array_to_device(arr_split, self.dev) if self.ss_gemv is not None else None
+        self.direct_gemv = False          # chosen by tune() when the shared-beta path exists
         self.plain_n = (nra, nca)
         if self.contraction == "semiseparable":
             self.semisep_config = (0, 0)      # (seg, lb): library defaults until tuned
@@ -465,6 +509,24 @@ class ScanBatch:
                 raise ValueError("grid too coarse for the FP32 factorised kernel (row block spans "
                                  f"r in [{st.gs.r[a]:.3g}, {st.gs.r[b]:.3g}]); use precision='fp64'")
 
+    def _contract_semisep(self, seg, lb, direct_gemv, out, ws, ws_bytes, sh):
+        """All FP64 contractions of the batch through the semiseparable kernel;
+        with direct_gemv the Filon/standard pairs of shared-beta energies take
+        their direct family from t_L = K_L beta (once per energy) and a dot."""
+        hip, lib, Lm = self.hip, self.lib, self.L_max
+        nra, nca = self.plain_n
+        jobs = self.plain_d_split if direct_gemv else self.plain_d
+        hip.check(lib.dwba_il_semisep_f64(self.plain_count, hip.ptr(jobs), nra, nca, Lm, seg, lb, hip.ptr(out),
+                                          hip.ptr(ws), ws_bytes, sh), "dwba_il_semisep_f64")
+        if direct_gemv:
+            gd, ng, dd, nd, items_d, ni = self.ss_gemv
+            hip.check(lib.dwba_semisep_gemv_f64(ng, hip.ptr(gd), Lm, sh), "dwba_semisep_gemv_f64")
+            if Lm <= 31:
+                hip.check(lib.dwba_ddot_tiled_f64(ni, hip.ptr(items_d), hip.ptr(dd), Lm, hip.ptr(out), sh),
+                          "dwba_ddot_tiled_f64")
+            else:
+                hip.check(lib.dwba_ddot_f64(nd, hip.ptr(dd), Lm, hip.ptr(out), sh), "dwba_ddot_f64")
+
     # ------------------------------------------------------------ launch
     def launch(self, stream=None, events=None):
         """Enqueue the whole scan on `stream` (torch's current by default).
@@ -505,9 +567,7 @@ class ScanBatch:
             nra, nca = self.plain_n
             if self.contraction == "semiseparable":
                 seg, lb = self.semisep_config
-                hip.check(lib.dwba_il_semisep_f64(self.plain_count, hip.ptr(self.plain_d), nra, nca, Lm, seg, lb,
-                                                  hip.ptr(self.out), hip.ptr(self.plain_ws), self.plain_ws_bytes, sh),
-                          "dwba_il_semisep_f64")
+                self._contract_semisep(seg, lb, self.direct_gemv, self.out, self.plain_ws, self.plain_ws_bytes, sh)
             else:
                 hip.check(lib.dwba_il_plain_f64_batch(self.plain_count, hip.ptr(self.plain_d), nra, nca, Lm,
                                                       hip.ptr(self.out), hip.ptr(self.plain_ws), self.plain_ws_bytes,
@@ -609,10 +669,12 @@ class ScanBatch:
         return wd, we
 
     def tune(self, tuner=None, steps: int = 3):
-        """Pick the semiseparable kernel's (seg, lb) for this batch with the
-        autotuner (cached per device and workload signature) and use them from
-        now on; a captured graph is captured again.  Returns the autotune
-        Choice, or None when the batch has no semiseparable contraction."""
+        """Pick the semiseparable kernel's (seg, lb) -- and, for energies with a
+        shared direct beta, whether their direct family goes through the O(N)
+        GEMV + tiled dot -- with the autotuner (cached per device and workload
+        signature), and use them from now on; a captured graph is captured
+        again.  Returns the autotune Choice, or None when the batch has no
+        semiseparable contraction."""
         if self.precision != "fp64" or self.contraction != "semiseparable" or not self.plain_count:
             return None
         from . import autotune as AT
@@ -620,8 +682,9 @@ class ScanBatch:
         tuner = tuner or AT.default_tuner()
         nra, nca = self.plain_n
         Lm = self.L_max
+        has_gemv = self.ss_gemv is not None
         key = AT.signature("semisep_f64", torch.cuda.get_device_name(self.dev), jobs=AT.bucket(self.plain_count),
-                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm)
+                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm, gemv=int(has_gemv))
         self.launch()                       # alpha/beta in place for the contraction
         torch.cuda.synchronize()
         out = torch.empty_like(self.out)
@@ -633,14 +696,16 @@ class ScanBatch:
                 nb = lib.dwba_il_semisep_f64_workspace_bytes(self.plain_count, nra, nca, Lm, seg)
                 wss[seg] = (torch.empty(max(nb, 1), dtype=torch.uint8, device=self.dev), nb)
             ws, nb = wss[seg]
-            hip.check(lib.dwba_il_semisep_f64(self.plain_count, hip.ptr(self.plain_d), nra, nca, Lm, seg,
-                                              int(c["lb"]), hip.ptr(out), hip.ptr(ws), nb, hip.stream_handle()),
-                      "dwba_il_semisep_f64")
+            self._contract_semisep(seg, int(c["lb"]), bool(c.get("gemv", 0)), out, ws, nb, hip.stream_handle())
 
-        choice = tuner.tune(key, AT.semisep_candidates(), run, steps=steps)
+        cands = AT.semisep_candidates()
+        if has_gemv:
+            cands = cands + [dict(c, gemv=1) for c in cands]
+        choice = tuner.tune(key, cands, run, steps=steps)
         del wss
         seg, lb = int(choice.params["seg"]), int(choice.params["lb"])
         self.semisep_config = (seg, lb)
+        self.direct_gemv = bool(choice.params.get("gemv", 0)) and has_gemv
         self.plain_ws_bytes = lib.dwba_il_semisep_f64_workspace_bytes(self.plain_count, nra, nca, Lm, seg)
         self.plain_ws = torch.empty(max(self.plain_ws_bytes, 1), dtype=torch.uint8, device=self.dev)
         self.tuning = choice

@@ -120,6 +120,80 @@ __global__ void finalize_kernel(const dwba_finalize_pair* __restrict__ pairs, in
     }
 }
 
+// Tiled form of ddot for pairs that share t (one energy): a workgroup takes
+// up to 64 pairs of one energy; per 64-point chunk the 64 alpha rows and the
+// 32 t rows are staged in LDS (coalesced loads, each read from HBM/L2 once per
+// tile instead of once per pair), and thread (pair, L-group of 8) accumulates
+// 8 multipoles.  items[k] = {first pair, pair count}; the pairs of an item have
+// the same t, ld and L_max <= 31.
+constexpr int TP = 64;     // pairs per tile
+constexpr int TK = 64;     // grid points per chunk
+
+__global__ __launch_bounds__(256) void ddot_tiled_kernel(const dwba_ddot_pair* __restrict__ pairs,
+                                                         const int2* __restrict__ items, int L_max,
+                                                         double* __restrict__ out) {
+    __shared__ double s_a[TK][TP + 1];
+    __shared__ __attribute__((aligned(16))) double s_t[TK][32];   // [point][L]: 8 L of a group in 4 b128 reads
+    __shared__ const double* s_ap[TP];
+    __shared__ int s_n;
+    const int2 it = items[blockIdx.x];
+    const int first = it.x, count = it.y;
+    const int tid = threadIdx.x;
+    const int p = tid & 63, g = tid >> 6;          // pair in the tile, L-group (a wavefront)
+    const double* t = pairs[first].t;
+    const int ld = pairs[first].ld;
+    if (tid < TP) s_ap[tid] = tid < count ? pairs[first + tid].alpha : nullptr;
+    if (tid == 0) {
+        int m = 0;
+        for (int q = 0; q < count; ++q) m = max(m, pairs[first + q].n);
+        s_n = m;
+    }
+    __syncthreads();
+    const int nmax = s_n;
+    const int nl = L_max + 1;
+    double acc[8];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) acc[l] = 0.0;
+    for (int i0 = 0; i0 < nmax; i0 += TK) {
+        // alpha is zero beyond each pair's n (and up to ld), so nmax bounds every row
+#pragma unroll
+        for (int q = 0; q < TP * TK / 256; ++q) {
+            const int idx = tid + 256 * q;
+            const int pr = idx >> 6, pt = idx & 63;
+            const int i = i0 + pt;
+            s_a[pt][pr] = (pr < count && i < nmax) ? s_ap[pr][i] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 32 * TK / 256; ++q) {
+            const int idx = tid + 256 * q;
+            const int L = idx >> 6, pt = idx & 63;
+            const int i = i0 + pt;
+            s_t[pt][L] = (L < nl && i < nmax) ? t[(size_t)L * ld + i] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int pt = 0; pt < TK; ++pt) {
+            const double a = s_a[pt][p];
+            const double2* tv = reinterpret_cast<const double2*>(&s_t[pt][8 * g]);
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const double2 v = tv[h];
+                acc[2 * h] = fma(a, v.x, acc[2 * h]);
+                acc[2 * h + 1] = fma(a, v.y, acc[2 * h + 1]);
+            }
+        }
+        __syncthreads();
+    }
+    if (p < count) {
+        double* o = out + (size_t)pairs[first + p].out_index * 2 * nl;
+#pragma unroll
+        for (int l = 0; l < 8; ++l) {
+            const int L = 8 * g + l;
+            if (L < nl) o[L] = acc[l];
+        }
+    }
+}
+
 int nl_bucket_g(int L_max) {
     static const int buckets[] = {1, 2, 4, 8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64};
     for (int b : buckets)
@@ -162,6 +236,17 @@ int32_t dwba_ddot_f64(int32_t n_pairs, const dwba_ddot_pair* d_pairs, int32_t L_
         default: return set_error(DWBA_ERR_ARG, "dwba_ddot_f64: L_max %d", L_max);
     }
     DWBA_LAUNCH_CHECK("ddot_kernel");
+    return DWBA_OK;
+}
+
+int32_t dwba_ddot_tiled_f64(int32_t n_items, const int32_t* d_items, const dwba_ddot_pair* d_pairs, int32_t L_max,
+                            double* d_out, void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_ddot_tiled_f64: call dwba_init first");
+    DWBA_REQUIRE(n_items > 0 && d_items && d_pairs && d_out && L_max >= 0 && L_max <= 31,
+                 "dwba_ddot_tiled_f64: bad arguments (L_max <= 31)");
+    hipLaunchKernelGGL(ddot_tiled_kernel, dim3(n_items), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), d_pairs,
+                       reinterpret_cast<const int2*>(d_items), L_max, d_out);
+    DWBA_LAUNCH_CHECK("ddot_tiled_kernel");
     return DWBA_OK;
 }
 

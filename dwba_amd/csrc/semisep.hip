@@ -41,6 +41,17 @@ __device__ inline int find_job(const int* __restrict__ off, int n_jobs, int t) {
     return lo;
 }
 
+// x^e, e >= 1, by squaring (runtime exponent).
+__device__ inline double pow_int_rt(double x, int e) {
+    double r = 1.0;
+    while (e > 0) {
+        if (e & 1) r *= x;
+        x *= x;
+        e >>= 1;
+    }
+    return r;
+}
+
 // q^L0 for L0 a multiple of LB (LB a power of two): LB-th power by squaring,
 // then L0/LB - 1 multiplications; every trip count is uniform (no selects).
 template <int LB>
@@ -143,6 +154,7 @@ __global__ __launch_bounds__(ST) void semisep_kernel(const dwba_plain_job* __res
     const int fam = local / w_per_fam;
     const int wt = local - fam * w_per_fam;
     if (J.n_rows <= 0) return;            // the join writes 0 for such jobs
+    if (fam == 0 && J.alpha_dir == nullptr) return;   // direct family computed elsewhere
     const double* __restrict__ al = fam ? J.alpha_ex : J.alpha_dir;
     const double* __restrict__ be = fam ? J.beta_ex : J.beta_dir;
     const int n = J.n_cols;
@@ -167,7 +179,7 @@ __global__ __launch_bounds__(ST) void semisep_kernel(const dwba_plain_job* __res
             const int L = L0 + l;
             dL[l] = L == 0 ? 1.0 : exp((double)L * ldl);
         }
-        double ir_cur = a < n ? 1.0 / J.r[a] : 1.0;
+        double ir_cur = 1.0 / J.r[min(a, n - 1)];
         const int am = min(b, max(a, nr));               // [a, am): alpha may be non-zero
         ss_range<LB, true>(J.r, al, be, n, a, am, L0, dL, A, Z, acc, Ca, Cb, P, ir_cur);
         ss_range<LB, false>(J.r, al, be, n, am, b, L0, dL, A, Z, acc, Ca, Cb, P, ir_cur);
@@ -208,6 +220,7 @@ __global__ void semisep_join(const dwba_plain_job* __restrict__ jobs, const int*
     const int k = blockIdx.x, fam = blockIdx.y, L = threadIdx.x;
     const int NL = L_max + 1;
     if (L >= NL) return;
+    if (fam == 0 && jobs[k].alpha_dir == nullptr) return;   // left to the caller
     const int w = tasks_of(jobs[k].n_cols, seg) / 2;
     double tot = 0.0, Ain = 0.0, Zin = 0.0;
     if (jobs[k].n_rows > 0) {
@@ -241,6 +254,95 @@ __global__ void semisep_plan(const dwba_plain_job* __restrict__ jobs, int n_jobs
     for (int k = k0; k < k1; ++k) {
         off[k] = run;
         run += tasks_of(jobs[k].n_cols, seg);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Shared-beta form (the direct family of an excitation scan, where every
+// pair of an energy has the same beta = u_f u_i w): t_L(i) = sum_j K_L(i, j) beta_j
+// for i < rows, all L, once per energy, in O(N) per L:
+//     t_L(i) = (A_i + d_L beta_i) / r_i + B_i,
+//     A_{i+1} = q_i^L (A_i + beta_i),              A_0 = 0          (j < i)
+//     B_{i-1} = q_{i-1}^L (B_i + beta_i / r_i),    B_{N-1} = 0      (j > i)
+// A workgroup of GT threads handles one (energy, block of LB multipoles);
+// a thread owns a contiguous range; ranges are joined by scans of the affine
+// maps through LDS (forward for A, backward for B), then every thread
+// recomputes its range with the carried values and writes t.
+constexpr int GT = 1024;
+
+template <int LB>
+__global__ __launch_bounds__(GT) void semisep_gemv_kernel(const dwba_dgemv_energy* __restrict__ en) {
+    __shared__ double s_m[GT], s_v[GT];
+    const dwba_dgemv_energy E = en[blockIdx.x];
+    const int L0 = blockIdx.y * LB;
+    if (L0 > E.L_max) return;
+    const int n = E.N, tid = threadIdx.x;
+    const int per = (n + GT - 1) / GT;
+    const int a = min(n, tid * per), b = min(n, a + per);
+    const double* __restrict__ r = E.r;
+    const double* __restrict__ be = E.beta;
+    const double ldl = log(1.0 - 1e-12);
+    for (int l = 0; l < LB; ++l) {
+        const int L = L0 + l;
+        if (L > E.L_max) break;
+        const double dL = L == 0 ? 1.0 : exp((double)L * ldl);
+        // ---- pass 1: local maps.  forward: A_b = Mf A_a + Af; backward: B_{a-1} = Mb B_{b-1} + Sb
+        double Af = 0.0, Mf = 1.0, Sb = 0.0, Mb = 1.0;
+        for (int i = a; i < b; ++i) {
+            const double q = i + 1 < n ? r[i] / r[i + 1] : 1.0;
+            const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
+            Af = qL * (Af + be[i]);
+            Mf *= qL;
+        }
+        for (int i = b - 1; i >= a; --i) {
+            const double q = i > 0 ? r[i - 1] / r[i] : 1.0;
+            const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
+            Sb = qL * (Sb + be[i] / r[i]);
+            Mb *= qL;
+        }
+        // ---- forward scan (inclusive) of (Mf, Af) over threads
+        s_m[tid] = Mf; s_v[tid] = Af;
+        __syncthreads();
+        for (int d = 1; d < GT; d <<= 1) {
+            double pm = 0.0, pv = 0.0;
+            const bool take = tid >= d;
+            if (take) { pm = s_m[tid - d]; pv = s_v[tid - d]; }
+            __syncthreads();
+            if (take) { s_v[tid] = fma(pv, s_m[tid], s_v[tid]); s_m[tid] = pm * s_m[tid]; }
+            __syncthreads();
+        }
+        const double A_in = tid > 0 ? s_v[tid - 1] : 0.0;
+        __syncthreads();
+        // ---- backward scan (inclusive from the right) of (Mb, Sb)
+        s_m[tid] = Mb; s_v[tid] = Sb;
+        __syncthreads();
+        for (int d = 1; d < GT; d <<= 1) {
+            double pm = 0.0, pv = 0.0;
+            const bool take = tid + d < GT;
+            if (take) { pm = s_m[tid + d]; pv = s_v[tid + d]; }
+            __syncthreads();
+            if (take) { s_v[tid] = fma(pv, s_m[tid], s_v[tid]); s_m[tid] = pm * s_m[tid]; }
+            __syncthreads();
+        }
+        const double B_in = tid + 1 < GT ? s_v[tid + 1] : 0.0;   // B at this range's last point
+        __syncthreads();
+        // ---- pass 2: forward part, then backward part added in place
+        double A = A_in;
+        double* __restrict__ t = E.t + (size_t)L * E.ld;
+        for (int i = a; i < b; ++i) {
+            const double ir = 1.0 / r[i];
+            if (i < E.rows) t[i] = (A + dL * be[i]) * ir;
+            const double q = i + 1 < n ? r[i] * (1.0 / r[i + 1]) : 1.0;
+            const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
+            A = qL * (A + be[i]);
+        }
+        double B = B_in;
+        for (int i = b - 1; i >= a; --i) {
+            if (i < E.rows) t[i] += B;
+            const double q = i > 0 ? r[i - 1] / r[i] : 1.0;
+            const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
+            B = qL * (B + be[i] / r[i]);
+        }
     }
 }
 
@@ -334,6 +436,17 @@ int32_t dwba_il_semisep_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const 
     }
     hipLaunchKernelGGL(semisep_join, dim3(n_jobs, 2), dim3(64), 0, s, d_jobs, d_off, lay.seg, L_max, d_summ, d_out);
     DWBA_LAUNCH_CHECK("semisep_join");
+    return DWBA_OK;
+}
+
+int32_t dwba_semisep_gemv_f64(int32_t n_energies, const dwba_dgemv_energy* d_energies, int32_t L_max,
+                              void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_semisep_gemv_f64: call dwba_init first");
+    DWBA_REQUIRE(n_energies > 0 && d_energies && L_max >= 0 && L_max <= 63, "dwba_semisep_gemv_f64: bad arguments");
+    constexpr int LB = 8;
+    hipLaunchKernelGGL(semisep_gemv_kernel<LB>, dim3(n_energies, (L_max + LB) / LB), dim3(GT), 0,
+                       reinterpret_cast<hipStream_t>(stream), d_energies);
+    DWBA_LAUNCH_CHECK("semisep_gemv_kernel");
     return DWBA_OK;
 }
 

@@ -105,9 +105,12 @@ int32_t dwba_il_plain_f64_batch(int32_t n_jobs, const dwba_plain_job* d_jobs,
  * evaluated as wavefront-parallel affine scans.  Replaces the same reference
  * code as dwba_il_plain_f64 (dwba_matrix_elements.py:1057-1080, :1113-1116,
  * :1243-1245, :1281-1289).  Requires 0 <= n_rows <= n_cols and d_r strictly
- * increasing and > 0.  Tuning (results agree to rounding for every choice):
- * seg = grid points per lane (0 = chosen from the work size; the workspace
- * size depends on it), lb = multipoles per register block (4 or 8, 0 = 8). */
+ * increasing and > 0.  A job whose alpha_dir is NULL has its direct family
+ * skipped (the direct outputs are left as they are; see
+ * dwba_semisep_gemv_f64).  Tuning (results agree to rounding for every
+ * choice): seg = grid points per lane (0 = chosen from the work size; the
+ * workspace size depends on it), lb = multipoles per register block (4 or 8,
+ * 0 = 8). */
 size_t dwba_il_semisep_f64_workspace_bytes(int32_t n_jobs, const int32_t* h_n_rows, const int32_t* h_n_cols,
                                            int32_t L_max, int32_t seg);
 int32_t dwba_il_semisep_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const int32_t* h_n_rows,
@@ -281,6 +284,14 @@ typedef struct dwba_dgemv_energy {
 int32_t dwba_dgemv_f64(int32_t n_energies, const dwba_dgemv_energy* d_energies, int32_t max_rows,
                        int32_t L_max, void* stream);
 
+/* Same output as dwba_dgemv_f64 in O(N) per (energy, L) by the
+ * semiseparable recurrences (forward for j < i, backward for j > i):
+ * t_L(i) = (A_i + (1-1e-12)^L beta_i) / r_i + B_i.  The direct family of an
+ * excitation scan has one beta per energy (dwba_matrix_elements.py:1136-1160,
+ * oscillatory_integrals.py:2571), so I_p(L) = alpha_p^T t_L (dwba_ddot_f64). */
+int32_t dwba_semisep_gemv_f64(int32_t n_energies, const dwba_dgemv_energy* d_energies, int32_t L_max,
+                              void* stream);
+
 /* d_out[out_index*2*(L_max+1) + L] = sum_{i<n} alpha[i] t[L*ld + i], one
  * workgroup per pair. */
 typedef struct dwba_ddot_pair {
@@ -290,6 +301,14 @@ typedef struct dwba_ddot_pair {
 } dwba_ddot_pair;
 int32_t dwba_ddot_f64(int32_t n_pairs, const dwba_ddot_pair* d_pairs, int32_t L_max, double* d_out,
                       void* stream);
+
+/* The same dot products for pairs grouped by shared t: d_items holds n_items
+ * int32 pairs {first, count} (count <= 64), each a run of consecutive
+ * d_pairs entries with the same t and ld; alpha must be zero from n up to
+ * the largest n of its item (it is read there).
+ * Each t row is read once per item instead of once per pair.  L_max <= 31. */
+int32_t dwba_ddot_tiled_f64(int32_t n_items, const int32_t* d_items, const dwba_ddot_pair* d_pairs, int32_t L_max,
+                            double* d_out, void* stream);
 
 /* Per-pair assembly of I_L in place in d_out (layout [pair][2][L_max+1]):
  * contractions zeroed when non-finite in the oscillatory branch, tail added

@@ -64,13 +64,15 @@ def test_semisep_random_jobs(L_max):
     print(f"  worst |d|/sum|aKb| {w:.2e}", flush=True)
 
 
+@pytest.mark.parametrize("lb", [0, 4])
 @pytest.mark.parametrize("seg", [1, 3, 64, 100000])
-def test_semisep_lane_split_independent(seg):
-    """The result does not depend on how the grid is cut into lanes and tasks."""
+def test_semisep_lane_split_independent(seg, lb):
+    """The result does not depend on how the grid is cut into lanes and tasks
+    or on the L-block width."""
     from dwba_amd import _hip
     host, dev = _jobs(SPECS, seed=5)
-    print(f"\n[semisep] seg={seg}", flush=True)
-    got = _hip.il_semisep_f64_batch(dev, 20, seg=seg).cpu().numpy()
+    print(f"\n[semisep] seg={seg} lb={lb:#x}", flush=True)
+    got = _hip.il_semisep_f64_batch(dev, 20, seg=seg, lb=lb).cpu().numpy()
     _check(host, got, 20)
 
 
@@ -110,3 +112,43 @@ def test_semisep_rejects_bad_shapes():
         _hip.il_semisep_f64_batch([(r, r, r, r, r, 101, 100)], 4)
     with pytest.raises(ValueError):
         _hip.il_semisep_f64_batch([(r, r, r, r, r, 10, 100)], 64)
+
+
+@pytest.mark.parametrize("L_max", [0, 9, 30])
+def test_semisep_gemv_matches_dense_gemv(L_max):
+    """t_L = K_L beta per energy: O(N) recurrences against the dense FP64 GEMV
+    kernel and the oracle's long-double K_L x."""
+    import torch
+    from dwba_amd import _hip
+    from dwba_amd.batch import DgemvEnergy
+    from oracle import gradient as OGr
+    rng = np.random.default_rng(L_max)
+    lib = _hip.load_library()
+    _hip.ensure_init()
+    specs = [(3000, 1800), (1201, 1201), (4500, 10), (2, 2)]
+    grids, descs_a, descs_b, outs = [], [], [], []
+    for n, rows in specs:
+        g = G.exp_grid(n, 1e-5, 1e3)
+        beta = rng.standard_normal(n) * g.w_simpson
+        ld = (n + 31) // 32 * 32
+        ta = torch.zeros((L_max + 1, ld), dtype=torch.float64, device="cuda")
+        tb = torch.zeros_like(ta)
+        r_d, b_d = _t(g.r), _t(beta)
+        grids.append((g.r, beta, rows, r_d, b_d))
+        descs_a.append(DgemvEnergy(_hip.ptr(r_d), _hip.ptr(b_d), _hip.ptr(ta), n, rows, ld, L_max))
+        descs_b.append(DgemvEnergy(_hip.ptr(r_d), _hip.ptr(b_d), _hip.ptr(tb), n, rows, ld, L_max))
+        outs.append((ta, tb))
+    da = _hip.struct_array_to_device((DgemvEnergy * len(descs_a))(*descs_a), torch.device("cuda"))
+    db = _hip.struct_array_to_device((DgemvEnergy * len(descs_b))(*descs_b), torch.device("cuda"))
+    print(f"\n[semisep gemv] L_max={L_max}", flush=True)
+    _hip.check(lib.dwba_semisep_gemv_f64(len(specs), _hip.ptr(da), L_max, _hip.stream_handle()), "gemv")
+    _hip.check(lib.dwba_dgemv_f64(len(specs), _hip.ptr(db), max(r for _, r in specs), L_max, _hip.stream_handle()),
+               "dgemv")
+    torch.cuda.synchronize()
+    for (r, beta, rows, *_), (ta, tb) in zip(grids, outs):
+        a = ta.cpu().numpy()[:, :rows]
+        b = tb.cpu().numpy()[:, :rows]
+        ref = OGr.kmatvec_all_L(r, beta, L_max).astype(np.float64)[:, :rows]
+        mag = OGr.kmatvec_all_L(r, np.abs(beta), L_max).astype(np.float64)[:, :rows]
+        assert np.all(np.abs(a - ref) <= 1e-13 * mag + 1e-300), np.max(np.abs(a - ref) / mag)
+        assert np.all(np.abs(b - ref) <= 1e-13 * mag + 1e-300)
