@@ -232,6 +232,43 @@ def _events_ms(fn, steps, stream):
     return e0.elapsed_time(e1) / steps
 
 
+def semisep_work_flop(b):
+    """Algorithmic flops of one semiseparable contraction stage of ScanBatch b:
+    11 per (grid point, L, family) for the recurrences; with the shared-beta
+    path, the direct families of its pairs cost one recurrence per energy
+    (forward and backward: 2 x 11 per point and L) plus a dot (2 per row and L)."""
+    from dwba_amd.planner import PREP_FILON, PREP_STANDARD
+    nl = b.L_max + 1
+    gemv = bool(getattr(b, "direct_gemv", False))
+    shared = set()
+    W = 0.0
+    if gemv:
+        for ei, st in enumerate(b.E):
+            ks = [k for k in st.members if b.pairs[k][2].mode in (PREP_FILON, PREP_STANDARD)]
+            if ks and len({id(b.pairs[k][5]) for k in st.members}) == 1:
+                shared.update(ks)
+                W += 2 * 11.0 * st.N * nl
+                W += sum(2.0 * b.pairs[k][2].N1 * nl for k in ks)
+    for k, (ei, _, plan, *_rest) in enumerate(b.pairs):
+        if plan.rows <= 0:
+            continue
+        fams = 1 if k in shared else 2
+        W += 11.0 * fams * plan.cols * nl
+    return {"flop": W, "gemv": gemv}
+
+
+def _committed_traffic(stage, workload):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration) and recorded in profiles/traffic.json."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        return d.get(workload, {}).get(stage)
+    except (OSError, ValueError):
+        return None
+
+
 def _scan_leg(scan, L_max, precision, contraction, steps, warmup, graph, stream, tune=True):
     """Plan/build one ScanBatch (autotuned when it has tunable kernels), time
     `steps` graph replays after `warmup`, and take per-stage times from one
@@ -294,11 +331,12 @@ def bench_batched(args, ionization=False):
                                            stream, tune=not args.no_tune)
     tuned = getattr(b64s, "tuning", None)
     log(f"fp64 semiseparable: {ms64s:.2f} ms/step, stages {st64s}, tuned {tuned.params if tuned else None}")
-    mag = b64s.magnitudes()
+    mag = None if args.no_fp32 else b64s.magnitudes()
     sizes = {"alpha_beta_bytes": 0}
     for (ei, _, plan, *_rest) in b64s.pairs:
         # alpha (rows) and beta (cols) of both families, FP64
         sizes["alpha_beta_bytes"] += 2 * 8 * (plan.rows + plan.cols)
+    ss_work = semisep_work_flop(b64s)
     del b64s
     result = {
         "metric": METRIC, "value": n_int / (ms64s * 1e-3), "unit": "integrals/s", "n_gpus": 1,
@@ -322,14 +360,22 @@ def bench_batched(args, ionization=False):
     }
     t_c = st64s.get("plain")
     if t_c:
-        byts = sizes["alpha_beta_bytes"]
-        result["roofline"] = {"bound": "hbm", "achieved": byts / (t_c * 1e-3) / 1e9, "peak": 8000.0,
-                              "unit": "GB/s", "frac": byts / (t_c * 1e-3) / 8e12, "traffic": None,
-                              "kernel": "semisep_kernel<8> + semisep_join (dwba_il_semisep_f64)",
-                              "algorithmic_bytes_per_launch": byts, "kernel_ms": t_c,
-                              "note": "algorithmic bytes = alpha (n_rows) + beta (n_cols) of both families, FP64, "
-                                      "read once; the contraction is O(N) per (pair, L)"}
+        W = ss_work["flop"]
+        kern = "semisep_kernel<8,4> + semisep_join" + (" + semisep_gemv_kernel + ddot_tiled_kernel"
+                                                       if ss_work["gemv"] else "")
+        result["roofline"] = {"bound": "fp64-valu", "achieved": W / (t_c * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+                              "unit": "TFLOP/s", "frac": W / (t_c * 1e-3) / PEAK_FP64,
+                              "traffic": _committed_traffic("contraction", "batched_64x400"),
+                              "kernel": kern, "stage": "contraction", "stage_ms": t_c,
+                              "work_flop_per_launch": W, "algorithmic_bytes_per_launch": sizes["alpha_beta_bytes"],
+                              "note": "11 flop per (grid point, L, family) for the recurrences (3 FMA, 2 add+mul, 1 "
+                                      "power step); 2 per (point, L) for the dot of the shared-beta path; "
+                                      "algorithmic bytes = alpha + beta of both families, FP64, read once"}
 
+    if args.no_fp32:
+        if not args.no_cpu_baseline:
+            result["cpu_baseline"] = cpu_baseline_batched(scan, L_max)
+        return result
     b32, ms32, st32, res32 = _scan_leg(scan, L_max, "fp32", "dense", args.steps, args.warmup, graph, stream)
     wd, we = b32.work_flop()
     del b32
@@ -412,14 +458,26 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-bf16", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-MFMA (and BF16x3, FP64 dense) legs")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="library defaults instead of autotuned launch parameters")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first copy 512 MiB device-to-device twice (calibration of HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the I_L path runs and is timed on a single MI355X; "
               "there is no multi-GPU mode to launch.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        import torch
+        x = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(2):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(f"[bench] copy probe: {x.numel() * 8} bytes read and written per copy", file=sys.stderr, flush=True)
+        del x, y
     if args.workload == "ionization" and args.energies == 64:
         args.energies = 48          # the ionization workload has 48 ejected-energy points
     result = {"batched": bench_batched, "ionization": lambda a: bench_batched(a, ionization=True),

@@ -32,6 +32,8 @@ import numpy as np
 
 from . import planner
 from .planner import GridStats, PREP_PLAIN, PREP_FILON, PREP_STANDARD, PREP_ZERO
+
+PREP_NO_ZERO_FILL = 0x10   # include/dwba_hip.h: the batch's per-pair vectors are zero-initialised once
 from .dwba_matrix_elements import (PairPlan, RadialDWBAIntegrals, TAIL_NONE, TAIL_OUTER, TAIL_LEGACY, plan_pair,
                                    _fit)
 
@@ -250,7 +252,8 @@ class ScanBatch:
                                   hip.ptr(uf_d), hip.ptr(ci_d), hip.ptr(cf_d), hip.ptr(st.vdiff_d), hip.ptr(nodes),
                                   hip.ptr(outs[0]), hip.ptr(outs[1]), hip.ptr(outs[2]), hip.ptr(outs[3]),
                                   hip.ptr(self.scalars[k]), lay.p_start, lay.p_step, lay.p_end, st.N, plan.N1,
-                                  lay.n_pnodes, plan.mode, plan.log1, plan.log2)
+                                  lay.n_pnodes, plan.mode | (PREP_NO_ZERO_FILL if plan.mode == PREP_FILON else 0),
+                                  plan.log1, plan.log2)
         self.prep_d = hip.struct_array_to_device(prep, dev)
         self.coul_d = torch.as_tensor(coul_par.ravel(), device=dev) if any_coul else None
         self.any_coul = any_coul
@@ -621,15 +624,16 @@ class ScanBatch:
         nca = (ctypes.c_int32 * n)()
         keep = []
         pos = {k: c for st in self.E for c, k in enumerate(st.members)}
+        absm = [tuple(m.abs() for m in (st.ad, st.bd, st.ae, st.be)) for st in self.E]   # 4 launches per energy
+        abs_plain = {k: v.abs() for k, v in self._plain_vec.items()}
+        keep += [absm, abs_plain]
         for k, (ei, pi, plan, ci, cf, bf) in enumerate(self.pairs):
             st = self.E[ei]
             if k in pos:
                 c = pos[k]
-                vec = [st.ad[c], st.bd[c], st.ae[c], st.be[c]]
+                av = [m[c] for m in absm[ei]]
             else:
-                vec = list(self._plain_vec[k])
-            av = [v.abs() for v in vec]
-            keep += av
+                av = list(abs_plain[k])
             g = int(self.out_index[k])
             arr[g] = hip.PlainJob(hip.ptr(st.r_d), hip.ptr(av[0]), hip.ptr(av[1]), hip.ptr(av[2]), hip.ptr(av[3]),
                                   plan.rows, plan.cols)

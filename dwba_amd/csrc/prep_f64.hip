@@ -154,11 +154,20 @@ __device__ void filon_alpha_beta(const dwba_prep_job& J, const CCTable& cc) {
     const double rho_d_last = clean(J.chi_f[N1 - 1] * J.chi_i[N1 - 1]);
     const double rho_e_last = clean(J.u_f[N1 - 1] * J.chi_i[N1 - 1]);
     const double rho2_last = clean(J.chi_f[N - 1] * J.u_i[N - 1]);
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        J.alpha_dir[i] = 0.0;
-        J.alpha_ex[i] = 0.0;
-        J.beta_ex[i] = 0.0;
+    // Without DWBA_PREP_NO_ZERO_FILL the three vectors are cleared and every round
+    // adds its contributions.  With it, only the points a round touches are
+    // written: set on first touch, added where two consecutive rounds meet;
+    // every other point keeps the zero it was allocated with (the same pair
+    // touches the same points on every call).
+    const bool fill = !(J.mode & DWBA_PREP_NO_ZERO_FILL);
+    if (fill) {
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+            J.alpha_dir[i] = 0.0;
+            J.alpha_ex[i] = 0.0;
+            J.beta_ex[i] = 0.0;
+        }
     }
+    int prev_ja_hi = -1, prev_jb_hi = -1;       // last points of the previous round's ranges
     const int per = CH_NODES / ncc;               // intervals per round
     for (int m0 = 0; m0 < n_int; m0 += per) {
         const int m_end = min(n_int, m0 + per);
@@ -214,9 +223,19 @@ __device__ void filon_alpha_beta(const dwba_prep_job& J, const CCTable& cc) {
             int q = lo;
             for (; q < nq && s_ja[q] == i - 1; ++q) { sd += s_ad1[q]; se += s_ae1[q]; }
             for (; q < nq && s_ja[q] == i; ++q) { sd += s_ad0[q]; se += s_ae0[q]; }
-            if (i < N1 && (sd != 0.0 || se != 0.0)) {
-                J.alpha_dir[i] += sd;
-                J.alpha_ex[i] += se;
+            if (i < N1) {
+                if (fill) {
+                    if (sd != 0.0 || se != 0.0) {
+                        J.alpha_dir[i] += sd;
+                        J.alpha_ex[i] += se;
+                    }
+                } else if (i <= prev_ja_hi) {
+                    J.alpha_dir[i] += sd;
+                    J.alpha_ex[i] += se;
+                } else {
+                    J.alpha_dir[i] = sd;
+                    J.alpha_ex[i] = se;
+                }
             }
         }
         for (int i = jb_lo + (int)threadIdx.x; i <= jb_hi; i += blockDim.x) {
@@ -226,8 +245,18 @@ __device__ void filon_alpha_beta(const dwba_prep_job& J, const CCTable& cc) {
             int q = lo;
             for (; q < nq && s_jb[q] == i - 1; ++q) sb += s_b1[q];
             for (; q < nq && s_jb[q] == i; ++q) sb += s_b0[q];
-            if (i < N && sb != 0.0) J.beta_ex[i] += sb;
+            if (i < N) {
+                if (fill) {
+                    if (sb != 0.0) J.beta_ex[i] += sb;
+                } else if (i <= prev_jb_hi) {
+                    J.beta_ex[i] += sb;
+                } else {
+                    J.beta_ex[i] = sb;
+                }
+            }
         }
+        prev_ja_hi = ja_hi;
+        prev_jb_hi = jb_hi;
     }
 }
 
@@ -264,7 +293,8 @@ __global__ __launch_bounds__(PREP_THREADS) void prep_f64_kernel(const dwba_prep_
     }
 
     // ---------------- alpha / beta ----------------
-    if (J.mode == DWBA_PREP_PLAIN) {
+    const int mode = J.mode & DWBA_PREP_MODE_MASK;
+    if (mode == DWBA_PREP_PLAIN) {
         for (int i = threadIdx.x; i < N; i += blockDim.x) {
             const double wl = i < N1 ? J.w[i] : 0.0;
             const double ui = J.u_i[i], uf = J.u_f[i], ci = J.chi_i[i], cf = J.chi_f[i];
@@ -275,7 +305,7 @@ __global__ __launch_bounds__(PREP_THREADS) void prep_f64_kernel(const dwba_prep_
         }
         return;
     }
-    if (J.mode == DWBA_PREP_ZERO) {
+    if (mode == DWBA_PREP_ZERO) {
         for (int i = threadIdx.x; i < N; i += blockDim.x) {
             J.alpha_dir[i] = 0.0; J.beta_dir[i] = 0.0; J.alpha_ex[i] = 0.0; J.beta_ex[i] = 0.0;
         }
@@ -284,7 +314,7 @@ __global__ __launch_bounds__(PREP_THREADS) void prep_f64_kernel(const dwba_prep_
     // beta_dir = (u_f u_i) w on the full grid for both STANDARD and FILON
     for (int i = threadIdx.x; i < N; i += blockDim.x)
         J.beta_dir[i] = clean(J.u_f[i] * J.u_i[i]) * J.w[i];
-    if (J.mode == DWBA_PREP_STANDARD) {
+    if (mode == DWBA_PREP_STANDARD) {
         for (int i = threadIdx.x; i < N; i += blockDim.x) {
             const double w = J.w[i];
             const double wl = i < N1 ? w : 0.0;
@@ -332,7 +362,7 @@ __global__ void coulomb_nodes_kernel(dwba_prep_job* __restrict__ jobs, int n_job
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_jobs) return;
     dwba_prep_job& J = jobs[k];
-    if (J.mode != DWBA_PREP_FILON || J.phase_nodes == nullptr) return;
+    if ((J.mode & DWBA_PREP_MODE_MASK) != DWBA_PREP_FILON || J.phase_nodes == nullptr) return;
     const double a = par[4 * k + 0], b = par[4 * k + 1], kt = par[4 * k + 2], eta = par[4 * k + 3];
     const double dphi = J.p_step;  // phase increment for Coulomb jobs
     double* nodes = const_cast<double*>(J.phase_nodes);

@@ -128,6 +128,13 @@ enum {
     DWBA_PREP_STANDARD = 2,  /* Filon branch fallback to weighted dot products       */
     DWBA_PREP_ZERO = 3       /* idx_limit < 2 in the Filon branch: integral is 0     */
 };
+/* Flag or-ed into dwba_prep_job.mode (FILON mode): skip clearing alpha_dir,
+ * alpha_ex and beta_ex and write only the points the Clenshaw-Curtis nodes
+ * touch.  For callers that keep one zero-initialised set of vectors per pair
+ * and prepare the same pair again (a captured scan): the untouched points are
+ * the same on every call and keep their zeros. */
+#define DWBA_PREP_NO_ZERO_FILL 0x10
+#define DWBA_PREP_MODE_MASK 0x0f
 
 /* One channel pair.  Device pointers; outputs are the alpha and beta vectors (N doubles
  * each) and scalars[4] = {sum rho2_dir_w, rho1_dir_w.V_diff, sum rho2_ex_w,
