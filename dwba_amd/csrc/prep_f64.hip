@@ -266,18 +266,38 @@ __global__ __launch_bounds__(PREP_THREADS) void prep_f64_kernel(const dwba_prep_
     const int N = J.N, N1 = J.N1;
     __shared__ double s_red[PREP_THREADS / 64][4];
 
-    // ---------------- L = 0 correction factors ----------------
+    // ---------------- one streaming pass over the grid ----------------
+    // L = 0 correction factors over the whole grid with w_lim = 0 beyond N1,
+    // as the reference's dot products do (a non-finite density beyond N1 still
+    // propagates), and every output that is a pointwise product of the inputs.
+    const int mode = J.mode & DWBA_PREP_MODE_MASK;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    // over the whole grid with w_lim = 0 beyond N1, as the reference's dot
-    // products do (a non-finite density beyond N1 still propagates)
+#pragma unroll 4
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
         const double ui = J.u_i[i], uf = J.u_f[i], ci = J.chi_i[i], cf = J.chi_f[i];
-        const double w = i < N1 ? J.w[i] : 0.0;
+        const double wf = J.w[i];
+        const double w = i < N1 ? wf : 0.0;
         const double vd = J.V_diff ? J.V_diff[i] : 0.0;
         acc[0] += (w * uf) * ui;              // rho2_dir_w
         acc[1] += ((w * cf) * ci) * vd;       // rho1_dir_w . V_diff
         acc[2] += (w * cf) * ui;              // rho2_ex_w
         acc[3] += ((w * uf) * ci) * vd;       // rho1_ex_w . V_diff
+        if (mode == DWBA_PREP_PLAIN) {
+            J.alpha_dir[i] = (w * cf) * ci;
+            J.beta_dir[i] = (w * uf) * ui;
+            J.alpha_ex[i] = (w * uf) * ci;
+            J.beta_ex[i] = (w * cf) * ui;
+        } else if (mode == DWBA_PREP_ZERO) {
+            J.alpha_dir[i] = 0.0; J.beta_dir[i] = 0.0; J.alpha_ex[i] = 0.0; J.beta_ex[i] = 0.0;
+        } else {
+            // beta_dir = (u_f u_i) w on the full grid for both STANDARD and FILON
+            J.beta_dir[i] = clean(uf * ui) * wf;
+            if (mode == DWBA_PREP_STANDARD) {
+                J.alpha_dir[i] = clean(cf * ci) * w;
+                J.alpha_ex[i] = clean(uf * ci) * w;
+                J.beta_ex[i] = clean(cf * ui) * wf;
+            }
+        }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -291,39 +311,7 @@ __global__ __launch_bounds__(PREP_THREADS) void prep_f64_kernel(const dwba_prep_
         for (int wv = 0; wv < PREP_THREADS / 64; ++wv) sum += s_red[wv][threadIdx.x];
         J.scalars[threadIdx.x] = sum;
     }
-
-    // ---------------- alpha / beta ----------------
-    const int mode = J.mode & DWBA_PREP_MODE_MASK;
-    if (mode == DWBA_PREP_PLAIN) {
-        for (int i = threadIdx.x; i < N; i += blockDim.x) {
-            const double wl = i < N1 ? J.w[i] : 0.0;
-            const double ui = J.u_i[i], uf = J.u_f[i], ci = J.chi_i[i], cf = J.chi_f[i];
-            J.alpha_dir[i] = (wl * cf) * ci;
-            J.beta_dir[i] = (wl * uf) * ui;
-            J.alpha_ex[i] = (wl * uf) * ci;
-            J.beta_ex[i] = (wl * cf) * ui;
-        }
-        return;
-    }
-    if (mode == DWBA_PREP_ZERO) {
-        for (int i = threadIdx.x; i < N; i += blockDim.x) {
-            J.alpha_dir[i] = 0.0; J.beta_dir[i] = 0.0; J.alpha_ex[i] = 0.0; J.beta_ex[i] = 0.0;
-        }
-        return;
-    }
-    // beta_dir = (u_f u_i) w on the full grid for both STANDARD and FILON
-    for (int i = threadIdx.x; i < N; i += blockDim.x)
-        J.beta_dir[i] = clean(J.u_f[i] * J.u_i[i]) * J.w[i];
-    if (mode == DWBA_PREP_STANDARD) {
-        for (int i = threadIdx.x; i < N; i += blockDim.x) {
-            const double w = J.w[i];
-            const double wl = i < N1 ? w : 0.0;
-            J.alpha_dir[i] = clean(J.chi_f[i] * J.chi_i[i]) * wl;
-            J.alpha_ex[i] = clean(J.u_f[i] * J.chi_i[i]) * wl;
-            J.beta_ex[i] = clean(J.chi_f[i] * J.u_i[i]) * w;
-        }
-        return;
-    }
+    if (mode != DWBA_PREP_FILON) return;
 
     // FILON
     filon_alpha_beta(J, cc);
