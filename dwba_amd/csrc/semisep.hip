@@ -76,7 +76,7 @@ __device__ inline int tasks_of(int n_cols, int seg) {
 // HAS_A: alpha_i may be non-zero (i < n_rows); beyond the rows only the beta
 // recurrence and the decay of Z remain.
 template <int LB, bool HAS_A>
-__device__ __forceinline__ void ss_point(double ri, double r_next, double a_i, double b_i, int L0, const double* dL,
+__device__ __forceinline__ void ss_point(double ri, double r_next, double a_i, double b_i, int L0, double& diag,
                                          double* A, double* Z, double* acc, double* Ca, double* Cb, double* P,
                                          double& ir_cur) {
     const double iri = ir_cur;
@@ -85,12 +85,12 @@ __device__ __forceinline__ void ss_point(double ri, double r_next, double a_i, d
     const double bw = b_i * iri;
     double qL = pow_block<LB>(q, L0);
     if (HAS_A) {
-        const double aw = a_i * iri, ab = a_i * bw;
+        const double aw = a_i * iri;
+        diag = fma(a_i, bw, diag);             // the diagonal term factorises: d_L sum_i alpha_i beta_i / r_i
 #pragma unroll
         for (int l = 0; l < LB; ++l) {
             acc[l] = fma(aw, A[l], acc[l]);
             acc[l] = fma(bw, Z[l], acc[l]);
-            acc[l] = fma(dL[l], ab, acc[l]);
             Ca[l] = fma(aw, P[l], Ca[l]);
             Cb[l] = fma(bw, P[l], Cb[l]);
             A[l] = qL * (A[l] + b_i);
@@ -117,7 +117,7 @@ __device__ __forceinline__ void ss_point(double ri, double r_next, double a_i, d
 template <int LB, bool HAS_A>
 __device__ __forceinline__ void ss_range(const double* __restrict__ r, const double* __restrict__ al,
                                          const double* __restrict__ be, int n, int i0, int i1, int L0,
-                                         const double* dL, double* A, double* Z, double* acc, double* Ca, double* Cb,
+                                         double& diag, double* A, double* Z, double* acc, double* Ca, double* Cb,
                                          double* P, double& ir_cur) {
     int i = i0;
 #pragma unroll 1
@@ -132,11 +132,11 @@ __device__ __forceinline__ void ss_range(const double* __restrict__ r, const dou
         }
 #pragma unroll
         for (int u = 0; u < GU; ++u)
-            ss_point<LB, HAS_A>(rr[u], rr[u + 1], av[u], bv[u], L0, dL, A, Z, acc, Ca, Cb, P, ir_cur);
+            ss_point<LB, HAS_A>(rr[u], rr[u + 1], av[u], bv[u], L0, diag, A, Z, acc, Ca, Cb, P, ir_cur);
     }
 #pragma unroll 1
     for (; i < i1; ++i)
-        ss_point<LB, HAS_A>(r[i], r[min(i + 1, n - 1)], HAS_A ? al[i] : 0.0, be[i], L0, dL, A, Z, acc, Ca, Cb, P,
+        ss_point<LB, HAS_A>(r[i], r[min(i + 1, n - 1)], HAS_A ? al[i] : 0.0, be[i], L0, diag, A, Z, acc, Ca, Cb, P,
                             ir_cur);
 }
 
@@ -172,17 +172,17 @@ __global__ __launch_bounds__(ST) void semisep_kernel(const dwba_plain_job* __res
 #pragma unroll
     for (int blk = 0; blk < NB; ++blk) {
         const int L0 = blk * LB;             // a constant after unrolling: q^L0 is straight-line code
-        double A[LB], Z[LB], acc[LB], Ca[LB], Cb[LB], P[LB], dL[LB];
+        double A[LB], Z[LB], acc[LB], Ca[LB], Cb[LB], P[LB];
 #pragma unroll
         for (int l = 0; l < LB; ++l) {
             A[l] = 0.0; Z[l] = 0.0; acc[l] = 0.0; Ca[l] = 0.0; Cb[l] = 0.0; P[l] = 1.0;
-            const int L = L0 + l;
-            dL[l] = L == 0 ? 1.0 : exp((double)L * ldl);
         }
+        double diag = 0.0;
         double ir_cur = 1.0 / J.r[min(a, n - 1)];
         const int am = min(b, max(a, nr));               // [a, am): alpha may be non-zero
-        ss_range<LB, true>(J.r, al, be, n, a, am, L0, dL, A, Z, acc, Ca, Cb, P, ir_cur);
-        ss_range<LB, false>(J.r, al, be, n, am, b, L0, dL, A, Z, acc, Ca, Cb, P, ir_cur);
+        ss_range<LB, true>(J.r, al, be, n, a, am, L0, diag, A, Z, acc, Ca, Cb, P, ir_cur);
+        ss_range<LB, false>(J.r, al, be, n, am, b, L0, diag, A, Z, acc, Ca, Cb, P, ir_cur);
+        const double dsum = wave_sum(diag);   // this task's sum_i alpha_i beta_i / r_i
         // join the lanes: inclusive scan of (M, A, Z) over lanes, M = P at lane end
 #pragma unroll
         for (int l = 0; l < LB; ++l) {
@@ -202,7 +202,9 @@ __global__ __launch_bounds__(ST) void semisep_kernel(const dwba_plain_job* __res
             // exclusive prefix: carry into this lane and the factor from the task start
             double Ain = __shfl_up(Ai, 1, 64), Zin = __shfl_up(Zi, 1, 64), Mpre = __shfl_up(M, 1, 64);
             if (lane == 0) { Ain = 0.0; Zin = 0.0; Mpre = 1.0; }
-            const double tot = wave_sum(acc[l] + Ain * Ca[l] + Zin * Cb[l]);
+            const int L = L0 + l;
+            const double tot = wave_sum(acc[l] + Ain * Ca[l] + Zin * Cb[l]) +
+                               (L == 0 ? 1.0 : exp((double)L * ldl)) * dsum;
             const double ca = wave_sum(Mpre * Ca[l]);
             const double cb = wave_sum(Mpre * Cb[l]);
             const double A_out = __shfl(Ai, 63, 64), Z_out = __shfl(Zi, 63, 64), M_out = __shfl(M, 63, 64);
