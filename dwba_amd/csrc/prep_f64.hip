@@ -59,204 +59,116 @@ __device__ inline int interval_of(const dwba_prep_job& J, double x, int n_int) {
     return m < 0 ? 0 : (m > n_int - 1 ? n_int - 1 : m);
 }
 
-// np.interp cell on a table xp[0..n-1] (ascending): largest j <= n-2 with
-// xp[j] <= X; X is already clipped to [xp[0], xp[n-1]].
-__device__ inline int cell_of(const double* xp, int n, double X) {
-    int lo = 0, hi = n - 2;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (xp[mid] <= X) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
-// Value of np.interp(X, xp, fp) at cell j (slope form of numpy's kernel).
-__device__ inline double interp_at(const double* xp, int n, double X, int j, double f0, double f1,
-                                   double f_last) {
-    if (X >= xp[n - 1]) return f_last;
-    if (X == xp[j]) return f0;
-    const double slope = (f1 - f0) / (xp[j + 1] - xp[j]);
-    return slope * (X - xp[j]) + f0;
-}
-
-// Largest j in [0, n-2] with xp[j] <= X, X in [xp[0], xp[n-1]] (np.interp's
-// cell).  The start is guessed from a uniform table and corrected by a short
-// walk; a long walk (strongly non-uniform table) falls back to bisection.
-__device__ inline int cell_fast(const double* xp, int n, double X) {
-    if (n <= 2) return 0;
-    const double x0 = xp[0], x1 = xp[n - 1];
-    double f = (X - x0) / (x1 - x0) * (double)(n - 1);
-    int j = f <= 0.0 ? 0 : (f >= (double)(n - 2) ? n - 2 : (int)f);
-    for (int s = 0; s < 16; ++s) {
-        if (xp[j] > X && j > 0) { --j; continue; }
-        if (j < n - 2 && xp[j + 1] <= X) { ++j; continue; }
-        return j;
-    }
-    return cell_of(xp, n, X);
-}
-
-// numpy searchsorted(r, x, 'left') clipped to [1, n-1], guessed in log r.
-__device__ inline int right_col(const double* r, const double* lr, int n, double x) {
-    int lo;   // number of r[j] < x
-    if (!(r[0] < x)) lo = 0;
-    else if (r[n - 1] < x) lo = n;
-    else {
-        // largest j with r[j] < x, in [0, n-2]
-        const double f = (log(x) - lr[0]) / (lr[n - 1] - lr[0]) * (double)(n - 1);
-        int j = f <= 0.0 ? 0 : (f >= (double)(n - 2) ? n - 2 : (int)f);
-        int s = 0;
-        for (; s < 16; ++s) {
-            if (!(r[j] < x) && j > 0) { --j; continue; }
-            if (j < n - 2 && r[j + 1] < x) { ++j; continue; }
-            break;
-        }
-        if (s == 16) {
-            int a = 0, b = n;
-            while (a < b) {
-                const int mid = (a + b) >> 1;
-                if (r[mid] < x) a = mid + 1; else b = mid;
-            }
-            j = a - 1;
-        }
-        lo = j + 1;
-    }
-    return lo < 1 ? 1 : (lo > n - 1 ? n - 1 : lo);
-}
-
 struct CCTable {
     double x[DWBA_MAX_CC];
     double w[DWBA_MAX_CC];
     int n;
 };
 
-constexpr int CH_NODES = 512;    // CC nodes per LDS round
-
-// FILON mode.  Every CC node c (interval m, node k, nodes ascending in x) is
-// evaluated once per pair: its alpha cell ja (np.interp on r[:N1]) with the
-// weights W f1(x) (1-t), W f1(x) t for direct and exchange, and its beta
-// columns il = ir - 1 (searchsorted, linear-in-r kernel interpolation) with
-// W f2(x) wl, W f2(x) wr.  Cells are non-decreasing along the node list, so
-// a grid point finds the nodes that touch it by bisection in LDS and adds
-// them in node order: deterministic, no atomics.
+// FILON mode as one pass over the grid cells [r_j, r_{j+1}) of r[:N1].
+// A CC node c at x in cell j (interval m, node k, weight W = w_k (b - a)/2)
+// adds
+//   alpha: W interp(rho1)(x) (1 - t), W interp(rho1)(x) t    to points j, j+1,
+//          interp(rho1)(x) = (1 - t) rho1_j + t rho1_{j+1}, t the np.interp
+//          weight in log r or r;
+//   beta_ex: W interp(rho2)(x) wl, W interp(rho2)(x) wr      to columns j, j+1,
+//          wl, wr the linear-in-r kernel weights.
+// Summed over the nodes of the cell this is a 2x2 operator on the densities
+// at j, j+1 whose coefficients (sums of W t^a (1-t)^b) need no density, so a
+// thread owns one cell, enumerates its nodes by position (nodes ascend in x
+// along (m, k)), and point i takes the cell-i part plus the cell-(i-1) part
+// from its neighbour.  Nodes lie in [r_0, r_{N1-1}]; the last cell is closed
+// on the right.  Every node lands in exactly one cell; because np.interp and
+// the kernel weights are continuous, a node on a cell edge gives the same
+// contribution (to rounding) from either side.
 __device__ void filon_alpha_beta(const dwba_prep_job& J, const CCTable& cc) {
-    __shared__ int s_ja[CH_NODES], s_jb[CH_NODES];
-    __shared__ double s_ad0[CH_NODES], s_ad1[CH_NODES], s_ae0[CH_NODES], s_ae1[CH_NODES];
-    __shared__ double s_b0[CH_NODES], s_b1[CH_NODES];
+    __shared__ double s_s1[PREP_THREADS][3];
     const int N = J.N, N1 = J.N1;
     const int n_int = J.n_pnodes - 1;
     const int ncc = cc.n;
     const double* r = J.r;
     const double* lr = J.logr;                    // log(max(r, 1e-300))
-    const double* xp1 = J.log1 ? lr : r;          // interpolation table on r[:N1]
-    const double* xp2 = J.log2 ? lr : r;          // interpolation table on r[:N]
-    const double r0 = r[0], r1e = r[N1 - 1], r2e = r[N - 1];
-    const double rlog0 = r0 > 1e-300 ? r0 : 1e-300;
-    const double rho_d_last = clean(J.chi_f[N1 - 1] * J.chi_i[N1 - 1]);
-    const double rho_e_last = clean(J.u_f[N1 - 1] * J.chi_i[N1 - 1]);
-    const double rho2_last = clean(J.chi_f[N - 1] * J.u_i[N - 1]);
-    // Without DWBA_PREP_NO_ZERO_FILL the three vectors are cleared and every round
-    // adds its contributions.  With it, only the points a round touches are
-    // written: set on first touch, added where two consecutive rounds meet;
-    // every other point keeps the zero it was allocated with (the same pair
-    // touches the same points on every call).
+    const bool lg1 = J.log1 != 0, lg2 = J.log2 != 0;
     const bool fill = !(J.mode & DWBA_PREP_NO_ZERO_FILL);
+    double carry_d = 0.0, carry_e = 0.0, carry_b = 0.0;   // cell-(i-1) parts across chunks
+    for (int base = 0; base < N1; base += PREP_THREADS) {
+        const int j = base + (int)threadIdx.x;
+        double s0d = 0.0, s1d = 0.0, s0e = 0.0, s1e = 0.0, s0b = 0.0, s1b = 0.0;
+        if (j < N1 - 1) {
+            const double rl = r[j], rr = r[j + 1];
+            const double ll = lr[j], lrr = lr[j + 1];
+            const bool last = (j == N1 - 2);
+            const double x1l = lg1 ? ll : rl, x1r = lg1 ? lrr : rr;
+            const double x2l = lg2 ? ll : rl, x2r = lg2 ? lrr : rr;
+            const double inv1 = x1r > x1l ? 1.0 / (x1r - x1l) : 0.0;
+            const double inv2 = x2r > x2l ? 1.0 / (x2r - x2l) : 0.0;
+            const double invr = 1.0 / (rr - rl + 1e-30);
+            double c00 = 0.0, c01 = 0.0, c11 = 0.0, b00 = 0.0, b01 = 0.0, b10 = 0.0, b11 = 0.0;
+            // first interval that can hold a node >= r_j (one back for rounding)
+            int m = interval_of(J, rl, n_int) - 1;
+            m = m < 0 ? 0 : m;
+            bool done = false;
+            for (; m < n_int && !done; ++m) {
+                const double a = phase_node(J, m), b = phase_node(J, m + 1);
+                if (a >= rr && !last) break;
+                if (!(b - a > 1e-12)) continue;       // zero-weight interval
+                const double hw = 0.5 * (b - a);
+                for (int k = 0; k < ncc; ++k) {
+                    // clamped into [a, b] so that x never decreases along the node list
+                    const double x = fmin(fmax(hw * (cc.x[k] + 1.0) + a, a), b);
+                    if (x < rl) continue;
+                    if (x >= rr && !last) { done = true; break; }
+                    const double W = cc.w[k] * hw;
+                    const double X = (lg1 || lg2) ? log(fmax(x, 1e-300)) : x;
+                    const double t1 = fmin(fmax(((lg1 ? X : x) - x1l) * inv1, 0.0), 1.0);
+                    const double t2 = lg1 == lg2 ? t1 : fmin(fmax(((lg2 ? X : x) - x2l) * inv2, 0.0), 1.0);
+                    const double wr = (x - rl) * invr, wl = 1.0 - wr;
+                    const double u1 = 1.0 - t1, u2 = 1.0 - t2;
+                    const double Wu = W * u1, Wt = W * t1;
+                    c00 = fma(Wu, u1, c00);
+                    c01 = fma(Wu, t1, c01);
+                    c11 = fma(Wt, t1, c11);
+                    const double Wl = W * wl, Wr = W * wr;
+                    b00 = fma(Wl, u2, b00);
+                    b01 = fma(Wl, t2, b01);
+                    b10 = fma(Wr, u2, b10);
+                    b11 = fma(Wr, t2, b11);
+                }
+            }
+            const double ci0 = J.chi_i[j], ci1 = J.chi_i[j + 1];
+            const double d0 = clean(J.chi_f[j] * ci0), d1 = clean(J.chi_f[j + 1] * ci1);
+            const double e0 = clean(J.u_f[j] * ci0), e1 = clean(J.u_f[j + 1] * ci1);
+            const double f0 = clean(J.chi_f[j] * J.u_i[j]), f1 = clean(J.chi_f[j + 1] * J.u_i[j + 1]);
+            s0d = c00 * d0 + c01 * d1;
+            s1d = c01 * d0 + c11 * d1;
+            s0e = c00 * e0 + c01 * e1;
+            s1e = c01 * e0 + c11 * e1;
+            s0b = b00 * f0 + b01 * f1;
+            s1b = b10 * f0 + b11 * f1;
+        }
+        s_s1[threadIdx.x][0] = s1d;
+        s_s1[threadIdx.x][1] = s1e;
+        s_s1[threadIdx.x][2] = s1b;
+        __syncthreads();
+        if (j < N1) {
+            const bool first = threadIdx.x == 0;
+            J.alpha_dir[j] = s0d + (first ? carry_d : s_s1[threadIdx.x - 1][0]);
+            J.alpha_ex[j] = s0e + (first ? carry_e : s_s1[threadIdx.x - 1][1]);
+            J.beta_ex[j] = s0b + (first ? carry_b : s_s1[threadIdx.x - 1][2]);
+        }
+        carry_d = s_s1[PREP_THREADS - 1][0];
+        carry_e = s_s1[PREP_THREADS - 1][1];
+        carry_b = s_s1[PREP_THREADS - 1][2];
+        __syncthreads();                          // s_s1 is rewritten by the next chunk
+    }
+    // Without DWBA_PREP_NO_ZERO_FILL the points beyond N1 (no node reaches
+    // them) are cleared; with it they keep the zeros they were allocated with.
     if (fill) {
-        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        for (int i = N1 + (int)threadIdx.x; i < N; i += PREP_THREADS) {
             J.alpha_dir[i] = 0.0;
             J.alpha_ex[i] = 0.0;
             J.beta_ex[i] = 0.0;
         }
-    }
-    int prev_ja_hi = -1, prev_jb_hi = -1;       // last points of the previous round's ranges
-    const int per = CH_NODES / ncc;               // intervals per round
-    for (int m0 = 0; m0 < n_int; m0 += per) {
-        const int m_end = min(n_int, m0 + per);
-        const int nq = (m_end - m0) * ncc;
-        __syncthreads();                          // previous round's gathers are done
-        for (int q = threadIdx.x; q < nq; q += blockDim.x) {
-            const int m = m0 + q / ncc, k = q - (q / ncc) * ncc;
-            const double a = phase_node(J, m), b = phase_node(J, m + 1);
-            const double hw = 0.5 * (b - a);
-            // clamped into [a, b] so that x never decreases along the node list
-            const double x = fmin(fmax(hw * (cc.x[k] + 1.0) + a, a), b);
-            const double W = (b - a > 1e-12) ? cc.w[k] * hw : 0.0;
-            // alpha: np.interp of rho1 on r[:N1]
-            const double xc = fmin(fmax(x, r0), r1e);
-            const double X = J.log1 ? log(fmax(xc, rlog0)) : xc;
-            const int j = cell_fast(xp1, N1, X);
-            double t;
-            if (X >= xp1[N1 - 1]) t = 1.0;
-            else if (X == xp1[j]) t = 0.0;
-            else t = (X - xp1[j]) / (xp1[j + 1] - xp1[j]);
-            const double fd0 = clean(J.chi_f[j] * J.chi_i[j]), fd1 = clean(J.chi_f[j + 1] * J.chi_i[j + 1]);
-            const double fe0 = clean(J.u_f[j] * J.chi_i[j]), fe1 = clean(J.u_f[j + 1] * J.chi_i[j + 1]);
-            const double vd = W * interp_at(xp1, N1, X, j, fd0, fd1, rho_d_last);
-            const double ve = W * interp_at(xp1, N1, X, j, fe0, fe1, rho_e_last);
-            s_ja[q] = j;
-            s_ad0[q] = vd * (1.0 - t);
-            s_ad1[q] = vd * t;
-            s_ae0[q] = ve * (1.0 - t);
-            s_ae1[q] = ve * t;
-            // beta_ex: kernel interpolation linear in r between columns il, ir
-            const int ir = right_col(r, lr, N, x);
-            const int il = ir - 1;
-            const double wr = (x - r[il]) / (r[ir] - r[il] + 1e-30);
-            const double wl = 1.0 - wr;
-            const double xc2 = fmin(fmax(x, r0), r2e);
-            const double X2 = J.log2 ? log(fmax(xc2, rlog0)) : xc2;
-            const int j2 = cell_fast(xp2, N, X2);
-            const double f0 = clean(J.chi_f[j2] * J.u_i[j2]), f1 = clean(J.chi_f[j2 + 1] * J.u_i[j2 + 1]);
-            const double v = W * interp_at(xp2, N, X2, j2, f0, f1, rho2_last);
-            s_jb[q] = il;
-            s_b0[q] = v * wl;
-            s_b1[q] = v * wr;
-        }
-        __syncthreads();
-        // gather onto the grid points this round touches
-        const int ja_lo = s_ja[0], ja_hi = s_ja[nq - 1] + 1;
-        const int jb_lo = s_jb[0], jb_hi = s_jb[nq - 1] + 1;
-        for (int i = ja_lo + (int)threadIdx.x; i <= ja_hi; i += blockDim.x) {
-            // nodes with ja == i-1 contribute their t-part, nodes with ja == i their (1-t)-part
-            int lo = 0, hi = nq;                  // first q with ja >= i-1
-            while (lo < hi) { const int mid = (lo + hi) >> 1; if (s_ja[mid] < i - 1) lo = mid + 1; else hi = mid; }
-            double sd = 0.0, se = 0.0;
-            int q = lo;
-            for (; q < nq && s_ja[q] == i - 1; ++q) { sd += s_ad1[q]; se += s_ae1[q]; }
-            for (; q < nq && s_ja[q] == i; ++q) { sd += s_ad0[q]; se += s_ae0[q]; }
-            if (i < N1) {
-                if (fill) {
-                    if (sd != 0.0 || se != 0.0) {
-                        J.alpha_dir[i] += sd;
-                        J.alpha_ex[i] += se;
-                    }
-                } else if (i <= prev_ja_hi) {
-                    J.alpha_dir[i] += sd;
-                    J.alpha_ex[i] += se;
-                } else {
-                    J.alpha_dir[i] = sd;
-                    J.alpha_ex[i] = se;
-                }
-            }
-        }
-        for (int i = jb_lo + (int)threadIdx.x; i <= jb_hi; i += blockDim.x) {
-            int lo = 0, hi = nq;
-            while (lo < hi) { const int mid = (lo + hi) >> 1; if (s_jb[mid] < i - 1) lo = mid + 1; else hi = mid; }
-            double sb = 0.0;
-            int q = lo;
-            for (; q < nq && s_jb[q] == i - 1; ++q) sb += s_b1[q];
-            for (; q < nq && s_jb[q] == i; ++q) sb += s_b0[q];
-            if (i < N) {
-                if (fill) {
-                    if (sb != 0.0) J.beta_ex[i] += sb;
-                } else if (i <= prev_jb_hi) {
-                    J.beta_ex[i] += sb;
-                } else {
-                    J.beta_ex[i] = sb;
-                }
-            }
-        }
-        prev_ja_hi = ja_hi;
-        prev_jb_hi = jb_hi;
     }
 }
 
