@@ -21,7 +21,8 @@
 // the lanes are combined in double-double; the result agrees with math.fsum
 // to ~1e-15 of the segment size.
 //
-// Lanes of the wavefront take segments round-robin.
+// Lanes of the wavefront take segments: contiguous runs on the all-L Filon
+// path, round-robin on the per-L Levin path.
 #include "common.hpp"
 
 namespace dwba {
@@ -361,33 +362,75 @@ __device__ double analytic_tail(double r_m, const dwba_tail_job& J, int L, doubl
     return tail * overlap;
 }
 
-// Filon segments of one (pair, sign), all L.  Lanes take segments
-// round-robin; a segment of neff (odd) equally spaced nodes is walked triple
-// by triple (the recursion of _filon_segment_complex), so only three nodes
-// are live at a time whatever neff is.  Per triple the exponentials and the
-// Filon coefficients are formed once and all L are swept with x^-(L+1) built
-// by repeated multiplication.  The envelope is M_L x^-(L+1); M_L multiplies
-// the finished sum.  Each lane adds its triples' real parts to per-L FP64
-// partial sums (a few hundred terms of similar size); the lanes are combined
-// in double-double.  The reference sums segments with math.fsum; the two
-// agree to ~1e-15 of the segment size.
+// x^-(L+1) sums of two nodes for L = 1 .. NL-1 (ix = 1/x, 0 where the
+// envelope is cut off, x <= 1e-6); M_L multiplies the whole sum at the end.
+template <int NL>
+__device__ __forceinline__ void sweep2(double* acc, double w0, double ix0, double w1, double ix1) {
+    double p0x = ix0 * ix0, p1x = ix1 * ix1;
+#pragma unroll
+    for (int L = 1; L < NL; ++L) {
+        acc[L] = fma(w0, p0x, acc[L]);
+        acc[L] = fma(w1, p1x, acc[L]);
+        p0x *= ix0;
+        p1x *= ix1;
+    }
+}
+
+// Filon segments of one (pair, sign), all L.  Each lane takes a contiguous
+// run of segments; a segment of neff (odd) equally spaced nodes is walked
+// triple by triple (the recursion of _filon_segment_complex).  Per triple
+// the real part of the Filon sum is wa f(x0) + wm f(x1) + wb f(x2), so a
+// node shared by two triples (or two segments) is swept once with the sum of
+// its weights.  The Filon coefficients depend on theta = omega * step only
+// and are formed once per segment; the node phases exp(i (omega x + p0))
+// advance by the rotation exp(i theta) from the segment start (the reference
+// evaluates each node phase directly: the two agree to a few ulp of the
+// phase).  Each lane adds to per-L FP64 partial sums (a few hundred terms of
+// similar size); the lanes are combined in double-double.  The reference
+// sums segments with math.fsum; the two agree to ~1e-15 of the segment size.
 template <int NL>
 __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, double om, double p0, double r_m,
                                             double r_max, double bstep, long nseg, int lane, double* acc) {
     const int nt = (neff - 1) / 2;
     const double inv_om = -1.0 / om;
-    for (long sg = lane; sg < nseg; sg += 64) {
+    const long per = (nseg + 63) / 64;
+    const long s_lo = (long)lane * per;
+    const long s_hi = s_lo + per < nseg ? s_lo + per : nseg;
+    double w_pend = 0.0, ix_pend = 0.0;           // weight still to sweep at the lane's current node
+    for (long sg = s_lo; sg < s_hi; ++sg) {
         const double a = __dadd_rn(__dmul_rn((double)sg, bstep), r_m);
         const double b = (sg + 1 == nseg) ? r_max : __dadd_rn(__dmul_rn((double)(sg + 1), bstep), r_m);
         const double step = (b - a) / (neff - 1);
         const double hseg = b - a;
         if (hseg < 1e-15) continue;
+        double ix0 = a > 1e-6 ? 1.0 / a : 0.0;
+        if (ix0 != ix_pend && w_pend != 0.0) {    // the pending node is not this segment's first
+            sweep2<NL>(acc, w_pend, ix_pend, 0.0, 0.0);
+            w_pend = 0.0;
+        }
         const bool small = fabs(om * hseg) < 1e-3;
         const double cmid = small ? cos(om * (a + b) / 2 + p0) : 0.0;
+        // rotation by theta = omega * step and the Filon coefficients at theta
+        double rs = 0.0, rc = 1.0, s0 = 0.0, c0 = 0.0;
+        double al = 0.0, be = 0.0, ga = 0.0;
+        if (!small) {
+            const double th = om * step;
+            sincos(th, &rs, &rc);
+            sincos(om * a + p0, &s0, &c0);
+            if (fabs(th) < 0.3) {
+                const double t2 = th * th, t3 = th * th * th;
+                al = 2 * t3 / 45 * (1 - t2 / 7 + t2 * t2 / 189);
+                be = 2.0 / 3 + 2 * t2 / 15 - 4 * (t2 * t2) / 105 + 2 * (t2 * t2 * t2) / 567;
+                ga = 4.0 / 3 - 2 * t2 / 15 + 1 * (t2 * t2) / 210 - (t2 * t2 * t2) / 11340;
+            } else {
+                const double sd = 2.0 * rs * rc, cd = 1.0 - 2.0 * rs * rs;   // sin, cos of 2 theta
+                const double t2 = th * th, it3 = 1.0 / (th * th * th);
+                al = (t2 + th * sd / 2 - 2 * (rs * rs)) * it3;
+                be = 2 * (th * (1 + cd) - sd) * it3;
+                ga = 4 * (rs - th * rc) * it3;
+            }
+        }
         double x0 = a;
-        double ix0 = x0 > 1e-6 ? 1.0 / x0 : 0.0;
-        double s0 = 0.0, c0 = 0.0;
-        if (!small) sincos(om * x0 + p0, &s0, &c0);
 #pragma unroll 1
         for (int t = 0; t < nt; ++t) {
             const int k1 = 2 * t + 1, k2 = 2 * t + 2;
@@ -396,7 +439,9 @@ __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, do
             const double ix1 = x1 > 1e-6 ? 1.0 / x1 : 0.0;
             const double ix2 = x2 > 1e-6 ? 1.0 / x2 : 0.0;
             const double h3 = x2 - x0;
-            // per-triple weights: re = wa f0 + wm f1 + wb f2 for every L
+            // phases at x1, x2 by rotation from x0
+            const double s1 = fma(s0, rc, c0 * rs), c1 = fma(c0, rc, -(s0 * rs));
+            const double s2 = fma(s1, rc, c1 * rs), c2 = fma(c1, rc, -(s1 * rs));
             double wa, wm, wb;
             if (small || fabs(om * h3) < 1e-3 || h3 < 1e-15) {
                 // simpson_odd on the triple times cos of the mid phase (of the
@@ -409,50 +454,23 @@ __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, do
                 wa = q * (2.0 - (h0divh1 != 0.0 ? 1.0 / h0divh1 : 0.0)) * wscale;
                 wm = q * (hsum * (hprod != 0.0 ? hsum / hprod : 0.0)) * wscale;
                 wb = q * (2.0 - h0divh1) * wscale;
-                if (!small) sincos(om * x2 + p0, &s0, &c0);   // node x2 starts the next triple
             } else {
-                double s1, c1, s2, c2;
-                sincos(om * x1 + p0, &s1, &c1);
-                sincos(om * x2 + p0, &s2, &c2);
-                const double th = om * h3 / 2;
-                double al, be, ga;
-                if (fabs(th) < 0.3) {
-                    const double t2 = th * th, t3 = th * th * th;
-                    al = 2 * t3 / 45 * (1 - t2 / 7 + t2 * t2 / 189);
-                    be = 2.0 / 3 + 2 * t2 / 15 - 4 * (t2 * t2) / 105 + 2 * (t2 * t2 * t2) / 567;
-                    ga = 4.0 / 3 - 2 * t2 / 15 + 1 * (t2 * t2) / 210 - (t2 * t2 * t2) / 11340;
-                } else {
-                    double st, ct, sd, cd;
-                    sincos(th, &st, &ct);
-                    sincos(2 * th, &sd, &cd);
-                    const double t2 = th * th, it3 = 1.0 / (th * th * th);
-                    al = (t2 + th * sd / 2 - 2 * (st * st)) * it3;
-                    be = 2 * (th * (1 + cd) - sd) * it3;
-                    ga = 4 * (st - th * ct) * it3;
-                }
                 // Re[(h/2)(al (-i/om)(f2 e2 - f0 e0) + be (f0 e0 + f2 e2) + ga f1 e1)]
                 const double hh = h3 / 2, cA = al * inv_om;
                 wa = hh * (cA * s0 + be * c0);
                 wb = hh * (-cA * s2 + be * c2);
                 wm = hh * (ga * c1);
-                s0 = s2; c0 = c2;
             }
-            // x^-(L+1) at L = 1 (ix = 0 where the envelope is cut off, x <= 1e-6);
-            // M_L multiplies the whole sum at the end
-            double p0x = ix0 * ix0, p1x = ix1 * ix1, p2x = ix2 * ix2;
-#pragma unroll
-            for (int L = 1; L < NL; ++L) {
-                acc[L] = fma(wa, p0x, acc[L]);
-                acc[L] = fma(wm, p1x, acc[L]);
-                acc[L] = fma(wb, p2x, acc[L]);
-                p0x *= ix0;
-                p1x *= ix1;
-                p2x *= ix2;
-            }
+            sweep2<NL>(acc, w_pend + wa, ix0, wm, ix1);
+            w_pend = wb;
+            ix_pend = ix2;
             x0 = x2;
             ix0 = ix2;
+            s0 = s2;
+            c0 = c2;
         }
     }
+    if (w_pend != 0.0) sweep2<NL>(acc, w_pend, ix_pend, 0.0, 0.0);
 }
 
 // Phases and the Filon / Levin choice of compute_outer_integral_oscillatory
