@@ -160,3 +160,13 @@ SEMISEP_LBS = (4, 8)
 
 def semisep_candidates() -> List[Dict[str, int]]:
     return [{"seg": s, "lb": b} for b in SEMISEP_LBS for s in SEMISEP_SEGS]
+
+
+def device_id(dev) -> str:
+    """Architecture and compute-unit count of a device: stable across
+    processes and profilers (the marketing name can read empty under a
+    profiler's interception layer)."""
+    import torch
+    p = torch.cuda.get_device_properties(dev)
+    arch = getattr(p, "gcnArchName", "") or p.name
+    return f"{arch.split(':')[0]}x{p.multi_processor_count}"

@@ -368,6 +368,7 @@ class ScanBatch:
                             hip.struct_array_to_device((DdotPair * len(dd))(*dd), self.dev), len(dd),
                             items_d, len(items))
         arr_split = (hip.PlainJob * self.n_pairs)()
+        nca_split = (ctypes.c_int32 * self.n_pairs)()
         for g, k in enumerate(order):
             ei, pi, plan, ci, cf, bf = self.pairs[k]
             st = self.E[ei]
@@ -375,17 +376,22 @@ class ScanBatch:
             arr[g] = hip.PlainJob(hip.ptr(st.r_d), hip.ptr(st.ad[c]), hip.ptr(st.bd[c]), hip.ptr(st.ae[c]),
                                   hip.ptr(st.be[c]), plan.rows, plan.cols)
             skip_dir = k in shared_direct      # direct family through the shared-beta GEMV instead
+            # exchange alone: alpha_ex and beta_ex of Filon / plain pairs vanish at
+            # and beyond N1, so the O(N) scan stops there
+            cols_split = plan.rows if (skip_dir and plan.mode in (PREP_FILON, PREP_PLAIN)) else plan.cols
             arr_split[g] = hip.PlainJob(hip.ptr(st.r_d), 0 if skip_dir else hip.ptr(st.ad[c]),
                                         0 if skip_dir else hip.ptr(st.bd[c]), hip.ptr(st.ae[c]), hip.ptr(st.be[c]),
-                                        plan.rows, plan.cols)
+                                        plan.rows, cols_split)
             nra[g], nca[g] = plan.rows, plan.cols
+            nca_split[g] = cols_split
         self.plain_d = hip.struct_array_to_device(arr, self.dev)
         self.plain_d_split = hip.struct_array_to_device(arr_split, self.dev) if self.ss_gemv is not None else None
+        self.plain_n_split = (nra, nca_split)
         self.direct_gemv = False          # chosen by tune() when the shared-beta path exists
         self.plain_n = (nra, nca)
         if self.contraction == "semiseparable":
             self.semisep_config = (0, 0)      # (seg, lb): library defaults until tuned
-            self.plain_ws_bytes = self.lib.dwba_il_semisep_f64_workspace_bytes(self.n_pairs, nra, nca, self.L_max, 0)
+            self.plain_ws_bytes = self._semisep_ws_bytes(0)
         else:
             self.plain_ws_bytes = self.lib.dwba_il_plain_f64_batch_workspace_bytes(self.n_pairs, nra, nca,
                                                                                    self.L_max)
@@ -512,12 +518,22 @@ class ScanBatch:
                 raise ValueError("grid too coarse for the FP32 factorised kernel (row block spans "
                                  f"r in [{st.gs.r[a]:.3g}, {st.gs.r[b]:.3g}]); use precision='fp64'")
 
+    def _semisep_ws_bytes(self, seg):
+        """Workspace of the semiseparable contraction for either job set (all
+        families, or exchange-only for the shared-beta pairs)."""
+        nra, nca = self.plain_n
+        nb = self.lib.dwba_il_semisep_f64_workspace_bytes(len(nra), nra, nca, self.L_max, seg)
+        if getattr(self, "plain_d_split", None) is not None:
+            nra_s, nca_s = self.plain_n_split
+            nb = max(nb, self.lib.dwba_il_semisep_f64_workspace_bytes(len(nra_s), nra_s, nca_s, self.L_max, seg))
+        return nb
+
     def _contract_semisep(self, seg, lb, direct_gemv, out, ws, ws_bytes, sh):
         """All FP64 contractions of the batch through the semiseparable kernel;
         with direct_gemv the Filon/standard pairs of shared-beta energies take
         their direct family from t_L = K_L beta (once per energy) and a dot."""
         hip, lib, Lm = self.hip, self.lib, self.L_max
-        nra, nca = self.plain_n
+        nra, nca = self.plain_n_split if direct_gemv else self.plain_n
         jobs = self.plain_d_split if direct_gemv else self.plain_d
         hip.check(lib.dwba_il_semisep_f64(self.plain_count, hip.ptr(jobs), nra, nca, Lm, seg, lb, hip.ptr(out),
                                           hip.ptr(ws), ws_bytes, sh), "dwba_il_semisep_f64")
@@ -687,7 +703,7 @@ class ScanBatch:
         nra, nca = self.plain_n
         Lm = self.L_max
         has_gemv = self.ss_gemv is not None
-        key = AT.signature("semisep_f64", torch.cuda.get_device_name(self.dev), jobs=AT.bucket(self.plain_count),
+        key = AT.signature("semisep_f64", AT.device_id(self.dev), jobs=AT.bucket(self.plain_count),
                            n=AT.bucket(float(np.mean(list(nca)))), L=Lm, gemv=int(has_gemv))
         self.launch()                       # alpha/beta in place for the contraction
         torch.cuda.synchronize()
@@ -697,7 +713,7 @@ class ScanBatch:
         def run(c):
             seg = int(c["seg"])
             if seg not in wss:
-                nb = lib.dwba_il_semisep_f64_workspace_bytes(self.plain_count, nra, nca, Lm, seg)
+                nb = self._semisep_ws_bytes(seg)
                 wss[seg] = (torch.empty(max(nb, 1), dtype=torch.uint8, device=self.dev), nb)
             ws, nb = wss[seg]
             self._contract_semisep(seg, int(c["lb"]), bool(c.get("gemv", 0)), out, ws, nb, hip.stream_handle())
@@ -710,7 +726,7 @@ class ScanBatch:
         seg, lb = int(choice.params["seg"]), int(choice.params["lb"])
         self.semisep_config = (seg, lb)
         self.direct_gemv = bool(choice.params.get("gemv", 0)) and has_gemv
-        self.plain_ws_bytes = lib.dwba_il_semisep_f64_workspace_bytes(self.plain_count, nra, nca, Lm, seg)
+        self.plain_ws_bytes = self._semisep_ws_bytes(seg)
         self.plain_ws = torch.empty(max(self.plain_ws_bytes, 1), dtype=torch.uint8, device=self.dev)
         self.tuning = choice
         if self.graph is not None:

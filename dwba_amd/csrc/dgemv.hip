@@ -124,7 +124,8 @@ __global__ void finalize_kernel(const dwba_finalize_pair* __restrict__ pairs, in
 // up to 64 pairs of one energy; per 64-point chunk the 64 alpha rows and the
 // 32 t rows are staged in LDS (coalesced loads, each read from HBM/L2 once per
 // tile instead of once per pair), and thread (pair, L-group of 8) accumulates
-// 8 multipoles.  items[k] = {first pair, pair count}; the pairs of an item have
+// 8 multipoles; the next chunk's loads are in flight while the current one is
+// summed.  items[k] = {first pair, pair count}; the pairs of an item have
 // the same t, ld and L_max <= 31.
 constexpr int TP = 64;     // pairs per tile
 constexpr int TK = 64;     // grid points per chunk
@@ -154,23 +155,40 @@ __global__ __launch_bounds__(256) void ddot_tiled_kernel(const dwba_ddot_pair* _
     double acc[8];
 #pragma unroll
     for (int l = 0; l < 8; ++l) acc[l] = 0.0;
-    for (int i0 = 0; i0 < nmax; i0 += TK) {
-        // alpha is zero beyond each pair's n (and up to ld), so nmax bounds every row
+    // chunk i0 is staged in registers while chunk i0 - TK is summed from LDS
+    constexpr int QA = TP * TK / 256, QT = 32 * TK / 256;
+    double ra[QA], rt[QT];
+    auto load = [&](int i0) {
 #pragma unroll
-        for (int q = 0; q < TP * TK / 256; ++q) {
+        for (int q = 0; q < QA; ++q) {
             const int idx = tid + 256 * q;
             const int pr = idx >> 6, pt = idx & 63;
             const int i = i0 + pt;
-            s_a[pt][pr] = (pr < count && i < nmax) ? s_ap[pr][i] : 0.0;
+            ra[q] = (pr < count && i < nmax) ? s_ap[pr][i] : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < 32 * TK / 256; ++q) {
+        for (int q = 0; q < QT; ++q) {
             const int idx = tid + 256 * q;
             const int L = idx >> 6, pt = idx & 63;
             const int i = i0 + pt;
-            s_t[pt][L] = (L < nl && i < nmax) ? t[(size_t)L * ld + i] : 0.0;
+            rt[q] = (L < nl && i < nmax) ? t[(size_t)L * ld + i] : 0.0;
+        }
+    };
+    // alpha is zero beyond each pair's n (and up to ld), so nmax bounds every row
+    if (nmax > 0) load(0);
+    for (int i0 = 0; i0 < nmax; i0 += TK) {
+#pragma unroll
+        for (int q = 0; q < QA; ++q) {
+            const int idx = tid + 256 * q;
+            s_a[idx & 63][idx >> 6] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < QT; ++q) {
+            const int idx = tid + 256 * q;
+            s_t[idx & 63][idx >> 6] = rt[q];
         }
         __syncthreads();
+        if (i0 + TK < nmax) load(i0 + TK);
 #pragma unroll 4
         for (int pt = 0; pt < TK; ++pt) {
             const double a = s_a[pt][p];

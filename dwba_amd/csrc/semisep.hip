@@ -266,85 +266,83 @@ __global__ void semisep_plan(const dwba_plain_job* __restrict__ jobs, int n_jobs
 //     t_L(i) = (A_i + d_L beta_i) / r_i + B_i,
 //     A_{i+1} = q_i^L (A_i + beta_i),              A_0 = 0          (j < i)
 //     B_{i-1} = q_{i-1}^L (B_i + beta_i / r_i),    B_{N-1} = 0      (j > i)
-// A workgroup of GT threads handles one (energy, block of LB multipoles);
-// a thread owns a contiguous range; ranges are joined by scans of the affine
-// maps through LDS (forward for A, backward for B), then every thread
-// recomputes its range with the carried values and writes t.
-constexpr int GT = 1024;
+// A workgroup of GT threads handles one (energy, L); a thread owns a
+// contiguous range; ranges are joined by scans of the affine maps through LDS
+// (forward for A, backward for B), then every thread recomputes its range
+// with the carried values and writes t.
+constexpr int GT = 256;
 
-template <int LB>
 __global__ __launch_bounds__(GT) void semisep_gemv_kernel(const dwba_dgemv_energy* __restrict__ en) {
     __shared__ double s_m[GT], s_v[GT];
     const dwba_dgemv_energy E = en[blockIdx.x];
-    const int L0 = blockIdx.y * LB;
-    if (L0 > E.L_max) return;
+    const int L = blockIdx.y;
+    if (L > E.L_max) return;
     const int n = E.N, tid = threadIdx.x;
     const int per = (n + GT - 1) / GT;
     const int a = min(n, tid * per), b = min(n, a + per);
     const double* __restrict__ r = E.r;
     const double* __restrict__ be = E.beta;
-    const double ldl = log(1.0 - 1e-12);
-    for (int l = 0; l < LB; ++l) {
-        const int L = L0 + l;
-        if (L > E.L_max) break;
-        const double dL = L == 0 ? 1.0 : exp((double)L * ldl);
-        // ---- pass 1: local maps.  forward: A_b = Mf A_a + Af; backward: B_{a-1} = Mb B_{b-1} + Sb
-        double Af = 0.0, Mf = 1.0, Sb = 0.0, Mb = 1.0;
-        for (int i = a; i < b; ++i) {
-            const double q = i + 1 < n ? r[i] / r[i + 1] : 1.0;
-            const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
-            Af = qL * (Af + be[i]);
-            Mf *= qL;
-        }
-        for (int i = b - 1; i >= a; --i) {
-            const double q = i > 0 ? r[i - 1] / r[i] : 1.0;
-            const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
-            Sb = qL * (Sb + be[i] / r[i]);
-            Mb *= qL;
-        }
-        // ---- forward scan (inclusive) of (Mf, Af) over threads
-        s_m[tid] = Mf; s_v[tid] = Af;
+    const double dL = L == 0 ? 1.0 : exp((double)L * log(1.0 - 1e-12));
+    // ---- pass 1, one sweep over [a, b): the forward map A_b = Mf A_a + Af and
+    // the backward map B_{a-1} = Mb B_{b-1} + Sb, with
+    //     Mb = prod_{i=a}^{b-1} q_{i-1}^L,  Sb = sum_{i=a}^{b-1} (beta_i / r_i) prod_{j=a}^{i} q_{j-1}^L
+    double Af = 0.0, Mf = 1.0, Sb = 0.0, Mb = 1.0;
+    double qprev = 1.0;                   // q_{i-1}^L, (r_{a-1} / r_a)^L at the start
+    if (a < b && a > 0 && L > 0) qprev = pow_int_rt(r[a - 1] / r[a], L);
+    for (int i = a; i < b; ++i) {
+        const double ri = r[i];
+        const double q = i + 1 < n ? ri / r[i + 1] : 1.0;
+        const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
+        const double bi = be[i];
+        Af = qL * (Af + bi);
+        Mf *= qL;
+        Mb *= qprev;
+        Sb = fma(bi / ri, Mb, Sb);
+        qprev = qL;
+    }
+    // ---- forward scan (inclusive) of (Mf, Af) over threads
+    s_m[tid] = Mf; s_v[tid] = Af;
+    __syncthreads();
+    for (int d = 1; d < GT; d <<= 1) {
+        double pm = 0.0, pv = 0.0;
+        const bool take = tid >= d;
+        if (take) { pm = s_m[tid - d]; pv = s_v[tid - d]; }
         __syncthreads();
-        for (int d = 1; d < GT; d <<= 1) {
-            double pm = 0.0, pv = 0.0;
-            const bool take = tid >= d;
-            if (take) { pm = s_m[tid - d]; pv = s_v[tid - d]; }
-            __syncthreads();
-            if (take) { s_v[tid] = fma(pv, s_m[tid], s_v[tid]); s_m[tid] = pm * s_m[tid]; }
-            __syncthreads();
-        }
-        const double A_in = tid > 0 ? s_v[tid - 1] : 0.0;
+        if (take) { s_v[tid] = fma(pv, s_m[tid], s_v[tid]); s_m[tid] = pm * s_m[tid]; }
         __syncthreads();
-        // ---- backward scan (inclusive from the right) of (Mb, Sb)
-        s_m[tid] = Mb; s_v[tid] = Sb;
+    }
+    const double A_in = tid > 0 ? s_v[tid - 1] : 0.0;
+    __syncthreads();
+    // ---- backward scan (inclusive from the right) of (Mb, Sb)
+    s_m[tid] = Mb; s_v[tid] = Sb;
+    __syncthreads();
+    for (int d = 1; d < GT; d <<= 1) {
+        double pm = 0.0, pv = 0.0;
+        const bool take = tid + d < GT;
+        if (take) { pm = s_m[tid + d]; pv = s_v[tid + d]; }
         __syncthreads();
-        for (int d = 1; d < GT; d <<= 1) {
-            double pm = 0.0, pv = 0.0;
-            const bool take = tid + d < GT;
-            if (take) { pm = s_m[tid + d]; pv = s_v[tid + d]; }
-            __syncthreads();
-            if (take) { s_v[tid] = fma(pv, s_m[tid], s_v[tid]); s_m[tid] = pm * s_m[tid]; }
-            __syncthreads();
-        }
-        const double B_in = tid + 1 < GT ? s_v[tid + 1] : 0.0;   // B at this range's last point
+        if (take) { s_v[tid] = fma(pv, s_m[tid], s_v[tid]); s_m[tid] = pm * s_m[tid]; }
         __syncthreads();
-        // ---- pass 2: forward part, then backward part added in place
-        double A = A_in;
-        double* __restrict__ t = E.t + (size_t)L * E.ld;
-        for (int i = a; i < b; ++i) {
-            const double ir = 1.0 / r[i];
-            if (i < E.rows) t[i] = (A + dL * be[i]) * ir;
-            const double q = i + 1 < n ? r[i] * (1.0 / r[i + 1]) : 1.0;
-            const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
-            A = qL * (A + be[i]);
-        }
-        double B = B_in;
-        for (int i = b - 1; i >= a; --i) {
-            if (i < E.rows) t[i] += B;
-            const double q = i > 0 ? r[i - 1] / r[i] : 1.0;
-            const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
-            B = qL * (B + be[i] / r[i]);
-        }
+    }
+    const double B_in = tid + 1 < GT ? s_v[tid + 1] : 0.0;   // B at this range's last point
+    // ---- pass 2: forward part, then backward part added in place
+    double A = A_in;
+    double* __restrict__ t = E.t + (size_t)L * E.ld;
+    for (int i = a; i < b; ++i) {
+        const double ri = r[i];
+        const double bi = be[i];
+        if (i < E.rows) t[i] = (A + dL * bi) / ri;
+        const double q = i + 1 < n ? ri / r[i + 1] : 1.0;
+        const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
+        A = qL * (A + bi);
+    }
+    double B = B_in;
+    for (int i = b - 1; i >= a; --i) {
+        const double ri = r[i];
+        if (i < E.rows) t[i] += B;
+        const double q = i > 0 ? r[i - 1] / ri : 1.0;
+        const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
+        B = qL * (B + be[i] / ri);
     }
 }
 
@@ -445,8 +443,7 @@ int32_t dwba_semisep_gemv_f64(int32_t n_energies, const dwba_dgemv_energy* d_ene
                               void* stream) {
     DWBA_REQUIRE(initialised(), "dwba_semisep_gemv_f64: call dwba_init first");
     DWBA_REQUIRE(n_energies > 0 && d_energies && L_max >= 0 && L_max <= 63, "dwba_semisep_gemv_f64: bad arguments");
-    constexpr int LB = 8;
-    hipLaunchKernelGGL(semisep_gemv_kernel<LB>, dim3(n_energies, (L_max + LB) / LB), dim3(GT), 0,
+    hipLaunchKernelGGL(semisep_gemv_kernel, dim3(n_energies, L_max + 1), dim3(GT), 0,
                        reinterpret_cast<hipStream_t>(stream), d_energies);
     DWBA_LAUNCH_CHECK("semisep_gemv_kernel");
     return DWBA_OK;
