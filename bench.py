@@ -232,12 +232,18 @@ def _events_ms(fn, steps, stream):
     return e0.elapsed_time(e1) / steps
 
 
+SS_FLOP = 9.0   # per (grid point, L, family): A and Z updates (2 x add+mul), q^L step (mul), 2 FMAs into the sum
+
+
 def semisep_work_flop(b):
     """Algorithmic flops of one semiseparable contraction stage of ScanBatch b:
-    11 per (grid point, L, family) for the recurrences; with the shared-beta
-    path, the direct families of its pairs cost one recurrence per energy
-    (forward and backward: 2 x 11 per point and L) plus a dot (2 per row and L)."""
-    from dwba_amd.planner import PREP_FILON, PREP_STANDARD
+    SS_FLOP per (grid point, L, family) of the sequential recurrences over the
+    points a family needs (exchange families of Filon / plain pairs end at N1
+    when their direct family goes through the shared-beta path); with that
+    path, the direct families of an energy cost one forward and one backward
+    recurrence plus the combination (SS_FLOP per point and L) and a dot per
+    pair (2 per row and L)."""
+    from dwba_amd.planner import PREP_FILON, PREP_PLAIN, PREP_STANDARD
     nl = b.L_max + 1
     gemv = bool(getattr(b, "direct_gemv", False))
     shared = set()
@@ -247,14 +253,17 @@ def semisep_work_flop(b):
             ks = [k for k in st.members if b.pairs[k][2].mode in (PREP_FILON, PREP_STANDARD)]
             if ks and len({id(b.pairs[k][5]) for k in st.members}) == 1:
                 shared.update(ks)
-                W += 2 * 11.0 * st.N * nl
+                W += SS_FLOP * st.N * nl
                 W += sum(2.0 * b.pairs[k][2].N1 * nl for k in ks)
     for k, (ei, _, plan, *_rest) in enumerate(b.pairs):
         if plan.rows <= 0:
             continue
-        fams = 1 if k in shared else 2
-        W += 11.0 * fams * plan.cols * nl
-    return {"flop": W, "gemv": gemv}
+        if k in shared:
+            ex = plan.rows if plan.mode in (PREP_FILON, PREP_PLAIN) else plan.cols
+            W += SS_FLOP * ex * nl
+        else:
+            W += SS_FLOP * 2 * plan.cols * nl
+    return {"flop": W, "gemv": gemv, "lanes": bool(getattr(b, "semisep_lanes", False))}
 
 
 def _committed_traffic(stage, workload):
@@ -361,15 +370,17 @@ def bench_batched(args, ionization=False):
     t_c = st64s.get("plain")
     if t_c:
         W = ss_work["flop"]
-        kern = "semisep_kernel<8,4> + semisep_join" + (" + semisep_gemv_kernel + ddot_tiled_kernel"
-                                                       if ss_work["gemv"] else "")
-        result["roofline"] = {"bound": "fp64-valu", "achieved": W / (t_c * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+        kern = ("semisep_lane_kernel<8,4>" if ss_work["lanes"] else "semisep_kernel<8,4> + semisep_join") + (
+            " + semisep_gemv_kernel + ddot_tiled_kernel" if ss_work["gemv"] else "")
+        result["roofline"] = {"bound": "mfma", "achieved": W / (t_c * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
                               "unit": "TFLOP/s", "frac": W / (t_c * 1e-3) / PEAK_FP64,
                               "traffic": _committed_traffic("contraction", "batched_64x400"),
                               "kernel": kern, "stage": "contraction", "stage_ms": t_c,
                               "work_flop_per_launch": W, "algorithmic_bytes_per_launch": sizes["alpha_beta_bytes"],
-                              "note": "11 flop per (grid point, L, family) for the recurrences (3 FMA, 2 add+mul, 1 "
-                                      "power step); 2 per (point, L) for the dot of the shared-beta path; "
+                              "note": "FP64 arithmetic on the vector ALUs (MI355X FP64 vector peak = FP64 matrix "
+                                      "peak = 78.6 TF/s); 9 flop per (grid point, L, family) for the sequential "
+                                      "recurrences (2 FMA, 2 add+mul, 1 power step); 2 per (point, L) for the dot "
+                                      "of the shared-beta path; "
                                       "algorithmic bytes = alpha + beta of both families, FP64, read once"}
 
     if args.no_fp32:

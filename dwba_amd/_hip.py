@@ -83,6 +83,7 @@ _SIGS = {
     "dwba_split_bf16": (c_i32, [c_vp, c_vp, c_vp, ctypes.c_int64, c_vp]),
     "dwba_dgemv_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_vp]),
     "dwba_semisep_gemv_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp]),
+    "dwba_il_semisep_lanes_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
     "dwba_ddot_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
     "dwba_ddot_tiled_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_vp, c_vp]),
     "dwba_batch_finalize": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
@@ -199,9 +200,10 @@ def il_plain_f64(r, alpha_dir, beta_dir, alpha_ex, beta_ex, n_rows: int, n_cols:
     return out
 
 
-def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None, lb: int = 0):
-    """dwba_il_semisep_f64 on the same job tuples as il_plain_f64_batch
-    (n_rows <= n_cols).  Returns a (n_jobs, 2, L_max+1) float64 CUDA tensor."""
+def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None, lb: int = 0, lanes: bool = False):
+    """dwba_il_semisep_f64 (or, with lanes, dwba_il_semisep_lanes_f64) on the
+    same job tuples as il_plain_f64_batch (n_rows <= n_cols).  Returns a
+    (n_jobs, 2, L_max+1) float64 CUDA tensor."""
     ensure_init()
     lib = load_library()
     n = len(jobs)
@@ -222,8 +224,13 @@ def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None, lb: int = 
     ws_bytes = lib.dwba_il_semisep_f64_workspace_bytes(n, nr, nc, L_max, seg)
     if ws_bytes == 0:
         raise ValueError("il_semisep_f64_batch: invalid shapes (need 0 <= n_rows <= n_cols, 0 <= L_max <= 63)")
-    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     out = torch.empty((n, 2, L_max + 1), dtype=torch.float64, device=dev)
+    if lanes:
+        check(lib.dwba_il_semisep_lanes_f64(n, ptr(d_jobs), L_max, ptr(out), stream_handle(stream)),
+              "dwba_il_semisep_lanes_f64")
+        out._dwba_keep = (d_jobs,)
+        return out
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     check(lib.dwba_il_semisep_f64(n, ptr(d_jobs), nr, nc, L_max, seg, lb, ptr(out), ptr(ws), ws_bytes,
                                   stream_handle(stream)), "dwba_il_semisep_f64")
     out._dwba_keep = (ws, d_jobs)

@@ -159,7 +159,9 @@ SEMISEP_LBS = (4, 8)
 
 
 def semisep_candidates() -> List[Dict[str, int]]:
-    return [{"seg": s, "lb": b} for b in SEMISEP_LBS for s in SEMISEP_SEGS]
+    """(seg, lb) of the wavefront-scan kernel, plus the lane-serial kernel
+    (one lane per job, family and block of multipoles; seg and lb unused)."""
+    return [{"seg": s, "lb": b} for b in SEMISEP_LBS for s in SEMISEP_SEGS] + [{"seg": 0, "lb": 8, "lanes": 1}]
 
 
 def device_id(dev) -> str:

@@ -391,6 +391,7 @@ class ScanBatch:
         self.plain_n = (nra, nca)
         if self.contraction == "semiseparable":
             self.semisep_config = (0, 0)      # (seg, lb): library defaults until tuned
+            self.semisep_lanes = False        # lane-serial kernel instead (chosen by tune())
             self.plain_ws_bytes = self._semisep_ws_bytes(0)
         else:
             self.plain_ws_bytes = self.lib.dwba_il_plain_f64_batch_workspace_bytes(self.n_pairs, nra, nca,
@@ -528,15 +529,19 @@ class ScanBatch:
             nb = max(nb, self.lib.dwba_il_semisep_f64_workspace_bytes(len(nra_s), nra_s, nca_s, self.L_max, seg))
         return nb
 
-    def _contract_semisep(self, seg, lb, direct_gemv, out, ws, ws_bytes, sh):
+    def _contract_semisep(self, seg, lb, direct_gemv, out, ws, ws_bytes, sh, lanes=False):
         """All FP64 contractions of the batch through the semiseparable kernel;
         with direct_gemv the Filon/standard pairs of shared-beta energies take
         their direct family from t_L = K_L beta (once per energy) and a dot."""
         hip, lib, Lm = self.hip, self.lib, self.L_max
         nra, nca = self.plain_n_split if direct_gemv else self.plain_n
         jobs = self.plain_d_split if direct_gemv else self.plain_d
-        hip.check(lib.dwba_il_semisep_f64(self.plain_count, hip.ptr(jobs), nra, nca, Lm, seg, lb, hip.ptr(out),
-                                          hip.ptr(ws), ws_bytes, sh), "dwba_il_semisep_f64")
+        if lanes:
+            hip.check(lib.dwba_il_semisep_lanes_f64(self.plain_count, hip.ptr(jobs), Lm, hip.ptr(out), sh),
+                      "dwba_il_semisep_lanes_f64")
+        else:
+            hip.check(lib.dwba_il_semisep_f64(self.plain_count, hip.ptr(jobs), nra, nca, Lm, seg, lb, hip.ptr(out),
+                                              hip.ptr(ws), ws_bytes, sh), "dwba_il_semisep_f64")
         if direct_gemv:
             gd, ng, dd, nd, items_d, ni = self.ss_gemv
             hip.check(lib.dwba_semisep_gemv_f64(ng, hip.ptr(gd), Lm, sh), "dwba_semisep_gemv_f64")
@@ -586,7 +591,8 @@ class ScanBatch:
             nra, nca = self.plain_n
             if self.contraction == "semiseparable":
                 seg, lb = self.semisep_config
-                self._contract_semisep(seg, lb, self.direct_gemv, self.out, self.plain_ws, self.plain_ws_bytes, sh)
+                self._contract_semisep(seg, lb, self.direct_gemv, self.out, self.plain_ws, self.plain_ws_bytes, sh,
+                                       self.semisep_lanes)
             else:
                 hip.check(lib.dwba_il_plain_f64_batch(self.plain_count, hip.ptr(self.plain_d), nra, nca, Lm,
                                                       hip.ptr(self.out), hip.ptr(self.plain_ws), self.plain_ws_bytes,
@@ -704,7 +710,7 @@ class ScanBatch:
         Lm = self.L_max
         has_gemv = self.ss_gemv is not None
         key = AT.signature("semisep_f64", AT.device_id(self.dev), jobs=AT.bucket(self.plain_count),
-                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm, gemv=int(has_gemv))
+                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm, gemv=int(has_gemv), v=2)
         self.launch()                       # alpha/beta in place for the contraction
         torch.cuda.synchronize()
         out = torch.empty_like(self.out)
@@ -716,7 +722,8 @@ class ScanBatch:
                 nb = self._semisep_ws_bytes(seg)
                 wss[seg] = (torch.empty(max(nb, 1), dtype=torch.uint8, device=self.dev), nb)
             ws, nb = wss[seg]
-            self._contract_semisep(seg, int(c["lb"]), bool(c.get("gemv", 0)), out, ws, nb, hip.stream_handle())
+            self._contract_semisep(seg, int(c["lb"]), bool(c.get("gemv", 0)), out, ws, nb, hip.stream_handle(),
+                                   bool(c.get("lanes", 0)))
 
         cands = AT.semisep_candidates()
         if has_gemv:
@@ -725,6 +732,7 @@ class ScanBatch:
         del wss
         seg, lb = int(choice.params["seg"]), int(choice.params["lb"])
         self.semisep_config = (seg, lb)
+        self.semisep_lanes = bool(choice.params.get("lanes", 0))
         self.direct_gemv = bool(choice.params.get("gemv", 0)) and has_gemv
         self.plain_ws_bytes = self._semisep_ws_bytes(seg)
         self.plain_ws = torch.empty(max(self.plain_ws_bytes, 1), dtype=torch.uint8, device=self.dev)

@@ -117,6 +117,15 @@ int32_t dwba_il_semisep_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const 
                             const int32_t* h_n_cols, int32_t L_max, int32_t seg, int32_t lb, double* d_out,
                             void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* Lane-serial form of dwba_il_semisep_f64: the same recurrences, one lane per
+ * (job, family, block of 8 multipoles) over the job's whole grid, so no
+ * workspace and no join; same jobs, output layout and NULL-alpha_dir rule,
+ * same values up to rounding order.  Fewer operations per grid point, less
+ * parallelism: the better choice when jobs are many (the autotuner decides).
+ * Replaces the same reference code as dwba_il_semisep_f64. */
+int32_t dwba_il_semisep_lanes_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, int32_t L_max, double* d_out,
+                                  void* stream);
+
 /* ------------------------------------------------------------------ */
 /* Full-Split preparation: alpha / beta vectors of a channel pair.       */
 /* ------------------------------------------------------------------ */

@@ -144,3 +144,10 @@ def test_direct_gemv_path_matches_full_recurrence():
     r1 = b.results()
     mag = b.magnitudes()
     assert np.all(np.abs(r1 - r0) <= 1e-12 * np.abs(r0) + 1e-14 * mag)
+    # the lane-serial kernel on both job sets (exchange-only scans end at N1)
+    b.semisep_lanes = True
+    for gemv in (True, False):
+        b.direct_gemv = gemv
+        b.run()
+        r2 = b.results()
+        assert np.all(np.abs(r2 - r0) <= 1e-12 * np.abs(r0) + 1e-14 * mag), gemv

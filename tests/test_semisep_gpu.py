@@ -76,6 +76,31 @@ def test_semisep_lane_split_independent(seg, lb):
     _check(host, got, 20)
 
 
+@pytest.mark.parametrize("L_max", [0, 7, 8, 30, 63])
+def test_semisep_lanes_random_jobs(L_max):
+    """Lane-serial kernel (one lane per job, family and multipole block)
+    against the oracle on the same jobs: empty rows, one-point grid, rows
+    shorter than the grid, L blocks that do not fill a power of two."""
+    from dwba_amd import _hip
+    host, dev = _jobs(SPECS, seed=100 + L_max)
+    print(f"\n[semisep lanes] {len(SPECS)} jobs L_max={L_max}", flush=True)
+    got = _hip.il_semisep_f64_batch(dev, L_max, lanes=True).cpu().numpy()
+    w = _check(host, got, L_max)
+    print(f"  worst |d|/sum|aKb| {w:.2e}", flush=True)
+
+
+def test_semisep_lanes_matches_scan_kernel():
+    from dwba_amd import _hip
+    host, dev = _jobs(SPECS, seed=11)
+    print("\n[semisep lanes] vs wavefront-scan kernel", flush=True)
+    a = _hip.il_semisep_f64_batch(dev, 30, lanes=True).cpu().numpy()
+    b = _hip.il_semisep_f64_batch(dev, 30).cpu().numpy()
+    for k, (r, vecs, n_rows, n) in enumerate(host):
+        for fam, (x, y) in enumerate(((vecs[0], vecs[1]), (vecs[2], vecs[3]))):
+            mag = semisep.abs_bound_all_L(r, x, y, 30)
+            assert np.all(np.abs(a[k, fam] - b[k, fam]) <= 1e-12 * np.abs(b[k, fam]) + 1e-13 * mag)
+
+
 def test_semisep_kernel_only_workload_n15000():
     from dwba_amd import _hip
     g, (ad, bd, ae, be, _) = S.kernel_only_workload(15000, seed=0)
