@@ -65,6 +65,55 @@ struct CCTable {
     int n;
 };
 
+// Ends [a, b] of phase interval m < n_pnodes - 1: the stored nodes (Coulomb
+// stepping) or numpy linspace(p_start, p_end, n_pnodes) (uniform).
+template <bool COUL>
+__device__ inline void interval_ends(const dwba_prep_job& J, int m, double& a, double& b) {
+    if (COUL) {
+        a = J.phase_nodes[m];
+        b = J.phase_nodes[m + 1];
+    } else {
+        a = __dadd_rn(__dmul_rn((double)m, J.p_step), J.p_start);
+        const double bn = __dadd_rn(__dmul_rn((double)(m + 1), J.p_step), J.p_start);
+        b = m + 1 == J.n_pnodes - 1 ? J.p_end : bn;
+    }
+}
+
+// Flattened index g = m ncc + k of the first CC node with x >= v (n_int ncc
+// when there is none).  Node k of interval m sits at x = clamp(hw (c_k + 1)
+// + a, a, b), hw = (b - a)/2, c_k ascending, so x never decreases along g.
+// Starts one interval before the one holding v, so a rounded interval index
+// never skips a node.
+template <bool COUL>
+__device__ inline int first_node_ge(const dwba_prep_job& J, const double* ccx, int ncc, int n_int, double v) {
+    int m = interval_of(J, v, n_int) - 1;
+    for (m = m < 0 ? 0 : m; m < n_int; ++m) {
+        double a, b;
+        interval_ends<COUL>(J, m, a, b);
+        if (b < v) continue;                      // every node of m is <= b < v
+        const double hw = 0.5 * (b - a);
+        for (int k = 0; k < ncc; ++k)
+            if (fmin(fmax(hw * (ccx[k] + 1.0) + a, a), b) >= v) return m * ncc + k;
+    }
+    return n_int * ncc;
+}
+
+// log(1 + u) for 0 <= u < 0.01: series to u^9 (truncation below 1e-19 u).
+__device__ inline double log1p_small(double u) {
+    double p = 1.0 / 9.0;
+    p = fma(p, u, -1.0 / 8.0);
+    p = fma(p, u, 1.0 / 7.0);
+    p = fma(p, u, -1.0 / 6.0);
+    p = fma(p, u, 1.0 / 5.0);
+    p = fma(p, u, -1.0 / 4.0);
+    p = fma(p, u, 1.0 / 3.0);
+    p = fma(p, u, -1.0 / 2.0);
+    p = fma(p, u, 1.0);
+    return p * u;
+}
+
+constexpr int NODE_BATCH = 4;    // CC nodes evaluated side by side per cell
+
 // FILON mode as one pass over the grid cells [r_j, r_{j+1}) of r[:N1].
 // A CC node c at x in cell j (interval m, node k, weight W = w_k (b - a)/2)
 // adds
@@ -75,54 +124,91 @@ struct CCTable {
 //          wl, wr the linear-in-r kernel weights.
 // Summed over the nodes of the cell this is a 2x2 operator on the densities
 // at j, j+1 whose coefficients (sums of W t^a (1-t)^b) need no density, so a
-// thread owns one cell, enumerates its nodes by position (nodes ascend in x
-// along (m, k)), and point i takes the cell-i part plus the cell-(i-1) part
-// from its neighbour.  Nodes lie in [r_0, r_{N1-1}]; the last cell is closed
-// on the right.  Every node lands in exactly one cell; because np.interp and
-// the kernel weights are continuous, a node on a cell edge gives the same
-// contribution (to rounding) from either side.
+// thread owns one cell, finds the index range of its nodes from their
+// positions (the upper end is the next cell's lower end, passed through LDS)
+// and evaluates them NODE_BATCH at a time as independent chains; point i
+// takes the cell-i part plus the cell-(i-1) part from its neighbour.  Nodes
+// lie in [r_0, r_{N1-1}]; the last cell is closed on the right.  Every node
+// lands in exactly one cell; because np.interp and the kernel weights are
+// continuous, a node on a cell edge gives the same contribution (to
+// rounding) from either side.  In log r the weight is
+// t = log(x / r_j) / (log r_{j+1} - log r_j); the numerator is the series
+// log1p((x - r_j) / r_j) in cells narrower than 1% of r_j (it differs from
+// the rounded difference log x - log r_j by less than that difference's own
+// rounding error).
+template <bool COUL>
 __device__ void filon_alpha_beta(const dwba_prep_job& J, const CCTable& cc) {
     __shared__ double s_s1[PREP_THREADS][3];
+    __shared__ double s_ccx[DWBA_MAX_CC], s_ccw[DWBA_MAX_CC];
+    __shared__ int s_glo[PREP_THREADS];
     const int N = J.N, N1 = J.N1;
     const int n_int = J.n_pnodes - 1;
     const int ncc = cc.n;
+    const int g_end = n_int * ncc;
+    const double inv_ncc = 1.0 / ncc;
     const double* r = J.r;
     const double* lr = J.logr;                    // log(max(r, 1e-300))
     const bool lg1 = J.log1 != 0, lg2 = J.log2 != 0;
+    const bool need_log = lg1 || lg2;
     const bool fill = !(J.mode & DWBA_PREP_NO_ZERO_FILL);
+    if (threadIdx.x < DWBA_MAX_CC) {              // per-lane node indices read the table from LDS
+        s_ccx[threadIdx.x] = threadIdx.x < ncc ? cc.x[threadIdx.x] : 0.0;
+        s_ccw[threadIdx.x] = threadIdx.x < ncc ? cc.w[threadIdx.x] : 0.0;
+    }
+    __syncthreads();
     double carry_d = 0.0, carry_e = 0.0, carry_b = 0.0;   // cell-(i-1) parts across chunks
     for (int base = 0; base < N1; base += PREP_THREADS) {
         const int j = base + (int)threadIdx.x;
+        const bool cell = j < N1 - 1;
+        const bool last = j == N1 - 2;
+        const int g_lo = !cell ? g_end : (j == 0 ? 0 : first_node_ge<COUL>(J, s_ccx, ncc, n_int, r[j]));
+        s_glo[threadIdx.x] = g_lo;
+        __syncthreads();
         double s0d = 0.0, s1d = 0.0, s0e = 0.0, s1e = 0.0, s0b = 0.0, s1b = 0.0;
-        if (j < N1 - 1) {
+        if (cell) {
+            const int g_hi = last ? g_end
+                                  : (threadIdx.x + 1 < PREP_THREADS ? s_glo[threadIdx.x + 1]
+                                                                    : first_node_ge<COUL>(J, s_ccx, ncc, n_int, r[j + 1]));
             const double rl = r[j], rr = r[j + 1];
             const double ll = lr[j], lrr = lr[j + 1];
-            const bool last = (j == N1 - 2);
-            const double x1l = lg1 ? ll : rl, x1r = lg1 ? lrr : rr;
-            const double x2l = lg2 ? ll : rl, x2r = lg2 ? lrr : rr;
-            const double inv1 = x1r > x1l ? 1.0 / (x1r - x1l) : 0.0;
-            const double inv2 = x2r > x2l ? 1.0 / (x2r - x2l) : 0.0;
+            const double inv_log = lrr > ll ? 1.0 / (lrr - ll) : 0.0;
+            const double inv_lin = rr > rl ? 1.0 / (rr - rl) : 0.0;
             const double invr = 1.0 / (rr - rl + 1e-30);
+            const bool ser = rl > 0.0 && (rr - rl) < 0.01 * rl;
+            const double inv_rl = ser ? 1.0 / rl : 0.0;
             double c00 = 0.0, c01 = 0.0, c11 = 0.0, b00 = 0.0, b01 = 0.0, b10 = 0.0, b11 = 0.0;
-            // first interval that can hold a node >= r_j (one back for rounding)
-            int m = interval_of(J, rl, n_int) - 1;
-            m = m < 0 ? 0 : m;
-            bool done = false;
-            for (; m < n_int && !done; ++m) {
-                const double a = phase_node(J, m), b = phase_node(J, m + 1);
-                if (a >= rr && !last) break;
-                if (!(b - a > 1e-12)) continue;       // zero-weight interval
-                const double hw = 0.5 * (b - a);
-                for (int k = 0; k < ncc; ++k) {
-                    // clamped into [a, b] so that x never decreases along the node list
-                    const double x = fmin(fmax(hw * (cc.x[k] + 1.0) + a, a), b);
-                    if (x < rl) continue;
-                    if (x >= rr && !last) { done = true; break; }
-                    const double W = cc.w[k] * hw;
-                    const double X = (lg1 || lg2) ? log(fmax(x, 1e-300)) : x;
-                    const double t1 = fmin(fmax(((lg1 ? X : x) - x1l) * inv1, 0.0), 1.0);
-                    const double t2 = lg1 == lg2 ? t1 : fmin(fmax(((lg2 ? X : x) - x2l) * inv2, 0.0), 1.0);
-                    const double wr = (x - rl) * invr, wl = 1.0 - wr;
+            for (int g0 = g_lo; g0 < g_hi; g0 += NODE_BATCH) {
+                double xs[NODE_BATCH], Ws[NODE_BATCH], dX[NODE_BATCH];
+#pragma unroll
+                for (int v = 0; v < NODE_BATCH; ++v) {
+                    const int g = min(g0 + v, g_hi - 1);  // a node past the range repeats with weight 0
+                    const int m = (int)(((double)g + 0.5) * inv_ncc);
+                    const int k = g - m * ncc;
+                    double a, b;
+                    interval_ends<COUL>(J, m, a, b);
+                    const double hw = 0.5 * (b - a);
+                    const double W = s_ccw[k] * hw;
+                    Ws[v] = (b - a > 1e-12 && g0 + v < g_hi) ? W : 0.0;
+                    xs[v] = fmin(fmax(hw * (s_ccx[k] + 1.0) + a, a), b);
+                    dX[v] = 0.0;
+                }
+                if (need_log) {
+                    if (ser) {
+#pragma unroll
+                        for (int v = 0; v < NODE_BATCH; ++v) dX[v] = log1p_small((xs[v] - rl) * inv_rl);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < NODE_BATCH; ++v) dX[v] = log(fmax(xs[v], 1e-300)) - ll;
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < NODE_BATCH; ++v) {
+                    const double W = Ws[v];
+                    const double dx = xs[v] - rl;
+                    const double tl = dX[v] * inv_log, tn = dx * inv_lin;
+                    const double t1 = fmin(fmax(lg1 ? tl : tn, 0.0), 1.0);
+                    const double t2 = fmin(fmax(lg2 ? tl : tn, 0.0), 1.0);
+                    const double wr = dx * invr, wl = 1.0 - wr;
                     const double u1 = 1.0 - t1, u2 = 1.0 - t2;
                     const double Wu = W * u1, Wt = W * t1;
                     c00 = fma(Wu, u1, c00);
@@ -226,7 +312,8 @@ __global__ __launch_bounds__(PREP_THREADS) void prep_f64_kernel(const dwba_prep_
     if (mode != DWBA_PREP_FILON) return;
 
     // FILON
-    filon_alpha_beta(J, cc);
+    if (J.phase_nodes) filon_alpha_beta<true>(J, cc);
+    else filon_alpha_beta<false>(J, cc);
 }
 
 // Multipole moments: block (set, L) reduces sum_j ((w_j r_j^L) u_f,j) u_i,j.
