@@ -83,7 +83,7 @@ _SIGS = {
     "dwba_split_bf16": (c_i32, [c_vp, c_vp, c_vp, ctypes.c_int64, c_vp]),
     "dwba_dgemv_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_vp]),
     "dwba_semisep_gemv_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp]),
-    "dwba_il_semisep_lanes_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
+    "dwba_il_semisep_lanes_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_vp, c_vp]),
     "dwba_ddot_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
     "dwba_ddot_tiled_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_vp, c_vp]),
     "dwba_batch_finalize": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
@@ -226,7 +226,7 @@ def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None, lb: int = 
         raise ValueError("il_semisep_f64_batch: invalid shapes (need 0 <= n_rows <= n_cols, 0 <= L_max <= 63)")
     out = torch.empty((n, 2, L_max + 1), dtype=torch.float64, device=dev)
     if lanes:
-        check(lib.dwba_il_semisep_lanes_f64(n, ptr(d_jobs), L_max, ptr(out), stream_handle(stream)),
+        check(lib.dwba_il_semisep_lanes_f64(n, ptr(d_jobs), L_max, lb, ptr(out), stream_handle(stream)),
               "dwba_il_semisep_lanes_f64")
         out._dwba_keep = (d_jobs,)
         return out

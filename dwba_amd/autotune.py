@@ -160,8 +160,9 @@ SEMISEP_LBS = (4, 8)
 
 def semisep_candidates() -> List[Dict[str, int]]:
     """(seg, lb) of the wavefront-scan kernel, plus the lane-serial kernel
-    (one lane per job, family and block of multipoles; seg and lb unused)."""
-    return [{"seg": s, "lb": b} for b in SEMISEP_LBS for s in SEMISEP_SEGS] + [{"seg": 0, "lb": 8, "lanes": 1}]
+    (one lane per job, family and block of lb multipoles; seg unused)."""
+    return [{"seg": s, "lb": b} for b in SEMISEP_LBS for s in SEMISEP_SEGS] + [
+        {"seg": 0, "lb": b, "lanes": 1} for b in SEMISEP_LBS]
 
 
 def device_id(dev) -> str:

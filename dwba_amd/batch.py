@@ -537,7 +537,7 @@ class ScanBatch:
         nra, nca = self.plain_n_split if direct_gemv else self.plain_n
         jobs = self.plain_d_split if direct_gemv else self.plain_d
         if lanes:
-            hip.check(lib.dwba_il_semisep_lanes_f64(self.plain_count, hip.ptr(jobs), Lm, hip.ptr(out), sh),
+            hip.check(lib.dwba_il_semisep_lanes_f64(self.plain_count, hip.ptr(jobs), Lm, lb, hip.ptr(out), sh),
                       "dwba_il_semisep_lanes_f64")
         else:
             hip.check(lib.dwba_il_semisep_f64(self.plain_count, hip.ptr(jobs), nra, nca, Lm, seg, lb, hip.ptr(out),

@@ -262,93 +262,147 @@ __global__ void semisep_plan(const dwba_plain_job* __restrict__ jobs, int n_jobs
 // ---------------------------------------------------------------------------
 // Lane-serial form: one lane owns one (job, family, block of LB multipoles)
 // stream and runs the recurrences over the job's whole grid, so there are no
-// carries to accumulate (Ca, Cb, P) and nothing to join: 7 FP64 operations
-// per grid point and multipole instead of 10.  Lanes are stream-major with
-// NBP (a power of two >= the block count) lanes per stream; streams
-// [0, n_jobs) are the direct families, [n_jobs, 2 n_jobs) the exchange ones,
-// so a batch whose direct families are computed elsewhere (alpha_dir NULL)
-// leaves whole wavefronts idle instead of half of every wavefront.  Loads
-// are grouped GU points at a time and the next group is in flight while the
-// current one is used.
+// carries to accumulate (Ca, Cb, P) and nothing to join.  The lane carries
+// A'_i = A_i + beta_i and Z'_i = Z_i + alpha_i, so each update is one FMA with
+// the next point's value,
+//     A'_{i+1} = q_i^L A'_i + beta_{i+1},   Z'_{i+1} = q_i^L Z'_i + alpha_{i+1},
+// and the sum is taken over (alpha_i A'_i + beta_i Z'_i) / r_i with the
+// diagonal coefficient d_L - 2 (alpha_i A_i + beta_i Z_i = alpha_i A'_i +
+// beta_i Z'_i - 2 alpha_i beta_i): 5 FP64 operations per grid point and
+// multipole; beyond the alpha rows only Z and the sum remain (3).  The
+// rewrite moves at most 2 eps sum |alpha_i beta_i| / r_i of rounding, a part
+// of the cancellation-free magnitude of the integral.  Lanes are
+// stream-major with NBP (a power of two >= the block count) lanes per
+// stream; streams [0, n_jobs) are the direct families, [n_jobs, 2 n_jobs)
+// the exchange ones, so a batch whose direct families are computed
+// elsewhere (alpha_dir NULL) leaves whole wavefronts idle instead of half of
+// every wavefront.  Loads are grouped LGU points at a time and the next group
+// is in flight while the current one is used.
 constexpr int LT = 256;          // threads per workgroup of the lane-serial kernel
 constexpr int LGU = 4;           // grid points per load group
 
-template <int LB, bool HAS_A>
-__device__ __forceinline__ void lane_point(double ri, double r_next, double a_i, double b_i, int blk, double& diag,
-                                           double* A, double* Z, double* acc, double& ir_cur, int nbp) {
+// q^(blk LB): LB-th power by squaring, then blk multiplications (uniform trip count).
+template <int LB>
+__device__ __forceinline__ double lane_q0(double q, int blk, int nbp) {
+    double qb = q;
+#pragma unroll
+    for (int m = 1; m < LB; m <<= 1) qb *= qb;
+    double q0 = 1.0;
+    for (int m = 0; m + 1 < nbp; ++m) q0 = m < blk ? q0 * qb : q0;
+    return q0;
+}
+
+// Point i with alpha: uses A'_i, Z'_i, advances them with alpha/beta at i+1.
+template <int LB>
+__device__ __forceinline__ void lane_point_a(double ri, double r_next, double a_i, double b_i, double a_next,
+                                             double b_next, int blk, int nbp, double& diag, double* Ap, double* Zp,
+                                             double* acc, double& ir_cur) {
     const double iri = ir_cur;
     const double ir_next = 1.0 / r_next;
     const double q = ri * ir_next;
-    const double bw = b_i * iri;
-    double qb = q;
+    const double aw = a_i * iri, bw = b_i * iri;
+    diag = fma(a_i, bw, diag);
+    double qL = lane_q0<LB>(q, blk, nbp);
 #pragma unroll
-    for (int m = 1; m < LB; m <<= 1) qb *= qb;       // q^LB
-    double qL = 1.0;                                 // q^(blk LB), uniform trip count
-    for (int m = 0; m + 1 < nbp; ++m) qL = m < blk ? qL * qb : qL;
-    if (HAS_A) {
-        const double aw = a_i * iri;
-        diag = fma(a_i, bw, diag);
-#pragma unroll
-        for (int l = 0; l < LB; ++l) {
-            acc[l] = fma(aw, A[l], acc[l]);
-            acc[l] = fma(bw, Z[l], acc[l]);
-            A[l] = qL * (A[l] + b_i);
-            Z[l] = qL * (Z[l] + a_i);
-            qL *= q;
-        }
-    } else {
-#pragma unroll
-        for (int l = 0; l < LB; ++l) {
-            acc[l] = fma(bw, Z[l], acc[l]);
-            A[l] = qL * (A[l] + b_i);
-            Z[l] *= qL;
-            qL *= q;
-        }
+    for (int l = 0; l < LB; ++l) {
+        acc[l] = fma(aw, Ap[l], acc[l]);
+        acc[l] = fma(bw, Zp[l], acc[l]);
+        Ap[l] = fma(qL, Ap[l], b_next);
+        Zp[l] = fma(qL, Zp[l], a_next);
+        qL *= q;
     }
     ir_cur = ir_next;
 }
 
-// Points [i0, i1) of one stream, LGU at a time with the next group prefetched.
-template <int LB, bool HAS_A>
-__device__ __forceinline__ void lane_range(const double* __restrict__ r, const double* __restrict__ al,
-                                           const double* __restrict__ be, int n, int i0, int i1, int blk, int nbp,
-                                           double& diag, double* A, double* Z, double* acc, double& ir_cur) {
-    int i = i0;
-    if (i + LGU <= i1) {
-        double cr[LGU + 1], ca[LGU], cb[LGU];
-        cr[0] = r[i];
+// Point i beyond the alpha rows (Z'_i = Z_i; A no longer contributes).
+template <int LB>
+__device__ __forceinline__ void lane_point_b(double ri, double r_next, double b_i, int blk, int nbp, double* Zp,
+                                             double* acc, double& ir_cur) {
+    const double iri = ir_cur;
+    const double ir_next = 1.0 / r_next;
+    const double q = ri * ir_next;
+    const double bw = b_i * iri;
+    double qL = lane_q0<LB>(q, blk, nbp);
 #pragma unroll
-        for (int u = 0; u < LGU; ++u) {
-            cr[u + 1] = r[min(i + u + 1, n - 1)];
-            ca[u] = HAS_A ? al[i + u] : 0.0;
-            cb[u] = be[i + u];
-        }
+    for (int l = 0; l < LB; ++l) {
+        acc[l] = fma(bw, Zp[l], acc[l]);
+        Zp[l] *= qL;
+        qL *= q;
+    }
+    ir_cur = ir_next;
+}
+
+// Points [0, nr) (alpha rows), LGU at a time with the next group prefetched;
+// every group also holds the values at its end point (the next group's first).
+template <int LB>
+__device__ __forceinline__ void lane_rows(const double* __restrict__ r, const double* __restrict__ al,
+                                          const double* __restrict__ be, int n, int nr, int blk, int nbp,
+                                          double& diag, double* Ap, double* Zp, double* acc, double& ir_cur) {
+    auto ld_r = [&](int i) { return r[min(i, n - 1)]; };
+    auto ld_a = [&](int i) { return i < nr ? al[i] : 0.0; };
+    auto ld_b = [&](int i) { return i < n ? be[i] : 0.0; };
+    int i = 0;
+    if (LGU <= nr) {
+        double cr[LGU + 1], ca[LGU + 1], cb[LGU + 1];
+#pragma unroll
+        for (int u = 0; u <= LGU; ++u) { cr[u] = ld_r(u); ca[u] = ld_a(u); cb[u] = ld_b(u); }
 #pragma unroll 1
-        for (; i + LGU <= i1; i += LGU) {
-            const bool more = i + 2 * LGU <= i1;
+        for (; i + LGU <= nr; i += LGU) {
+            const bool more = i + 2 * LGU <= nr;
             double xr[LGU], xa[LGU], xb[LGU];
             if (more) {
 #pragma unroll
                 for (int u = 0; u < LGU; ++u) {
-                    xr[u] = r[min(i + LGU + u + 1, n - 1)];
-                    xa[u] = HAS_A ? al[i + LGU + u] : 0.0;
-                    xb[u] = be[i + LGU + u];
+                    const int k = i + LGU + 1 + u;
+                    xr[u] = ld_r(k); xa[u] = ld_a(k); xb[u] = ld_b(k);
                 }
             }
 #pragma unroll
             for (int u = 0; u < LGU; ++u)
-                lane_point<LB, HAS_A>(cr[u], cr[u + 1], ca[u], cb[u], blk, diag, A, Z, acc, ir_cur, nbp);
+                lane_point_a<LB>(cr[u], cr[u + 1], ca[u], cb[u], ca[u + 1], cb[u + 1], blk, nbp, diag, Ap, Zp, acc,
+                                 ir_cur);
             if (more) {
-                cr[0] = cr[LGU];
+                cr[0] = cr[LGU]; ca[0] = ca[LGU]; cb[0] = cb[LGU];
 #pragma unroll
-                for (int u = 0; u < LGU; ++u) { cr[u + 1] = xr[u]; ca[u] = xa[u]; cb[u] = xb[u]; }
+                for (int u = 0; u < LGU; ++u) { cr[u + 1] = xr[u]; ca[u + 1] = xa[u]; cb[u + 1] = xb[u]; }
             }
         }
     }
 #pragma unroll 1
-    for (; i < i1; ++i)
-        lane_point<LB, HAS_A>(r[i], r[min(i + 1, n - 1)], HAS_A ? al[i] : 0.0, be[i], blk, diag, A, Z, acc, ir_cur,
-                              nbp);
+    for (; i < nr; ++i)
+        lane_point_a<LB>(r[i], ld_r(i + 1), al[i], be[i], ld_a(i + 1), ld_b(i + 1), blk, nbp, diag, Ap, Zp, acc,
+                         ir_cur);
+}
+
+// Points [nr, n): beta and the decaying Z only.
+template <int LB>
+__device__ __forceinline__ void lane_tail(const double* __restrict__ r, const double* __restrict__ be, int n, int i0,
+                                          int blk, int nbp, double* Zp, double* acc, double& ir_cur) {
+    int i = i0;
+    if (i + LGU <= n) {
+        double cr[LGU + 1], cb[LGU];
+        cr[0] = r[i];
+#pragma unroll
+        for (int u = 0; u < LGU; ++u) { cr[u + 1] = r[min(i + u + 1, n - 1)]; cb[u] = be[i + u]; }
+#pragma unroll 1
+        for (; i + LGU <= n; i += LGU) {
+            const bool more = i + 2 * LGU <= n;
+            double xr[LGU], xb[LGU];
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < LGU; ++u) { xr[u] = r[min(i + LGU + u + 1, n - 1)]; xb[u] = be[i + LGU + u]; }
+            }
+#pragma unroll
+            for (int u = 0; u < LGU; ++u) lane_point_b<LB>(cr[u], cr[u + 1], cb[u], blk, nbp, Zp, acc, ir_cur);
+            if (more) {
+                cr[0] = cr[LGU];
+#pragma unroll
+                for (int u = 0; u < LGU; ++u) { cr[u + 1] = xr[u]; cb[u] = xb[u]; }
+            }
+        }
+    }
+#pragma unroll 1
+    for (; i < n; ++i) lane_point_b<LB>(r[i], r[min(i + 1, n - 1)], be[i], blk, nbp, Zp, acc, ir_cur);
 }
 
 template <int LB, int NBP>
@@ -366,23 +420,27 @@ __global__ __launch_bounds__(LT) void semisep_lane_kernel(const dwba_plain_job* 
     const dwba_plain_job J = jobs[k];
     if (fam == 0 && J.alpha_dir == nullptr) return;    // direct family computed elsewhere
     double* __restrict__ o = out + (size_t)k * 2 * NL + fam * NL;
-    double A[LB], Z[LB], acc[LB];
-#pragma unroll
-    for (int l = 0; l < LB; ++l) { A[l] = 0.0; Z[l] = 0.0; acc[l] = 0.0; }
+    double Ap[LB], Zp[LB], acc[LB];
     double diag = 0.0;
     const int n = J.n_cols, nr = min(J.n_rows, J.n_cols);
+#pragma unroll
+    for (int l = 0; l < LB; ++l) acc[l] = 0.0;
     if (nr > 0 && n > 0) {
         const double* __restrict__ al = fam ? J.alpha_ex : J.alpha_dir;
         const double* __restrict__ be = fam ? J.beta_ex : J.beta_dir;
+        const double a0 = al[0], b0 = be[0];
+#pragma unroll
+        for (int l = 0; l < LB; ++l) { Ap[l] = b0; Zp[l] = a0; }   // A_0 = Z_0 = 0
         double ir_cur = 1.0 / J.r[0];
-        lane_range<LB, true>(J.r, al, be, n, 0, nr, blk, NBP, diag, A, Z, acc, ir_cur);
-        lane_range<LB, false>(J.r, al, be, n, nr, n, blk, NBP, diag, A, Z, acc, ir_cur);
+        lane_rows<LB>(J.r, al, be, n, nr, blk, NBP, diag, Ap, Zp, acc, ir_cur);
+        lane_tail<LB>(J.r, be, n, nr, blk, NBP, Zp, acc, ir_cur);
     }
     const double ldl = log(1.0 - 1e-12);
 #pragma unroll
     for (int l = 0; l < LB; ++l) {
         const int L = L0 + l;
-        if (L < NL) o[L] = nr > 0 ? acc[l] + (L == 0 ? 1.0 : exp((double)L * ldl)) * diag : 0.0;
+        const double dL = L == 0 ? 1.0 : exp((double)L * ldl);
+        if (L < NL) o[L] = nr > 0 ? fma(dL - 2.0, diag, acc[l]) : 0.0;
     }
 }
 
@@ -566,22 +624,24 @@ int32_t dwba_il_semisep_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const 
     return DWBA_OK;
 }
 
-int32_t dwba_il_semisep_lanes_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, int32_t L_max, double* d_out,
-                                  void* stream) {
+int32_t dwba_il_semisep_lanes_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, int32_t L_max, int32_t lb,
+                                  double* d_out, void* stream) {
     DWBA_REQUIRE(initialised(), "dwba_il_semisep_lanes_f64: call dwba_init first");
     DWBA_REQUIRE(n_jobs > 0 && d_jobs && d_out && L_max >= 0 && L_max <= 63,
                  "dwba_il_semisep_lanes_f64: bad arguments (n_jobs=%d, L_max=%d)", n_jobs, L_max);
-    constexpr int LB = 8;
-    const int nb = (L_max + LB) / LB;
-    const int nbp = nb <= 1 ? 1 : (nb <= 2 ? 2 : (nb <= 4 ? 4 : 8));
+    if (lb == 0) lb = 8;
+    DWBA_REQUIRE(lb == 4 || lb == 8, "dwba_il_semisep_lanes_f64: lb must be 0, 4 or 8 (got %d)", lb);
+    const int nb = (L_max + lb) / lb;
+    int nbp = 1;
+    while (nbp < nb) nbp <<= 1;                      // 1 .. 16
     const long lanes = 2L * n_jobs * nbp;
     const long blocks = (lanes + LT - 1) / LT;
     DWBA_REQUIRE(blocks <= INT32_MAX, "dwba_il_semisep_lanes_f64: too many jobs (%d)", n_jobs);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    switch (nbp) {
-#define C(P) case P: hipLaunchKernelGGL((semisep_lane_kernel<LB, P>), dim3((unsigned)blocks), dim3(LT), 0, s, d_jobs, \
-                                        n_jobs, L_max, d_out); break;
-        C(1) C(2) C(4) C(8)
+    switch (lb * 100 + nbp) {
+#define C(B, P) case B * 100 + P: hipLaunchKernelGGL((semisep_lane_kernel<B, P>), dim3((unsigned)blocks), dim3(LT), 0, \
+                                                     s, d_jobs, n_jobs, L_max, d_out); break;
+        C(8, 1) C(8, 2) C(8, 4) C(8, 8) C(4, 1) C(4, 2) C(4, 4) C(4, 8) C(4, 16)
 #undef C
         default: return set_error(DWBA_ERR_ARG, "dwba_il_semisep_lanes_f64: L_max %d", L_max);
     }

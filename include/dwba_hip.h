@@ -122,9 +122,10 @@ int32_t dwba_il_semisep_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const 
  * workspace and no join; same jobs, output layout and NULL-alpha_dir rule,
  * same values up to rounding order.  Fewer operations per grid point, less
  * parallelism: the better choice when jobs are many (the autotuner decides).
+ * lb = multipoles per lane (4 or 8, 0 = 8): 4 doubles the lanes.
  * Replaces the same reference code as dwba_il_semisep_f64. */
-int32_t dwba_il_semisep_lanes_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, int32_t L_max, double* d_out,
-                                  void* stream);
+int32_t dwba_il_semisep_lanes_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, int32_t L_max, int32_t lb,
+                                  double* d_out, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Full-Split preparation: alpha / beta vectors of a channel pair.       */

@@ -76,15 +76,16 @@ def test_semisep_lane_split_independent(seg, lb):
     _check(host, got, 20)
 
 
+@pytest.mark.parametrize("lb", [0, 4])
 @pytest.mark.parametrize("L_max", [0, 7, 8, 30, 63])
-def test_semisep_lanes_random_jobs(L_max):
+def test_semisep_lanes_random_jobs(L_max, lb):
     """Lane-serial kernel (one lane per job, family and multipole block)
     against the oracle on the same jobs: empty rows, one-point grid, rows
     shorter than the grid, L blocks that do not fill a power of two."""
     from dwba_amd import _hip
     host, dev = _jobs(SPECS, seed=100 + L_max)
-    print(f"\n[semisep lanes] {len(SPECS)} jobs L_max={L_max}", flush=True)
-    got = _hip.il_semisep_f64_batch(dev, L_max, lanes=True).cpu().numpy()
+    print(f"\n[semisep lanes] {len(SPECS)} jobs L_max={L_max} lb={lb}", flush=True)
+    got = _hip.il_semisep_f64_batch(dev, L_max, lanes=True, lb=lb).cpu().numpy()
     w = _check(host, got, L_max)
     print(f"  worst |d|/sum|aKb| {w:.2e}", flush=True)
 
