@@ -247,23 +247,60 @@ def semisep_work_flop(b):
     nl = b.L_max + 1
     gemv = bool(getattr(b, "direct_gemv", False))
     shared = set()
-    W = 0.0
+    W_gemv = 0.0
     if gemv:
         for ei, st in enumerate(b.E):
             ks = [k for k in st.members if b.pairs[k][2].mode in (PREP_FILON, PREP_STANDARD)]
             if ks and len({id(b.pairs[k][5]) for k in st.members}) == 1:
                 shared.update(ks)
-                W += SS_FLOP * st.N * nl
-                W += sum(2.0 * b.pairs[k][2].N1 * nl for k in ks)
+                W_gemv += SS_FLOP * st.N * nl
+                W_gemv += sum(2.0 * b.pairs[k][2].N1 * nl for k in ks)
+    W_scan = 0.0
+    bytes_scan = 0.0
     for k, (ei, _, plan, *_rest) in enumerate(b.pairs):
         if plan.rows <= 0:
             continue
         if k in shared:
             ex = plan.rows if plan.mode in (PREP_FILON, PREP_PLAIN) else plan.cols
-            W += SS_FLOP * ex * nl
+            W_scan += SS_FLOP * ex * nl
+            bytes_scan += 8.0 * (plan.rows + ex)                    # alpha_ex, beta_ex
         else:
-            W += SS_FLOP * 2 * plan.cols * nl
-    return {"flop": W, "gemv": gemv, "lanes": bool(getattr(b, "semisep_lanes", False))}
+            W_scan += SS_FLOP * 2 * plan.cols * nl
+            bytes_scan += 2 * 8.0 * (plan.rows + plan.cols)         # alpha, beta of both families
+    return {"flop": W_scan + W_gemv, "scan_flop": W_scan, "scan_bytes": bytes_scan, "gemv": gemv,
+            "lanes": bool(getattr(b, "semisep_lanes", False))}
+
+
+def scan_kernel_ms(b, steps, stream):
+    """Average duration (HIP events on the launch stream) of the batch's
+    per-pair recurrence kernel alone -- the lane-serial kernel (table-driven
+    when the batch uses tables) or the wavefront-scan kernel pair -- with
+    the batch's tuned configuration, after a full launch has written alpha/beta."""
+    hip, lib, Lm = b.hip, b.lib, b.L_max
+    seg, lb = b.semisep_config
+    jobs = b.plain_d_split if b.direct_gemv else b.plain_d
+    sh = hip.stream_handle(stream)
+    if b.semisep_lanes and b.lane_tables:
+        lbv = lb or 8
+        gd, ng, max_n, ptrs, _ = b._lane_table_set(lbv)
+        hip.check(lib.dwba_semisep_lane_tables_f64(ng, hip.ptr(gd), max_n, Lm, lbv, sh), "tables")
+        fn = lambda: hip.check(lib.dwba_il_semisep_lanes_tab_f64(b.plain_count, hip.ptr(jobs), hip.ptr(ptrs), Lm,
+                                                                 lbv, hip.ptr(b.out), sh), "lanes_tab")
+        name = f"semisep_lane_tab_kernel<{lbv},*>"
+    elif b.semisep_lanes:
+        fn = lambda: hip.check(lib.dwba_il_semisep_lanes_f64(b.plain_count, hip.ptr(jobs), Lm, lb, hip.ptr(b.out),
+                                                             sh), "lanes")
+        name = f"semisep_lane_kernel<{lb or 8},*>"
+    else:
+        nra, nca = b.plain_n_split if b.direct_gemv else b.plain_n
+        fn = lambda: hip.check(lib.dwba_il_semisep_f64(b.plain_count, hip.ptr(jobs), nra, nca, Lm, seg, lb,
+                                                       hip.ptr(b.out), hip.ptr(b.plain_ws), b.plain_ws_bytes, sh),
+                               "semisep")
+        name = "semisep_plan + semisep_kernel + semisep_join"
+    fn()
+    ms = _events_ms(fn, steps, stream)
+    b.launch()                     # leave a complete result in b.out
+    return name, ms
 
 
 def _committed_traffic(stage, workload):
@@ -346,7 +383,9 @@ def bench_batched(args, ionization=False):
         # alpha (rows) and beta (cols) of both families, FP64
         sizes["alpha_beta_bytes"] += 2 * 8 * (plan.rows + plan.cols)
     ss_work = semisep_work_flop(b64s)
-    del b64s
+    dot_kind = getattr(b64s, "dot_kind", "tiled")
+    kernel_timing = scan_kernel_ms(b64s, max(args.steps, 5), stream)
+    log(f"scan kernel {kernel_timing[0]}: {kernel_timing[1]:.3f} ms")
     result = {
         "metric": METRIC, "value": n_int / (ms64s * 1e-3), "unit": "integrals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms64s, "higher_is_better": True,
@@ -365,23 +404,38 @@ def bench_batched(args, ionization=False):
         "stage_ms": st64s,
         "autotune": ({"params": tuned.params, "ms": tuned.ms, "cached": tuned.cached, "candidates": tuned.timings}
                      if tuned else None),
+        "tail_overlap": ({"mode": b64s.overlap, "candidates_ms": {str(m): t["ms"] for m, t in
+                                                                   zip((None, "start", "after_prep"),
+                                                                       b64s.tuning_overlap.timings)}
+                          if b64s.tuning_overlap.timings else None}
+                         if getattr(b64s, "tuning_overlap", None) else None),
         "host_note": "host planning, uploads and autotuning precede the timed region",
     }
+    del b64s
     t_c = st64s.get("plain")
     if t_c:
         W = ss_work["flop"]
-        kern = ("semisep_lane_kernel<8,4>" if ss_work["lanes"] else "semisep_kernel<8,4> + semisep_join") + (
-            " + semisep_gemv_kernel + ddot_tiled_kernel" if ss_work["gemv"] else "")
-        result["roofline"] = {"bound": "mfma", "achieved": W / (t_c * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
-                              "unit": "TFLOP/s", "frac": W / (t_c * 1e-3) / PEAK_FP64,
-                              "traffic": _committed_traffic("contraction", "batched_64x400"),
-                              "kernel": kern, "stage": "contraction", "stage_ms": t_c,
-                              "work_flop_per_launch": W, "algorithmic_bytes_per_launch": sizes["alpha_beta_bytes"],
-                              "note": "FP64 arithmetic on the vector ALUs (MI355X FP64 vector peak = FP64 matrix "
-                                      "peak = 78.6 TF/s); 9 flop per (grid point, L, family) for the sequential "
-                                      "recurrences (2 FMA, 2 add+mul, 1 power step); 2 per (point, L) for the dot "
-                                      "of the shared-beta path; "
-                                      "algorithmic bytes = alpha + beta of both families, FP64, read once"}
+        kname, k_ms = kernel_timing
+        Wk = ss_work["scan_flop"]
+        wl = "ionization_48x400" if ionization else "batched_64x400"
+        result["roofline"] = {"bound": "mfma", "achieved": Wk / (k_ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+                              "unit": "TFLOP/s", "frac": Wk / (k_ms * 1e-3) / PEAK_FP64,
+                              "traffic": _committed_traffic("scan_kernel", wl),
+                              "kernel": kname, "kernel_ms": k_ms, "work_flop_per_launch": Wk,
+                              "algorithmic_bytes_per_launch": ss_work["scan_bytes"],
+                              "hbm_gbs_algorithmic": ss_work["scan_bytes"] / (k_ms * 1e-3) / 1e9,
+                              "note": "dominant contraction kernel, timed alone with HIP events on its launch "
+                                      "stream (average over the timed launches); FP64 arithmetic on the vector "
+                                      "ALUs (MI355X FP64 vector peak = FP64 matrix peak = 78.6 TF/s); 9 flop per "
+                                      "(grid point, L, family) of the recurrences it runs; algorithmic bytes = "
+                                      "the alpha and beta it reads, FP64, once",
+                              "contraction_stage": {
+                                  "ms": t_c, "achieved_tflops": W / (t_c * 1e-3) / 1e12,
+                                  "frac": W / (t_c * 1e-3) / PEAK_FP64, "work_flop": W,
+                                  "kernels": kname + (" + semisep_gemv_kernel + ddot_" + dot_kind + "_kernel"
+                                                      if ss_work["gemv"] else ""),
+                                  "note": "all contraction kernels of a step (recurrences, shared-beta GEMV and "
+                                          "per-pair dots at 2 flop per point and L)"}}
 
     if args.no_fp32:
         if not args.no_cpu_baseline:

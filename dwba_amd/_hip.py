@@ -86,6 +86,10 @@ _SIGS = {
     "dwba_il_semisep_lanes_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_vp, c_vp]),
     "dwba_ddot_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
     "dwba_ddot_tiled_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_vp, c_vp]),
+    "dwba_ddot_mfma_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_vp, c_vp]),
+    "dwba_semisep_lane_table_stride": (c_i32, [c_i32, c_i32]),
+    "dwba_semisep_lane_tables_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_i32, c_vp]),
+    "dwba_il_semisep_lanes_tab_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp]),
     "dwba_batch_finalize": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
     "dwba_il_backward_f64": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
     "dwba_il_backward_f32": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),

@@ -55,6 +55,11 @@ class DgemvEnergy(ctypes.Structure):
                 ("N", ctypes.c_int32), ("rows", ctypes.c_int32), ("ld", ctypes.c_int32), ("L_max", ctypes.c_int32)]
 
 
+class LaneGrid(ctypes.Structure):
+    """dwba_lane_grid of include/dwba_hip.h."""
+    _fields_ = [("r", ctypes.c_void_p), ("tab", ctypes.c_void_p), ("n", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
 class DdotPair(ctypes.Structure):
     _fields_ = [("alpha", ctypes.c_void_p), ("t", ctypes.c_void_p), ("n", ctypes.c_int32), ("ld", ctypes.c_int32),
                 ("out_index", ctypes.c_int32), ("pad_", ctypes.c_int32)]
@@ -132,6 +137,12 @@ class ScanBatch:
         torch.cuda.synchronize()
         self.build_seconds = time.perf_counter() - t0
         self.graph = None
+        # "after_prep": the tail chain (moments, outer tails) runs on a second
+        # stream, forked after prep and joined before finalize, so it overlaps
+        # the contractions; "start": forked before prep; None: one stream.
+        self.overlap = "after_prep"
+        self._side = None
+        self.tuning_overlap = None
 
     # ------------------------------------------------------------ planning
     def _plan(self, energies, use_osc, method, phase_inc, mgf, k_thr):
@@ -367,6 +378,15 @@ class ScanBatch:
             self.ss_gemv = (hip.struct_array_to_device((DgemvEnergy * len(dg))(*dg), self.dev), len(dg),
                             hip.struct_array_to_device((DdotPair * len(dd))(*dd), self.dev), len(dd),
                             items_d, len(items))
+            # tiles of <= 16 pairs for the matrix-core dot
+            items16 = []
+            first = 0
+            for cnt in dd_counts:
+                for q in range(0, cnt, 16):
+                    items16.append((first + q, min(16, cnt - q)))
+                first += cnt
+            self.items16 = (self.torch.as_tensor(np.asarray(items16, dtype=np.int32).reshape(-1, 2),
+                                                 device=self.dev), len(items16))
         arr_split = (hip.PlainJob * self.n_pairs)()
         nca_split = (ctypes.c_int32 * self.n_pairs)()
         for g, k in enumerate(order):
@@ -386,12 +406,16 @@ class ScanBatch:
             nca_split[g] = cols_split
         self.plain_d = hip.struct_array_to_device(arr, self.dev)
         self.plain_d_split = hip.struct_array_to_device(arr_split, self.dev) if self.ss_gemv is not None else None
+        self._job_energy = [self.pairs[k][0] for k in order]     # grid of each job, in job order
+        self._lane_tabs = {}                                      # lb -> (grids, n, max_n, tab pointer array)
         self.plain_n_split = (nra, nca_split)
         self.direct_gemv = False          # chosen by tune() when the shared-beta path exists
+        self.dot_kind = "mfma"            # per-pair dots of that path: "mfma" (FP64 matrix cores) or "tiled"
         self.plain_n = (nra, nca)
         if self.contraction == "semiseparable":
             self.semisep_config = (0, 0)      # (seg, lb): library defaults until tuned
             self.semisep_lanes = False        # lane-serial kernel instead (chosen by tune())
+            self.lane_tables = True           # that kernel reads per-grid 1/r, q, q^(lb b) tables
             self.plain_ws_bytes = self._semisep_ws_bytes(0)
         else:
             self.plain_ws_bytes = self.lib.dwba_il_plain_f64_batch_workspace_bytes(self.n_pairs, nra, nca,
@@ -529,6 +553,28 @@ class ScanBatch:
             nb = max(nb, self.lib.dwba_il_semisep_f64_workspace_bytes(len(nra_s), nra_s, nca_s, self.L_max, seg))
         return nb
 
+    def _lane_table_set(self, lb):
+        """Per-energy tables of the table-driven lane kernel for multipole
+        block lb (built on first use): device grid descriptors and the
+        job -> table pointer array."""
+        hit = self._lane_tabs.get(lb)
+        if hit is not None:
+            return hit
+        torch, hip, lib = self.torch, self.hip, self.lib
+        stride = lib.dwba_semisep_lane_table_stride(self.L_max, lb)
+        if stride <= 0:
+            raise ValueError(f"lane tables: bad (L_max, lb) = ({self.L_max}, {lb})")
+        tabs, grids = [], (LaneGrid * len(self.E))()
+        for ei, st in enumerate(self.E):
+            t = torch.empty(st.N * stride, dtype=torch.float64, device=self.dev)
+            tabs.append(t)
+            grids[ei] = LaneGrid(hip.ptr(st.r_d), hip.ptr(t), st.N, 0)
+        ptrs = torch.as_tensor(np.asarray([hip.ptr(tabs[ei]) for ei in self._job_energy], dtype=np.int64),
+                               device=self.dev)
+        ent = (hip.struct_array_to_device(grids, self.dev), len(self.E), max(st.N for st in self.E), ptrs, tabs)
+        self._lane_tabs[lb] = ent
+        return ent
+
     def _contract_semisep(self, seg, lb, direct_gemv, out, ws, ws_bytes, sh, lanes=False):
         """All FP64 contractions of the batch through the semiseparable kernel;
         with direct_gemv the Filon/standard pairs of shared-beta energies take
@@ -536,7 +582,14 @@ class ScanBatch:
         hip, lib, Lm = self.hip, self.lib, self.L_max
         nra, nca = self.plain_n_split if direct_gemv else self.plain_n
         jobs = self.plain_d_split if direct_gemv else self.plain_d
-        if lanes:
+        if lanes and self.lane_tables:
+            lbv = lb or 8
+            gd, ng, max_n, ptrs, _ = self._lane_table_set(lbv)
+            hip.check(lib.dwba_semisep_lane_tables_f64(ng, hip.ptr(gd), max_n, Lm, lbv, sh),
+                      "dwba_semisep_lane_tables_f64")
+            hip.check(lib.dwba_il_semisep_lanes_tab_f64(self.plain_count, hip.ptr(jobs), hip.ptr(ptrs), Lm, lbv,
+                                                        hip.ptr(out), sh), "dwba_il_semisep_lanes_tab_f64")
+        elif lanes:
             hip.check(lib.dwba_il_semisep_lanes_f64(self.plain_count, hip.ptr(jobs), Lm, lb, hip.ptr(out), sh),
                       "dwba_il_semisep_lanes_f64")
         else:
@@ -545,7 +598,11 @@ class ScanBatch:
         if direct_gemv:
             gd, ng, dd, nd, items_d, ni = self.ss_gemv
             hip.check(lib.dwba_semisep_gemv_f64(ng, hip.ptr(gd), Lm, sh), "dwba_semisep_gemv_f64")
-            if Lm <= 31:
+            if Lm <= 31 and self.dot_kind == "mfma":
+                it16, n16 = self.items16
+                hip.check(lib.dwba_ddot_mfma_f64(n16, hip.ptr(it16), hip.ptr(dd), Lm, hip.ptr(out), sh),
+                          "dwba_ddot_mfma_f64")
+            elif Lm <= 31:
                 hip.check(lib.dwba_ddot_tiled_f64(ni, hip.ptr(items_d), hip.ptr(dd), Lm, hip.ptr(out), sh),
                           "dwba_ddot_tiled_f64")
             else:
@@ -566,25 +623,53 @@ class ScanBatch:
                 ev = torch.cuda.Event(enable_timing=True)
                 ev.record(strm)
                 events[(name, which)] = ev
-        mark("total", "start")
-        mark("prep", "start")
         Lm = self.L_max
+
+        def tails(h):
+            if not self.n_tail:
+                return
+            hip.check(lib.dwba_moments_f64(self.n_mom_sets, hip.ptr(self.mom_ptr[0]), hip.ptr(self.mom_ptr[1]),
+                                           hip.ptr(self.mom_ptr[2]), hip.ptr(self.mom_ptr[3]), hip.ptr(self.mom_n),
+                                           Lm, hip.ptr(self.moments), h), "dwba_moments_f64")
+            seg, dmax = planner.filon_limits()
+            Dp, _ = hip.f64_array(self.levin[0])
+            Xp, _ = hip.f64_array(self.levin[1])
+            hip.check(lib.dwba_outer_tail_f64(self.n_tail, hip.ptr(self.tail_d), Lm, seg, dmax, Dp, Xp,
+                                              hip.ptr(self.tail_ws), self.tail_ws_bytes, h), "dwba_outer_tail_f64")
+
+        # The tail chain reads only bound densities, phases and its own
+        # moments -- nothing prep or the contractions write -- so without
+        # stage events it runs on a side stream (default priority) forked
+        # from the launch stream and joined before finalize.
+        fork = events is None and bool(self.n_tail) and self.overlap in ("start", "after_prep")
+        ev_join = None
+
+        def fork_tails():
+            nonlocal ev_join
+            if self._side is None:
+                self._side = torch.cuda.Stream(device=self.dev)
+            ev_fork = torch.cuda.Event()
+            ev_fork.record(strm)
+            self._side.wait_event(ev_fork)
+            tails(hip.stream_handle(self._side))
+            ev_join = torch.cuda.Event()
+            ev_join.record(self._side)
+
+        mark("total", "start")
+        if fork and self.overlap == "start":
+            fork_tails()
+        mark("prep", "start")
         cxp, _ = hip.f64_array(self.cc[0])
         cwp, _ = hip.f64_array(self.cc[1])
         hip.check(lib.dwba_fullsplit_prep_f64(self.n_pairs, hip.ptr(self.prep_d), hip.ptr(self.coul_d),
                                               int(self.any_coul), cxp, cwp, self.CC_nodes, sh),
                   "dwba_fullsplit_prep_f64")
         mark("prep", "end")
+        if fork and self.overlap == "after_prep":
+            fork_tails()
         mark("tail", "start")
-        if self.n_tail:
-            hip.check(lib.dwba_moments_f64(self.n_mom_sets, hip.ptr(self.mom_ptr[0]), hip.ptr(self.mom_ptr[1]),
-                                           hip.ptr(self.mom_ptr[2]), hip.ptr(self.mom_ptr[3]), hip.ptr(self.mom_n),
-                                           Lm, hip.ptr(self.moments), sh), "dwba_moments_f64")
-            seg, dmax = planner.filon_limits()
-            Dp, _ = hip.f64_array(self.levin[0])
-            Xp, _ = hip.f64_array(self.levin[1])
-            hip.check(lib.dwba_outer_tail_f64(self.n_tail, hip.ptr(self.tail_d), Lm, seg, dmax, Dp, Xp,
-                                              hip.ptr(self.tail_ws), self.tail_ws_bytes, sh), "dwba_outer_tail_f64")
+        if not fork:
+            tails(sh)
         mark("tail", "end")
         mark("plain", "start")
         if self.plain_count:
@@ -621,6 +706,8 @@ class ScanBatch:
                              Lm, hip.ptr(self.partial), hip.ptr(self.counter), self.xg_max_cb,
                              hip.ptr(self.out), 0, sh), "dwba_xgemm")
                 mark("xgemm", "end")
+        if ev_join is not None:
+            strm.wait_event(ev_join)
         hip.check(lib.dwba_batch_finalize(self.n_pairs, hip.ptr(self.fin_d), Lm, hip.ptr(self.out), sh),
                   "dwba_batch_finalize")
         mark("total", "end")
@@ -710,7 +797,7 @@ class ScanBatch:
         Lm = self.L_max
         has_gemv = self.ss_gemv is not None
         key = AT.signature("semisep_f64", AT.device_id(self.dev), jobs=AT.bucket(self.plain_count),
-                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm, gemv=int(has_gemv), v=2)
+                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm, gemv=int(has_gemv), v=3)
         self.launch()                       # alpha/beta in place for the contraction
         torch.cuda.synchronize()
         out = torch.empty_like(self.out)
@@ -737,6 +824,20 @@ class ScanBatch:
         self.plain_ws_bytes = self._semisep_ws_bytes(seg)
         self.plain_ws = torch.empty(max(self.plain_ws_bytes, 1), dtype=torch.uint8, device=self.dev)
         self.tuning = choice
+        if self.n_tail:
+            # where the tail chain forks off the launch stream (none / before prep / after prep)
+            modes = (None, "start", "after_prep")
+            okey = AT.signature("scan_overlap", AT.device_id(self.dev), jobs=AT.bucket(self.n_pairs),
+                                tails=AT.bucket(self.n_tail), n=AT.bucket(float(np.mean(list(nca)))), L=Lm,
+                                c=f"{seg}.{lb}.{int(self.semisep_lanes)}.{int(self.direct_gemv)}", v=1)
+
+            def run_overlap(c):
+                self.overlap = modes[int(c["overlap"])]
+                self.launch()
+
+            oc = tuner.tune(okey, [{"overlap": i} for i in range(len(modes))], run_overlap, steps=steps)
+            self.overlap = modes[int(oc.params["overlap"])]
+            self.tuning_overlap = oc
         if self.graph is not None:
             self.graph = None
             self.capture()

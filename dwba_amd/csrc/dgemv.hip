@@ -212,6 +212,125 @@ __global__ __launch_bounds__(256) void ddot_tiled_kernel(const dwba_ddot_pair* _
     }
 }
 
+// The same tiled dots on the FP64 matrix cores: for an item of <= 16 pairs
+// sharing t, out[p][L] = sum_i alpha_p[i] t[L][i] is a 16 x 32 x n GEMM.
+// v_mfma_f64_16x16x4_f64 takes A (16 x 4) with A[m][k] in lane m + 16k and
+// B (4 x 16) with B[k][j] in lane j + 16k, and returns D (16 x 16) with
+// D[lane/16 + 4v][lane % 16] in element v (measured on gfx950; the FP32
+// 16x16x4 form interleaves rows the other way).  The order of the summed
+// points is free as long as A and B agree, so lane (m, k) reads MS
+// consecutive points k*MS .. k*MS+MS-1 of a 4*MS-point step straight from
+// global memory (pair m for A, multipoles m and m + 16 for B): no LDS, no
+// broadcast reads.  The MW wavefronts of a workgroup take interleaved steps
+// and are summed through LDS at the end.
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int MQ = 16;        // pairs per item
+constexpr int MS = 4;         // points per lane and step
+constexpr int MSTEP = 4 * MS; // points per wavefront step
+constexpr int MW = 4;         // wavefronts per workgroup
+
+// NT = 1 or 2 tiles of 16 multipoles, fixed at compile time: no branch may
+// separate MFMAs from the reads of their accumulators (the wait states the
+// compiler inserts before those reads are only right on straight-line code).
+template <int NT>
+__global__ __launch_bounds__(64 * MW) void ddot_mfma_kernel(const dwba_ddot_pair* __restrict__ pairs,
+                                                            const int2* __restrict__ items, int n_items, int L_max,
+                                                            double* __restrict__ out) {
+    __shared__ double s_red[MW - 1][4 * NT][64];
+    // consecutive items (one energy's pairs, sharing t) on one XCD: blocks
+    // are dealt to the 8 XCDs round-robin
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = nb / 8, rem = nb % 8, xcd = b % 8;
+    const int item = (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + b / 8;
+    if (item >= n_items) return;
+    const int2 it = items[item];
+    const int first = it.x, count = it.y;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m = lane & 15, kq = lane >> 4;
+    const int nl = L_max + 1;
+    const double* t = pairs[first].t;
+    const size_t ld = (size_t)pairs[first].ld;
+    int nmax = 0;
+    for (int q = 0; q < count; ++q) nmax = max(nmax, pairs[first + q].n);
+    // lanes past the item's pairs or past L_max read a valid row (pair 0,
+    // multipole 0) whose results land in output rows / columns never stored
+    const double* ap = pairs[first + (m < count ? m : 0)].alpha;
+    const double* tp[NT];
+#pragma unroll
+    for (int c = 0; c < NT; ++c) tp[c] = t + (size_t)(m + 16 * c < nl ? m + 16 * c : 0) * ld;
+    v4d acc[NT];
+#pragma unroll
+    for (int c = 0; c < NT; ++c) acc[c] = v4d{0.0, 0.0, 0.0, 0.0};
+    const int n_full = nmax / MSTEP;
+    // one step: MS points per lane, loaded then multiplied, branch-free
+    auto step_full = [&](int st) {
+        const int i = st * MSTEP + kq * MS;
+        double a[MS], bb[NT][MS];
+#pragma unroll
+        for (int u = 0; u < MS; ++u) {
+            a[u] = ap[i + u];
+#pragma unroll
+            for (int c = 0; c < NT; ++c) bb[c][u] = tp[c][i + u];
+        }
+#pragma unroll
+        for (int u = 0; u < MS; ++u)
+#pragma unroll
+            for (int c = 0; c < NT; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bb[c][u], acc[c], 0, 0, 0);
+    };
+    int st = w;
+    for (; st + MW < n_full; st += 2 * MW) {
+        step_full(st);
+        step_full(st + MW);
+    }
+    if (st < n_full) step_full(st);
+    // the last, partial step (points below nmax only): clamped reads, masked values
+    if (n_full * MSTEP < nmax && n_full % MW == w) {
+        const int i = n_full * MSTEP + kq * MS;
+        double a[MS], bb[NT][MS];
+#pragma unroll
+        for (int u = 0; u < MS; ++u) {
+            const int j = min(i + u, nmax - 1);
+            const bool in = i + u < nmax;
+            const double va = ap[j];
+            a[u] = in ? va : 0.0;
+#pragma unroll
+            for (int c = 0; c < NT; ++c) {
+                const double vb = tp[c][j];
+                bb[c][u] = in ? vb : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < MS; ++u)
+#pragma unroll
+            for (int c = 0; c < NT; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bb[c][u], acc[c], 0, 0, 0);
+    }
+    if (w > 0) {
+#pragma unroll
+        for (int c = 0; c < NT; ++c)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) s_red[w - 1][4 * c + v][lane] = acc[c][v];
+    }
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+        for (int q = 0; q < MW - 1; ++q)
+#pragma unroll
+            for (int c = 0; c < NT; ++c)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[c][v] += s_red[q][4 * c + v][lane];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = kq + 4 * v;       // pair in the item
+            if (row < count) {
+                double* o = out + (size_t)pairs[first + row].out_index * 2 * nl;
+#pragma unroll
+                for (int c = 0; c < NT; ++c)
+                    if (m + 16 * c < nl) o[m + 16 * c] = acc[c][v];
+            }
+        }
+    }
+}
+
 int nl_bucket_g(int L_max) {
     static const int buckets[] = {1, 2, 4, 8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64};
     for (int b : buckets)
@@ -265,6 +384,23 @@ int32_t dwba_ddot_tiled_f64(int32_t n_items, const int32_t* d_items, const dwba_
     hipLaunchKernelGGL(ddot_tiled_kernel, dim3(n_items), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), d_pairs,
                        reinterpret_cast<const int2*>(d_items), L_max, d_out);
     DWBA_LAUNCH_CHECK("ddot_tiled_kernel");
+    return DWBA_OK;
+}
+
+int32_t dwba_ddot_mfma_f64(int32_t n_items, const int32_t* d_items, const dwba_ddot_pair* d_pairs, int32_t L_max,
+                           double* d_out, void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_ddot_mfma_f64: call dwba_init first");
+    DWBA_REQUIRE(n_items > 0 && d_items && d_pairs && d_out && L_max >= 0 && L_max <= 31,
+                 "dwba_ddot_mfma_f64: bad arguments (L_max <= 31)");
+    if (L_max < 16)
+        hipLaunchKernelGGL(ddot_mfma_kernel<1>, dim3(n_items), dim3(64 * MW), 0,
+                           reinterpret_cast<hipStream_t>(stream), d_pairs, reinterpret_cast<const int2*>(d_items),
+                           n_items, L_max, d_out);
+    else
+        hipLaunchKernelGGL(ddot_mfma_kernel<2>, dim3(n_items), dim3(64 * MW), 0,
+                           reinterpret_cast<hipStream_t>(stream), d_pairs, reinterpret_cast<const int2*>(d_items),
+                           n_items, L_max, d_out);
+    DWBA_LAUNCH_CHECK("ddot_mfma_kernel");
     return DWBA_OK;
 }
 

@@ -445,6 +445,171 @@ __global__ __launch_bounds__(LT) void semisep_lane_kernel(const dwba_plain_job* 
 }
 
 // ---------------------------------------------------------------------------
+// Table-driven lane-serial form.  1 / r_i, q_i and q_i^(LB blk) depend only
+// on the grid, so one pass per grid writes them (with the same operations the
+// kernel above uses, hence bit-identical sums) and every lane of every pair on
+// that grid reads them instead of repeating a correctly rounded division and
+// the power chain at each point.  Row i of a table holds LS = 2 + NBP doubles:
+//     1 / r_i,  q_i = r_i * (1 / r_{i+1}),  (q_i^LB)^b for b = 0 .. NBP-1
+// (r_{i+1} clamped to the last point; the last row's q is never used).
+__global__ __launch_bounds__(256) void semisep_lane_table_kernel(const dwba_lane_grid* __restrict__ grids, int lb,
+                                                                 int nbp) {
+    const dwba_lane_grid G = grids[blockIdx.y];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= G.n) return;
+    const int ls = 2 + nbp;
+    const double ri = G.r[i];
+    const double ir_next = 1.0 / G.r[min(i + 1, G.n - 1)];
+    const double q = ri * ir_next;
+    double qb = q;
+    for (int m = 1; m < lb; m <<= 1) qb *= qb;
+    double* __restrict__ t = G.tab + (size_t)i * ls;
+    t[0] = 1.0 / ri;
+    t[1] = q;
+    double q0 = 1.0;
+    for (int b = 0; b < nbp; ++b) {
+        t[2 + b] = q0;
+        q0 *= qb;
+    }
+}
+
+// One point with alpha (the arithmetic of lane_point_a, factors from the table).
+template <int LB>
+__device__ __forceinline__ void lane_point_at(double iri, double q, double q0, double a_i, double b_i, double a_next,
+                                              double b_next, double& diag, double* Ap, double* Zp, double* acc) {
+    const double aw = a_i * iri, bw = b_i * iri;
+    diag = fma(a_i, bw, diag);
+    double qL = q0;
+#pragma unroll
+    for (int l = 0; l < LB; ++l) {
+        acc[l] = fma(aw, Ap[l], acc[l]);
+        acc[l] = fma(bw, Zp[l], acc[l]);
+        Ap[l] = fma(qL, Ap[l], b_next);
+        Zp[l] = fma(qL, Zp[l], a_next);
+        qL *= q;
+    }
+}
+
+// One point beyond the alpha rows (the arithmetic of lane_point_b).
+template <int LB>
+__device__ __forceinline__ void lane_point_bt(double iri, double q, double q0, double b_i, double* Zp, double* acc) {
+    const double bw = b_i * iri;
+    double qL = q0;
+#pragma unroll
+    for (int l = 0; l < LB; ++l) {
+        acc[l] = fma(bw, Zp[l], acc[l]);
+        Zp[l] *= qL;
+        qL *= q;
+    }
+}
+
+template <int LB, int NBP>
+__global__ __launch_bounds__(LT) void semisep_lane_tab_kernel(const dwba_plain_job* __restrict__ jobs,
+                                                              const double* const* __restrict__ tabs, int n_jobs,
+                                                              int L_max, double* __restrict__ out) {
+    constexpr int LS = 2 + NBP;
+    const long gl = (long)blockIdx.x * LT + threadIdx.x;
+    const long stream = gl / NBP;
+    const int blk = (int)(gl - stream * NBP);
+    if (stream >= 2L * n_jobs) return;
+    const int fam = stream >= n_jobs ? 1 : 0;
+    const int k = (int)(fam ? stream - n_jobs : stream);
+    const int NL = L_max + 1;
+    const int L0 = blk * LB;
+    if (L0 >= NL) return;
+    const dwba_plain_job J = jobs[k];
+    if (fam == 0 && J.alpha_dir == nullptr) return;    // direct family computed elsewhere
+    double* __restrict__ o = out + (size_t)k * 2 * NL + fam * NL;
+    double Ap[LB], Zp[LB], acc[LB];
+    double diag = 0.0;
+    const int n = J.n_cols, nr = min(J.n_rows, J.n_cols);
+#pragma unroll
+    for (int l = 0; l < LB; ++l) acc[l] = 0.0;
+    if (nr > 0 && n > 0) {
+        const double* __restrict__ al = fam ? J.alpha_ex : J.alpha_dir;
+        const double* __restrict__ be = fam ? J.beta_ex : J.beta_dir;
+        const double* __restrict__ T = tabs[k];
+        auto tir = [&](int i) { return T[(size_t)i * LS]; };
+        auto tq = [&](int i) { return T[(size_t)i * LS + 1]; };
+        auto tq0 = [&](int i) { return T[(size_t)i * LS + 2 + blk]; };
+        auto ld_a = [&](int i) { return i < nr ? al[i] : 0.0; };
+        auto ld_b = [&](int i) { return i < n ? be[i] : 0.0; };
+        const double a0 = al[0], b0 = be[0];
+#pragma unroll
+        for (int l = 0; l < LB; ++l) { Ap[l] = b0; Zp[l] = a0; }   // A_0 = Z_0 = 0
+        // points [0, nr): groups of LGU with the next group's loads in flight
+        int i = 0;
+        if (LGU <= nr) {
+            double gi[LGU], gq[LGU], g0[LGU], ca[LGU + 1], cb[LGU + 1];
+#pragma unroll
+            for (int u = 0; u < LGU; ++u) { gi[u] = tir(u); gq[u] = tq(u); g0[u] = tq0(u); }
+#pragma unroll
+            for (int u = 0; u <= LGU; ++u) { ca[u] = ld_a(u); cb[u] = ld_b(u); }
+#pragma unroll 1
+            for (; i + LGU <= nr; i += LGU) {
+                const bool more = i + 2 * LGU <= nr;
+                double xi[LGU], xq[LGU], x0[LGU], xa[LGU], xb[LGU];
+                if (more) {
+#pragma unroll
+                    for (int u = 0; u < LGU; ++u) {
+                        const int kk = i + LGU + u;
+                        xi[u] = tir(kk); xq[u] = tq(kk); x0[u] = tq0(kk);
+                        xa[u] = ld_a(kk + 1); xb[u] = ld_b(kk + 1);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < LGU; ++u)
+                    lane_point_at<LB>(gi[u], gq[u], g0[u], ca[u], cb[u], ca[u + 1], cb[u + 1], diag, Ap, Zp, acc);
+                if (more) {
+                    ca[0] = ca[LGU]; cb[0] = cb[LGU];
+#pragma unroll
+                    for (int u = 0; u < LGU; ++u) {
+                        gi[u] = xi[u]; gq[u] = xq[u]; g0[u] = x0[u];
+                        ca[u + 1] = xa[u]; cb[u + 1] = xb[u];
+                    }
+                }
+            }
+        }
+#pragma unroll 1
+        for (; i < nr; ++i)
+            lane_point_at<LB>(tir(i), tq(i), tq0(i), al[i], be[i], ld_a(i + 1), ld_b(i + 1), diag, Ap, Zp, acc);
+        // points [nr, n): beta and the decaying Z only
+        if (i + LGU <= n) {
+            double gi[LGU], gq[LGU], g0[LGU], cb[LGU];
+#pragma unroll
+            for (int u = 0; u < LGU; ++u) { gi[u] = tir(i + u); gq[u] = tq(i + u); g0[u] = tq0(i + u); cb[u] = be[i + u]; }
+#pragma unroll 1
+            for (; i + LGU <= n; i += LGU) {
+                const bool more = i + 2 * LGU <= n;
+                double xi[LGU], xq[LGU], x0[LGU], xb[LGU];
+                if (more) {
+#pragma unroll
+                    for (int u = 0; u < LGU; ++u) {
+                        const int kk = i + LGU + u;
+                        xi[u] = tir(kk); xq[u] = tq(kk); x0[u] = tq0(kk); xb[u] = be[kk];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < LGU; ++u) lane_point_bt<LB>(gi[u], gq[u], g0[u], cb[u], Zp, acc);
+                if (more) {
+#pragma unroll
+                    for (int u = 0; u < LGU; ++u) { gi[u] = xi[u]; gq[u] = xq[u]; g0[u] = x0[u]; cb[u] = xb[u]; }
+                }
+            }
+        }
+#pragma unroll 1
+        for (; i < n; ++i) lane_point_bt<LB>(tir(i), tq(i), tq0(i), be[i], Zp, acc);
+    }
+    const double ldl = log(1.0 - 1e-12);
+#pragma unroll
+    for (int l = 0; l < LB; ++l) {
+        const int L = L0 + l;
+        const double dL = L == 0 ? 1.0 : exp((double)L * ldl);
+        if (L < NL) o[L] = nr > 0 ? fma(dL - 2.0, diag, acc[l]) : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Shared-beta form (the direct family of an excitation scan, where every
 // pair of an energy has the same beta = u_f u_i w): t_L(i) = sum_j K_L(i, j) beta_j
 // for i < rows, all L, once per energy, in O(N) per L:
@@ -646,6 +811,56 @@ int32_t dwba_il_semisep_lanes_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, 
         default: return set_error(DWBA_ERR_ARG, "dwba_il_semisep_lanes_f64: L_max %d", L_max);
     }
     DWBA_LAUNCH_CHECK("semisep_lane_kernel");
+    return DWBA_OK;
+}
+
+static int lane_nbp(int32_t L_max, int32_t lb) {
+    const int nb = (L_max + lb) / lb;
+    int nbp = 1;
+    while (nbp < nb) nbp <<= 1;                      // 1 .. 16
+    return nbp;
+}
+
+int32_t dwba_semisep_lane_table_stride(int32_t L_max, int32_t lb) {
+    if (lb == 0) lb = 8;
+    if ((lb != 4 && lb != 8) || L_max < 0 || L_max > 63) return -1;
+    return 2 + lane_nbp(L_max, lb);
+}
+
+int32_t dwba_semisep_lane_tables_f64(int32_t n_grids, const dwba_lane_grid* d_grids, int32_t max_n, int32_t L_max,
+                                     int32_t lb, void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_semisep_lane_tables_f64: call dwba_init first");
+    if (lb == 0) lb = 8;
+    DWBA_REQUIRE(n_grids > 0 && n_grids <= 65535 && d_grids && max_n > 0 && L_max >= 0 && L_max <= 63 &&
+                     (lb == 4 || lb == 8),
+                 "dwba_semisep_lane_tables_f64: bad arguments (n_grids=%d, max_n=%d, L_max=%d, lb=%d)", n_grids,
+                 max_n, L_max, lb);
+    hipLaunchKernelGGL(semisep_lane_table_kernel, dim3((max_n + 255) / 256, n_grids), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), d_grids, lb, lane_nbp(L_max, lb));
+    DWBA_LAUNCH_CHECK("semisep_lane_table_kernel");
+    return DWBA_OK;
+}
+
+int32_t dwba_il_semisep_lanes_tab_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const double* const* d_tabs,
+                                      int32_t L_max, int32_t lb, double* d_out, void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_il_semisep_lanes_tab_f64: call dwba_init first");
+    DWBA_REQUIRE(n_jobs > 0 && d_jobs && d_tabs && d_out && L_max >= 0 && L_max <= 63,
+                 "dwba_il_semisep_lanes_tab_f64: bad arguments (n_jobs=%d, L_max=%d)", n_jobs, L_max);
+    if (lb == 0) lb = 8;
+    DWBA_REQUIRE(lb == 4 || lb == 8, "dwba_il_semisep_lanes_tab_f64: lb must be 0, 4 or 8 (got %d)", lb);
+    const int nbp = lane_nbp(L_max, lb);
+    const long lanes = 2L * n_jobs * nbp;
+    const long blocks = (lanes + LT - 1) / LT;
+    DWBA_REQUIRE(blocks <= INT32_MAX, "dwba_il_semisep_lanes_tab_f64: too many jobs (%d)", n_jobs);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    switch (lb * 100 + nbp) {
+#define C(B, P) case B * 100 + P: hipLaunchKernelGGL((semisep_lane_tab_kernel<B, P>), dim3((unsigned)blocks), \
+                                                     dim3(LT), 0, s, d_jobs, d_tabs, n_jobs, L_max, d_out); break;
+        C(8, 1) C(8, 2) C(8, 4) C(8, 8) C(4, 1) C(4, 2) C(4, 4) C(4, 8) C(4, 16)
+#undef C
+        default: return set_error(DWBA_ERR_ARG, "dwba_il_semisep_lanes_tab_f64: L_max %d", L_max);
+    }
+    DWBA_LAUNCH_CHECK("semisep_lane_tab_kernel");
     return DWBA_OK;
 }
 

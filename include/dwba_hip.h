@@ -127,6 +127,27 @@ int32_t dwba_il_semisep_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const 
 int32_t dwba_il_semisep_lanes_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, int32_t L_max, int32_t lb,
                                   double* d_out, void* stream);
 
+/* Table-driven form of dwba_il_semisep_lanes_f64 (same jobs, output and FP64
+ * operations, so bit-identical results): the grid factors 1/r_i, q_i and
+ * q_i^(lb*b) are read from a per-grid table instead of being recomputed by
+ * every lane.  dwba_semisep_lane_tables_f64 writes, for each grid g with
+ * n points, tab[i*S + 0] = 1/r_i, tab[i*S + 1] = r_i * (1/r_{i+1}) (index
+ * clamped to n-1), tab[i*S + 2 + b] = (q_i^lb)^b for b < S - 2, where
+ * S = dwba_semisep_lane_table_stride(L_max, lb).  d_tabs[k] (device array of
+ * device pointers) is the table of job k's grid, covering >= n_cols points.
+ * Replaces the same reference code as dwba_il_semisep_f64. */
+typedef struct dwba_lane_grid {
+    const double* r;          /* n points, increasing */
+    double* tab;              /* n * stride doubles */
+    int32_t n;
+    int32_t pad_;
+} dwba_lane_grid;
+int32_t dwba_semisep_lane_table_stride(int32_t L_max, int32_t lb);
+int32_t dwba_semisep_lane_tables_f64(int32_t n_grids, const dwba_lane_grid* d_grids, int32_t max_n, int32_t L_max,
+                                     int32_t lb, void* stream);
+int32_t dwba_il_semisep_lanes_tab_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const double* const* d_tabs,
+                                      int32_t L_max, int32_t lb, double* d_out, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* Full-Split preparation: alpha / beta vectors of a channel pair.       */
 /* ------------------------------------------------------------------ */
@@ -326,6 +347,15 @@ int32_t dwba_ddot_f64(int32_t n_pairs, const dwba_ddot_pair* d_pairs, int32_t L_
  * Each t row is read once per item instead of once per pair.  L_max <= 31. */
 int32_t dwba_ddot_tiled_f64(int32_t n_items, const int32_t* d_items, const dwba_ddot_pair* d_pairs, int32_t L_max,
                             double* d_out, void* stream);
+
+/* The same dot products on the FP64 matrix cores (v_mfma_f64_16x16x4_f64):
+ * items of count <= 16 pairs sharing t and ld; as for the tiled form, alpha
+ * must be zero from n up to the largest n of its item (it is read there);
+ * nothing at or beyond that largest n is read.  L_max <= 31.  Replaces, as
+ * dwba_ddot_tiled_f64 does, the per-pair direct dot of the Filon direct
+ * branch (oscillatory_integrals.py:2571-2644 via dwba_matrix_elements.py:1136-1160). */
+int32_t dwba_ddot_mfma_f64(int32_t n_items, const int32_t* d_items, const dwba_ddot_pair* d_pairs, int32_t L_max,
+                           double* d_out, void* stream);
 
 /* Per-pair assembly of I_L in place in d_out (layout [pair][2][L_max+1]):
  * contractions zeroed when non-finite in the oscillatory branch, tail added

@@ -130,20 +130,28 @@ def test_autotuned_batch_matches_default(tmp_path, monkeypatch):
     assert ch2.cached and ch2.params == ch.params
 
 
-def test_direct_gemv_path_matches_full_recurrence():
-    """The shared-beta direct family (O(N) GEMV once per energy + tiled dot)
-    equals the per-pair recurrence to rounding."""
+@pytest.mark.parametrize("L_max", [12, 20, 31])
+def test_direct_gemv_path_matches_full_recurrence(L_max):
+    """The shared-beta direct family (O(N) GEMV once per energy + a dot per
+    pair, on the FP64 matrix cores or tiled on the VALUs) equals the per-pair
+    recurrence to rounding.  70 pairs per energy leave a partial 16-pair item;
+    L_max 12 uses one 16-multipole MFMA tile, 20 and 31 two."""
     from dwba_amd.batch import ScanBatch
     scan = _small_scan(6, n_e=2, n_p=70)
-    b = ScanBatch(scan, 12, precision="fp64", oscillatory_method="full_split")
+    b = ScanBatch(scan, L_max, precision="fp64", oscillatory_method="full_split")
     assert b.ss_gemv is not None
     b.run()
     r0 = b.results()
-    b.direct_gemv = True
-    b.run()
-    r1 = b.results()
     mag = b.magnitudes()
-    assert np.all(np.abs(r1 - r0) <= 1e-12 * np.abs(r0) + 1e-14 * mag)
+    for kind in ("mfma", "tiled"):
+        b.direct_gemv = True
+        b.dot_kind = kind
+        b.out.zero_()
+        b.run()
+        r1 = b.results()
+        excess = np.abs(r1 - r0) / (1e-12 * np.abs(r0) + 1e-14 * mag)
+        w = np.unravel_index(np.argmax(excess), excess.shape)
+        assert excess[w] <= 1.0, (kind, w, r1[w], r0[w], mag[w], int(np.sum(excess > 1)))
     # the lane-serial kernel on both job sets (exchange-only scans end at N1)
     b.semisep_lanes = True
     for gemv in (True, False):
@@ -151,3 +159,91 @@ def test_direct_gemv_path_matches_full_recurrence():
         b.run()
         r2 = b.results()
         assert np.all(np.abs(r2 - r0) <= 1e-12 * np.abs(r0) + 1e-14 * mag), gemv
+
+
+@pytest.mark.parametrize("mode", [None, "start", "after_prep"])
+def test_tail_overlap_modes(mode):
+    """The tail chain on a side stream (forked before or after prep, joined
+    before finalize) gives the single-stream result exactly, eagerly and as
+    replays of a captured graph with eager work in between."""
+    import torch
+    from dwba_amd.batch import ScanBatch
+    scan = _small_scan(7, n_e=3, n_p=25)
+    b = ScanBatch(scan, 10, precision="fp64", oscillatory_method="full_split")
+    assert b.n_tail > 0
+    b.overlap = None
+    b.launch()
+    ref = b.results().copy()
+    b.overlap = mode
+    b.out.zero_()
+    b.tails.zero_()
+    b.launch()
+    assert np.array_equal(b.results(), ref)
+    b.capture()
+    for rep in range(3):
+        b.out.zero_()
+        b.tails.zero_()
+        torch.cuda.synchronize()
+        b.run()
+        torch.cuda.synchronize()
+        assert np.array_equal(b.results(), ref), (mode, rep)
+        _ = (torch.ones(1000, device="cuda") * rep).sum().item()
+
+
+def test_lane_tables_bit_identical():
+    """The table-driven lane kernel reads 1/r, q and q^(lb b) computed once
+    per grid with the same operations the lane kernel uses: identical bits,
+    for both multipole block sizes and both job sets."""
+    from dwba_amd.batch import ScanBatch
+    scan = _small_scan(8, n_e=3, n_p=30)
+    b = ScanBatch(scan, 30, precision="fp64", oscillatory_method="full_split")
+    b.semisep_lanes = True
+    for lb in (4, 8):
+        b.semisep_config = (0, lb)
+        for gemv in (False, True):
+            b.direct_gemv = gemv and b.ss_gemv is not None
+            b.lane_tables = False
+            b.out.zero_()
+            b.run()
+            r0 = b.results()
+            b.lane_tables = True
+            b.out.zero_()
+            b.run()
+            r1 = b.results()
+            assert np.array_equal(r0, r1), (lb, gemv)
+
+
+@pytest.mark.parametrize("count,L_max,nmax", [(16, 12, 1000), (6, 12, 37), (16, 31, 515), (11, 20, 16), (16, 30, 3)])
+def test_ddot_mfma_against_numpy(count, L_max, nmax):
+    """dwba_ddot_mfma_f64 through the C ABI on one item of synthetic rows:
+    out[p][L] = sum_{i < n_p} alpha_p[i] t[L][i], alpha zero from n_p to nmax."""
+    import ctypes
+    import torch
+    from dwba_amd import _hip
+    from dwba_amd.batch import DdotPair
+    rng = np.random.default_rng(count * 1000 + L_max + nmax)
+    nl = L_max + 1
+    ld = ((nmax + 31) // 32) * 32
+    ns = rng.integers(0, nmax + 1, size=count)
+    ns[0] = nmax
+    A = np.zeros((count, ld))
+    for p in range(count):
+        A[p, :ns[p]] = rng.standard_normal(ns[p])
+    T = np.zeros((nl, ld))
+    T[:, :nmax] = rng.standard_normal((nl, nmax))
+    dev = torch.device("cuda", _hip.ensure_init(0))
+    A_d = torch.as_tensor(A, device=dev)
+    T_d = torch.as_tensor(T, device=dev)
+    out = torch.full((count, 2, nl), np.nan, dtype=torch.float64, device=dev)
+    pairs = (DdotPair * count)(*[DdotPair(_hip.ptr(A_d[p]), _hip.ptr(T_d), int(ns[p]), ld, p, 0)
+                                 for p in range(count)])
+    pairs_d = _hip.struct_array_to_device(pairs, dev)
+    items = torch.as_tensor(np.asarray([[0, count]], dtype=np.int32), device=dev)
+    lib = _hip.load_library()
+    _hip.check(lib.dwba_ddot_mfma_f64(1, _hip.ptr(items), _hip.ptr(pairs_d), L_max, _hip.ptr(out),
+                                      _hip.stream_handle()), "dwba_ddot_mfma_f64")
+    got = out.cpu().numpy()[:, 0, :]
+    ref = A[:, :nmax] @ T[:, :nmax].T
+    scale = np.abs(A[:, :nmax]) @ np.abs(T[:, :nmax]).T
+    bad = np.argwhere(~(np.abs(got - ref) <= 1e-14 * scale + 1e-300))
+    assert bad.size == 0, ("wrong (pair, L) cells", bad.tolist()[:40], len(bad))
