@@ -282,11 +282,13 @@ def scan_kernel_ms(b, steps, stream):
     sh = hip.stream_handle(stream)
     if b.semisep_lanes and b.lane_tables:
         lbv = lb or 8
-        gd, ng, max_n, ptrs, _ = b._lane_table_set(lbv)
+        gd, ng, max_n, _ = b._lane_table_set(lbv)
+        sj, ptrs, oidx = b._lane_job_set(lbv, b.direct_gemv)
         hip.check(lib.dwba_semisep_lane_tables_f64(ng, hip.ptr(gd), max_n, Lm, lbv, sh), "tables")
-        fn = lambda: hip.check(lib.dwba_il_semisep_lanes_tab_f64(b.plain_count, hip.ptr(jobs), hip.ptr(ptrs), Lm,
-                                                                 lbv, hip.ptr(b.out), sh), "lanes_tab")
-        name = f"semisep_lane_tab_kernel<{lbv},*>"
+        kfn = lib.dwba_il_semisep_lanes_split_f64 if b.lane_split else lib.dwba_il_semisep_lanes_tab_f64
+        fn = lambda: hip.check(kfn(b.plain_count, hip.ptr(sj), hip.ptr(ptrs), hip.ptr(oidx), Lm, lbv, hip.ptr(b.out),
+                                   sh), "lanes")
+        name = f"semisep_lane_{'split' if b.lane_split else 'tab'}_kernel<{lbv},*>"
     elif b.semisep_lanes:
         fn = lambda: hip.check(lib.dwba_il_semisep_lanes_f64(b.plain_count, hip.ptr(jobs), Lm, lb, hip.ptr(b.out),
                                                              sh), "lanes")
@@ -384,7 +386,7 @@ def bench_batched(args, ionization=False):
         sizes["alpha_beta_bytes"] += 2 * 8 * (plan.rows + plan.cols)
     ss_work = semisep_work_flop(b64s)
     dot_kind = getattr(b64s, "dot_kind", "tiled")
-    kernel_timing = scan_kernel_ms(b64s, max(args.steps, 5), stream)
+    kernel_timing = scan_kernel_ms(b64s, max(args.steps, 20), stream)
     log(f"scan kernel {kernel_timing[0]}: {kernel_timing[1]:.3f} ms")
     result = {
         "metric": METRIC, "value": n_int / (ms64s * 1e-3), "unit": "integrals/s", "n_gpus": 1,
@@ -404,11 +406,9 @@ def bench_batched(args, ionization=False):
         "stage_ms": st64s,
         "autotune": ({"params": tuned.params, "ms": tuned.ms, "cached": tuned.cached, "candidates": tuned.timings}
                      if tuned else None),
-        "tail_overlap": ({"mode": b64s.overlap, "candidates_ms": {str(m): t["ms"] for m, t in
-                                                                   zip((None, "start", "after_prep"),
-                                                                       b64s.tuning_overlap.timings)}
-                          if b64s.tuning_overlap.timings else None}
-                         if getattr(b64s, "tuning_overlap", None) else None),
+        "step_tuning": ({"tail_fork": str(b64s.overlap), "direct_gemv": bool(b64s.direct_gemv),
+                         "dot": b64s.dot_kind, **b64s.tuning_step}
+                        if getattr(b64s, "tuning_step", None) else None),
         "host_note": "host planning, uploads and autotuning precede the timed region",
     }
     del b64s

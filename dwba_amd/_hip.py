@@ -89,7 +89,8 @@ _SIGS = {
     "dwba_ddot_mfma_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_vp, c_vp]),
     "dwba_semisep_lane_table_stride": (c_i32, [c_i32, c_i32]),
     "dwba_semisep_lane_tables_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_i32, c_vp]),
-    "dwba_il_semisep_lanes_tab_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp]),
+    "dwba_il_semisep_lanes_tab_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp]),
+    "dwba_il_semisep_lanes_split_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp]),
     "dwba_batch_finalize": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
     "dwba_il_backward_f64": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
     "dwba_il_backward_f32": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
@@ -204,9 +205,13 @@ def il_plain_f64(r, alpha_dir, beta_dir, alpha_ex, beta_ex, n_rows: int, n_cols:
     return out
 
 
-def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None, lb: int = 0, lanes: bool = False):
+def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None, lb: int = 0, lanes: bool = False,
+                         tables: bool = False, split: bool = False):
     """dwba_il_semisep_f64 (or, with lanes, dwba_il_semisep_lanes_f64) on the
-    same job tuples as il_plain_f64_batch (n_rows <= n_cols).  Returns a
+    same job tuples as il_plain_f64_batch (n_rows <= n_cols).  With lanes and
+    tables: one grid table per job (dwba_semisep_lane_tables_f64) and the
+    table-driven kernel (split: the two-wavefront form), the jobs passed in
+    reverse order with d_out_index mapping them back.  Returns a
     (n_jobs, 2, L_max+1) float64 CUDA tensor."""
     ensure_init()
     lib = load_library()
@@ -229,6 +234,35 @@ def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None, lb: int = 
     if ws_bytes == 0:
         raise ValueError("il_semisep_f64_batch: invalid shapes (need 0 <= n_rows <= n_cols, 0 <= L_max <= 63)")
     out = torch.empty((n, 2, L_max + 1), dtype=torch.float64, device=dev)
+    if lanes and tables:
+        import numpy as np
+        from .batch import LaneGrid
+        lbv = lb or 8
+        stride = lib.dwba_semisep_lane_table_stride(L_max, lbv)
+        if stride <= 0:
+            raise ValueError(f"il_semisep_f64_batch: bad (L_max, lb) = ({L_max}, {lb})")
+        tabs = [torch.empty(max(int(nc[k]), 1) * stride, dtype=torch.float64, device=dev) for k in range(n)]
+        live = [k for k in range(n) if nc[k] > 0]
+        grids = (LaneGrid * max(len(live), 1))()
+        for g, k in enumerate(live):
+            grids[g] = LaneGrid(ptr(jobs[k][0]), ptr(tabs[k]), int(nc[k]), 0)
+        keep = [tabs]
+        if live:
+            d_grids = struct_array_to_device(grids, dev)
+            keep.append(d_grids)
+            check(lib.dwba_semisep_lane_tables_f64(len(live), ptr(d_grids), max(int(nc[k]) for k in live), L_max,
+                                                   lbv, stream_handle(stream)), "dwba_semisep_lane_tables_f64")
+        rev = (PlainJob * n)()
+        for p in range(n):
+            rev[p] = arr[n - 1 - p]
+        d_rev = struct_array_to_device(rev, dev)
+        d_tabs = torch.as_tensor(np.asarray([ptr(tabs[n - 1 - p]) for p in range(n)], dtype=np.int64), device=dev)
+        d_oidx = torch.as_tensor(np.arange(n - 1, -1, -1, dtype=np.int32), device=dev)
+        fn = lib.dwba_il_semisep_lanes_split_f64 if split else lib.dwba_il_semisep_lanes_tab_f64
+        check(fn(n, ptr(d_rev), ptr(d_tabs), ptr(d_oidx), L_max, lbv, ptr(out), stream_handle(stream)),
+              "dwba_il_semisep_lanes_split_f64" if split else "dwba_il_semisep_lanes_tab_f64")
+        out._dwba_keep = (d_jobs, d_rev, d_tabs, d_oidx, keep)
+        return out
     if lanes:
         check(lib.dwba_il_semisep_lanes_f64(n, ptr(d_jobs), L_max, lb, ptr(out), stream_handle(stream)),
               "dwba_il_semisep_lanes_f64")

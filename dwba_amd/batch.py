@@ -407,7 +407,9 @@ class ScanBatch:
         self.plain_d = hip.struct_array_to_device(arr, self.dev)
         self.plain_d_split = hip.struct_array_to_device(arr_split, self.dev) if self.ss_gemv is not None else None
         self._job_energy = [self.pairs[k][0] for k in order]     # grid of each job, in job order
-        self._lane_tabs = {}                                      # lb -> (grids, n, max_n, tab pointer array)
+        self._jobs_host = (arr, arr_split)                        # host copies, job order
+        self._lane_tabs = {}                                      # lb -> per-grid tables
+        self._lane_sets = {}                                      # (lb, split) -> jobs sorted longest first
         self.plain_n_split = (nra, nca_split)
         self.direct_gemv = False          # chosen by tune() when the shared-beta path exists
         self.dot_kind = "mfma"            # per-pair dots of that path: "mfma" (FP64 matrix cores) or "tiled"
@@ -416,6 +418,7 @@ class ScanBatch:
             self.semisep_config = (0, 0)      # (seg, lb): library defaults until tuned
             self.semisep_lanes = False        # lane-serial kernel instead (chosen by tune())
             self.lane_tables = True           # that kernel reads per-grid 1/r, q, q^(lb b) tables
+            self.lane_split = False           # and splits each range between two wavefronts (tune())
             self.plain_ws_bytes = self._semisep_ws_bytes(0)
         else:
             self.plain_ws_bytes = self.lib.dwba_il_plain_f64_batch_workspace_bytes(self.n_pairs, nra, nca,
@@ -569,13 +572,35 @@ class ScanBatch:
             t = torch.empty(st.N * stride, dtype=torch.float64, device=self.dev)
             tabs.append(t)
             grids[ei] = LaneGrid(hip.ptr(st.r_d), hip.ptr(t), st.N, 0)
-        ptrs = torch.as_tensor(np.asarray([hip.ptr(tabs[ei]) for ei in self._job_energy], dtype=np.int64),
-                               device=self.dev)
-        ent = (hip.struct_array_to_device(grids, self.dev), len(self.E), max(st.N for st in self.E), ptrs, tabs)
+        ent = (hip.struct_array_to_device(grids, self.dev), len(self.E), max(st.N for st in self.E), tabs)
         self._lane_tabs[lb] = ent
         return ent
 
-    def _contract_semisep(self, seg, lb, direct_gemv, out, ws, ws_bytes, sh, lanes=False):
+    def _lane_job_set(self, lb, split):
+        """Jobs of the table-driven lane kernel sorted by length, longest
+        first (wavefronts of equal-length pairs, long ones dispatched first),
+        with their table pointers and output rows."""
+        key = (lb, bool(split))
+        hit = self._lane_sets.get(key)
+        if hit is not None:
+            return hit
+        torch, hip = self.torch, self.hip
+        _, _, _, tabs = self._lane_table_set(lb)
+        src = self._jobs_host[1 if split else 0]
+        n = self.plain_count
+        perm = sorted(range(n), key=lambda g: -int(src[g].n_cols))
+        arr = (hip.PlainJob * n)()
+        for p, g in enumerate(perm):
+            arr[p] = src[g]
+        jobs_d = hip.struct_array_to_device(arr, self.dev)
+        ptrs = torch.as_tensor(np.asarray([hip.ptr(tabs[self._job_energy[g]]) for g in perm], dtype=np.int64),
+                               device=self.dev)
+        oidx = torch.as_tensor(np.asarray(perm, dtype=np.int32), device=self.dev)
+        ent = (jobs_d, ptrs, oidx)
+        self._lane_sets[key] = ent
+        return ent
+
+    def _contract_semisep(self, seg, lb, direct_gemv, out, ws, ws_bytes, sh, lanes=False, split=None):
         """All FP64 contractions of the batch through the semiseparable kernel;
         with direct_gemv the Filon/standard pairs of shared-beta energies take
         their direct family from t_L = K_L beta (once per energy) and a dot."""
@@ -584,11 +609,14 @@ class ScanBatch:
         jobs = self.plain_d_split if direct_gemv else self.plain_d
         if lanes and self.lane_tables:
             lbv = lb or 8
-            gd, ng, max_n, ptrs, _ = self._lane_table_set(lbv)
+            gd, ng, max_n, _ = self._lane_table_set(lbv)
+            sj, ptrs, oidx = self._lane_job_set(lbv, direct_gemv)
             hip.check(lib.dwba_semisep_lane_tables_f64(ng, hip.ptr(gd), max_n, Lm, lbv, sh),
                       "dwba_semisep_lane_tables_f64")
-            hip.check(lib.dwba_il_semisep_lanes_tab_f64(self.plain_count, hip.ptr(jobs), hip.ptr(ptrs), Lm, lbv,
-                                                        hip.ptr(out), sh), "dwba_il_semisep_lanes_tab_f64")
+            split = self.lane_split if split is None else split
+            fn = lib.dwba_il_semisep_lanes_split_f64 if split else lib.dwba_il_semisep_lanes_tab_f64
+            hip.check(fn(self.plain_count, hip.ptr(sj), hip.ptr(ptrs), hip.ptr(oidx), Lm, lbv, hip.ptr(out), sh),
+                      "dwba_il_semisep_lanes_split_f64" if split else "dwba_il_semisep_lanes_tab_f64")
         elif lanes:
             hip.check(lib.dwba_il_semisep_lanes_f64(self.plain_count, hip.ptr(jobs), Lm, lb, hip.ptr(out), sh),
                       "dwba_il_semisep_lanes_f64")
@@ -797,7 +825,7 @@ class ScanBatch:
         Lm = self.L_max
         has_gemv = self.ss_gemv is not None
         key = AT.signature("semisep_f64", AT.device_id(self.dev), jobs=AT.bucket(self.plain_count),
-                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm, gemv=int(has_gemv), v=3)
+                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm, gemv=int(has_gemv), v=4)
         self.launch()                       # alpha/beta in place for the contraction
         torch.cuda.synchronize()
         out = torch.empty_like(self.out)
@@ -810,7 +838,7 @@ class ScanBatch:
                 wss[seg] = (torch.empty(max(nb, 1), dtype=torch.uint8, device=self.dev), nb)
             ws, nb = wss[seg]
             self._contract_semisep(seg, int(c["lb"]), bool(c.get("gemv", 0)), out, ws, nb, hip.stream_handle(),
-                                   bool(c.get("lanes", 0)))
+                                   bool(c.get("lanes", 0)), bool(c.get("split", 0)))
 
         cands = AT.semisep_candidates()
         if has_gemv:
@@ -820,24 +848,52 @@ class ScanBatch:
         seg, lb = int(choice.params["seg"]), int(choice.params["lb"])
         self.semisep_config = (seg, lb)
         self.semisep_lanes = bool(choice.params.get("lanes", 0))
+        self.lane_split = bool(choice.params.get("split", 0))
         self.direct_gemv = bool(choice.params.get("gemv", 0)) and has_gemv
         self.plain_ws_bytes = self._semisep_ws_bytes(seg)
         self.plain_ws = torch.empty(max(self.plain_ws_bytes, 1), dtype=torch.uint8, device=self.dev)
         self.tuning = choice
-        if self.n_tail:
-            # where the tail chain forks off the launch stream (none / before prep / after prep)
-            modes = (None, "start", "after_prep")
-            okey = AT.signature("scan_overlap", AT.device_id(self.dev), jobs=AT.bucket(self.n_pairs),
-                                tails=AT.bucket(self.n_tail), n=AT.bucket(float(np.mean(list(nca)))), L=Lm,
-                                c=f"{seg}.{lb}.{int(self.semisep_lanes)}.{int(self.direct_gemv)}", v=1)
+        self.tuning_step = None
+        # Final choice on whole-step time: the fastest contraction with and
+        # without the shared-beta direct path, each with the tail chain
+        # forked before or after prep or not at all (the contraction alone
+        # does not show how much of it the overlapped tails hide).
+        modes = (None, "start", "after_prep") if self.n_tail else (None,)
+        finalists = [choice.params]
+        for want in ((0, 0), (0, 1), (1, 0), (1, 1)):      # (shared-beta path, split lanes)
+            same = [t for t in choice.timings
+                    if (int(t["params"].get("gemv", 0)), int(t["params"].get("split", 0))) == want]
+            if same:
+                best = min(same, key=lambda t: t["ms"])["params"]
+                if best not in finalists:
+                    finalists.append(best)
+        jkey = AT.signature("scan_step", AT.device_id(self.dev), jobs=AT.bucket(self.n_pairs),
+                            tails=AT.bucket(self.n_tail), n=AT.bucket(float(np.mean(list(nca)))), L=Lm,
+                            c="|".join(f"{f['seg']}.{f['lb']}.{int(f.get('lanes', 0))}.{int(f.get('split', 0))}."
+                                       f"{int(f.get('gemv', 0))}"
+                                       for f in finalists), v=2)
 
-            def run_overlap(c):
-                self.overlap = modes[int(c["overlap"])]
-                self.launch()
+        def apply(c):
+            f = finalists[int(c["c"])]
+            self.semisep_config = (int(f["seg"]), int(f["lb"]))
+            self.semisep_lanes = bool(f.get("lanes", 0))
+            self.lane_split = bool(f.get("split", 0))
+            self.direct_gemv = bool(f.get("gemv", 0)) and has_gemv
+            self.plain_ws_bytes = self._semisep_ws_bytes(int(f["seg"]))
+            if self.plain_ws.numel() < self.plain_ws_bytes:
+                self.plain_ws = torch.empty(max(self.plain_ws_bytes, 1), dtype=torch.uint8, device=self.dev)
+            self.overlap = modes[int(c["overlap"])]
 
-            oc = tuner.tune(okey, [{"overlap": i} for i in range(len(modes))], run_overlap, steps=steps)
-            self.overlap = modes[int(oc.params["overlap"])]
-            self.tuning_overlap = oc
+        def run_step(c):
+            apply(c)
+            self.launch()
+
+        cands = [{"c": i, "overlap": j} for i in range(len(finalists)) for j in range(len(modes))]
+        oc = tuner.tune(jkey, cands, run_step, steps=steps)
+        apply(oc.params)
+        self.tuning_overlap = oc
+        self.tuning_step = {"finalists": finalists, "modes": [str(m) for m in modes], "choice": oc.params,
+                            "ms": oc.ms, "timings": oc.timings}
         if self.graph is not None:
             self.graph = None
             self.capture()

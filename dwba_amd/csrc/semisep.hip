@@ -505,7 +505,8 @@ __device__ __forceinline__ void lane_point_bt(double iri, double q, double q0, d
 
 template <int LB, int NBP>
 __global__ __launch_bounds__(LT) void semisep_lane_tab_kernel(const dwba_plain_job* __restrict__ jobs,
-                                                              const double* const* __restrict__ tabs, int n_jobs,
+                                                              const double* const* __restrict__ tabs,
+                                                              const int* __restrict__ out_index, int n_jobs,
                                                               int L_max, double* __restrict__ out) {
     constexpr int LS = 2 + NBP;
     const long gl = (long)blockIdx.x * LT + threadIdx.x;
@@ -519,7 +520,8 @@ __global__ __launch_bounds__(LT) void semisep_lane_tab_kernel(const dwba_plain_j
     if (L0 >= NL) return;
     const dwba_plain_job J = jobs[k];
     if (fam == 0 && J.alpha_dir == nullptr) return;    // direct family computed elsewhere
-    double* __restrict__ o = out + (size_t)k * 2 * NL + fam * NL;
+    const int ko = out_index ? out_index[k] : k;
+    double* __restrict__ o = out + (size_t)ko * 2 * NL + fam * NL;
     double Ap[LB], Zp[LB], acc[LB];
     double diag = 0.0;
     const int n = J.n_cols, nr = min(J.n_rows, J.n_cols);
@@ -606,6 +608,212 @@ __global__ __launch_bounds__(LT) void semisep_lane_tab_kernel(const dwba_plain_j
         const int L = L0 + l;
         const double dL = L == 0 ? 1.0 : exp((double)L * ldl);
         if (L < NL) o[L] = nr > 0 ? fma(dL - 2.0, diag, acc[l]) : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Split-range form of the table-driven lane kernel: the points of a (pair,
+// family, block) are divided at m between two wavefronts of one workgroup.
+// The first runs [0, m) exactly as above.  The second starts at m from zero
+// carries and also accumulates, with P_i = prod_{m <= j < i} q_j^L,
+//     Ca = sum_{i >= m} alpha_i / r_i P_i,   Cb = sum_{i >= m} beta_i / r_i P_i,
+// which is what the true carries A_m, Z_m add to its part (the recurrences are
+// linear: A_i = A0_i + A_m P_i); after a barrier it reads the first wave's
+// A'_m = A_m + beta_m, Z'_m = Z_m + alpha_m, sum and diagonal from LDS and
+// finishes.  m balances the two waves' operation counts (5 per point and
+// multipole before m, 8 after, 5 in the beta-only range), so the longest
+// chain is ~0.6 of the single-lane one at 1.3x the total work.
+template <int LB, int LS, bool WC>
+__device__ __forceinline__ void split_point_a(const double* __restrict__ t, int blk, double a_i, double b_i,
+                                              double a_next, double b_next, double& diag, double* Ap, double* Zp,
+                                              double* acc, double* Ca, double* Cb, double* P) {
+    const double iri = t[0], q = t[1], q0 = t[2 + blk];
+    const double aw = a_i * iri, bw = b_i * iri;
+    diag = fma(a_i, bw, diag);
+    double qL = q0;
+#pragma unroll
+    for (int l = 0; l < LB; ++l) {
+        acc[l] = fma(aw, Ap[l], acc[l]);
+        acc[l] = fma(bw, Zp[l], acc[l]);
+        Ap[l] = fma(qL, Ap[l], b_next);
+        Zp[l] = fma(qL, Zp[l], a_next);
+        if (WC) {
+            Ca[l] = fma(aw, P[l], Ca[l]);
+            Cb[l] = fma(bw, P[l], Cb[l]);
+            P[l] *= qL;
+        }
+        qL *= q;
+    }
+}
+
+template <int LB, int LS>
+__device__ __forceinline__ void split_point_b(const double* __restrict__ t, int blk, double b_i, double* Zp,
+                                              double* acc, double* Cb, double* P) {
+    const double iri = t[0], q = t[1], q0 = t[2 + blk];
+    const double bw = b_i * iri;
+    double qL = q0;
+#pragma unroll
+    for (int l = 0; l < LB; ++l) {
+        acc[l] = fma(bw, Zp[l], acc[l]);
+        Zp[l] *= qL;
+        Cb[l] = fma(bw, P[l], Cb[l]);
+        P[l] *= qL;
+        qL *= q;
+    }
+}
+
+// alpha points [i0, i1), LGU at a time with the next group's values in flight
+template <int LB, int LS, bool WC>
+__device__ __forceinline__ void split_rows(const double* __restrict__ T, const double* __restrict__ al,
+                                           const double* __restrict__ be, int n, int nr, int i0, int i1, int blk,
+                                           double& diag, double* Ap, double* Zp, double* acc, double* Ca, double* Cb,
+                                           double* P) {
+    auto ld_a = [&](int i) { return i < nr ? al[i] : 0.0; };
+    auto ld_b = [&](int i) { return i < n ? be[i] : 0.0; };
+    int i = i0;
+    if (i + LGU <= i1) {
+        double ca[LGU + 1], cb[LGU + 1];
+#pragma unroll
+        for (int u = 0; u <= LGU; ++u) { ca[u] = ld_a(i + u); cb[u] = ld_b(i + u); }
+#pragma unroll 1
+        for (; i + LGU <= i1; i += LGU) {
+            const bool more = i + 2 * LGU <= i1;
+            double xa[LGU], xb[LGU];
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < LGU; ++u) { xa[u] = ld_a(i + LGU + 1 + u); xb[u] = ld_b(i + LGU + 1 + u); }
+            }
+#pragma unroll
+            for (int u = 0; u < LGU; ++u)
+                split_point_a<LB, LS, WC>(T + (size_t)(i + u) * LS, blk, ca[u], cb[u], ca[u + 1], cb[u + 1], diag,
+                                          Ap, Zp, acc, Ca, Cb, P);
+            if (more) {
+                ca[0] = ca[LGU]; cb[0] = cb[LGU];
+#pragma unroll
+                for (int u = 0; u < LGU; ++u) { ca[u + 1] = xa[u]; cb[u + 1] = xb[u]; }
+            }
+        }
+    }
+#pragma unroll 1
+    for (; i < i1; ++i)
+        split_point_a<LB, LS, WC>(T + (size_t)i * LS, blk, al[i], be[i], ld_a(i + 1), ld_b(i + 1), diag, Ap, Zp,
+                                  acc, Ca, Cb, P);
+}
+
+// beta-only points [i0, n) (second wave only)
+template <int LB, int LS>
+__device__ __forceinline__ void split_tail(const double* __restrict__ T, const double* __restrict__ be, int n, int i0,
+                                           int blk, double* Zp, double* acc, double* Cb, double* P) {
+    int i = i0;
+    if (i + LGU <= n) {
+        double cb[LGU];
+#pragma unroll
+        for (int u = 0; u < LGU; ++u) cb[u] = be[i + u];
+#pragma unroll 1
+        for (; i + LGU <= n; i += LGU) {
+            const bool more = i + 2 * LGU <= n;
+            double xb[LGU];
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < LGU; ++u) xb[u] = be[i + LGU + u];
+            }
+#pragma unroll
+            for (int u = 0; u < LGU; ++u) split_point_b<LB, LS>(T + (size_t)(i + u) * LS, blk, cb[u], Zp, acc, Cb, P);
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < LGU; ++u) cb[u] = xb[u];
+            }
+        }
+    }
+#pragma unroll 1
+    for (; i < n; ++i) split_point_b<LB, LS>(T + (size_t)i * LS, blk, be[i], Zp, acc, Cb, P);
+}
+
+// Split point: equal operation counts for the two waves (see above).
+template <int LB>
+__device__ __forceinline__ int split_at(int nr, int n) {
+    const long c0 = 5 * LB + 8, c1 = 8 * LB + 8, cb = 5 * LB + 8;
+    const long m = (c1 * (long)nr + cb * (long)(n - nr)) / (c0 + c1);
+    return (int)min((long)nr, max(0L, m));
+}
+
+template <int LB, int NBP>
+__global__ __launch_bounds__(256) void semisep_lane_split_kernel(const dwba_plain_job* __restrict__ jobs,
+                                                                 const double* const* __restrict__ tabs,
+                                                                 const int* __restrict__ out_index, int n_jobs,
+                                                                 int L_max, double* __restrict__ out) {
+    constexpr int LS = 2 + NBP;
+    constexpr int NX = 3 * LB + 1;                 // A', Z', sum per multipole + diagonal
+    __shared__ double s_x[2][NX][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int grp = w >> 1, h = w & 1;             // wave pair grp, part h of the range
+    const long gt = (long)blockIdx.x * 128 + grp * 64 + lane;
+    const long stream = gt / NBP;
+    const int blk = (int)(gt - stream * NBP);
+    const int NL = L_max + 1;
+    const int L0 = blk * LB;
+    bool active = stream < 2L * n_jobs && L0 < NL;
+    int fam = 0, k = 0;
+    if (active) {
+        fam = stream >= n_jobs ? 1 : 0;
+        k = (int)(fam ? stream - n_jobs : stream);
+        if (fam == 0 && jobs[k].alpha_dir == nullptr) active = false;   // direct family computed elsewhere
+    }
+    double Ap[LB], Zp[LB], acc[LB], Ca[LB], Cb[LB], P[LB];
+    double diag = 0.0;
+#pragma unroll
+    for (int l = 0; l < LB; ++l) { Ap[l] = 0.0; Zp[l] = 0.0; acc[l] = 0.0; Ca[l] = 0.0; Cb[l] = 0.0; P[l] = 1.0; }
+    int n = 0, nr = 0, m = 0;
+    const double* __restrict__ al = nullptr;
+    const double* __restrict__ be = nullptr;
+    if (active) {
+        const dwba_plain_job J = jobs[k];
+        n = J.n_cols;
+        nr = min(J.n_rows, J.n_cols);
+        m = split_at<LB>(nr, n);
+        al = fam ? J.alpha_ex : J.alpha_dir;
+        be = fam ? J.beta_ex : J.beta_dir;
+        if (nr > 0 && n > 0) {
+            const double* __restrict__ T = tabs[k];
+            if (h == 0) {
+                const double a0 = al[0], b0 = be[0];
+#pragma unroll
+                for (int l = 0; l < LB; ++l) { Ap[l] = b0; Zp[l] = a0; }     // A_0 = Z_0 = 0
+                split_rows<LB, LS, false>(T, al, be, n, nr, 0, m, blk, diag, Ap, Zp, acc, Ca, Cb, P);
+            } else {
+                const double am = m < nr ? al[m] : 0.0, bm = m < n ? be[m] : 0.0;
+#pragma unroll
+                for (int l = 0; l < LB; ++l) { Ap[l] = bm; Zp[l] = am; }     // zero carries at m
+                split_rows<LB, LS, true>(T, al, be, n, nr, m, nr, blk, diag, Ap, Zp, acc, Ca, Cb, P);
+                split_tail<LB, LS>(T, be, n, nr, blk, Zp, acc, Cb, P);
+            }
+        }
+    }
+    if (h == 0) {
+#pragma unroll
+        for (int l = 0; l < LB; ++l) {
+            s_x[grp][l][lane] = Ap[l];
+            s_x[grp][LB + l][lane] = Zp[l];
+            s_x[grp][2 * LB + l][lane] = acc[l];
+        }
+        s_x[grp][3 * LB][lane] = diag;
+    }
+    __syncthreads();
+    if (h == 1 && active) {
+        const int ko = out_index ? out_index[k] : k;
+        double* __restrict__ o = out + (size_t)ko * 2 * NL + fam * NL;
+        const double am = (nr > 0 && m < nr) ? al[m] : 0.0, bm = (nr > 0 && m < n) ? be[m] : 0.0;
+        const double dg = diag + s_x[grp][3 * LB][lane];
+        const double ldl = log(1.0 - 1e-12);
+#pragma unroll
+        for (int l = 0; l < LB; ++l) {
+            const double A_m = s_x[grp][l][lane] - bm;
+            const double Z_m = s_x[grp][LB + l][lane] - am;
+            const double tot = fma(A_m, Ca[l], fma(Z_m, Cb[l], acc[l] + s_x[grp][2 * LB + l][lane]));
+            const int L = L0 + l;
+            const double dL = L == 0 ? 1.0 : exp((double)L * ldl);
+            if (L < NL) o[L] = nr > 0 ? fma(dL - 2.0, dg, tot) : 0.0;
+        }
     }
 }
 
@@ -842,7 +1050,8 @@ int32_t dwba_semisep_lane_tables_f64(int32_t n_grids, const dwba_lane_grid* d_gr
 }
 
 int32_t dwba_il_semisep_lanes_tab_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const double* const* d_tabs,
-                                      int32_t L_max, int32_t lb, double* d_out, void* stream) {
+                                      const int32_t* d_out_index, int32_t L_max, int32_t lb, double* d_out,
+                                      void* stream) {
     DWBA_REQUIRE(initialised(), "dwba_il_semisep_lanes_tab_f64: call dwba_init first");
     DWBA_REQUIRE(n_jobs > 0 && d_jobs && d_tabs && d_out && L_max >= 0 && L_max <= 63,
                  "dwba_il_semisep_lanes_tab_f64: bad arguments (n_jobs=%d, L_max=%d)", n_jobs, L_max);
@@ -855,12 +1064,38 @@ int32_t dwba_il_semisep_lanes_tab_f64(int32_t n_jobs, const dwba_plain_job* d_jo
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     switch (lb * 100 + nbp) {
 #define C(B, P) case B * 100 + P: hipLaunchKernelGGL((semisep_lane_tab_kernel<B, P>), dim3((unsigned)blocks), \
-                                                     dim3(LT), 0, s, d_jobs, d_tabs, n_jobs, L_max, d_out); break;
+                                                     dim3(LT), 0, s, d_jobs, d_tabs, d_out_index, n_jobs, L_max, \
+                                                     d_out); break;
         C(8, 1) C(8, 2) C(8, 4) C(8, 8) C(4, 1) C(4, 2) C(4, 4) C(4, 8) C(4, 16)
 #undef C
         default: return set_error(DWBA_ERR_ARG, "dwba_il_semisep_lanes_tab_f64: L_max %d", L_max);
     }
     DWBA_LAUNCH_CHECK("semisep_lane_tab_kernel");
+    return DWBA_OK;
+}
+
+int32_t dwba_il_semisep_lanes_split_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const double* const* d_tabs,
+                                        const int32_t* d_out_index, int32_t L_max, int32_t lb, double* d_out,
+                                        void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_il_semisep_lanes_split_f64: call dwba_init first");
+    DWBA_REQUIRE(n_jobs > 0 && d_jobs && d_tabs && d_out && L_max >= 0 && L_max <= 63,
+                 "dwba_il_semisep_lanes_split_f64: bad arguments (n_jobs=%d, L_max=%d)", n_jobs, L_max);
+    if (lb == 0) lb = 8;
+    DWBA_REQUIRE(lb == 4 || lb == 8, "dwba_il_semisep_lanes_split_f64: lb must be 0, 4 or 8 (got %d)", lb);
+    const int nbp = lane_nbp(L_max, lb);
+    const long tasks = 2L * n_jobs * nbp;            // (job, family, block); two wavefronts per 64 of them
+    const long blocks = (tasks + 127) / 128;
+    DWBA_REQUIRE(blocks <= INT32_MAX, "dwba_il_semisep_lanes_split_f64: too many jobs (%d)", n_jobs);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    switch (lb * 100 + nbp) {
+#define C(B, P) case B * 100 + P: hipLaunchKernelGGL((semisep_lane_split_kernel<B, P>), dim3((unsigned)blocks), \
+                                                     dim3(256), 0, s, d_jobs, d_tabs, d_out_index, n_jobs, L_max, \
+                                                     d_out); break;
+        C(8, 1) C(8, 2) C(8, 4) C(8, 8) C(4, 1) C(4, 2) C(4, 4) C(4, 8) C(4, 16)
+#undef C
+        default: return set_error(DWBA_ERR_ARG, "dwba_il_semisep_lanes_split_f64: L_max %d", L_max);
+    }
+    DWBA_LAUNCH_CHECK("semisep_lane_split_kernel");
     return DWBA_OK;
 }
 

@@ -193,10 +193,13 @@ def test_tail_overlap_modes(mode):
 def test_lane_tables_bit_identical():
     """The table-driven lane kernel reads 1/r, q and q^(lb b) computed once
     per grid with the same operations the lane kernel uses: identical bits,
-    for both multipole block sizes and both job sets."""
+    for both multipole block sizes and both job sets.  Its split-range form
+    (two wavefronts per range, joined through LDS) agrees to rounding."""
     from dwba_amd.batch import ScanBatch
     scan = _small_scan(8, n_e=3, n_p=30)
     b = ScanBatch(scan, 30, precision="fp64", oscillatory_method="full_split")
+    b.run()
+    mag = b.magnitudes()
     b.semisep_lanes = True
     for lb in (4, 8):
         b.semisep_config = (0, lb)
@@ -207,10 +210,18 @@ def test_lane_tables_bit_identical():
             b.run()
             r0 = b.results()
             b.lane_tables = True
+            b.lane_split = False
             b.out.zero_()
             b.run()
             r1 = b.results()
             assert np.array_equal(r0, r1), (lb, gemv)
+            # the range split between two wavefronts: same sums, other order
+            b.lane_split = True
+            b.out.zero_()
+            b.run()
+            r2 = b.results()
+            assert np.all(np.abs(r2 - r0) <= 1e-12 * np.abs(r0) + 1e-14 * mag), (lb, gemv)
+            b.lane_split = False
 
 
 @pytest.mark.parametrize("count,L_max,nmax", [(16, 12, 1000), (6, 12, 37), (16, 31, 515), (11, 20, 16), (16, 30, 3)])
