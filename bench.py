@@ -483,12 +483,13 @@ def bench_batched(args, ionization=False):
     return result
 
 
-def cpu_baseline_batched(scan, L_max, n_pairs=2, sample_L=(0, 1)):
+def cpu_baseline_batched(scan, L_max, max_pairs=8, sample_L=(0, 1, 2, 3, 4), budget_s=10.0):
     """Oracle's faithful restatement of radial_ME_all_L (materialises N x N
     K_L, Filon / filon_exchange node interpolations with N1 x n_cc gathers,
-    per-L outer tails with math.fsum) on a sample: the smallest-N energy,
-    n_pairs pairs, L in sample_L; the per-(pair, L) cost is extrapolated to
-    the full batch (sum over energies scaled by (N_e / N_sample)^2)."""
+    per-L outer tails with math.fsum) on a sample: pairs of the smallest-N
+    energy, one call each for L in sample_L, until about budget_s seconds of
+    CPU work (at least one pair, at most max_pairs); the per-(pair, L) cost is
+    extrapolated to the full batch (sum over energies scaled by (N_e / N)^2)."""
     from oracle import radial
     model, ncpu, blas = cpu_info()
     e = min(range(len(scan)), key=lambda i: len(scan[i].grid.r))
@@ -496,16 +497,20 @@ def cpu_baseline_batched(scan, L_max, n_pairs=2, sample_L=(0, 1)):
     g = en.grid
     Lm = max(sample_L)
     t0 = time.perf_counter()
-    for pr in en.pairs[:n_pairs]:
+    done = 0
+    for pr in en.pairs[:max_pairs]:
         radial.radial_ME_all_L_faithful(g.r, g.w_simpson, g.w_trapz, en.V_core, en.U_i, en.bound_i, en.bound_f,
                                         pr[0], pr[1], Lm, oscillatory_method="full_split", U_f_array=en.U_f)
+        done += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
     t = time.perf_counter() - t0
     n_s = len(g.r)
-    per_pair_L = t / (n_pairs * len(sample_L))
+    per_pair_L = t / (done * len(sample_L))
     total = sum(len(x.pairs) * (L_max + 1) * per_pair_L * (len(x.grid.r) / n_s) ** 2 for x in scan)
     n_int = sum(len(x.pairs) for x in scan) * (L_max + 1) * 2
     return {"value": n_int / total, "unit": "integrals/s", "cores": int(blas or 1), "kind": "port",
-            "sample": f"faithful radial_ME_all_L (full_split) restatement, {n_pairs} pairs x L={list(sample_L)} "
+            "sample": f"faithful radial_ME_all_L (full_split) restatement, {done} pairs x L={list(sample_L)} "
                       f"at N={n_s} ({t:.1f} s), extrapolated to the whole batch by (N_e/N)^2 and pair/L counts; "
                       f"BLAS threads={blas}; host {model}, os.cpu_count={ncpu}"}
 
@@ -513,7 +518,7 @@ def cpu_baseline_batched(scan, L_max, n_pairs=2, sample_L=(0, 1)):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=15000)
     ap.add_argument("--seed", type=int, default=0)

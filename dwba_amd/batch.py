@@ -860,9 +860,12 @@ class ScanBatch:
         # does not show how much of it the overlapped tails hide).
         modes = (None, "start", "after_prep") if self.n_tail else (None,)
         finalists = [choice.params]
-        for want in ((0, 0), (0, 1), (1, 0), (1, 1)):      # (shared-beta path, split lanes)
+        kinds = {(int(t["params"].get("gemv", 0)), int(t["params"].get("lanes", 0)), int(t["params"].get("split", 0)))
+                 for t in choice.timings}
+        for want in sorted(kinds):                         # (shared-beta path, lane kernel, split lanes)
             same = [t for t in choice.timings
-                    if (int(t["params"].get("gemv", 0)), int(t["params"].get("split", 0))) == want]
+                    if (int(t["params"].get("gemv", 0)), int(t["params"].get("lanes", 0)),
+                        int(t["params"].get("split", 0))) == want]
             if same:
                 best = min(same, key=lambda t: t["ms"])["params"]
                 if best not in finalists:
@@ -871,7 +874,7 @@ class ScanBatch:
                             tails=AT.bucket(self.n_tail), n=AT.bucket(float(np.mean(list(nca)))), L=Lm,
                             c="|".join(f"{f['seg']}.{f['lb']}.{int(f.get('lanes', 0))}.{int(f.get('split', 0))}."
                                        f"{int(f.get('gemv', 0))}"
-                                       for f in finalists), v=2)
+                                       for f in finalists), v=3)
 
         def apply(c):
             f = finalists[int(c["c"])]
