@@ -285,10 +285,17 @@ def scan_kernel_ms(b, steps, stream):
         gd, ng, max_n, _ = b._lane_table_set(lbv)
         sj, ptrs, oidx = b._lane_job_set(lbv, b.direct_gemv)
         hip.check(lib.dwba_semisep_lane_tables_f64(ng, hip.ptr(gd), max_n, Lm, lbv, sh), "tables")
-        kfn = lib.dwba_il_semisep_lanes_split_f64 if b.lane_split else lib.dwba_il_semisep_lanes_tab_f64
-        fn = lambda: hip.check(kfn(b.plain_count, hip.ptr(sj), hip.ptr(ptrs), hip.ptr(oidx), Lm, lbv, hip.ptr(b.out),
-                                   sh), "lanes")
-        name = f"semisep_lane_{'split' if b.lane_split else 'tab'}_kernel<{lbv},*>"
+        if b.lane_split:
+            occ = 3 if int(b.lane_split) == 3 else 2
+            fn = lambda: hip.check(lib.dwba_il_semisep_lanes_split_f64(b.plain_count, hip.ptr(sj), hip.ptr(ptrs),
+                                                                       hip.ptr(oidx), Lm, lbv, occ, hip.ptr(b.out),
+                                                                       sh), "lanes_split")
+            name = f"semisep_lane_split_kernel<{lbv},*,{occ}>"
+        else:
+            fn = lambda: hip.check(lib.dwba_il_semisep_lanes_tab_f64(b.plain_count, hip.ptr(sj), hip.ptr(ptrs),
+                                                                     hip.ptr(oidx), Lm, lbv, hip.ptr(b.out), sh),
+                                   "lanes_tab")
+            name = f"semisep_lane_tab_kernel<{lbv},*>"
     elif b.semisep_lanes:
         fn = lambda: hip.check(lib.dwba_il_semisep_lanes_f64(b.plain_count, hip.ptr(jobs), Lm, lb, hip.ptr(b.out),
                                                              sh), "lanes")

@@ -90,7 +90,7 @@ _SIGS = {
     "dwba_semisep_lane_table_stride": (c_i32, [c_i32, c_i32]),
     "dwba_semisep_lane_tables_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_i32, c_vp]),
     "dwba_il_semisep_lanes_tab_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp]),
-    "dwba_il_semisep_lanes_split_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp]),
+    "dwba_il_semisep_lanes_split_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
     "dwba_batch_finalize": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
     "dwba_il_backward_f64": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
     "dwba_il_backward_f32": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
@@ -206,7 +206,7 @@ def il_plain_f64(r, alpha_dir, beta_dir, alpha_ex, beta_ex, n_rows: int, n_cols:
 
 
 def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None, lb: int = 0, lanes: bool = False,
-                         tables: bool = False, split: bool = False):
+                         tables: bool = False, split: bool = False, waves_per_simd: int = 0):
     """dwba_il_semisep_f64 (or, with lanes, dwba_il_semisep_lanes_f64) on the
     same job tuples as il_plain_f64_batch (n_rows <= n_cols).  With lanes and
     tables: one grid table per job (dwba_semisep_lane_tables_f64) and the
@@ -258,9 +258,13 @@ def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None, lb: int = 
         d_rev = struct_array_to_device(rev, dev)
         d_tabs = torch.as_tensor(np.asarray([ptr(tabs[n - 1 - p]) for p in range(n)], dtype=np.int64), device=dev)
         d_oidx = torch.as_tensor(np.arange(n - 1, -1, -1, dtype=np.int32), device=dev)
-        fn = lib.dwba_il_semisep_lanes_split_f64 if split else lib.dwba_il_semisep_lanes_tab_f64
-        check(fn(n, ptr(d_rev), ptr(d_tabs), ptr(d_oidx), L_max, lbv, ptr(out), stream_handle(stream)),
-              "dwba_il_semisep_lanes_split_f64" if split else "dwba_il_semisep_lanes_tab_f64")
+        if split:
+            check(lib.dwba_il_semisep_lanes_split_f64(n, ptr(d_rev), ptr(d_tabs), ptr(d_oidx), L_max, lbv,
+                                                      waves_per_simd, ptr(out), stream_handle(stream)),
+                  "dwba_il_semisep_lanes_split_f64")
+        else:
+            check(lib.dwba_il_semisep_lanes_tab_f64(n, ptr(d_rev), ptr(d_tabs), ptr(d_oidx), L_max, lbv, ptr(out),
+                                                    stream_handle(stream)), "dwba_il_semisep_lanes_tab_f64")
         out._dwba_keep = (d_jobs, d_rev, d_tabs, d_oidx, keep)
         return out
     if lanes:

@@ -161,10 +161,11 @@ SEMISEP_LBS = (4, 8)
 def semisep_candidates() -> List[Dict[str, int]]:
     """(seg, lb) of the wavefront-scan kernel, plus the lane-serial kernel
     (one lane per job, family and block of lb multipoles; seg unused), whole
-    range per lane or split between two wavefronts."""
+    range per lane or split between two wavefronts (split = the occupancy,
+    2 or 3 waves per SIMD, that kernel is compiled for)."""
     return [{"seg": s, "lb": b} for b in SEMISEP_LBS for s in SEMISEP_SEGS] + [
         {"seg": 0, "lb": b, "lanes": 1} for b in SEMISEP_LBS] + [
-        {"seg": 0, "lb": b, "lanes": 1, "split": 1} for b in SEMISEP_LBS]
+        {"seg": 0, "lb": b, "lanes": 1, "split": w} for b in SEMISEP_LBS for w in (2, 3)]
 
 
 def device_id(dev) -> str:

@@ -418,7 +418,8 @@ class ScanBatch:
             self.semisep_config = (0, 0)      # (seg, lb): library defaults until tuned
             self.semisep_lanes = False        # lane-serial kernel instead (chosen by tune())
             self.lane_tables = True           # that kernel reads per-grid 1/r, q, q^(lb b) tables
-            self.lane_split = False           # and splits each range between two wavefronts (tune())
+            self.lane_split = 0               # and splits each range between two wavefronts (tune()): 0 no,
+                                              # 2 or 3 the occupancy (waves per SIMD) the split kernel targets
             self.plain_ws_bytes = self._semisep_ws_bytes(0)
         else:
             self.plain_ws_bytes = self.lib.dwba_il_plain_f64_batch_workspace_bytes(self.n_pairs, nra, nca,
@@ -613,10 +614,15 @@ class ScanBatch:
             sj, ptrs, oidx = self._lane_job_set(lbv, direct_gemv)
             hip.check(lib.dwba_semisep_lane_tables_f64(ng, hip.ptr(gd), max_n, Lm, lbv, sh),
                       "dwba_semisep_lane_tables_f64")
-            split = self.lane_split if split is None else split
-            fn = lib.dwba_il_semisep_lanes_split_f64 if split else lib.dwba_il_semisep_lanes_tab_f64
-            hip.check(fn(self.plain_count, hip.ptr(sj), hip.ptr(ptrs), hip.ptr(oidx), Lm, lbv, hip.ptr(out), sh),
-                      "dwba_il_semisep_lanes_split_f64" if split else "dwba_il_semisep_lanes_tab_f64")
+            split = self.lane_split if split is None else split     # 0: whole ranges, 2 or 3: waves per SIMD
+            if split:
+                hip.check(lib.dwba_il_semisep_lanes_split_f64(self.plain_count, hip.ptr(sj), hip.ptr(ptrs),
+                                                              hip.ptr(oidx), Lm, lbv, 3 if int(split) == 3 else 2,
+                                                              hip.ptr(out), sh), "dwba_il_semisep_lanes_split_f64")
+            else:
+                hip.check(lib.dwba_il_semisep_lanes_tab_f64(self.plain_count, hip.ptr(sj), hip.ptr(ptrs),
+                                                            hip.ptr(oidx), Lm, lbv, hip.ptr(out), sh),
+                          "dwba_il_semisep_lanes_tab_f64")
         elif lanes:
             hip.check(lib.dwba_il_semisep_lanes_f64(self.plain_count, hip.ptr(jobs), Lm, lb, hip.ptr(out), sh),
                       "dwba_il_semisep_lanes_f64")
@@ -825,7 +831,7 @@ class ScanBatch:
         Lm = self.L_max
         has_gemv = self.ss_gemv is not None
         key = AT.signature("semisep_f64", AT.device_id(self.dev), jobs=AT.bucket(self.plain_count),
-                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm, gemv=int(has_gemv), v=4)
+                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm, gemv=int(has_gemv), v=5)
         self.launch()                       # alpha/beta in place for the contraction
         torch.cuda.synchronize()
         out = torch.empty_like(self.out)
@@ -838,7 +844,7 @@ class ScanBatch:
                 wss[seg] = (torch.empty(max(nb, 1), dtype=torch.uint8, device=self.dev), nb)
             ws, nb = wss[seg]
             self._contract_semisep(seg, int(c["lb"]), bool(c.get("gemv", 0)), out, ws, nb, hip.stream_handle(),
-                                   bool(c.get("lanes", 0)), bool(c.get("split", 0)))
+                                   bool(c.get("lanes", 0)), int(c.get("split", 0)))
 
         cands = AT.semisep_candidates()
         if has_gemv:
@@ -848,7 +854,7 @@ class ScanBatch:
         seg, lb = int(choice.params["seg"]), int(choice.params["lb"])
         self.semisep_config = (seg, lb)
         self.semisep_lanes = bool(choice.params.get("lanes", 0))
-        self.lane_split = bool(choice.params.get("split", 0))
+        self.lane_split = int(choice.params.get("split", 0))
         self.direct_gemv = bool(choice.params.get("gemv", 0)) and has_gemv
         self.plain_ws_bytes = self._semisep_ws_bytes(seg)
         self.plain_ws = torch.empty(max(self.plain_ws_bytes, 1), dtype=torch.uint8, device=self.dev)
@@ -880,7 +886,7 @@ class ScanBatch:
             f = finalists[int(c["c"])]
             self.semisep_config = (int(f["seg"]), int(f["lb"]))
             self.semisep_lanes = bool(f.get("lanes", 0))
-            self.lane_split = bool(f.get("split", 0))
+            self.lane_split = int(f.get("split", 0))
             self.direct_gemv = bool(f.get("gemv", 0)) and has_gemv
             self.plain_ws_bytes = self._semisep_ws_bytes(int(f["seg"]))
             if self.plain_ws.numel() < self.plain_ws_bytes:

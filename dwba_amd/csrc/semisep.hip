@@ -623,11 +623,10 @@ __global__ __launch_bounds__(LT) void semisep_lane_tab_kernel(const dwba_plain_j
 // finishes.  m balances the two waves' operation counts (5 per point and
 // multipole before m, 8 after, 5 in the beta-only range), so the longest
 // chain is ~0.6 of the single-lane one at 1.3x the total work.
-template <int LB, int LS, bool WC>
-__device__ __forceinline__ void split_point_a(const double* __restrict__ t, int blk, double a_i, double b_i,
+template <int LB, bool WC>
+__device__ __forceinline__ void split_point_a(double iri, double q, double q0, double a_i, double b_i,
                                               double a_next, double b_next, double& diag, double* Ap, double* Zp,
                                               double* acc, double* Ca, double* Cb, double* P) {
-    const double iri = t[0], q = t[1], q0 = t[2 + blk];
     const double aw = a_i * iri, bw = b_i * iri;
     diag = fma(a_i, bw, diag);
     double qL = q0;
@@ -646,10 +645,9 @@ __device__ __forceinline__ void split_point_a(const double* __restrict__ t, int 
     }
 }
 
-template <int LB, int LS>
-__device__ __forceinline__ void split_point_b(const double* __restrict__ t, int blk, double b_i, double* Zp,
-                                              double* acc, double* Cb, double* P) {
-    const double iri = t[0], q = t[1], q0 = t[2 + blk];
+template <int LB>
+__device__ __forceinline__ void split_point_b(double iri, double q, double q0, double b_i, double* Zp, double* acc,
+                                              double* Cb, double* P) {
     const double bw = b_i * iri;
     double qL = q0;
 #pragma unroll
@@ -662,7 +660,9 @@ __device__ __forceinline__ void split_point_b(const double* __restrict__ t, int 
     }
 }
 
-// alpha points [i0, i1), LGU at a time with the next group's values in flight
+// alpha points [i0, i1), LGU at a time with the next group's alpha / beta in
+// flight (the table rows are shared by the wavefront's pairs of one grid and
+// read at use: prefetching them too spills registers)
 template <int LB, int LS, bool WC>
 __device__ __forceinline__ void split_rows(const double* __restrict__ T, const double* __restrict__ al,
                                            const double* __restrict__ be, int n, int nr, int i0, int i1, int blk,
@@ -670,6 +670,10 @@ __device__ __forceinline__ void split_rows(const double* __restrict__ T, const d
                                            double* P) {
     auto ld_a = [&](int i) { return i < nr ? al[i] : 0.0; };
     auto ld_b = [&](int i) { return i < n ? be[i] : 0.0; };
+    auto pt = [&](int i, double a_i, double b_i, double a_n, double b_n) {
+        const double* t = T + (size_t)i * LS;
+        split_point_a<LB, WC>(t[0], t[1], t[2 + blk], a_i, b_i, a_n, b_n, diag, Ap, Zp, acc, Ca, Cb, P);
+    };
     int i = i0;
     if (i + LGU <= i1) {
         double ca[LGU + 1], cb[LGU + 1];
@@ -684,9 +688,7 @@ __device__ __forceinline__ void split_rows(const double* __restrict__ T, const d
                 for (int u = 0; u < LGU; ++u) { xa[u] = ld_a(i + LGU + 1 + u); xb[u] = ld_b(i + LGU + 1 + u); }
             }
 #pragma unroll
-            for (int u = 0; u < LGU; ++u)
-                split_point_a<LB, LS, WC>(T + (size_t)(i + u) * LS, blk, ca[u], cb[u], ca[u + 1], cb[u + 1], diag,
-                                          Ap, Zp, acc, Ca, Cb, P);
+            for (int u = 0; u < LGU; ++u) pt(i + u, ca[u], cb[u], ca[u + 1], cb[u + 1]);
             if (more) {
                 ca[0] = ca[LGU]; cb[0] = cb[LGU];
 #pragma unroll
@@ -695,15 +697,17 @@ __device__ __forceinline__ void split_rows(const double* __restrict__ T, const d
         }
     }
 #pragma unroll 1
-    for (; i < i1; ++i)
-        split_point_a<LB, LS, WC>(T + (size_t)i * LS, blk, al[i], be[i], ld_a(i + 1), ld_b(i + 1), diag, Ap, Zp,
-                                  acc, Ca, Cb, P);
+    for (; i < i1; ++i) pt(i, al[i], be[i], ld_a(i + 1), ld_b(i + 1));
 }
 
 // beta-only points [i0, n) (second wave only)
 template <int LB, int LS>
 __device__ __forceinline__ void split_tail(const double* __restrict__ T, const double* __restrict__ be, int n, int i0,
                                            int blk, double* Zp, double* acc, double* Cb, double* P) {
+    auto pt = [&](int i, double b_i) {
+        const double* t = T + (size_t)i * LS;
+        split_point_b<LB>(t[0], t[1], t[2 + blk], b_i, Zp, acc, Cb, P);
+    };
     int i = i0;
     if (i + LGU <= n) {
         double cb[LGU];
@@ -718,7 +722,7 @@ __device__ __forceinline__ void split_tail(const double* __restrict__ T, const d
                 for (int u = 0; u < LGU; ++u) xb[u] = be[i + LGU + u];
             }
 #pragma unroll
-            for (int u = 0; u < LGU; ++u) split_point_b<LB, LS>(T + (size_t)(i + u) * LS, blk, cb[u], Zp, acc, Cb, P);
+            for (int u = 0; u < LGU; ++u) pt(i + u, cb[u]);
             if (more) {
 #pragma unroll
                 for (int u = 0; u < LGU; ++u) cb[u] = xb[u];
@@ -726,7 +730,7 @@ __device__ __forceinline__ void split_tail(const double* __restrict__ T, const d
         }
     }
 #pragma unroll 1
-    for (; i < n; ++i) split_point_b<LB, LS>(T + (size_t)i * LS, blk, be[i], Zp, acc, Cb, P);
+    for (; i < n; ++i) pt(i, be[i]);
 }
 
 // Split point: equal operation counts for the two waves (see above).
@@ -737,8 +741,9 @@ __device__ __forceinline__ int split_at(int nr, int n) {
     return (int)min((long)nr, max(0L, m));
 }
 
-template <int LB, int NBP>
-__global__ __launch_bounds__(256) void semisep_lane_split_kernel(const dwba_plain_job* __restrict__ jobs,
+template <int LB, int NBP, int OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
+void semisep_lane_split_kernel(const dwba_plain_job* __restrict__ jobs,
                                                                  const double* const* __restrict__ tabs,
                                                                  const int* __restrict__ out_index, int n_jobs,
                                                                  int L_max, double* __restrict__ out) {
@@ -901,6 +906,108 @@ __global__ __launch_bounds__(GT) void semisep_gemv_kernel(const dwba_dgemv_energ
         const double q = i > 0 ? r[i - 1] / ri : 1.0;
         const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
         B = qL * (B + be[i] / ri);
+    }
+}
+
+// The same t_L in coalesced chunks: a workgroup per (energy, L) sweeps the
+// grid in chunks of GT points, one point per thread.  Point i's step of the
+// forward recurrence is the affine map x -> q_i^L x + q_i^L beta_i (of the
+// backward one x -> q_{i-1}^L x + q_{i-1}^L beta_i / r_i); a chunk composes
+// them with an in-wave shuffle scan and a scan of the four wave totals in
+// LDS, applies the carry from the previous chunk, and writes t (forward
+// sweep) or adds to it (backward sweep) with coalesced accesses.
+struct Aff { double m, v; };                     // x -> m x + v
+
+__device__ __forceinline__ Aff aff_then(Aff first, Aff second) {   // second after first
+    return Aff{second.m * first.m, fma(second.m, first.v, second.v)};
+}
+
+// Inclusive scan over the workgroup in thread order (rev: reversed order);
+// returns this thread's exclusive prefix, and the whole chunk in *tot.
+__device__ __forceinline__ Aff block_scan_excl(Aff x, bool rev, double* s_m, double* s_v, Aff* tot) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int pos = rev ? 63 - lane : lane;                  // position in the scan order within the wave
+    Aff inc = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int src = rev ? lane + d : lane - d;
+        const double pm = __shfl(inc.m, src < 0 || src > 63 ? lane : src, 64);
+        const double pv = __shfl(inc.v, src < 0 || src > 63 ? lane : src, 64);
+        if (pos >= d) inc = aff_then(Aff{pm, pv}, inc);
+    }
+    const int last = rev ? 0 : 63;                            // lane holding the wave's total
+    if (lane == last) { s_m[w] = inc.m; s_v[w] = inc.v; }
+    __syncthreads();
+    constexpr int NW = GT / 64;
+    Aff pre{1.0, 0.0};                                        // earlier waves in scan order
+    for (int q = 0; q < NW; ++q) {
+        const int wq = rev ? NW - 1 - q : q;
+        if (wq == w) break;
+        pre = aff_then(pre, Aff{s_m[wq], s_v[wq]});
+    }
+    Aff all = Aff{1.0, 0.0};
+    for (int q = 0; q < NW; ++q) {
+        const int wq = rev ? NW - 1 - q : q;
+        all = aff_then(all, Aff{s_m[wq], s_v[wq]});
+    }
+    *tot = all;
+    // exclusive = the previous element's inclusive value (pre for the wave's first)
+    const int src = rev ? lane + 1 : lane - 1;
+    const double em = __shfl(inc.m, src < 0 || src > 63 ? lane : src, 64);
+    const double ev = __shfl(inc.v, src < 0 || src > 63 ? lane : src, 64);
+    const Aff ex = pos == 0 ? Aff{1.0, 0.0} : Aff{em, ev};
+    __syncthreads();                                          // s_m / s_v are reused by the next call
+    return aff_then(pre, ex);
+}
+
+__global__ __launch_bounds__(GT) void semisep_gemv_scan_kernel(const dwba_dgemv_energy* __restrict__ en) {
+    __shared__ double s_m[GT / 64], s_v[GT / 64];
+    const dwba_dgemv_energy E = en[blockIdx.x];
+    const int L = blockIdx.y;
+    if (L > E.L_max) return;
+    const int n = E.N, rows = E.rows, tid = threadIdx.x;
+    const double* __restrict__ r = E.r;
+    const double* __restrict__ be = E.beta;
+    double* __restrict__ t = E.t + (size_t)L * E.ld;
+    const double dL = L == 0 ? 1.0 : exp((double)L * log(1.0 - 1e-12));
+    const int top = min(rows, n);                             // t is needed for i < rows only
+    // forward: A_{i+1} = q_i^L (A_i + beta_i), A_0 = 0; t_i = (A_i + d_L beta_i) / r_i
+    double carry = 0.0;
+    for (int c0 = 0; c0 < top; c0 += GT) {
+        const int i = c0 + tid;
+        Aff f{1.0, 0.0};
+        double ri = 1.0, bi = 0.0;
+        if (i < n) {
+            ri = r[i];
+            bi = be[i];
+            const double q = i + 1 < n ? ri / r[i + 1] : 1.0;
+            const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
+            f = Aff{qL, qL * bi};
+        }
+        Aff tot;
+        const Aff ex = block_scan_excl(f, false, s_m, s_v, &tot);
+        const double A = fma(ex.m, carry, ex.v);
+        if (i < top) t[i] = (A + dL * bi) / ri;
+        carry = fma(tot.m, carry, tot.v);
+    }
+    // backward: B_{i-1} = q_{i-1}^L (B_i + beta_i / r_i), B_{n-1} = 0; t_i += B_i.
+    // Chunks from the top of the grid down; points >= top only feed the carry.
+    carry = 0.0;
+    const int nch = (n + GT - 1) / GT;
+    for (int c = nch - 1; c >= 0; --c) {
+        const int i = c * GT + tid;
+        Aff g{1.0, 0.0};
+        if (i < n) {
+            const double ri = r[i];
+            const double q = i > 0 ? r[i - 1] / ri : 1.0;
+            const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
+            g = Aff{qL, qL * (be[i] / ri)};
+        }
+        Aff tot;
+        const Aff ex = block_scan_excl(g, true, s_m, s_v, &tot);
+        const double B = fma(ex.m, carry, ex.v);
+        if (i < top) t[i] += B;
+        carry = fma(tot.m, carry, tot.v);
     }
 }
 
@@ -1075,8 +1182,8 @@ int32_t dwba_il_semisep_lanes_tab_f64(int32_t n_jobs, const dwba_plain_job* d_jo
 }
 
 int32_t dwba_il_semisep_lanes_split_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const double* const* d_tabs,
-                                        const int32_t* d_out_index, int32_t L_max, int32_t lb, double* d_out,
-                                        void* stream) {
+                                        const int32_t* d_out_index, int32_t L_max, int32_t lb,
+                                        int32_t waves_per_simd, double* d_out, void* stream) {
     DWBA_REQUIRE(initialised(), "dwba_il_semisep_lanes_split_f64: call dwba_init first");
     DWBA_REQUIRE(n_jobs > 0 && d_jobs && d_tabs && d_out && L_max >= 0 && L_max <= 63,
                  "dwba_il_semisep_lanes_split_f64: bad arguments (n_jobs=%d, L_max=%d)", n_jobs, L_max);
@@ -1087,11 +1194,15 @@ int32_t dwba_il_semisep_lanes_split_f64(int32_t n_jobs, const dwba_plain_job* d_
     const long blocks = (tasks + 127) / 128;
     DWBA_REQUIRE(blocks <= INT32_MAX, "dwba_il_semisep_lanes_split_f64: too many jobs (%d)", n_jobs);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    switch (lb * 100 + nbp) {
-#define C(B, P) case B * 100 + P: hipLaunchKernelGGL((semisep_lane_split_kernel<B, P>), dim3((unsigned)blocks), \
-                                                     dim3(256), 0, s, d_jobs, d_tabs, d_out_index, n_jobs, L_max, \
-                                                     d_out); break;
-        C(8, 1) C(8, 2) C(8, 4) C(8, 8) C(4, 1) C(4, 2) C(4, 4) C(4, 8) C(4, 16)
+    if (waves_per_simd == 0) waves_per_simd = 2;
+    DWBA_REQUIRE(waves_per_simd == 2 || waves_per_simd == 3,
+                 "dwba_il_semisep_lanes_split_f64: waves_per_simd must be 0, 2 or 3 (got %d)", waves_per_simd);
+    switch (waves_per_simd * 10000 + lb * 100 + nbp) {
+#define C(O, B, P) case O * 10000 + B * 100 + P: \
+        hipLaunchKernelGGL((semisep_lane_split_kernel<B, P, O>), dim3((unsigned)blocks), dim3(256), 0, s, d_jobs, \
+                           d_tabs, d_out_index, n_jobs, L_max, d_out); break;
+        C(2, 8, 1) C(2, 8, 2) C(2, 8, 4) C(2, 8, 8) C(2, 4, 1) C(2, 4, 2) C(2, 4, 4) C(2, 4, 8) C(2, 4, 16)
+        C(3, 8, 1) C(3, 8, 2) C(3, 8, 4) C(3, 8, 8) C(3, 4, 1) C(3, 4, 2) C(3, 4, 4) C(3, 4, 8) C(3, 4, 16)
 #undef C
         default: return set_error(DWBA_ERR_ARG, "dwba_il_semisep_lanes_split_f64: L_max %d", L_max);
     }
@@ -1103,9 +1214,9 @@ int32_t dwba_semisep_gemv_f64(int32_t n_energies, const dwba_dgemv_energy* d_ene
                               void* stream) {
     DWBA_REQUIRE(initialised(), "dwba_semisep_gemv_f64: call dwba_init first");
     DWBA_REQUIRE(n_energies > 0 && d_energies && L_max >= 0 && L_max <= 63, "dwba_semisep_gemv_f64: bad arguments");
-    hipLaunchKernelGGL(semisep_gemv_kernel, dim3(n_energies, L_max + 1), dim3(GT), 0,
+    hipLaunchKernelGGL(semisep_gemv_scan_kernel, dim3(n_energies, L_max + 1), dim3(GT), 0,
                        reinterpret_cast<hipStream_t>(stream), d_energies);
-    DWBA_LAUNCH_CHECK("semisep_gemv_kernel");
+    DWBA_LAUNCH_CHECK("semisep_gemv_scan_kernel");
     return DWBA_OK;
 }
 

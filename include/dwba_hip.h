@@ -155,10 +155,12 @@ int32_t dwba_il_semisep_lanes_tab_f64(int32_t n_jobs, const dwba_plain_job* d_jo
  * wavefronts (the second starts from zero carries, accumulates the
  * propagation sums of the first's carries and joins it through LDS): a
  * shorter dependency chain per wavefront for 1.3x the operations; same
- * arguments and tables, same values up to rounding order. */
+ * arguments and tables, same values up to rounding order.  waves_per_simd
+ * (0 = 2, or 3): the occupancy the kernel is compiled for (3 caps it at
+ * 168 VGPRs). */
 int32_t dwba_il_semisep_lanes_split_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const double* const* d_tabs,
-                                        const int32_t* d_out_index, int32_t L_max, int32_t lb, double* d_out,
-                                        void* stream);
+                                        const int32_t* d_out_index, int32_t L_max, int32_t lb,
+                                        int32_t waves_per_simd, double* d_out, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Full-Split preparation: alpha / beta vectors of a channel pair.       */

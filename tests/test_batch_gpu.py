@@ -210,18 +210,19 @@ def test_lane_tables_bit_identical():
             b.run()
             r0 = b.results()
             b.lane_tables = True
-            b.lane_split = False
+            b.lane_split = 0
             b.out.zero_()
             b.run()
             r1 = b.results()
             assert np.array_equal(r0, r1), (lb, gemv)
             # the range split between two wavefronts: same sums, other order
-            b.lane_split = True
-            b.out.zero_()
-            b.run()
-            r2 = b.results()
-            assert np.all(np.abs(r2 - r0) <= 1e-12 * np.abs(r0) + 1e-14 * mag), (lb, gemv)
-            b.lane_split = False
+            for occ in (2, 3):
+                b.lane_split = occ
+                b.out.zero_()
+                b.run()
+                r2 = b.results()
+                assert np.all(np.abs(r2 - r0) <= 1e-12 * np.abs(r0) + 1e-14 * mag), (lb, gemv, occ)
+            b.lane_split = 0
 
 
 @pytest.mark.parametrize("count,L_max,nmax", [(16, 12, 1000), (6, 12, 37), (16, 31, 515), (11, 20, 16), (16, 30, 3)])

@@ -90,18 +90,20 @@ def test_semisep_lanes_random_jobs(L_max, lb):
     print(f"  worst |d|/sum|aKb| {w:.2e}", flush=True)
 
 
-@pytest.mark.parametrize("split", [False, True])
+@pytest.mark.parametrize("split", [0, 2, 3])
 @pytest.mark.parametrize("lb", [0, 4])
 @pytest.mark.parametrize("L_max", [0, 7, 30, 63])
 def test_semisep_lane_tables_random_jobs(L_max, lb, split):
     """Table-driven lane kernel, whole range per lane or split between two
-    wavefronts, jobs given in reverse order through the output index, against
+    wavefronts (compiled for 2 or 3 waves per SIMD), jobs given in reverse
+    order through the output index, against
     the oracle: empty rows, one- and two-point grids, rows shorter than the
     grid (beta-only ranges), split points at the ends."""
     from dwba_amd import _hip
     host, dev = _jobs(SPECS, seed=200 + L_max)
     print(f"\n[semisep lane tables] {len(SPECS)} jobs L_max={L_max} lb={lb} split={split}", flush=True)
-    got = _hip.il_semisep_f64_batch(dev, L_max, lanes=True, lb=lb, tables=True, split=split).cpu().numpy()
+    got = _hip.il_semisep_f64_batch(dev, L_max, lanes=True, lb=lb, tables=True, split=bool(split),
+                                    waves_per_simd=split).cpu().numpy()
     w = _check(host, got, L_max)
     print(f"  worst |d|/sum|aKb| {w:.2e}", flush=True)
 
