@@ -266,6 +266,8 @@ class ScanBatch:
                                   lay.n_pnodes, plan.mode | (PREP_NO_ZERO_FILL if plan.mode == PREP_FILON else 0),
                                   plan.log1, plan.log2)
         self.prep_d = hip.struct_array_to_device(prep, dev)
+        self._prep_host = prep
+        self.prep_d_gemv = None           # prep jobs for the shared-beta direct path (_build_fp64)
         self.coul_d = torch.as_tensor(coul_par.ravel(), device=dev) if any_coul else None
         self.any_coul = any_coul
         self._keep += node_bufs
@@ -345,6 +347,7 @@ class ScanBatch:
         # direct beta (u_f u_i w) for its Filon/standard pairs, so their direct
         # family is t_L = K_L beta once per energy (O(N)) and a dot per pair
         shared_direct = set()
+        beta_owner = set()                # the pair of each energy whose beta_dir the GEMV reads
         dg, dd, dd_counts = [], [], []
         max_rows = 0
         self.t_bufs = []
@@ -359,6 +362,7 @@ class ScanBatch:
                 self.t_bufs.append(t)
                 dg.append(DgemvEnergy(hip.ptr(st.r_d), hip.ptr(st.bd[pos[ks[0]]]), hip.ptr(t), st.N, rows_d, st.ld,
                                       self.L_max))
+                beta_owner.add(ks[0])
                 max_rows = max(max_rows, rows_d)
                 for k in ks:
                     shared_direct.add(k)
@@ -406,6 +410,14 @@ class ScanBatch:
             nca_split[g] = cols_split
         self.plain_d = hip.struct_array_to_device(arr, self.dev)
         self.plain_d_split = hip.struct_array_to_device(arr_split, self.dev) if self.ss_gemv is not None else None
+        if self.ss_gemv is not None:
+            # with the shared-beta path only one pair per energy writes beta_dir
+            pj = (hip.PrepJob * self.n_pairs)()
+            for k in range(self.n_pairs):
+                pj[k] = self._prep_host[k]
+                if k in shared_direct and k not in beta_owner:
+                    pj[k].beta_dir = 0
+            self.prep_d_gemv = hip.struct_array_to_device(pj, self.dev)
         self._job_energy = [self.pairs[k][0] for k in order]     # grid of each job, in job order
         self._jobs_host = (arr, arr_split)                        # host copies, job order
         self._lane_tabs = {}                                      # lb -> per-grid tables
@@ -695,7 +707,9 @@ class ScanBatch:
         mark("prep", "start")
         cxp, _ = hip.f64_array(self.cc[0])
         cwp, _ = hip.f64_array(self.cc[1])
-        hip.check(lib.dwba_fullsplit_prep_f64(self.n_pairs, hip.ptr(self.prep_d), hip.ptr(self.coul_d),
+        gemv = getattr(self, "direct_gemv", False) and self.prep_d_gemv is not None
+        prep_jobs = self.prep_d_gemv if gemv else self.prep_d
+        hip.check(lib.dwba_fullsplit_prep_f64(self.n_pairs, hip.ptr(prep_jobs), hip.ptr(self.coul_d),
                                               int(self.any_coul), cxp, cwp, self.CC_nodes, sh),
                   "dwba_fullsplit_prep_f64")
         mark("prep", "end")
@@ -762,6 +776,13 @@ class ScanBatch:
         in the order the pairs were given."""
         torch, hip, lib = self.torch, self.hip, self.lib
         n = self.n_pairs
+        if getattr(self, "direct_gemv", False) and self.prep_d_gemv is not None:
+            # the shared-beta path leaves beta_dir of most pairs unwritten: write all
+            cxp, _ = hip.f64_array(self.cc[0])
+            cwp, _ = hip.f64_array(self.cc[1])
+            hip.check(lib.dwba_fullsplit_prep_f64(n, hip.ptr(self.prep_d), hip.ptr(self.coul_d), int(self.any_coul),
+                                                  cxp, cwp, self.CC_nodes, hip.stream_handle()),
+                      "dwba_fullsplit_prep_f64")
         arr = (hip.PlainJob * n)()
         nra = (ctypes.c_int32 * n)()
         nca = (ctypes.c_int32 * n)()

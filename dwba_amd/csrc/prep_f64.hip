@@ -289,7 +289,7 @@ __global__ __launch_bounds__(PREP_THREADS) void prep_f64_kernel(const dwba_prep_
             J.alpha_dir[i] = 0.0; J.beta_dir[i] = 0.0; J.alpha_ex[i] = 0.0; J.beta_ex[i] = 0.0;
         } else {
             // beta_dir = (u_f u_i) w on the full grid for both STANDARD and FILON
-            J.beta_dir[i] = clean(uf * ui) * wf;
+            if (J.beta_dir) J.beta_dir[i] = clean(uf * ui) * wf;   // NULL: a shared beta written by another pair
             if (mode == DWBA_PREP_STANDARD) {
                 J.alpha_dir[i] = clean(cf * ci) * w;
                 J.alpha_ex[i] = clean(uf * ci) * w;

@@ -199,7 +199,8 @@ typedef struct dwba_prep_job {
     const double* V_diff;     /* V_core - U_i, or NULL */
     const double* phase_nodes;
     double* alpha_dir;
-    double* beta_dir;
+    double* beta_dir;         /* may be NULL in FILON / STANDARD mode: beta_dir = (u_f u_i) w is then not
+                                 written (pairs of one energy that share it need it once) */
     double* alpha_ex;
     double* beta_ex;
     double* scalars;

@@ -147,6 +147,8 @@ def test_direct_gemv_path_matches_full_recurrence(L_max):
         b.direct_gemv = True
         b.dot_kind = kind
         b.out.zero_()
+        for st in b.E:             # prep rewrites only the beta_dir the shared path reads
+            st.bd.zero_()
         b.run()
         r1 = b.results()
         excess = np.abs(r1 - r0) / (1e-12 * np.abs(r0) + 1e-14 * mag)
