@@ -376,13 +376,31 @@ __device__ __forceinline__ void sweep2(double* acc, double w0, double ix0, doubl
     }
 }
 
+// sin, cos of theta and the Filon coefficients alpha, beta, gamma at theta
+// (_filon_segment_complex's formulas, Taylor series below |theta| = 0.3).
+__device__ __forceinline__ void filon_coeffs(double th, double& rs, double& rc, double& al, double& be, double& ga) {
+    sincos(th, &rs, &rc);
+    if (fabs(th) < 0.3) {
+        const double t2 = th * th, t3 = th * th * th;
+        al = 2 * t3 / 45 * (1 - t2 / 7 + t2 * t2 / 189);
+        be = 2.0 / 3 + 2 * t2 / 15 - 4 * (t2 * t2) / 105 + 2 * (t2 * t2 * t2) / 567;
+        ga = 4.0 / 3 - 2 * t2 / 15 + 1 * (t2 * t2) / 210 - (t2 * t2 * t2) / 11340;
+    } else {
+        const double sd = 2.0 * rs * rc, cd = 1.0 - 2.0 * rs * rs;   // sin, cos of 2 theta
+        const double t2 = th * th, it3 = 1.0 / (th * th * th);
+        al = (t2 + th * sd / 2 - 2 * (rs * rs)) * it3;
+        be = 2 * (th * (1 + cd) - sd) * it3;
+        ga = 4 * (rs - th * rc) * it3;
+    }
+}
+
 // Filon segments of one (pair, sign), all L.  Each lane takes a contiguous
 // run of segments; a segment of neff (odd) equally spaced nodes is walked
 // triple by triple (the recursion of _filon_segment_complex).  Per triple
 // the real part of the Filon sum is wa f(x0) + wm f(x1) + wb f(x2), so a
 // node shared by two triples (or two segments) is swept once with the sum of
 // its weights.  The Filon coefficients depend on theta = omega * step only
-// and are formed once per segment; the node phases exp(i (omega x + p0))
+// and are formed once (the last segment apart); the node phases exp(i (omega x + p0))
 // advance by the rotation exp(i theta) from the segment start (the reference
 // evaluates each node phase directly: the two agree to a few ulp of the
 // phase).  Each lane adds to per-L FP64 partial sums (a few hundred terms of
@@ -397,6 +415,11 @@ __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, do
     const long s_lo = (long)lane * per;
     const long s_hi = s_lo + per < nseg ? s_lo + per : nseg;
     double w_pend = 0.0, ix_pend = 0.0;           // weight still to sweep at the lane's current node
+    // Segments are [r_m + s bstep, r_m + (s+1) bstep], so theta = omega * step
+    // is one value up to rounding: its sine, cosine and Filon coefficients are
+    // formed once; only the last segment (ending at r_max) gets its own.
+    double rs_u, rc_u, al_u, be_u, ga_u;
+    filon_coeffs(om * (bstep / (neff - 1)), rs_u, rc_u, al_u, be_u, ga_u);
     for (long sg = s_lo; sg < s_hi; ++sg) {
         const double a = __dadd_rn(__dmul_rn((double)sg, bstep), r_m);
         const double b = (sg + 1 == nseg) ? r_max : __dadd_rn(__dmul_rn((double)(sg + 1), bstep), r_m);
@@ -414,21 +437,12 @@ __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, do
         double rs = 0.0, rc = 1.0, s0 = 0.0, c0 = 0.0;
         double al = 0.0, be = 0.0, ga = 0.0;
         if (!small) {
-            const double th = om * step;
-            sincos(th, &rs, &rc);
-            sincos(om * a + p0, &s0, &c0);
-            if (fabs(th) < 0.3) {
-                const double t2 = th * th, t3 = th * th * th;
-                al = 2 * t3 / 45 * (1 - t2 / 7 + t2 * t2 / 189);
-                be = 2.0 / 3 + 2 * t2 / 15 - 4 * (t2 * t2) / 105 + 2 * (t2 * t2 * t2) / 567;
-                ga = 4.0 / 3 - 2 * t2 / 15 + 1 * (t2 * t2) / 210 - (t2 * t2 * t2) / 11340;
+            if (sg + 1 == nseg) {
+                filon_coeffs(om * step, rs, rc, al, be, ga);
             } else {
-                const double sd = 2.0 * rs * rc, cd = 1.0 - 2.0 * rs * rs;   // sin, cos of 2 theta
-                const double t2 = th * th, it3 = 1.0 / (th * th * th);
-                al = (t2 + th * sd / 2 - 2 * (rs * rs)) * it3;
-                be = 2 * (th * (1 + cd) - sd) * it3;
-                ga = 4 * (rs - th * rc) * it3;
+                rs = rs_u; rc = rc_u; al = al_u; be = be_u; ga = ga_u;
             }
+            sincos(om * a + p0, &s0, &c0);
         }
         double x0 = a;
 #pragma unroll 1
