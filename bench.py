@@ -283,7 +283,7 @@ def scan_kernel_ms(b, steps, stream):
     if b.semisep_lanes and b.lane_tables:
         lbv = lb or 8
         gd, ng, max_n, _ = b._lane_table_set(lbv)
-        sj, ptrs, oidx = b._lane_job_set(lbv, b.direct_gemv)
+        sj, ptrs, oidx, _ = b._lane_job_set(lbv, b.direct_gemv)
         hip.check(lib.dwba_semisep_lane_tables_f64(ng, hip.ptr(gd), max_n, Lm, lbv, sh), "tables")
         if b.lane_split:
             occ = 3 if int(b.lane_split) == 3 else 2
@@ -414,6 +414,7 @@ def bench_batched(args, ionization=False):
         "autotune": ({"params": tuned.params, "ms": tuned.ms, "cached": tuned.cached, "candidates": tuned.timings}
                      if tuned else None),
         "step_tuning": ({"tail_fork": str(b64s.overlap), "direct_gemv": bool(b64s.direct_gemv),
+                         "pipelined": bool(b64s.pipeline and b64s.pipeline_ok()),
                          "dot": b64s.dot_kind, **b64s.tuning_step}
                         if getattr(b64s, "tuning_step", None) else None),
         "host_note": "host planning, uploads and autotuning precede the timed region",

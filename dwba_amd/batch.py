@@ -128,6 +128,11 @@ class ScanBatch:
         self.precision = precision
         self.CC_nodes = int(CC_nodes)
         self._keep = []
+        # defaults of the FP64 contraction options (the FP64 build and tune() set them)
+        self.direct_gemv = False
+        self.pipeline = False
+        self._pipe = None
+        self._side2 = None
         t0 = time.perf_counter()
         self._plan(energies, use_oscillatory_quadrature, oscillatory_method, phase_increment,
                    min_grid_fraction, k_threshold)
@@ -142,6 +147,7 @@ class ScanBatch:
         # the contractions; "start": forked before prep; None: one stream.
         self.overlap = "after_prep"
         self._side = None
+        self._side2 = None
         self.tuning_overlap = None
 
     # ------------------------------------------------------------ planning
@@ -348,7 +354,7 @@ class ScanBatch:
         # family is t_L = K_L beta once per energy (O(N)) and a dot per pair
         shared_direct = set()
         beta_owner = set()                # the pair of each energy whose beta_dir the GEMV reads
-        dg, dd, dd_counts = [], [], []
+        dg, dd, dd_counts, dg_energy = [], [], [], []
         max_rows = 0
         self.t_bufs = []
         if self.contraction == "semiseparable":
@@ -358,6 +364,7 @@ class ScanBatch:
                     continue
                 rows_d = max(self.pairs[k][2].N1 for k in ks)
                 dd_counts.append(len(ks))
+                dg_energy.append(ei)
                 t = self.torch.zeros((self.L_max + 1, st.ld), dtype=self.torch.float64, device=self.dev)
                 self.t_bufs.append(t)
                 dg.append(DgemvEnergy(hip.ptr(st.r_d), hip.ptr(st.bd[pos[ks[0]]]), hip.ptr(t), st.N, rows_d, st.ld,
@@ -384,10 +391,12 @@ class ScanBatch:
                             items_d, len(items))
             # tiles of <= 16 pairs for the matrix-core dot
             items16 = []
+            self._item16_energy = []
             first = 0
-            for cnt in dd_counts:
+            for cnt, ei in zip(dd_counts, dg_energy):
                 for q in range(0, cnt, 16):
                     items16.append((first + q, min(16, cnt - q)))
+                    self._item16_energy.append(ei)
                 first += cnt
             self.items16 = (self.torch.as_tensor(np.asarray(items16, dtype=np.int32).reshape(-1, 2),
                                                  device=self.dev), len(items16))
@@ -419,6 +428,9 @@ class ScanBatch:
                     pj[k].beta_dir = 0
             self.prep_d_gemv = hip.struct_array_to_device(pj, self.dev)
         self._job_energy = [self.pairs[k][0] for k in order]     # grid of each job, in job order
+        self._dg_energy = dg_energy
+        self.pipeline = False             # two-half pipelined FP64 step (tune())
+        self._pipe = self._pipe_plan()
         self._jobs_host = (arr, arr_split)                        # host copies, job order
         self._lane_tabs = {}                                      # lb -> per-grid tables
         self._lane_sets = {}                                      # (lb, split) -> jobs sorted longest first
@@ -589,19 +601,43 @@ class ScanBatch:
         self._lane_tabs[lb] = ent
         return ent
 
-    def _lane_job_set(self, lb, split):
+    def _pipe_plan(self):
+        """Split of the energies into two halves of about equal prep work
+        (pairs x grid points) for the pipelined step: prep of the second half
+        runs beside the contractions of the first.  None when there is
+        nothing to split."""
+        nE = len(self.E)
+        if nE < 2 or self.contraction != "semiseparable":
+            return None
+        cost = np.zeros(nE)
+        for (ei, _, plan, *_r) in self.pairs:
+            cost[ei] += self.E[ei].N
+        cum = np.cumsum(cost)
+        eA = int(np.searchsorted(cum, 0.5 * cum[-1])) + 1
+        eA = min(max(eA, 1), nE - 1)
+        kA = next((k for k, p in enumerate(self.pairs) if p[0] >= eA), len(self.pairs))
+        dgA = sum(1 for e in getattr(self, "_dg_energy", []) if e < eA)
+        itA = sum(1 for e in getattr(self, "_item16_energy", []) if e < eA)
+        return {"eA": eA, "kA": kA, "dgA": dgA, "itA": itA}
+
+    def _lane_job_set(self, lb, split, half=None):
         """Jobs of the table-driven lane kernel sorted by length, longest
         first (wavefronts of equal-length pairs, long ones dispatched first),
-        with their table pointers and output rows."""
-        key = (lb, bool(split))
+        with their table pointers and output rows; half 0 / 1: only the jobs
+        of that half of the energies (pipelined step)."""
+        key = (lb, bool(split), half)
         hit = self._lane_sets.get(key)
         if hit is not None:
             return hit
         torch, hip = self.torch, self.hip
         _, _, _, tabs = self._lane_table_set(lb)
         src = self._jobs_host[1 if split else 0]
-        n = self.plain_count
-        perm = sorted(range(n), key=lambda g: -int(src[g].n_cols))
+        jobs = range(self.plain_count)
+        if half is not None:
+            eA = self._pipe["eA"]
+            jobs = [g for g in jobs if (self._job_energy[g] >= eA) == (half == 1)]
+        perm = sorted(jobs, key=lambda g: -int(src[g].n_cols))
+        n = len(perm)
         arr = (hip.PlainJob * n)()
         for p, g in enumerate(perm):
             arr[p] = src[g]
@@ -609,7 +645,7 @@ class ScanBatch:
         ptrs = torch.as_tensor(np.asarray([hip.ptr(tabs[self._job_energy[g]]) for g in perm], dtype=np.int64),
                                device=self.dev)
         oidx = torch.as_tensor(np.asarray(perm, dtype=np.int32), device=self.dev)
-        ent = (jobs_d, ptrs, oidx)
+        ent = (jobs_d, ptrs, oidx, n)
         self._lane_sets[key] = ent
         return ent
 
@@ -623,7 +659,7 @@ class ScanBatch:
         if lanes and self.lane_tables:
             lbv = lb or 8
             gd, ng, max_n, _ = self._lane_table_set(lbv)
-            sj, ptrs, oidx = self._lane_job_set(lbv, direct_gemv)
+            sj, ptrs, oidx, _ = self._lane_job_set(lbv, direct_gemv)
             hip.check(lib.dwba_semisep_lane_tables_f64(ng, hip.ptr(gd), max_n, Lm, lbv, sh),
                       "dwba_semisep_lane_tables_f64")
             split = self.lane_split if split is None else split     # 0: whole ranges, 2 or 3: waves per SIMD
@@ -653,6 +689,45 @@ class ScanBatch:
                           "dwba_ddot_tiled_f64")
             else:
                 hip.check(lib.dwba_ddot_f64(nd, hip.ptr(dd), Lm, hip.ptr(out), sh), "dwba_ddot_f64")
+
+    def pipeline_ok(self):
+        """The pipelined step needs the table-driven lane kernel and, with the
+        shared-beta path, the matrix-core dots (per-half job, GEMV and item
+        ranges exist for those)."""
+        return (self._pipe is not None and self.precision == "fp64" and self.contraction == "semiseparable"
+                and self.semisep_lanes and self.lane_tables and self.L_max <= 31
+                and (not self.direct_gemv or self.dot_kind == "mfma"))
+
+    def _contract_half(self, half, out, sh):
+        """The contraction of one half of the energies (pipelined step): the
+        shared-beta GEMV and dots of its energies, then the lane kernel on
+        its jobs.  The grid tables must already be written."""
+        hip, lib, Lm = self.hip, self.lib, self.L_max
+        seg, lb = self.semisep_config
+        lbv = lb or 8
+        P = self._pipe
+        if self.direct_gemv:
+            gd, ng, dd, nd, _, _ = self.ss_gemv
+            it16, n16 = self.items16
+            g0, gn = (0, P["dgA"]) if half == 0 else (P["dgA"], ng - P["dgA"])
+            i0, i_n = (0, P["itA"]) if half == 0 else (P["itA"], n16 - P["itA"])
+            if gn > 0:
+                hip.check(lib.dwba_semisep_gemv_f64(gn, hip.ptr(gd) + g0 * ctypes.sizeof(DgemvEnergy), Lm, sh),
+                          "dwba_semisep_gemv_f64")
+            if i_n > 0:
+                hip.check(lib.dwba_ddot_mfma_f64(i_n, hip.ptr(it16) + i0 * 8, hip.ptr(dd), Lm, hip.ptr(out), sh),
+                          "dwba_ddot_mfma_f64")
+        _, _, _, _ = self._lane_table_set(lbv)
+        sj, ptrs, oidx, n = self._lane_job_set(lbv, self.direct_gemv, half)
+        if n == 0:
+            return
+        if self.lane_split:
+            hip.check(lib.dwba_il_semisep_lanes_split_f64(n, hip.ptr(sj), hip.ptr(ptrs), hip.ptr(oidx), Lm, lbv,
+                                                          3 if int(self.lane_split) == 3 else 2, hip.ptr(out), sh),
+                      "dwba_il_semisep_lanes_split_f64")
+        else:
+            hip.check(lib.dwba_il_semisep_lanes_tab_f64(n, hip.ptr(sj), hip.ptr(ptrs), hip.ptr(oidx), Lm, lbv,
+                                                        hip.ptr(out), sh), "dwba_il_semisep_lanes_tab_f64")
 
     # ------------------------------------------------------------ launch
     def launch(self, stream=None, events=None):
@@ -709,18 +784,54 @@ class ScanBatch:
         cwp, _ = hip.f64_array(self.cc[1])
         gemv = getattr(self, "direct_gemv", False) and self.prep_d_gemv is not None
         prep_jobs = self.prep_d_gemv if gemv else self.prep_d
-        hip.check(lib.dwba_fullsplit_prep_f64(self.n_pairs, hip.ptr(prep_jobs), hip.ptr(self.coul_d),
-                                              int(self.any_coul), cxp, cwp, self.CC_nodes, sh),
-                  "dwba_fullsplit_prep_f64")
+
+        def prep(k0, n, h):
+            if n <= 0:
+                return
+            coul = hip.ptr(self.coul_d) + 4 * 8 * k0 if self.any_coul else 0
+            hip.check(lib.dwba_fullsplit_prep_f64(n, hip.ptr(prep_jobs) + k0 * ctypes.sizeof(hip.PrepJob), coul,
+                                                  int(self.any_coul), cxp, cwp, self.CC_nodes, h),
+                      "dwba_fullsplit_prep_f64")
+
+        # Pipelined step (no stage events): prep of the second half of the
+        # energies runs on the launch stream while the first half is
+        # contracted on a second stream, which then takes the second half.
+        pipe = events is None and self.pipeline and self.plain_count and self.pipeline_ok()
+        if pipe:
+            kA = self._pipe["kA"]
+            if self._side2 is None:
+                self._side2 = torch.cuda.Stream(device=self.dev)
+            s2, sh2 = self._side2, hip.stream_handle(self._side2)
+            prep(0, kA, sh)
+            ev_a = torch.cuda.Event()
+            ev_a.record(strm)
+            if fork and self.overlap == "after_prep":
+                fork_tails()
+            prep(kA, self.n_pairs - kA, sh)
+            ev_b = torch.cuda.Event()
+            ev_b.record(strm)
+            s2.wait_event(ev_a)
+            lbv = self.semisep_config[1] or 8
+            gd, ng, max_n, _ = self._lane_table_set(lbv)
+            hip.check(lib.dwba_semisep_lane_tables_f64(ng, hip.ptr(gd), max_n, Lm, lbv, sh2),
+                      "dwba_semisep_lane_tables_f64")
+            self._contract_half(0, self.out, sh2)
+            s2.wait_event(ev_b)
+            self._contract_half(1, self.out, sh2)
+            ev_c = torch.cuda.Event()
+            ev_c.record(s2)
+            strm.wait_event(ev_c)
+        else:
+            prep(0, self.n_pairs, sh)
         mark("prep", "end")
-        if fork and self.overlap == "after_prep":
+        if fork and self.overlap == "after_prep" and not pipe:
             fork_tails()
         mark("tail", "start")
         if not fork:
             tails(sh)
         mark("tail", "end")
         mark("plain", "start")
-        if self.plain_count:
+        if self.plain_count and not pipe:
             nra, nca = self.plain_n
             if self.contraction == "semiseparable":
                 seg, lb = self.semisep_config
@@ -901,7 +1012,7 @@ class ScanBatch:
                             tails=AT.bucket(self.n_tail), n=AT.bucket(float(np.mean(list(nca)))), L=Lm,
                             c="|".join(f"{f['seg']}.{f['lb']}.{int(f.get('lanes', 0))}.{int(f.get('split', 0))}."
                                        f"{int(f.get('gemv', 0))}"
-                                       for f in finalists), v=3)
+                                       for f in finalists), v=4)
 
         def apply(c):
             f = finalists[int(c["c"])]
@@ -913,12 +1024,16 @@ class ScanBatch:
             if self.plain_ws.numel() < self.plain_ws_bytes:
                 self.plain_ws = torch.empty(max(self.plain_ws_bytes, 1), dtype=torch.uint8, device=self.dev)
             self.overlap = modes[int(c["overlap"])]
+            self.pipeline = bool(c.get("pipe", 0))
 
         def run_step(c):
             apply(c)
             self.launch()
 
         cands = [{"c": i, "overlap": j} for i in range(len(finalists)) for j in range(len(modes))]
+        if self._pipe is not None:       # the two-half pipelined step, for the lane-kernel finalists
+            cands += [{"c": i, "overlap": j, "pipe": 1} for i, f in enumerate(finalists) if f.get("lanes", 0)
+                      for j in range(len(modes))]
         oc = tuner.tune(jkey, cands, run_step, steps=steps)
         apply(oc.params)
         self.tuning_overlap = oc

@@ -261,3 +261,38 @@ def test_ddot_mfma_against_numpy(count, L_max, nmax):
     scale = np.abs(A[:, :nmax]) @ np.abs(T[:, :nmax]).T
     bad = np.argwhere(~(np.abs(got - ref) <= 1e-14 * scale + 1e-300))
     assert bad.size == 0, ("wrong (pair, L) cells", bad.tolist()[:40], len(bad))
+
+
+@pytest.mark.parametrize("gemv", [False, True])
+def test_pipelined_step_bit_identical(gemv):
+    """The two-half pipelined step (prep of the second half of the energies
+    beside the contraction of the first, on two streams, tails on a third)
+    gives the single-stream result bit for bit, eagerly and as graph replays."""
+    import torch
+    from dwba_amd.batch import ScanBatch
+    scan = _small_scan(9, n_e=4, n_p=24)
+    b = ScanBatch(scan, 20, precision="fp64", oscillatory_method="full_split")
+    assert b._pipe is not None and 0 < b._pipe["kA"] < b.n_pairs
+    b.semisep_config = (0, 8)
+    b.semisep_lanes = True
+    b.lane_split = 2
+    b.direct_gemv = gemv and b.ss_gemv is not None
+    b.overlap = None
+    b.pipeline = False
+    b.launch()
+    ref = b.results().copy()
+    for mode in (None, "start", "after_prep"):
+        b.overlap = mode
+        b.pipeline = True
+        assert b.pipeline_ok()
+        b.out.zero_()
+        b.launch()
+        assert np.array_equal(b.results(), ref), mode
+    b.capture()
+    for rep in range(3):
+        b.out.zero_()
+        torch.cuda.synchronize()
+        b.run()
+        torch.cuda.synchronize()
+        assert np.array_equal(b.results(), ref), rep
+        _ = (torch.ones(1000, device="cuda") * rep).sum().item()
