@@ -7,12 +7,13 @@ launch stream (warmup + median of a few runs); the fastest wins and is stored
 in a JSON cache so later runs of the same signature skip the search.
 
 Cache file: $DWBA_AUTOTUNE_CACHE, else ~/.cache/dwba_amd/autotune.json.
-Setting DWBA_AUTOTUNE=off makes every lookup miss and nothing is timed: the
-callers then keep the library defaults.
+Setting DWBA_AUTOTUNE=off makes every lookup miss, and ScanBatch.tune()
+then times nothing and keeps the library defaults.
 
 Every parameter tuned here changes only the order of FP64 operations (how
-the grid is cut into lanes and how many multipoles share registers), never
-the quadrature, so tuned and default runs agree to rounding.
+the grid is cut into lanes, how many multipoles share registers, which
+contraction kernel runs, which streams the stages run on), never the
+quadrature, so tuned and default runs agree to rounding.
 """
 from __future__ import annotations
 
@@ -162,10 +163,12 @@ def semisep_candidates() -> List[Dict[str, int]]:
     """(seg, lb) of the wavefront-scan kernel, plus the lane-serial kernel
     (one lane per job, family and block of lb multipoles; seg unused), whole
     range per lane or split between two wavefronts (split = the occupancy,
-    2 or 3 waves per SIMD, that kernel is compiled for)."""
+    2 or 3 waves per SIMD, that kernel is compiled for); order 1: the lane
+    kernel's jobs grouped by energy instead of sorted by length alone."""
     return [{"seg": s, "lb": b} for b in SEMISEP_LBS for s in SEMISEP_SEGS] + [
         {"seg": 0, "lb": b, "lanes": 1} for b in SEMISEP_LBS] + [
-        {"seg": 0, "lb": b, "lanes": 1, "split": w} for b in SEMISEP_LBS for w in (2, 3)]
+        {"seg": 0, "lb": b, "lanes": 1, "split": w} for b in SEMISEP_LBS for w in (2, 3)] + [
+        {"seg": 0, "lb": 8, "lanes": 1, "split": w, "order": 1} for w in (0, 2)]
 
 
 def device_id(dev) -> str:

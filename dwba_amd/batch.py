@@ -442,6 +442,7 @@ class ScanBatch:
             self.semisep_config = (0, 0)      # (seg, lb): library defaults until tuned
             self.semisep_lanes = False        # lane-serial kernel instead (chosen by tune())
             self.lane_tables = True           # that kernel reads per-grid 1/r, q, q^(lb b) tables
+            self.lane_order = "length"        # job order of that kernel: "length" or "energy" (tune())
             self.lane_split = 0               # and splits each range between two wavefronts (tune()): 0 no,
                                               # 2 or 3 the occupancy (waves per SIMD) the split kernel targets
             self.plain_ws_bytes = self._semisep_ws_bytes(0)
@@ -624,8 +625,11 @@ class ScanBatch:
         """Jobs of the table-driven lane kernel sorted by length, longest
         first (wavefronts of equal-length pairs, long ones dispatched first),
         with their table pointers and output rows; half 0 / 1: only the jobs
-        of that half of the energies (pipelined step)."""
-        key = (lb, bool(split), half)
+        of that half of the energies (pipelined step).  lane_order "energy":
+        pairs of one energy kept together (energies with the longest pairs
+        first, longest first within each), so a wavefront's grid-table reads
+        hit one table."""
+        key = (lb, bool(split), half, self.lane_order)
         hit = self._lane_sets.get(key)
         if hit is not None:
             return hit
@@ -636,7 +640,14 @@ class ScanBatch:
         if half is not None:
             eA = self._pipe["eA"]
             jobs = [g for g in jobs if (self._job_energy[g] >= eA) == (half == 1)]
-        perm = sorted(jobs, key=lambda g: -int(src[g].n_cols))
+        if self.lane_order == "energy":
+            emax = {}
+            for g in jobs:
+                e = self._job_energy[g]
+                emax[e] = max(emax.get(e, 0), int(src[g].n_cols))
+            perm = sorted(jobs, key=lambda g: (-emax[self._job_energy[g]], self._job_energy[g], -int(src[g].n_cols)))
+        else:
+            perm = sorted(jobs, key=lambda g: -int(src[g].n_cols))
         n = len(perm)
         arr = (hip.PlainJob * n)()
         for p, g in enumerate(perm):
@@ -717,7 +728,6 @@ class ScanBatch:
             if i_n > 0:
                 hip.check(lib.dwba_ddot_mfma_f64(i_n, hip.ptr(it16) + i0 * 8, hip.ptr(dd), Lm, hip.ptr(out), sh),
                           "dwba_ddot_mfma_f64")
-        _, _, _, _ = self._lane_table_set(lbv)
         sj, ptrs, oidx, n = self._lane_job_set(lbv, self.direct_gemv, half)
         if n == 0:
             return
@@ -957,13 +967,15 @@ class ScanBatch:
         if self.precision != "fp64" or self.contraction != "semiseparable" or not self.plain_count:
             return None
         from . import autotune as AT
+        if not AT.enabled():
+            return None                     # DWBA_AUTOTUNE=off: library defaults, nothing timed
         torch, hip, lib = self.torch, self.hip, self.lib
         tuner = tuner or AT.default_tuner()
         nra, nca = self.plain_n
         Lm = self.L_max
         has_gemv = self.ss_gemv is not None
         key = AT.signature("semisep_f64", AT.device_id(self.dev), jobs=AT.bucket(self.plain_count),
-                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm, gemv=int(has_gemv), v=5)
+                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm, gemv=int(has_gemv), v=6)
         self.launch()                       # alpha/beta in place for the contraction
         torch.cuda.synchronize()
         out = torch.empty_like(self.out)
@@ -975,6 +987,7 @@ class ScanBatch:
                 nb = self._semisep_ws_bytes(seg)
                 wss[seg] = (torch.empty(max(nb, 1), dtype=torch.uint8, device=self.dev), nb)
             ws, nb = wss[seg]
+            self.lane_order = "energy" if c.get("order", 0) else "length"
             self._contract_semisep(seg, int(c["lb"]), bool(c.get("gemv", 0)), out, ws, nb, hip.stream_handle(),
                                    bool(c.get("lanes", 0)), int(c.get("split", 0)))
 
@@ -987,6 +1000,7 @@ class ScanBatch:
         self.semisep_config = (seg, lb)
         self.semisep_lanes = bool(choice.params.get("lanes", 0))
         self.lane_split = int(choice.params.get("split", 0))
+        self.lane_order = "energy" if choice.params.get("order", 0) else "length"
         self.direct_gemv = bool(choice.params.get("gemv", 0)) and has_gemv
         self.plain_ws_bytes = self._semisep_ws_bytes(seg)
         self.plain_ws = torch.empty(max(self.plain_ws_bytes, 1), dtype=torch.uint8, device=self.dev)
@@ -998,12 +1012,10 @@ class ScanBatch:
         # does not show how much of it the overlapped tails hide).
         modes = (None, "start", "after_prep") if self.n_tail else (None,)
         finalists = [choice.params]
-        kinds = {(int(t["params"].get("gemv", 0)), int(t["params"].get("lanes", 0)), int(t["params"].get("split", 0)))
-                 for t in choice.timings}
-        for want in sorted(kinds):                         # (shared-beta path, lane kernel, split lanes)
-            same = [t for t in choice.timings
-                    if (int(t["params"].get("gemv", 0)), int(t["params"].get("lanes", 0)),
-                        int(t["params"].get("split", 0))) == want]
+        kind = lambda q: (int(q.get("gemv", 0)), int(q.get("lanes", 0)), int(q.get("split", 0)), int(q.get("order", 0)))
+        kinds = {kind(t["params"]) for t in choice.timings}
+        for want in sorted(kinds):                         # (shared-beta path, lane kernel, split lanes, job order)
+            same = [t for t in choice.timings if kind(t["params"]) == want]
             if same:
                 best = min(same, key=lambda t: t["ms"])["params"]
                 if best not in finalists:
@@ -1011,14 +1023,15 @@ class ScanBatch:
         jkey = AT.signature("scan_step", AT.device_id(self.dev), jobs=AT.bucket(self.n_pairs),
                             tails=AT.bucket(self.n_tail), n=AT.bucket(float(np.mean(list(nca)))), L=Lm,
                             c="|".join(f"{f['seg']}.{f['lb']}.{int(f.get('lanes', 0))}.{int(f.get('split', 0))}."
-                                       f"{int(f.get('gemv', 0))}"
-                                       for f in finalists), v=4)
+                                       f"{int(f.get('order', 0))}.{int(f.get('gemv', 0))}"
+                                       for f in finalists), v=5)
 
         def apply(c):
             f = finalists[int(c["c"])]
             self.semisep_config = (int(f["seg"]), int(f["lb"]))
             self.semisep_lanes = bool(f.get("lanes", 0))
             self.lane_split = int(f.get("split", 0))
+            self.lane_order = "energy" if f.get("order", 0) else "length"
             self.direct_gemv = bool(f.get("gemv", 0)) and has_gemv
             self.plain_ws_bytes = self._semisep_ws_bytes(int(f["seg"]))
             if self.plain_ws.numel() < self.plain_ws_bytes:

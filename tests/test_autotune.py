@@ -50,3 +50,14 @@ def test_unwritable_cache_is_harmless(tmp_path, monkeypatch):
     t = AT.Autotuner()
     ch = t.tune("k", [{"seg": 1}], lambda c: None, timer=lambda fn, s: 1.0)
     assert ch.params == {"seg": 1}
+
+
+def test_autotune_off_keeps_defaults(monkeypatch):
+    """DWBA_AUTOTUNE=off: lookups miss (and ScanBatch.tune() returns None
+    before timing anything, keeping the library defaults)."""
+    from dwba_amd import autotune as AT
+    monkeypatch.setenv("DWBA_AUTOTUNE", "off")
+    assert not AT.enabled()
+    t = AT.Autotuner(persist=False)
+    t.store("k", AT.Choice(params={"seg": 1}, ms=1.0, timings=[]))
+    assert t.lookup("k") is None

@@ -217,6 +217,12 @@ def test_lane_tables_bit_identical():
             b.run()
             r1 = b.results()
             assert np.array_equal(r0, r1), (lb, gemv)
+            # jobs grouped by energy instead of sorted by length: same bits
+            b.lane_order = "energy"
+            b.out.zero_()
+            b.run()
+            assert np.array_equal(r0, b.results()), (lb, gemv, "energy order")
+            b.lane_order = "length"
             # the range split between two wavefronts: same sums, other order
             for occ in (2, 3):
                 b.lane_split = occ
