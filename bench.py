@@ -440,7 +440,7 @@ def bench_batched(args, ionization=False):
                               "contraction_stage": {
                                   "ms": t_c, "achieved_tflops": W / (t_c * 1e-3) / 1e12,
                                   "frac": W / (t_c * 1e-3) / PEAK_FP64, "work_flop": W,
-                                  "kernels": kname + (" + semisep_gemv_kernel + ddot_" + dot_kind + "_kernel"
+                                  "kernels": kname + (" + semisep_gemv_scan_kernel + ddot_" + dot_kind + "_kernel"
                                                       if ss_work["gemv"] else ""),
                                   "note": "all contraction kernels of a step (recurrences, shared-beta GEMV and "
                                           "per-pair dots at 2 flop per point and L)"}}

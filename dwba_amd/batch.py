@@ -958,12 +958,17 @@ class ScanBatch:
         return wd, we
 
     def tune(self, tuner=None, steps: int = 3):
-        """Pick the semiseparable kernel's (seg, lb) -- and, for energies with a
-        shared direct beta, whether their direct family goes through the O(N)
-        GEMV + tiled dot -- with the autotuner (cached per device and workload
-        signature), and use them from now on; a captured graph is captured
-        again.  Returns the autotune Choice, or None when the batch has no
-        semiseparable contraction."""
+        """Choose how the FP64 step runs, with the autotuner (cached per device
+        and workload signature), and use it from now on; a captured graph is
+        captured again.  Stage 1 times the contraction alone over every
+        candidate: the wavefront-scan kernel (seg, lb), the lane kernel (lb,
+        whole or split ranges at 2 or 3 waves per SIMD, job order), each with
+        and without the shared-beta direct path (O(N) GEMV + matrix-core dots)
+        where an energy has one direct beta.  Stage 2 times whole steps: the
+        best candidate of each kind x the tail fork point x (lane kernels)
+        one-stream or pipelined.  Returns stage 1's Choice (stage 2's is
+        self.tuning_step), or None when the batch has no semiseparable
+        contraction or DWBA_AUTOTUNE=off."""
         if self.precision != "fp64" or self.contraction != "semiseparable" or not self.plain_count:
             return None
         from . import autotune as AT
@@ -1006,10 +1011,11 @@ class ScanBatch:
         self.plain_ws = torch.empty(max(self.plain_ws_bytes, 1), dtype=torch.uint8, device=self.dev)
         self.tuning = choice
         self.tuning_step = None
-        # Final choice on whole-step time: the fastest contraction with and
-        # without the shared-beta direct path, each with the tail chain
-        # forked before or after prep or not at all (the contraction alone
-        # does not show how much of it the overlapped tails hide).
+        # Final choice on whole-step time: the fastest contraction of each kind
+        # (shared-beta path or not, lane kernel or not, split or not, job
+        # order), each with the tail chain forked before or after prep or not
+        # at all, the lane kernels also pipelined (the contraction alone does
+        # not show how much of it the overlapped stages hide).
         modes = (None, "start", "after_prep") if self.n_tail else (None,)
         finalists = [choice.params]
         kind = lambda q: (int(q.get("gemv", 0)), int(q.get("lanes", 0)), int(q.get("split", 0)), int(q.get("order", 0)))
