@@ -34,6 +34,7 @@ from . import planner
 from .planner import GridStats, PREP_PLAIN, PREP_FILON, PREP_STANDARD, PREP_ZERO
 
 PREP_NO_ZERO_FILL = 0x10   # include/dwba_hip.h: the batch's per-pair vectors are zero-initialised once
+PREP_FILON_GROUPED = 0x20  # include/dwba_hip.h: Filon alpha/beta from the group cell sums instead
 from .dwba_matrix_elements import (PairPlan, RadialDWBAIntegrals, TAIL_NONE, TAIL_OUTER, TAIL_LEGACY, plan_pair,
                                    _fit)
 
@@ -58,6 +59,12 @@ class DgemvEnergy(ctypes.Structure):
 class LaneGrid(ctypes.Structure):
     """dwba_lane_grid of include/dwba_hip.h."""
     _fields_ = [("r", ctypes.c_void_p), ("tab", ctypes.c_void_p), ("n", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+class FilonGroup(ctypes.Structure):
+    """dwba_filon_group of include/dwba_hip.h."""
+    _fields_ = [("coef", ctypes.c_void_p), ("ld_coef", ctypes.c_int32), ("rep", ctypes.c_int32),
+                ("first", ctypes.c_int32), ("count", ctypes.c_int32)]
 
 
 class DdotPair(ctypes.Structure):
@@ -274,6 +281,9 @@ class ScanBatch:
         self.prep_d = hip.struct_array_to_device(prep, dev)
         self._prep_host = prep
         self.prep_d_gemv = None           # prep jobs for the shared-beta direct path (_build_fp64)
+        self._gemv_nobeta = set()         # pairs whose beta_dir that path does not read
+        self._prep_dev = {}               # (shared-beta path, grouped Filon) -> device prep jobs
+        self._build_filon_groups()
         self.coul_d = torch.as_tensor(coul_par.ravel(), device=dev) if any_coul else None
         self.any_coul = any_coul
         self._keep += node_bufs
@@ -421,12 +431,8 @@ class ScanBatch:
         self.plain_d_split = hip.struct_array_to_device(arr_split, self.dev) if self.ss_gemv is not None else None
         if self.ss_gemv is not None:
             # with the shared-beta path only one pair per energy writes beta_dir
-            pj = (hip.PrepJob * self.n_pairs)()
-            for k in range(self.n_pairs):
-                pj[k] = self._prep_host[k]
-                if k in shared_direct and k not in beta_owner:
-                    pj[k].beta_dir = 0
-            self.prep_d_gemv = hip.struct_array_to_device(pj, self.dev)
+            self._gemv_nobeta = set(shared_direct) - beta_owner
+            self.prep_d_gemv = self._prep_jobs(True, False)
         self._job_energy = [self.pairs[k][0] for k in order]     # grid of each job, in job order
         self._dg_energy = dg_energy
         self.pipeline = False             # two-half pipelined FP64 step (tune())
@@ -602,6 +608,61 @@ class ScanBatch:
         self._lane_tabs[lb] = ent
         return ent
 
+    def _build_filon_groups(self):
+        """Filon pairs of one energy with the same split index, phase layout
+        and interpolation kinds share their cell sums: groups for
+        dwba_filon_cells_f64 / dwba_filon_apply_f64 (prep_grouped)."""
+        torch, hip = self.torch, self.hip
+        self.prep_grouped = False
+        self.filon_groups = None
+        self._grouped_pairs = set()
+        groups = {}
+        for k, (ei, pi, plan, *_r) in enumerate(self.pairs):
+            if plan.mode != PREP_FILON or plan.N1 < 2:
+                continue
+            lay = plan.layout
+            key = (ei, plan.N1, plan.log1, plan.log2, lay.p_start, lay.p_step, lay.p_end, lay.n_pnodes,
+                   bool(lay.coulomb), lay.k_total if lay.coulomb else 0.0, lay.eta_total if lay.coulomb else 0.0)
+            groups.setdefault(key, []).append(k)
+        if not groups or len(groups) > 65535:
+            return
+        gl = list(groups.values())
+        lds = [_round_up(self.pairs[g[0]][2].N1, 32) for g in gl]
+        coef = torch.empty(7 * sum(lds), dtype=torch.float64, device=self.dev)
+        arr = (FilonGroup * len(gl))()
+        members, off, first = [], 0, 0
+        for i, (g, ld) in enumerate(zip(gl, lds)):
+            arr[i] = FilonGroup(hip.ptr(coef) + 8 * off, ld, g[0], first, len(g))
+            members += g
+            off += 7 * ld
+            first += len(g)
+        self.filon_groups = (hip.struct_array_to_device(arr, self.dev), len(gl),
+                             torch.as_tensor(np.asarray(members, dtype=np.int32), device=self.dev),
+                             max(self.pairs[g[0]][2].N1 for g in gl), coef)
+        self._grouped_pairs = set(members)
+
+    def _prep_jobs(self, gemv, grouped):
+        """Device prep jobs: with gemv, beta_dir only where the shared-beta
+        path reads it; with grouped, the grouped Filon pairs skip the per-pair
+        cell pass (dwba_filon_apply_f64 writes their alpha / beta_ex)."""
+        key = (bool(gemv), bool(grouped))
+        if key == (False, False):
+            return self.prep_d
+        hit = self._prep_dev.get(key)
+        if hit is not None:
+            return hit
+        hip = self.hip
+        pj = (hip.PrepJob * self.n_pairs)()
+        for k in range(self.n_pairs):
+            pj[k] = self._prep_host[k]
+            if gemv and k in self._gemv_nobeta:
+                pj[k].beta_dir = 0
+            if grouped and k in self._grouped_pairs:
+                pj[k].mode = pj[k].mode | PREP_FILON_GROUPED
+        d = hip.struct_array_to_device(pj, self.dev)
+        self._prep_dev[key] = d
+        return d
+
     def _pipe_plan(self):
         """Split of the energies into two halves of about equal prep work
         (pairs x grid points) for the pipelined step: prep of the second half
@@ -706,7 +767,7 @@ class ScanBatch:
         shared-beta path, the matrix-core dots (per-half job, GEMV and item
         ranges exist for those)."""
         return (self._pipe is not None and self.precision == "fp64" and self.contraction == "semiseparable"
-                and self.semisep_lanes and self.lane_tables and self.L_max <= 31
+                and not self.prep_grouped and self.semisep_lanes and self.lane_tables and self.L_max <= 31
                 and (not self.direct_gemv or self.dot_kind == "mfma"))
 
     def _contract_half(self, half, out, sh):
@@ -793,7 +854,8 @@ class ScanBatch:
         cxp, _ = hip.f64_array(self.cc[0])
         cwp, _ = hip.f64_array(self.cc[1])
         gemv = getattr(self, "direct_gemv", False) and self.prep_d_gemv is not None
-        prep_jobs = self.prep_d_gemv if gemv else self.prep_d
+        grouped = self.prep_grouped and self.filon_groups is not None
+        prep_jobs = self._prep_jobs(gemv, grouped)
 
         def prep(k0, n, h):
             if n <= 0:
@@ -802,6 +864,12 @@ class ScanBatch:
             hip.check(lib.dwba_fullsplit_prep_f64(n, hip.ptr(prep_jobs) + k0 * ctypes.sizeof(hip.PrepJob), coul,
                                                   int(self.any_coul), cxp, cwp, self.CC_nodes, h),
                       "dwba_fullsplit_prep_f64")
+            if grouped:                    # whole batch only (the pipelined step does not group)
+                gd, ng, mem, max_n1, _ = self.filon_groups
+                hip.check(lib.dwba_filon_cells_f64(ng, hip.ptr(gd), hip.ptr(prep_jobs), cxp, cwp, self.CC_nodes, h),
+                          "dwba_filon_cells_f64")
+                hip.check(lib.dwba_filon_apply_f64(ng, hip.ptr(gd), hip.ptr(prep_jobs), hip.ptr(mem), max_n1, h),
+                          "dwba_filon_apply_f64")
 
         # Pipelined step (no stage events): prep of the second half of the
         # energies runs on the launch stream while the first half is
@@ -897,8 +965,9 @@ class ScanBatch:
         in the order the pairs were given."""
         torch, hip, lib = self.torch, self.hip, self.lib
         n = self.n_pairs
-        if getattr(self, "direct_gemv", False) and self.prep_d_gemv is not None:
+        if (getattr(self, "direct_gemv", False) and self.prep_d_gemv is not None) or self.prep_grouped:
             # the shared-beta path leaves beta_dir of most pairs unwritten: write all
+            # (the per-pair prep also recomputes grouped pairs' alpha / beta_ex)
             cxp, _ = hip.f64_array(self.cc[0])
             cwp, _ = hip.f64_array(self.cc[1])
             hip.check(lib.dwba_fullsplit_prep_f64(n, hip.ptr(self.prep_d), hip.ptr(self.coul_d), int(self.any_coul),
@@ -1030,7 +1099,7 @@ class ScanBatch:
                             tails=AT.bucket(self.n_tail), n=AT.bucket(float(np.mean(list(nca)))), L=Lm,
                             c="|".join(f"{f['seg']}.{f['lb']}.{int(f.get('lanes', 0))}.{int(f.get('split', 0))}."
                                        f"{int(f.get('order', 0))}.{int(f.get('gemv', 0))}"
-                                       for f in finalists), v=5)
+                                       for f in finalists), v=6)
 
         def apply(c):
             f = finalists[int(c["c"])]
@@ -1044,6 +1113,7 @@ class ScanBatch:
                 self.plain_ws = torch.empty(max(self.plain_ws_bytes, 1), dtype=torch.uint8, device=self.dev)
             self.overlap = modes[int(c["overlap"])]
             self.pipeline = bool(c.get("pipe", 0))
+            self.prep_grouped = bool(c.get("grp", 0))
 
         def run_step(c):
             apply(c)
@@ -1053,6 +1123,8 @@ class ScanBatch:
         if self._pipe is not None:       # the two-half pipelined step, for the lane-kernel finalists
             cands += [{"c": i, "overlap": j, "pipe": 1} for i, f in enumerate(finalists) if f.get("lanes", 0)
                       for j in range(len(modes))]
+        if self.filon_groups is not None:   # Filon cell sums once per group of pairs
+            cands += [{"c": i, "overlap": j, "grp": 1} for i in range(len(finalists)) for j in range(len(modes))]
         oc = tuner.tune(jkey, cands, run_step, steps=steps)
         apply(oc.params)
         self.tuning_overlap = oc

@@ -136,6 +136,70 @@ constexpr int NODE_BATCH = 4;    // CC nodes evaluated side by side per cell
 // log1p((x - r_j) / r_j) in cells narrower than 1% of r_j (it differs from
 // the rounded difference log x - log r_j by less than that difference's own
 // rounding error).
+// The 2x2 interpolation sums of cell j's CC nodes g_lo .. g_hi - 1 (see
+// filon_alpha_beta below), evaluated NODE_BATCH nodes at a time.
+template <bool COUL>
+__device__ __forceinline__ void cell_sums(const dwba_prep_job& J, const double* s_ccx, const double* s_ccw, int ncc,
+                                          double inv_ncc, bool lg1, bool lg2, int j, int g_lo, int g_hi,
+                                          double& c00, double& c01, double& c11, double& b00, double& b01,
+                                          double& b10, double& b11) {
+    const double* r = J.r;
+    const double* lr = J.logr;
+    const bool need_log = lg1 || lg2;
+    const double rl = r[j], rr = r[j + 1];
+    const double ll = lr[j], lrr = lr[j + 1];
+    const double inv_log = lrr > ll ? 1.0 / (lrr - ll) : 0.0;
+    const double inv_lin = rr > rl ? 1.0 / (rr - rl) : 0.0;
+    const double invr = 1.0 / (rr - rl + 1e-30);
+    const bool ser = rl > 0.0 && (rr - rl) < 0.01 * rl;
+    const double inv_rl = ser ? 1.0 / rl : 0.0;
+    c00 = 0.0; c01 = 0.0; c11 = 0.0; b00 = 0.0; b01 = 0.0; b10 = 0.0; b11 = 0.0;
+    for (int g0 = g_lo; g0 < g_hi; g0 += NODE_BATCH) {
+        double xs[NODE_BATCH], Ws[NODE_BATCH], dX[NODE_BATCH];
+#pragma unroll
+        for (int v = 0; v < NODE_BATCH; ++v) {
+            const int g = min(g0 + v, g_hi - 1);  // a node past the range repeats with weight 0
+            const int m = (int)(((double)g + 0.5) * inv_ncc);
+            const int k = g - m * ncc;
+            double a, b;
+            interval_ends<COUL>(J, m, a, b);
+            const double hw = 0.5 * (b - a);
+            const double W = s_ccw[k] * hw;
+            Ws[v] = (b - a > 1e-12 && g0 + v < g_hi) ? W : 0.0;
+            xs[v] = fmin(fmax(hw * (s_ccx[k] + 1.0) + a, a), b);
+            dX[v] = 0.0;
+        }
+        if (need_log) {
+            if (ser) {
+#pragma unroll
+                for (int v = 0; v < NODE_BATCH; ++v) dX[v] = log1p_small((xs[v] - rl) * inv_rl);
+            } else {
+#pragma unroll
+                for (int v = 0; v < NODE_BATCH; ++v) dX[v] = log(fmax(xs[v], 1e-300)) - ll;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NODE_BATCH; ++v) {
+            const double W = Ws[v];
+            const double dx = xs[v] - rl;
+            const double tl = dX[v] * inv_log, tn = dx * inv_lin;
+            const double t1 = fmin(fmax(lg1 ? tl : tn, 0.0), 1.0);
+            const double t2 = fmin(fmax(lg2 ? tl : tn, 0.0), 1.0);
+            const double wr = dx * invr, wl = 1.0 - wr;
+            const double u1 = 1.0 - t1, u2 = 1.0 - t2;
+            const double Wu = W * u1, Wt = W * t1;
+            c00 = fma(Wu, u1, c00);
+            c01 = fma(Wu, t1, c01);
+            c11 = fma(Wt, t1, c11);
+            const double Wl = W * wl, Wr = W * wr;
+            b00 = fma(Wl, u2, b00);
+            b01 = fma(Wl, t2, b01);
+            b10 = fma(Wr, u2, b10);
+            b11 = fma(Wr, t2, b11);
+        }
+    }
+}
+
 template <bool COUL>
 __device__ void filon_alpha_beta(const dwba_prep_job& J, const CCTable& cc) {
     __shared__ double s_s1[PREP_THREADS][3];
@@ -147,9 +211,7 @@ __device__ void filon_alpha_beta(const dwba_prep_job& J, const CCTable& cc) {
     const int g_end = n_int * ncc;
     const double inv_ncc = 1.0 / ncc;
     const double* r = J.r;
-    const double* lr = J.logr;                    // log(max(r, 1e-300))
     const bool lg1 = J.log1 != 0, lg2 = J.log2 != 0;
-    const bool need_log = lg1 || lg2;
     const bool fill = !(J.mode & DWBA_PREP_NO_ZERO_FILL);
     if (threadIdx.x < DWBA_MAX_CC) {              // per-lane node indices read the table from LDS
         s_ccx[threadIdx.x] = threadIdx.x < ncc ? cc.x[threadIdx.x] : 0.0;
@@ -169,58 +231,8 @@ __device__ void filon_alpha_beta(const dwba_prep_job& J, const CCTable& cc) {
             const int g_hi = last ? g_end
                                   : (threadIdx.x + 1 < PREP_THREADS ? s_glo[threadIdx.x + 1]
                                                                     : first_node_ge<COUL>(J, s_ccx, ncc, n_int, r[j + 1]));
-            const double rl = r[j], rr = r[j + 1];
-            const double ll = lr[j], lrr = lr[j + 1];
-            const double inv_log = lrr > ll ? 1.0 / (lrr - ll) : 0.0;
-            const double inv_lin = rr > rl ? 1.0 / (rr - rl) : 0.0;
-            const double invr = 1.0 / (rr - rl + 1e-30);
-            const bool ser = rl > 0.0 && (rr - rl) < 0.01 * rl;
-            const double inv_rl = ser ? 1.0 / rl : 0.0;
-            double c00 = 0.0, c01 = 0.0, c11 = 0.0, b00 = 0.0, b01 = 0.0, b10 = 0.0, b11 = 0.0;
-            for (int g0 = g_lo; g0 < g_hi; g0 += NODE_BATCH) {
-                double xs[NODE_BATCH], Ws[NODE_BATCH], dX[NODE_BATCH];
-#pragma unroll
-                for (int v = 0; v < NODE_BATCH; ++v) {
-                    const int g = min(g0 + v, g_hi - 1);  // a node past the range repeats with weight 0
-                    const int m = (int)(((double)g + 0.5) * inv_ncc);
-                    const int k = g - m * ncc;
-                    double a, b;
-                    interval_ends<COUL>(J, m, a, b);
-                    const double hw = 0.5 * (b - a);
-                    const double W = s_ccw[k] * hw;
-                    Ws[v] = (b - a > 1e-12 && g0 + v < g_hi) ? W : 0.0;
-                    xs[v] = fmin(fmax(hw * (s_ccx[k] + 1.0) + a, a), b);
-                    dX[v] = 0.0;
-                }
-                if (need_log) {
-                    if (ser) {
-#pragma unroll
-                        for (int v = 0; v < NODE_BATCH; ++v) dX[v] = log1p_small((xs[v] - rl) * inv_rl);
-                    } else {
-#pragma unroll
-                        for (int v = 0; v < NODE_BATCH; ++v) dX[v] = log(fmax(xs[v], 1e-300)) - ll;
-                    }
-                }
-#pragma unroll
-                for (int v = 0; v < NODE_BATCH; ++v) {
-                    const double W = Ws[v];
-                    const double dx = xs[v] - rl;
-                    const double tl = dX[v] * inv_log, tn = dx * inv_lin;
-                    const double t1 = fmin(fmax(lg1 ? tl : tn, 0.0), 1.0);
-                    const double t2 = fmin(fmax(lg2 ? tl : tn, 0.0), 1.0);
-                    const double wr = dx * invr, wl = 1.0 - wr;
-                    const double u1 = 1.0 - t1, u2 = 1.0 - t2;
-                    const double Wu = W * u1, Wt = W * t1;
-                    c00 = fma(Wu, u1, c00);
-                    c01 = fma(Wu, t1, c01);
-                    c11 = fma(Wt, t1, c11);
-                    const double Wl = W * wl, Wr = W * wr;
-                    b00 = fma(Wl, u2, b00);
-                    b01 = fma(Wl, t2, b01);
-                    b10 = fma(Wr, u2, b10);
-                    b11 = fma(Wr, t2, b11);
-                }
-            }
+            double c00, c01, c11, b00, b01, b10, b11;
+            cell_sums<COUL>(J, s_ccx, s_ccw, ncc, inv_ncc, lg1, lg2, j, g_lo, g_hi, c00, c01, c11, b00, b01, b10, b11);
             const double ci0 = J.chi_i[j], ci1 = J.chi_i[j + 1];
             const double d0 = clean(J.chi_f[j] * ci0), d1 = clean(J.chi_f[j + 1] * ci1);
             const double e0 = clean(J.u_f[j] * ci0), e1 = clean(J.u_f[j + 1] * ci1);
@@ -309,11 +321,109 @@ __global__ __launch_bounds__(PREP_THREADS) void prep_f64_kernel(const dwba_prep_
         for (int wv = 0; wv < PREP_THREADS / 64; ++wv) sum += s_red[wv][threadIdx.x];
         J.scalars[threadIdx.x] = sum;
     }
-    if (mode != DWBA_PREP_FILON) return;
+    if (mode != DWBA_PREP_FILON || (J.mode & DWBA_PREP_FILON_GROUPED)) return;
 
     // FILON
     if (J.phase_nodes) filon_alpha_beta<true>(J, cc);
     else filon_alpha_beta<false>(J, cc);
+}
+
+// Cell sums of a group of FILON jobs sharing their geometry: one workgroup
+// per group, the cell pass of filon_alpha_beta without densities; the 7
+// sums of cell j go to coef[q * ld_coef + j].
+template <bool COUL>
+__device__ void filon_cells(const dwba_filon_group& G, const dwba_prep_job& J, const CCTable& cc) {
+    __shared__ double s_ccx[DWBA_MAX_CC], s_ccw[DWBA_MAX_CC];
+    __shared__ int s_glo[PREP_THREADS];
+    const int N1 = J.N1;
+    const int n_int = J.n_pnodes - 1;
+    const int ncc = cc.n;
+    const int g_end = n_int * ncc;
+    const double inv_ncc = 1.0 / ncc;
+    const double* r = J.r;
+    const bool lg1 = J.log1 != 0, lg2 = J.log2 != 0;
+    if (threadIdx.x < DWBA_MAX_CC) {
+        s_ccx[threadIdx.x] = threadIdx.x < ncc ? cc.x[threadIdx.x] : 0.0;
+        s_ccw[threadIdx.x] = threadIdx.x < ncc ? cc.w[threadIdx.x] : 0.0;
+    }
+    __syncthreads();
+    double* __restrict__ co = G.coef;
+    const size_t ld = (size_t)G.ld_coef;
+    for (int base = 0; base < N1; base += PREP_THREADS) {
+        const int j = base + (int)threadIdx.x;
+        const bool cell = j < N1 - 1;
+        const bool last = j == N1 - 2;
+        const int g_lo = !cell ? g_end : (j == 0 ? 0 : first_node_ge<COUL>(J, s_ccx, ncc, n_int, r[j]));
+        s_glo[threadIdx.x] = g_lo;
+        __syncthreads();
+        if (cell) {
+            const int g_hi = last ? g_end
+                                  : (threadIdx.x + 1 < PREP_THREADS ? s_glo[threadIdx.x + 1]
+                                                                    : first_node_ge<COUL>(J, s_ccx, ncc, n_int, r[j + 1]));
+            double c00, c01, c11, b00, b01, b10, b11;
+            cell_sums<COUL>(J, s_ccx, s_ccw, ncc, inv_ncc, lg1, lg2, j, g_lo, g_hi, c00, c01, c11, b00, b01, b10, b11);
+            co[j] = c00;
+            co[ld + j] = c01;
+            co[2 * ld + j] = c11;
+            co[3 * ld + j] = b00;
+            co[4 * ld + j] = b01;
+            co[5 * ld + j] = b10;
+            co[6 * ld + j] = b11;
+        }
+        __syncthreads();                          // s_glo is rewritten by the next chunk
+    }
+}
+
+__global__ __launch_bounds__(PREP_THREADS) void filon_cells_kernel(const dwba_filon_group* __restrict__ groups,
+                                                                   const dwba_prep_job* __restrict__ jobs,
+                                                                   CCTable cc) {
+    const dwba_filon_group G = groups[blockIdx.x];
+    const dwba_prep_job J = jobs[G.rep];
+    if (J.phase_nodes) filon_cells<true>(G, J, cc);
+    else filon_cells<false>(G, J, cc);
+}
+
+// alpha_dir, alpha_ex, beta_ex of every member of a group: one thread per
+// point j < N1, the group's cell sums loaded once and applied to each member
+// (point j: cell j's part c00 rho_j + c01 rho_j+1 plus cell j-1's part
+// c01 rho_j-1 + c11 rho_j, the same sums filon_alpha_beta forms).
+__global__ __launch_bounds__(PREP_THREADS) void filon_apply_kernel(const dwba_filon_group* __restrict__ groups,
+                                                                   const dwba_prep_job* __restrict__ jobs,
+                                                                   const int* __restrict__ members) {
+    const dwba_filon_group G = groups[blockIdx.y];
+    const int N1 = jobs[G.rep].N1;
+    const int j = blockIdx.x * PREP_THREADS + (int)threadIdx.x;
+    if (j >= N1) return;
+    const size_t ld = (size_t)G.ld_coef;
+    const double* __restrict__ co = G.coef;
+    const bool here = j < N1 - 1, prev = j >= 1;
+    const double c00 = here ? co[j] : 0.0, c01 = here ? co[ld + j] : 0.0;
+    const double p01 = prev ? co[ld + j - 1] : 0.0, p11 = prev ? co[2 * ld + j - 1] : 0.0;
+    const double b00 = here ? co[3 * ld + j] : 0.0, b01 = here ? co[4 * ld + j] : 0.0;
+    const double q10 = prev ? co[5 * ld + j - 1] : 0.0, q11 = prev ? co[6 * ld + j - 1] : 0.0;
+    for (int m = 0; m < G.count; ++m) {
+        const dwba_prep_job J = jobs[members[G.first + m]];
+        const double ci = J.chi_i[j], cf = J.chi_f[j], uf = J.u_f[j], ui = J.u_i[j];
+        const double d = clean(cf * ci), e = clean(uf * ci), f = clean(cf * ui);
+        double s0d = 0.0, s0e = 0.0, s0b = 0.0, s1d = 0.0, s1e = 0.0, s1b = 0.0;
+        if (here) {
+            const double ci1 = J.chi_i[j + 1], cf1 = J.chi_f[j + 1];
+            const double d1 = clean(cf1 * ci1), e1 = clean(J.u_f[j + 1] * ci1), f1 = clean(cf1 * J.u_i[j + 1]);
+            s0d = c00 * d + c01 * d1;
+            s0e = c00 * e + c01 * e1;
+            s0b = b00 * f + b01 * f1;
+        }
+        if (prev) {
+            const double ci0 = J.chi_i[j - 1], cf0 = J.chi_f[j - 1];
+            const double d0 = clean(cf0 * ci0), e0 = clean(J.u_f[j - 1] * ci0), f0 = clean(cf0 * J.u_i[j - 1]);
+            s1d = p01 * d0 + p11 * d;
+            s1e = p01 * e0 + p11 * e;
+            s1b = q10 * f0 + q11 * f;
+        }
+        J.alpha_dir[j] = s0d + s1d;
+        J.alpha_ex[j] = s0e + s1e;
+        J.beta_ex[j] = s0b + s1b;
+    }
 }
 
 // Multipole moments: block (set, L) reduces sum_j ((w_j r_j^L) u_f,j) u_i,j.
@@ -373,16 +483,22 @@ __global__ void coulomb_nodes_kernel(dwba_prep_job* __restrict__ jobs, int n_job
 
 }  // namespace
 
-int launch_prep(int n_jobs, dwba_prep_job* d_jobs, const double* d_coulomb_par, int any_coulomb,
-                const double* cc_x, const double* cc_w, int n_cc, hipStream_t s) {
-    CCTable cc{};
-    if (n_cc < 2 || n_cc > DWBA_MAX_CC) return set_error(DWBA_ERR_ARG, "prep: CC node count %d outside [2, %d]", n_cc, DWBA_MAX_CC);
-    // nodes ascending in x (the FILON gather relies on it; the sum order is free)
+// CC table with nodes ascending in x (the FILON gather relies on it; the sum order is free).
+bool make_cc(const double* cc_x, const double* cc_w, int n_cc, CCTable* cc) {
+    if (n_cc < 2 || n_cc > DWBA_MAX_CC) return false;
     int ord[DWBA_MAX_CC];
     for (int k = 0; k < n_cc; ++k) ord[k] = k;
     std::sort(ord, ord + n_cc, [&](int p, int q) { return cc_x[p] < cc_x[q]; });
-    for (int k = 0; k < n_cc; ++k) { cc.x[k] = cc_x[ord[k]]; cc.w[k] = cc_w[ord[k]]; }
-    cc.n = n_cc;
+    for (int k = 0; k < n_cc; ++k) { cc->x[k] = cc_x[ord[k]]; cc->w[k] = cc_w[ord[k]]; }
+    cc->n = n_cc;
+    return true;
+}
+
+int launch_prep(int n_jobs, dwba_prep_job* d_jobs, const double* d_coulomb_par, int any_coulomb,
+                const double* cc_x, const double* cc_w, int n_cc, hipStream_t s) {
+    CCTable cc{};
+    if (!make_cc(cc_x, cc_w, n_cc, &cc))
+        return set_error(DWBA_ERR_ARG, "prep: CC node count %d outside [2, %d]", n_cc, DWBA_MAX_CC);
     if (any_coulomb) {
         hipLaunchKernelGGL(coulomb_nodes_kernel, dim3((n_jobs + 63) / 64), dim3(64), 0, s, d_jobs, n_jobs, d_coulomb_par);
         DWBA_LAUNCH_CHECK("coulomb_nodes_kernel");
@@ -414,4 +530,31 @@ extern "C" int32_t dwba_fullsplit_prep_f64(int32_t n_jobs, dwba_prep_job* d_jobs
     DWBA_REQUIRE(!any_coulomb || d_coulomb_par, "dwba_fullsplit_prep_f64: Coulomb jobs need d_coulomb_par");
     return dwba::launch_prep(n_jobs, d_jobs, d_coulomb_par, any_coulomb, h_cc_x, h_cc_w, n_cc,
                              reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int32_t dwba_filon_cells_f64(int32_t n_groups, const dwba_filon_group* d_groups,
+                                       const dwba_prep_job* d_jobs, const double* h_cc_x, const double* h_cc_w,
+                                       int32_t n_cc, void* stream) {
+    DWBA_REQUIRE(dwba::initialised(), "dwba_filon_cells_f64: call dwba_init first");
+    DWBA_REQUIRE(n_groups > 0 && d_groups && d_jobs && h_cc_x && h_cc_w, "dwba_filon_cells_f64: bad arguments");
+    dwba::CCTable cc{};
+    DWBA_REQUIRE(dwba::make_cc(h_cc_x, h_cc_w, n_cc, &cc), "dwba_filon_cells_f64: CC node count %d outside [2, %d]",
+                 n_cc, DWBA_MAX_CC);
+    hipLaunchKernelGGL(dwba::filon_cells_kernel, dim3(n_groups), dim3(dwba::PREP_THREADS), 0,
+                       reinterpret_cast<hipStream_t>(stream), d_groups, d_jobs, cc);
+    DWBA_LAUNCH_CHECK("filon_cells_kernel");
+    return DWBA_OK;
+}
+
+extern "C" int32_t dwba_filon_apply_f64(int32_t n_groups, const dwba_filon_group* d_groups,
+                                       const dwba_prep_job* d_jobs, const int32_t* d_members, int32_t max_n1,
+                                       void* stream) {
+    DWBA_REQUIRE(dwba::initialised(), "dwba_filon_apply_f64: call dwba_init first");
+    DWBA_REQUIRE(n_groups > 0 && n_groups <= 65535 && d_groups && d_jobs && d_members && max_n1 > 0,
+                 "dwba_filon_apply_f64: bad arguments (n_groups=%d, max_n1=%d)", n_groups, max_n1);
+    hipLaunchKernelGGL(dwba::filon_apply_kernel, dim3((max_n1 + dwba::PREP_THREADS - 1) / dwba::PREP_THREADS, n_groups),
+                       dim3(dwba::PREP_THREADS), 0, reinterpret_cast<hipStream_t>(stream), d_groups, d_jobs,
+                       d_members);
+    DWBA_LAUNCH_CHECK("filon_apply_kernel");
+    return DWBA_OK;
 }

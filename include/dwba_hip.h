@@ -179,6 +179,8 @@ enum {
  * and prepare the same pair again (a captured scan): the untouched points are
  * the same on every call and keep their zeros. */
 #define DWBA_PREP_NO_ZERO_FILL 0x10
+#define DWBA_PREP_FILON_GROUPED 0x20   /* FILON: streaming pass only; alpha_dir, alpha_ex, beta_ex come from
+                                          dwba_filon_cells_f64 + dwba_filon_apply_f64 */
 #define DWBA_PREP_MODE_MASK 0x0f
 
 /* One channel pair.  Device pointers; outputs are the alpha and beta vectors (N doubles
@@ -224,6 +226,34 @@ typedef struct dwba_prep_job {
 int32_t dwba_fullsplit_prep_f64(int32_t n_jobs, dwba_prep_job* d_jobs, const double* d_coulomb_par,
                                 int32_t any_coulomb, const double* h_cc_x, const double* h_cc_w,
                                 int32_t n_cc, void* stream);
+
+/* Filon cell coefficients shared by a group of FILON jobs with the same grid,
+ * split index N1, phase layout and interpolation kinds (pairs of one energy
+ * that differ only in their densities).  For cell j < N1 - 1 the CC nodes in
+ * [r_j, r_j+1) contribute to alpha at points j, j+1 through the 2x2 sums
+ * c00, c01, c11 (of W (1-t)^2, W t (1-t), W t^2) and to beta_ex through
+ * b00, b01, b10, b11 (of W wl (1-t), W wl t, W wr (1-t), W wr t), exactly as
+ * dwba_fullsplit_prep_f64 forms them per pair.  coef holds these 7 rows at
+ * stride ld_coef (>= N1); rep is the index in d_jobs of one member (its
+ * phase nodes, for Coulomb stepping, must already be generated, i.e. the
+ * prep launch with DWBA_PREP_FILON_GROUPED jobs runs first); members[first ..
+ * first + count) are the jobs the apply step writes.  Replaces the per-pair
+ * node interpolations of oscillatory_kernel_integral_2d
+ * (oscillatory_integrals.py:2578-2644, :2667-2770). */
+typedef struct dwba_filon_group {
+    double* coef;
+    int32_t ld_coef;
+    int32_t rep;
+    int32_t first;
+    int32_t count;
+} dwba_filon_group;
+int32_t dwba_filon_cells_f64(int32_t n_groups, const dwba_filon_group* d_groups, const dwba_prep_job* d_jobs,
+                             const double* h_cc_x, const double* h_cc_w, int32_t n_cc, void* stream);
+/* alpha_dir, alpha_ex, beta_ex of every member job from its group's cells:
+ * point j takes cell j's part (c00 rho_j + c01 rho_j+1) and cell j-1's part
+ * (c01 rho_j-1 + c11 rho_j); max_n1 bounds every group's N1. */
+int32_t dwba_filon_apply_f64(int32_t n_groups, const dwba_filon_group* d_groups, const dwba_prep_job* d_jobs,
+                             const int32_t* d_members, int32_t max_n1, void* stream);
 
 /* Multipole moments M_L = sum_j ((w_j r_j^L) u_f,j) u_i,j for L = 0..L_max of
  * n_sets density pairs (device pointer arrays d_r/d_w/d_u_i/d_u_f of length

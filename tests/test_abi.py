@@ -52,7 +52,7 @@ def test_struct_layouts_match_header(tmp_path):
         "dwba_plain_job": _hip.PlainJob, "dwba_prep_job": _hip.PrepJob, "dwba_tail_job": _hip.TailJob,
         "dwba_xgemm_energy": batch.XgemmEnergy, "dwba_dgemv_energy": batch.DgemvEnergy,
         "dwba_ddot_pair": batch.DdotPair, "dwba_finalize_pair": batch.FinalizePair,
-        "dwba_lane_grid": batch.LaneGrid,
+        "dwba_lane_grid": batch.LaneGrid, "dwba_filon_group": batch.FilonGroup,
     }
     src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, cls in structs.items():

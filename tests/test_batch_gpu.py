@@ -302,3 +302,35 @@ def test_pipelined_step_bit_identical(gemv):
         torch.cuda.synchronize()
         assert np.array_equal(b.results(), ref), rep
         _ = (torch.ones(1000, device="cuda") * rep).sum().item()
+
+
+@pytest.mark.parametrize("ionization", [False, True])
+def test_grouped_filon_prep_matches_per_pair(ionization):
+    """Filon cell sums formed once per group of pairs (same energy, split
+    index and layout) and applied to each pair give the per-pair prep's
+    alpha / beta to rounding, hence the same integrals; the stacked vectors
+    are cleared first so every value must be rewritten."""
+    from dwba_amd.batch import ScanBatch
+    scan = _small_scan(10, n_e=3, n_p=30, ionization=ionization)
+    b = ScanBatch(scan, 12, precision="fp64", oscillatory_method="full_split")
+    assert b.filon_groups is not None
+    ng, n_members = b.filon_groups[1], len(b._grouped_pairs)
+    print(f"\n[grouped prep] {ng} groups for {n_members} Filon pairs", flush=True)
+    assert ng < n_members
+    b.run()
+    r0 = b.results()
+    mag = b.magnitudes()
+    vecs0 = [[m.clone() for m in (st.ad, st.ae, st.be)] for st in b.E]
+    b.prep_grouped = True
+    for st in b.E:
+        for m in (st.ad, st.ae, st.be):
+            m.zero_()
+    b.out.zero_()
+    b.run()
+    r1 = b.results()
+    for st, v0 in zip(b.E, vecs0):
+        for m, m0 in zip((st.ad, st.ae, st.be), v0):
+            a, c = m.cpu().numpy(), m0.cpu().numpy()
+            tol = 1e-12 * np.abs(c) + 1e-15 * np.max(np.abs(c)) + 1e-300
+            assert np.all(np.abs(a - c) <= tol), np.max(np.abs(a - c) / tol)
+    assert np.all(np.abs(r1 - r0) <= 1e-12 * np.abs(r0) + 1e-14 * mag)
