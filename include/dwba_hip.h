@@ -157,7 +157,7 @@ int32_t dwba_il_semisep_lanes_tab_f64(int32_t n_jobs, const dwba_plain_job* d_jo
  * shorter dependency chain per wavefront for 1.3x the operations; same
  * arguments and tables, same values up to rounding order.  waves_per_simd
  * (0 = 2, or 3): the occupancy the kernel is compiled for (3 caps it at
- * 168 VGPRs). */
+ * 168 VGPRs).  Replaces the same reference code as dwba_il_semisep_f64. */
 int32_t dwba_il_semisep_lanes_split_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const double* const* d_tabs,
                                         const int32_t* d_out_index, int32_t L_max, int32_t lb,
                                         int32_t waves_per_simd, double* d_out, void* stream);
@@ -251,7 +251,9 @@ int32_t dwba_filon_cells_f64(int32_t n_groups, const dwba_filon_group* d_groups,
                              const double* h_cc_x, const double* h_cc_w, int32_t n_cc, void* stream);
 /* alpha_dir, alpha_ex, beta_ex of every member job from its group's cells:
  * point j takes cell j's part (c00 rho_j + c01 rho_j+1) and cell j-1's part
- * (c01 rho_j-1 + c11 rho_j); max_n1 bounds every group's N1. */
+ * (c01 rho_j-1 + c11 rho_j); max_n1 bounds every group's N1.  With
+ * dwba_filon_cells_f64 it replaces the same reference code (the density
+ * interpolations at the nodes, oscillatory_integrals.py:2578-2644, :2667-2770). */
 int32_t dwba_filon_apply_f64(int32_t n_groups, const dwba_filon_group* d_groups, const dwba_prep_job* d_jobs,
                              const int32_t* d_members, int32_t max_n1, void* stream);
 
