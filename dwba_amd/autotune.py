@@ -104,10 +104,12 @@ class Autotuner:
     # ------------------------------------------------------------ search
     def tune(self, key: str, candidates: Sequence[Dict[str, int]], run: Callable[[Dict[str, int]], None],
              timer: Optional[Callable[[Callable[[], None], int], float]] = None, steps: int = 3,
-             warmup: int = 1) -> Choice:
+             warmup: int = 1, margin: float = 0.0) -> Choice:
         """Fastest candidate for `key` (from the cache when present).
         timer(fn, steps) -> ms per call; defaults to HIP events on torch's
-        current stream."""
+        current stream.  margin > 0: the earliest candidate (in the given
+        order) within (1 + margin) of the fastest wins, so timing noise does
+        not displace a preferred first candidate."""
         hit = self.lookup(key)
         if hit is not None:
             return hit
@@ -124,6 +126,11 @@ class Autotuner:
             timings.append({"params": dict(cand), "ms": ms})
             if best is None or ms < best[1]:
                 best = (dict(cand), ms)
+        if margin > 0:
+            for t in timings:
+                if t["ms"] <= best[1] * (1.0 + margin):
+                    best = (dict(t["params"]), t["ms"])
+                    break
         choice = Choice(params=best[0], ms=best[1], timings=timings)
         self.store(key, choice)
         return choice

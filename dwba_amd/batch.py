@@ -35,6 +35,7 @@ from .planner import GridStats, PREP_PLAIN, PREP_FILON, PREP_STANDARD, PREP_ZERO
 
 PREP_NO_ZERO_FILL = 0x10   # include/dwba_hip.h: the batch's per-pair vectors are zero-initialised once
 PREP_FILON_GROUPED = 0x20  # include/dwba_hip.h: Filon alpha/beta from the group cell sums instead
+GROUP_SLICE = 16           # members per workgroup of the grouped Filon apply pass
 from .dwba_matrix_elements import (PairPlan, RadialDWBAIntegrals, TAIL_NONE, TAIL_OUTER, TAIL_LEGACY, plan_pair,
                                    _fit)
 
@@ -624,7 +625,8 @@ class ScanBatch:
             key = (ei, plan.N1, plan.log1, plan.log2, lay.p_start, lay.p_step, lay.p_end, lay.n_pnodes,
                    bool(lay.coulomb), lay.k_total if lay.coulomb else 0.0, lay.eta_total if lay.coulomb else 0.0)
             groups.setdefault(key, []).append(k)
-        if not groups or len(groups) > 65535:
+        n_slices = sum((len(g) + GROUP_SLICE - 1) // GROUP_SLICE for g in groups.values())
+        if not groups or n_slices > 65535:
             return
         gl = list(groups.values())
         lds = [_round_up(self.pairs[g[0]][2].N1, 32) for g in gl]
@@ -636,9 +638,18 @@ class ScanBatch:
             members += g
             off += 7 * ld
             first += len(g)
+        # the apply pass takes slices of <= GROUP_SLICE members of a group
+        # (same cell sums): more workgroups, shorter per-thread member loops
+        sl = []
+        for i in range(len(gl)):
+            G = arr[i]
+            for q in range(0, G.count, GROUP_SLICE):
+                sl.append(FilonGroup(G.coef, G.ld_coef, G.rep, G.first + q, min(GROUP_SLICE, G.count - q)))
+        slices = (FilonGroup * len(sl))(*sl)
         self.filon_groups = (hip.struct_array_to_device(arr, self.dev), len(gl),
                              torch.as_tensor(np.asarray(members, dtype=np.int32), device=self.dev),
-                             max(self.pairs[g[0]][2].N1 for g in gl), coef)
+                             max(self.pairs[g[0]][2].N1 for g in gl), coef,
+                             hip.struct_array_to_device(slices, self.dev), len(sl))
         self._grouped_pairs = set(members)
 
     def _prep_jobs(self, gemv, grouped):
@@ -865,10 +876,10 @@ class ScanBatch:
                                                   int(self.any_coul), cxp, cwp, self.CC_nodes, h),
                       "dwba_fullsplit_prep_f64")
             if grouped:                    # whole batch only (the pipelined step does not group)
-                gd, ng, mem, max_n1, _ = self.filon_groups
+                gd, ng, mem, max_n1, _, sd, ns = self.filon_groups
                 hip.check(lib.dwba_filon_cells_f64(ng, hip.ptr(gd), hip.ptr(prep_jobs), cxp, cwp, self.CC_nodes, h),
                           "dwba_filon_cells_f64")
-                hip.check(lib.dwba_filon_apply_f64(ng, hip.ptr(gd), hip.ptr(prep_jobs), hip.ptr(mem), max_n1, h),
+                hip.check(lib.dwba_filon_apply_f64(ns, hip.ptr(sd), hip.ptr(prep_jobs), hip.ptr(mem), max_n1, h),
                           "dwba_filon_apply_f64")
 
         # Pipelined step (no stage events): prep of the second half of the
@@ -1099,7 +1110,7 @@ class ScanBatch:
                             tails=AT.bucket(self.n_tail), n=AT.bucket(float(np.mean(list(nca)))), L=Lm,
                             c="|".join(f"{f['seg']}.{f['lb']}.{int(f.get('lanes', 0))}.{int(f.get('split', 0))}."
                                        f"{int(f.get('order', 0))}.{int(f.get('gemv', 0))}"
-                                       for f in finalists), v=6)
+                                       for f in finalists), v=8)
 
         def apply(c):
             f = finalists[int(c["c"])]
@@ -1125,7 +1136,8 @@ class ScanBatch:
                       for j in range(len(modes))]
         if self.filon_groups is not None:   # Filon cell sums once per group of pairs
             cands += [{"c": i, "overlap": j, "grp": 1} for i in range(len(finalists)) for j in range(len(modes))]
-        oc = tuner.tune(jkey, cands, run_step, steps=steps)
+        # stage 1's winner comes first; another layout must beat it by 2 %
+        oc = tuner.tune(jkey, cands, run_step, steps=max(steps, 5), margin=0.02)
         apply(oc.params)
         self.tuning_overlap = oc
         self.tuning_step = {"finalists": finalists, "modes": [str(m) for m in modes], "choice": oc.params,

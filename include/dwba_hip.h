@@ -249,7 +249,9 @@ typedef struct dwba_filon_group {
 } dwba_filon_group;
 int32_t dwba_filon_cells_f64(int32_t n_groups, const dwba_filon_group* d_groups, const dwba_prep_job* d_jobs,
                              const double* h_cc_x, const double* h_cc_w, int32_t n_cc, void* stream);
-/* alpha_dir, alpha_ex, beta_ex of every member job from its group's cells:
+/* alpha_dir, alpha_ex, beta_ex of every member job from its group's cells
+ * (d_groups may hold slices of groups: entries with the same coef and rep
+ * and disjoint member ranges, one workgroup row each):
  * point j takes cell j's part (c00 rho_j + c01 rho_j+1) and cell j-1's part
  * (c01 rho_j-1 + c11 rho_j); max_n1 bounds every group's N1.  With
  * dwba_filon_cells_f64 it replaces the same reference code (the density

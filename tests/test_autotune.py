@@ -61,3 +61,19 @@ def test_autotune_off_keeps_defaults(monkeypatch):
     t = AT.Autotuner(persist=False)
     t.store("k", AT.Choice(params={"seg": 1}, ms=1.0, timings=[]))
     assert t.lookup("k") is None
+
+
+def test_autotune_margin_prefers_earlier_candidate(tmp_path):
+    """With a margin, the first candidate within it of the fastest wins."""
+    from dwba_amd import autotune as AT
+    t = AT.Autotuner(path=str(tmp_path / "t.json"))
+    times = {0: 10.0, 1: 9.9, 2: 9.0}
+    timer = lambda fn, steps: (fn(), times[cur[0]])[1]
+    cur = [0]
+    run = lambda c: cur.__setitem__(0, c["i"])
+    c = t.tune("k1", [{"i": 0}, {"i": 1}], run, timer=timer, warmup=0, margin=0.02)
+    assert c.params == {"i": 0}            # 10.0 is within 2 % of 9.9
+    c = t.tune("k2", [{"i": 0}, {"i": 2}], run, timer=timer, warmup=0, margin=0.02)
+    assert c.params == {"i": 2}            # 9.0 beats 10.0 by more than 2 %
+    c = t.tune("k3", [{"i": 0}, {"i": 1}], run, timer=timer, warmup=0)
+    assert c.params == {"i": 1}            # no margin: the fastest
