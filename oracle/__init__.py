@@ -42,6 +42,6 @@ How each function is pinned (checks live in tests/test_oracle_*.py):
                            envelopes with linear phase (textbook Filon)
   tail.levin_segment       exactness on f = p' + i w p (Levin collocation)
   tail.outer_integral,     golden vectors written by tests/golden/make_golden.py
-  radial (composite C1)    (imports the reference in the survey container
-                           only, CPU path)
+  radial (composite, H     (which imports the reference's CPU modules where they
+  1s->2p at 50 eV)         are importable; the fixtures travel, the reference does not)
 """
