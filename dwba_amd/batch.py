@@ -617,18 +617,10 @@ class ScanBatch:
         self.prep_grouped = False
         self.filon_groups = None
         self._grouped_pairs = set()
-        groups = {}
-        for k, (ei, pi, plan, *_r) in enumerate(self.pairs):
-            if plan.mode != PREP_FILON or plan.N1 < 2:
-                continue
-            lay = plan.layout
-            key = (ei, plan.N1, plan.log1, plan.log2, lay.p_start, lay.p_step, lay.p_end, lay.n_pnodes,
-                   bool(lay.coulomb), lay.k_total if lay.coulomb else 0.0, lay.eta_total if lay.coulomb else 0.0)
-            groups.setdefault(key, []).append(k)
-        n_slices = sum((len(g) + GROUP_SLICE - 1) // GROUP_SLICE for g in groups.values())
-        if not groups or n_slices > 65535:
+        gl = planner.filon_groups((ei, plan) for (ei, _, plan, *_r) in self.pairs)
+        n_slices = sum((len(g) + GROUP_SLICE - 1) // GROUP_SLICE for g in gl)
+        if not gl or n_slices > 65535:
             return
-        gl = list(groups.values())
         lds = [_round_up(self.pairs[g[0]][2].N1, 32) for g in gl]
         coef = torch.empty(7 * sum(lds), dtype=torch.float64, device=self.dev)
         arr = (FilonGroup * len(gl))()

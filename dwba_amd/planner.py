@@ -275,3 +275,21 @@ def filon_layout(gs: GridStats, n1: int, k_total: float, phase_inc: float, eta_t
     if not (b - a) > 1e-12:
         return FilonLayout(PREP_STANDARD)
     return FilonLayout(PREP_FILON, a, dphi, b, 0, True, k_total, eta_total)
+
+
+def filon_groups(plans) -> list:
+    """Groups of FILON pairs whose node geometry is the same: one energy
+    (same grid), split index N1, phase layout and interpolation kinds.
+    plans: iterable of (energy index, PairPlan-like with mode, N1, log1, log2,
+    layout).  Returns lists of positions in `plans`, in first-seen order;
+    pairs in other modes or with N1 < 2 belong to no group.  The cell sums of
+    the Filon alpha/beta depend only on these fields, not on the densities."""
+    groups = {}
+    for pos, (ei, plan) in enumerate(plans):
+        if plan.mode != PREP_FILON or plan.N1 < 2:
+            continue
+        lay = plan.layout
+        key = (ei, plan.N1, plan.log1, plan.log2, lay.p_start, lay.p_step, lay.p_end, lay.n_pnodes,
+               bool(lay.coulomb), lay.k_total if lay.coulomb else 0.0, lay.eta_total if lay.coulomb else 0.0)
+        groups.setdefault(key, []).append(pos)
+    return list(groups.values())

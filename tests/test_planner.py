@@ -88,3 +88,37 @@ def test_filon_limits_env(monkeypatch):
     seg, d = planner.filon_limits()
     assert seg == 128 and d == np.pi / 16
     assert T.filon_max_segments() == seg and T.filon_max_effective_dphi() == d
+
+
+def test_filon_groups_share_geometry():
+    """Grouping of Filon pairs for the shared cell sums: every member of a
+    group has the same energy, split index, layout and interpolation kinds;
+    no pair is in two groups; non-Filon pairs are in none; on a benchmark
+    energy the ~400 pairs fall into a few dozen groups."""
+    import numpy as np
+    from dwba_amd import synthetic as S, planner as P
+    from dwba_amd.dwba_matrix_elements import plan_pair, _fit
+    scan = S.excitation_scan(n_energies=2, n_pairs=120, n_range=(4000, 6000), l_max=40, k_range=(0.5, 8.0), seed=3)
+    plans = []
+    for ei, en in enumerate(scan):
+        r = np.asarray(en.grid.r)
+        gs = P.GridStats.of(r, en.grid.w_trapz)
+        N = len(r)
+        Ui, ui, uf = _fit("U_i", en.U_i, N), _fit("u", en.bound_i.u_of_r, N), _fit("u", en.bound_f.u_of_r, N)
+        ext = {}
+        for pr in en.pairs:
+            plans.append((ei, plan_pair(gs, Ui, None, ui, uf, pr[0], pr[1], 30, True, "full_split", 1.5708, 0.1, 0.5,
+                                        True, ext)))
+    groups = P.filon_groups(plans)
+    seen = set()
+    for g in groups:
+        e0, p0 = plans[g[0]]
+        for pos in g:
+            e, p = plans[pos]
+            assert pos not in seen
+            seen.add(pos)
+            assert e == e0 and p.N1 == p0.N1 and p.log1 == p0.log1 and p.log2 == p0.log2
+            assert p.layout == p0.layout
+    for pos, (e, p) in enumerate(plans):
+        assert (pos in seen) == (p.mode == P.PREP_FILON and p.N1 >= 2)
+    assert 0 < len(groups) < len(seen) // 2
