@@ -23,6 +23,14 @@
 // (M, A) -> (M1 M2, A1 M2 + A2), M = (r_a / r_b)^L; tasks of one job are joined
 // in order by a second kernel.  L is processed in blocks of LB values held in
 // registers (6 * LB doubles per lane); each block re-reads the lane's range.
+//
+// Other forms of the same contraction in this file:
+//   semisep_lane_kernel        one lane per (job, family, L-block) over the whole range;
+//   semisep_lane_tab_kernel    the same reading 1/r, q, q^(LB b) from per-grid tables
+//                              (semisep_lane_table_kernel), jobs in any order via an output index;
+//   semisep_lane_split_kernel  each range split between two wavefronts joined through LDS;
+//   semisep_gemv_scan_kernel   t_L = K_L beta for one shared beta per energy (chunked affine
+//                              scans), for the direct family of pairs that share it.
 #include "common.hpp"
 
 namespace dwba {
