@@ -837,88 +837,10 @@ void semisep_lane_split_kernel(const dwba_plain_job* __restrict__ jobs,
 //     t_L(i) = (A_i + d_L beta_i) / r_i + B_i,
 //     A_{i+1} = q_i^L (A_i + beta_i),              A_0 = 0          (j < i)
 //     B_{i-1} = q_{i-1}^L (B_i + beta_i / r_i),    B_{N-1} = 0      (j > i)
-// A workgroup of GT threads handles one (energy, L); a thread owns a
-// contiguous range; ranges are joined by scans of the affine maps through LDS
-// (forward for A, backward for B), then every thread recomputes its range
-// with the carried values and writes t.
 constexpr int GT = 256;
 
-__global__ __launch_bounds__(GT) void semisep_gemv_kernel(const dwba_dgemv_energy* __restrict__ en) {
-    __shared__ double s_m[GT], s_v[GT];
-    const dwba_dgemv_energy E = en[blockIdx.x];
-    const int L = blockIdx.y;
-    if (L > E.L_max) return;
-    const int n = E.N, tid = threadIdx.x;
-    const int per = (n + GT - 1) / GT;
-    const int a = min(n, tid * per), b = min(n, a + per);
-    const double* __restrict__ r = E.r;
-    const double* __restrict__ be = E.beta;
-    const double dL = L == 0 ? 1.0 : exp((double)L * log(1.0 - 1e-12));
-    // ---- pass 1, one sweep over [a, b): the forward map A_b = Mf A_a + Af and
-    // the backward map B_{a-1} = Mb B_{b-1} + Sb, with
-    //     Mb = prod_{i=a}^{b-1} q_{i-1}^L,  Sb = sum_{i=a}^{b-1} (beta_i / r_i) prod_{j=a}^{i} q_{j-1}^L
-    double Af = 0.0, Mf = 1.0, Sb = 0.0, Mb = 1.0;
-    double qprev = 1.0;                   // q_{i-1}^L, (r_{a-1} / r_a)^L at the start
-    if (a < b && a > 0 && L > 0) qprev = pow_int_rt(r[a - 1] / r[a], L);
-    for (int i = a; i < b; ++i) {
-        const double ri = r[i];
-        const double q = i + 1 < n ? ri / r[i + 1] : 1.0;
-        const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
-        const double bi = be[i];
-        Af = qL * (Af + bi);
-        Mf *= qL;
-        Mb *= qprev;
-        Sb = fma(bi / ri, Mb, Sb);
-        qprev = qL;
-    }
-    // ---- forward scan (inclusive) of (Mf, Af) over threads
-    s_m[tid] = Mf; s_v[tid] = Af;
-    __syncthreads();
-    for (int d = 1; d < GT; d <<= 1) {
-        double pm = 0.0, pv = 0.0;
-        const bool take = tid >= d;
-        if (take) { pm = s_m[tid - d]; pv = s_v[tid - d]; }
-        __syncthreads();
-        if (take) { s_v[tid] = fma(pv, s_m[tid], s_v[tid]); s_m[tid] = pm * s_m[tid]; }
-        __syncthreads();
-    }
-    const double A_in = tid > 0 ? s_v[tid - 1] : 0.0;
-    __syncthreads();
-    // ---- backward scan (inclusive from the right) of (Mb, Sb)
-    s_m[tid] = Mb; s_v[tid] = Sb;
-    __syncthreads();
-    for (int d = 1; d < GT; d <<= 1) {
-        double pm = 0.0, pv = 0.0;
-        const bool take = tid + d < GT;
-        if (take) { pm = s_m[tid + d]; pv = s_v[tid + d]; }
-        __syncthreads();
-        if (take) { s_v[tid] = fma(pv, s_m[tid], s_v[tid]); s_m[tid] = pm * s_m[tid]; }
-        __syncthreads();
-    }
-    const double B_in = tid + 1 < GT ? s_v[tid + 1] : 0.0;   // B at this range's last point
-    // ---- pass 2: forward part, then backward part added in place
-    double A = A_in;
-    double* __restrict__ t = E.t + (size_t)L * E.ld;
-    for (int i = a; i < b; ++i) {
-        const double ri = r[i];
-        const double bi = be[i];
-        if (i < E.rows) t[i] = (A + dL * bi) / ri;
-        const double q = i + 1 < n ? ri / r[i + 1] : 1.0;
-        const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
-        A = qL * (A + bi);
-    }
-    double B = B_in;
-    for (int i = b - 1; i >= a; --i) {
-        const double ri = r[i];
-        if (i < E.rows) t[i] += B;
-        const double q = i > 0 ? r[i - 1] / ri : 1.0;
-        const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
-        B = qL * (B + be[i] / ri);
-    }
-}
-
-// The same t_L in coalesced chunks: a workgroup per (energy, L) sweeps the
-// grid in chunks of GT points, one point per thread.  Point i's step of the
+// A workgroup per (energy, L) sweeps the grid in chunks of GT points, one
+// point per thread.  Point i's step of the
 // forward recurrence is the affine map x -> q_i^L x + q_i^L beta_i (of the
 // backward one x -> q_{i-1}^L x + q_{i-1}^L beta_i / r_i); a chunk composes
 // them with an in-wave shuffle scan and a scan of the four wave totals in
