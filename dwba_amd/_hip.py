@@ -68,6 +68,7 @@ _SIGS = {
     "dwba_last_error": (ctypes.c_char_p, []),
     "dwba_il_plain_f64_workspace_bytes": (c_sz, [c_i32, c_i32, c_i32]),
     "dwba_il_plain_f64": (c_i32, [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_sz, c_vp]),
+    "dwba_il_plain_f64_noalloc": (c_i32, [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]),
     "dwba_il_plain_f64_batch_workspace_bytes": (c_sz, [c_i32, c_i32p, c_i32p, c_i32]),
     "dwba_il_plain_f64_batch": (c_i32, [c_i32, c_vp, c_i32p, c_i32p, c_i32, c_vp, c_vp, c_sz, c_vp]),
     "dwba_il_semisep_f64_workspace_bytes": (c_sz, [c_i32, c_i32p, c_i32p, c_i32, c_i32]),

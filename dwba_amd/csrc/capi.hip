@@ -39,6 +39,8 @@ int32_t dwba_init(int32_t device) {
     hipDeviceProp_t p;
     DWBA_HIP_CHECK(hipGetDeviceProperties(&p, device));
     dwba::g_cu = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
+    const int rc = dwba::noalloc_scratch_setup(device);
+    if (rc != DWBA_OK) return rc;
     dwba::g_init = 1;
     dwba::g_err.clear();
     return DWBA_OK;

@@ -44,6 +44,10 @@ inline size_t align_up(size_t x, size_t a = 256) { return (x + a - 1) / a * a; }
 // Number of CUs of the current device (recorded by dwba_init).
 int device_cu_count();
 
+// Records the current device's address of the fixed scratch array of
+// dwba_il_plain_f64_noalloc (plain_f64.hip); called by dwba_init.
+int noalloc_scratch_setup(int device);
+
 // Deterministic wavefront-wide sum of a double (xor butterfly over 64 lanes).
 __device__ inline double wave_sum(double v) {
 #pragma unroll

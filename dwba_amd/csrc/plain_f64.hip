@@ -20,6 +20,12 @@
 // kernel adds them per job in a fixed order (bit-reproducible results).
 #include "common.hpp"
 
+#include <atomic>
+
+// Device scratch of dwba_il_plain_f64_noalloc (size in include/dwba_hip.h);
+// each GPU holds its own copy until the process ends.
+static __device__ __align__(256) unsigned char scratch[DWBA_NOALLOC_SCRATCH_BYTES];
+
 namespace dwba {
 namespace {
 
@@ -262,7 +268,54 @@ int launch_plain(const PlainPlan& p, int n_jobs, const dwba_plain_job* d_jobs, i
     return DWBA_OK;
 }
 
+// One job through the plain kernels with the caller's workspace `ws` of
+// `ws_bytes` bytes: the job descriptor is stored into the workspace by a
+// kernel (no host-device copy), so the call can be captured in a HIP graph.
+int plain_single(const char* name, const double* d_r, int32_t n_rows, int32_t n_cols, const double* d_alpha_dir,
+                 const double* d_beta_dir, const double* d_alpha_ex, const double* d_beta_ex, int32_t L_max,
+                 double* d_out, char* ws, size_t ws_bytes, hipStream_t s) {
+    DWBA_REQUIRE(initialised(), "%s: call dwba_init first", name);
+    DWBA_REQUIRE(L_max >= 0 && L_max <= 63, "%s: L_max=%d outside [0, 63]", name, L_max);
+    DWBA_REQUIRE(n_rows >= 0 && n_cols >= n_rows, "%s: need 0 <= n_rows <= n_cols (got %d, %d)", name, n_rows,
+                 n_cols);
+    DWBA_REQUIRE(d_out != nullptr, "%s: null output", name);
+    DWBA_REQUIRE(n_cols == 0 || (d_r && d_alpha_dir && d_beta_dir && d_alpha_ex && d_beta_ex),
+                 "%s: null input pointer", name);
+    PlainPlan p = make_plan(1, &n_rows, &n_cols, L_max);
+    if (ws_bytes < p.total())
+        return set_error(DWBA_ERR_WORKSPACE, "%s: workspace %zu < %zu bytes", name, ws_bytes, p.total());
+    dwba_plain_job* d_job = reinterpret_cast<dwba_plain_job*>(ws + p.off_bytes + p.partial_bytes);
+    dwba_plain_job job{d_r, d_alpha_dir, d_beta_dir, d_alpha_ex, d_beta_ex, n_rows, n_cols};
+    hipLaunchKernelGGL(plain_f64_store_job, dim3(1), dim3(64), 0, s, job, d_job);
+    DWBA_LAUNCH_CHECK("plain_f64_store_job");
+    return launch_plain(p, 1, d_job, L_max, d_out, ws, s);
+}
+
+// Scratch of dwba_il_plain_f64_noalloc.  `scratch` names the array in device
+// code only; on the host it is a stand-in symbol, so host code works with the
+// device address that hipGetSymbolAddress returns, recorded per device by
+// noalloc_scratch_setup (called from dwba_init with that device current).
+// For one job the plain kernels use at most 256 B of block offsets, 2*64
+// doubles per workgroup for <= 16 workgroups per CU, and one job descriptor:
+// about 4 MiB on 256 CUs.  Nothing in it needs to start at zero (the plan
+// kernel writes every offset, each workgroup every one of its partial sums).
+constexpr int kMaxScratchDevices = 64;
+std::atomic<void*> g_scratch_ptr[kMaxScratchDevices];
+
 }  // namespace
+
+int noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxScratchDevices)
+        return set_error(DWBA_ERR_ARG, "dwba_init: device %d beyond the %d devices the scratch table covers", device,
+                         kMaxScratchDevices);
+    if (g_scratch_ptr[device].load() != nullptr) return DWBA_OK;
+    void* p = nullptr;
+    DWBA_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (p == nullptr) return set_error(DWBA_ERR_HIP, "dwba_init: hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[device].store(p);
+    return DWBA_OK;
+}
+
 }  // namespace dwba
 
 using namespace dwba;
@@ -278,24 +331,23 @@ int32_t dwba_il_plain_f64(const double* d_r, int32_t n_rows, int32_t n_cols, con
                           const double* d_beta_dir, const double* d_alpha_ex, const double* d_beta_ex,
                           int32_t L_max, double* d_out, void* d_workspace, size_t workspace_bytes,
                           void* stream) {
-    DWBA_REQUIRE(initialised(), "dwba_il_plain_f64: call dwba_init first");
-    DWBA_REQUIRE(L_max >= 0 && L_max <= 63, "dwba_il_plain_f64: L_max=%d outside [0, 63]", L_max);
-    DWBA_REQUIRE(n_rows >= 0 && n_cols >= n_rows,
-                 "dwba_il_plain_f64: need 0 <= n_rows <= n_cols (got %d, %d)", n_rows, n_cols);
-    DWBA_REQUIRE(d_out != nullptr && d_workspace != nullptr, "dwba_il_plain_f64: null output/workspace");
-    DWBA_REQUIRE(n_cols == 0 || (d_r && d_alpha_dir && d_beta_dir && d_alpha_ex && d_beta_ex),
-                 "dwba_il_plain_f64: null input pointer");
-    PlainPlan p = make_plan(1, &n_rows, &n_cols, L_max);
-    if (workspace_bytes < p.total())
-        return set_error(DWBA_ERR_WORKSPACE, "dwba_il_plain_f64: workspace %zu < %zu bytes", workspace_bytes,
-                         p.total());
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    char* ws = reinterpret_cast<char*>(d_workspace);
-    dwba_plain_job* d_job = reinterpret_cast<dwba_plain_job*>(ws + p.off_bytes + p.partial_bytes);
-    dwba_plain_job job{d_r, d_alpha_dir, d_beta_dir, d_alpha_ex, d_beta_ex, n_rows, n_cols};
-    hipLaunchKernelGGL(plain_f64_store_job, dim3(1), dim3(64), 0, s, job, d_job);
-    DWBA_LAUNCH_CHECK("plain_f64_store_job");
-    return launch_plain(p, 1, d_job, L_max, d_out, ws, s);
+    DWBA_REQUIRE(d_workspace != nullptr, "dwba_il_plain_f64: null workspace");
+    return plain_single("dwba_il_plain_f64", d_r, n_rows, n_cols, d_alpha_dir, d_beta_dir, d_alpha_ex, d_beta_ex,
+                        L_max, d_out, reinterpret_cast<char*>(d_workspace), workspace_bytes,
+                        reinterpret_cast<hipStream_t>(stream));
+}
+
+int32_t dwba_il_plain_f64_noalloc(const double* d_r, int32_t n_rows, int32_t n_cols, const double* d_alpha_dir,
+                                  const double* d_beta_dir, const double* d_alpha_ex, const double* d_beta_ex,
+                                  int32_t L_max, double* d_out, void* stream) {
+    int dev = -1;
+    DWBA_HIP_CHECK(hipGetDevice(&dev));
+    char* base = (dev >= 0 && dev < kMaxScratchDevices) ? static_cast<char*>(g_scratch_ptr[dev].load()) : nullptr;
+    DWBA_REQUIRE(base != nullptr, "dwba_il_plain_f64_noalloc: no scratch address for device %d (call dwba_init(%d))",
+                 dev, dev);
+    return plain_single("dwba_il_plain_f64_noalloc", d_r, n_rows, n_cols, d_alpha_dir, d_beta_dir, d_alpha_ex,
+                        d_beta_ex, L_max, d_out, base, DWBA_NOALLOC_SCRATCH_BYTES,
+                        reinterpret_cast<hipStream_t>(stream));
 }
 
 size_t dwba_il_plain_f64_batch_workspace_bytes(int32_t n_jobs, const int32_t* h_n_rows,

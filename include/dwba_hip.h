@@ -72,6 +72,23 @@ int32_t dwba_il_plain_f64(const double* d_r, int32_t n_rows, int32_t n_cols,
                           int32_t L_max, double* d_out,
                           void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* The same as dwba_il_plain_f64 with no workspace argument and no allocation:
+ * no counterpart in the reference.  It is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  The kernels' partial sums go to a fixed
+ * library-owned device array of DWBA_NOALLOC_SCRATCH_BYTES per GPU (held
+ * until the process ends; its address on each device is taken by dwba_init,
+ * which must have been called for the current device).  All calls on one
+ * GPU share that array, so calls that overlap in time on one GPU (two
+ * streams, two threads) race: serialise them.  Returns DWBA_ERR_WORKSPACE if
+ * the launch would need more scratch than that (more than 512 CUs).  Same
+ * arguments and bit-identical results as dwba_il_plain_f64. */
+#define DWBA_NOALLOC_SCRATCH_BYTES (8u << 20)
+int32_t dwba_il_plain_f64_noalloc(const double* d_r, int32_t n_rows, int32_t n_cols,
+                                  const double* d_alpha_dir, const double* d_beta_dir,
+                                  const double* d_alpha_ex, const double* d_beta_ex,
+                                  int32_t L_max, double* d_out, void* stream);
+
 /* Batched form: n_jobs independent channel pairs (possibly on different
  * grids) in ONE launch.  d_jobs is a DEVICE array of n_jobs descriptors whose
  * pointers are device pointers; h_n_rows / h_n_cols are HOST copies of the
