@@ -103,9 +103,23 @@ def match_index(r, k, l, tol=1e-12) -> int:
     return i if i < len(r) else len(r) - 1
 
 
-def make_wave(r, k, l, delta, with_match=True) -> Wave:
+def coulomb_eta_sigma(k, l, z_ion):
+    """Sommerfeld parameter eta = -z_ion / k of a wave in the field of an ion
+    of charge z_ion and its Coulomb phase sigma_l = arg Gamma(l + 1 + i eta)."""
+    if not z_ion:
+        return 0.0, 0.0
+    from scipy.special import loggamma
+    eta = -float(z_ion) / float(k)
+    return eta, float(np.imag(loggamma(l + 1 + 1j * eta)))
+
+
+def make_wave(r, k, l, delta, with_match=True, z_ion=0.0) -> Wave:
+    """Synthetic wave chi_l; with z_ion != 0 it carries the Coulomb eta and
+    sigma_l of an ionic target (its shape is the same synthetic form: only
+    the phase metadata the quadrature reads changes)."""
+    eta, sigma = coulomb_eta_sigma(k, l, z_ion)
     return Wave(l=int(l), k_au=float(k), chi_of_r=wave_values(r, k, l, delta), phase_shift=float(delta),
-                idx_match=match_index(r, k, l) if with_match else -1)
+                idx_match=match_index(r, k, l) if with_match else -1, eta=eta, sigma_l=sigma)
 
 
 @dataclass
@@ -152,11 +166,13 @@ def h_potentials(r):
 
 
 def excitation_scan(n_energies=64, n_pairs=400, n_range=(4000, 15000), l_max=40, k_range=(0.5, 8.0),
-                    seed=0, with_match=True):
+                    seed=0, with_match=True, z_ion=0.0):
     """Batched precision-sweep workload: per energy its own exp grid
     N ~ U{n_range} on [1e-5, 1000] a0, (k_i, k_f) ~ U[k_range] per energy,
     n_pairs channel pairs with (l_i, l_f) ~ U{0..l_max}, delta ~ U[0, pi]
-    per wave; bound pair u_1s -> u_2p; waves carry match points."""
+    per wave; bound pair u_1s -> u_2p; waves carry match points.  z_ion != 0:
+    the waves carry the Coulomb eta and sigma_l of an ion of that charge
+    (Coulomb-stepped Filon phase nodes)."""
     from .batch import EnergyInput
     rng = np.random.default_rng(seed)
     out = []
@@ -170,18 +186,19 @@ def excitation_scan(n_energies=64, n_pairs=400, n_range=(4000, 15000), l_max=40,
         for _ in range(n_pairs):
             li, lf = rng.integers(0, l_max + 1, size=2)
             di, df = rng.uniform(0.0, np.pi, size=2)
-            pairs.append((make_wave(r, ki, li, di, with_match), make_wave(r, kf, lf, df, with_match)))
+            pairs.append((make_wave(r, ki, li, di, with_match, z_ion), make_wave(r, kf, lf, df, with_match, z_ion)))
         out.append(EnergyInput(grid=g, V_core=V, U_i=U, bound_i=Bound(u1s(r), l=0, energy_au=-0.5),
                                bound_f=Bound(u2p(r), l=1, energy_au=-0.125), pairs=pairs, U_f=U))
     return out
 
 
-def ionization_scan(n_points=48, n_triples=400, n_range=(6000, 15000), l_max=30, seed=0, k_i=2.0):
+def ionization_scan(n_points=48, n_triples=400, n_range=(6000, 15000), l_max=30, seed=0, k_i=2.0, z_ion=0.0):
     """Ionization-style workload: Slater bound state (zeta = 0.6) and three
     continuum waves (incident k_i, ejected k_e ~ U[0.2, 4], scattered k_f with
     k_f^2 = k_i^2 - k_e^2 - 2*0.18) per ejected-energy point on its own grid
     N ~ U{n_range}; n_triples (l_i, l_f, l_ej) ~ U{0..l_max} per point, drawn
-    with l_i + l_f + l_ej even.  The ejected wave takes the place of u_f."""
+    with l_i + l_f + l_ej even.  The ejected wave takes the place of u_f.
+    z_ion != 0: Coulomb eta and sigma_l on every wave, as in excitation_scan."""
     from .batch import EnergyInput
     rng = np.random.default_rng(seed)
     out = []
@@ -198,7 +215,8 @@ def ionization_scan(n_points=48, n_triples=400, n_range=(6000, 15000), l_max=30,
             if (li + lf + le) % 2:
                 continue
             di, df, de = rng.uniform(0.0, np.pi, size=3)
-            pairs.append((make_wave(r, k_i, li, di), make_wave(r, kf, lf, df), make_wave(r, ke, le, de)))
+            pairs.append((make_wave(r, k_i, li, di, z_ion=z_ion), make_wave(r, kf, lf, df, z_ion=z_ion),
+                          make_wave(r, ke, le, de, z_ion=z_ion)))
         out.append(EnergyInput(grid=g, V_core=V, U_i=U, bound_i=Bound(slater(r, 0.6), l=0, energy_au=-0.18),
                                bound_f=pairs[0][2], pairs=pairs, U_f=U))
     return out

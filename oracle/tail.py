@@ -198,12 +198,69 @@ def filon_integral(f_func, omega, phase_offset, r_start, r_end, delta_phi=np.pi 
         scale = int(np.ceil(eff / dmax))
         n_eff = min(15, max(n_nodes_per_segment, n_nodes_per_segment + 2 * (scale - 1)))
     bounds = np.linspace(r_start, r_end, n_seg + 1)
+    f_vec = getattr(f_func, "vec", None)
+    if f_vec is not None:
+        return _ksum(_filon_segments_vec(f_vec, bounds, n_eff, omega, phase_offset))
     contrib = []
     for i in range(n_seg):
         nodes = np.linspace(bounds[i], bounds[i + 1], n_eff)
         fv = np.array([f_func(x) for x in nodes], dtype=float)
         contrib.append(filon_segment(fv, nodes, omega, phase_offset))
     return _ksum(contrib)
+
+
+def _filon_segments_vec(f_vec, bounds, n_eff, omega, phase_offset):
+    """filon_segment over every segment [bounds[i], bounds[i+1]] at once:
+    the same nodes (np.linspace arithmetic: start + i*step, last node = stop),
+    the same per-triple Filon coefficients and the same left-to-right sum of
+    the triples of a segment; a segment with |omega h| < 1e-3 (Simpson
+    branch) or an even node count falls back to filon_segment.  Returns the
+    per-segment complex contributions in segment order."""
+    a, b = bounds[:-1], bounds[1:]
+    step = (b - a) / (n_eff - 1)
+    nodes = np.arange(n_eff)[None, :] * step[:, None] + a[:, None]
+    nodes[:, -1] = b
+    fv = f_vec(nodes)
+    h_seg = nodes[:, -1] - nodes[:, 0]
+    out = np.zeros(len(a), dtype=complex)
+    slow = (h_seg < 1e-15) | (np.abs(omega * h_seg) < 1e-3)
+    if n_eff % 2 == 0 or n_eff < 3:
+        slow[:] = True
+    fast = ~slow
+    if np.any(fast):
+        R, F = nodes[fast], fv[fast]
+        res = np.zeros(R.shape[0], dtype=complex)
+        for i in range(0, n_eff - 2, 2):
+            r0, r1, r2 = R[:, i], R[:, i + 1], R[:, i + 2]
+            f0, f1, f2 = F[:, i], F[:, i + 1], F[:, i + 2]
+            h = r2 - r0
+            theta = omega * h / 2
+            t2 = theta ** 2
+            small = np.abs(theta) < 0.3
+            with np.errstate(divide="ignore", invalid="ignore"):
+                sin_t, cos_t = np.sin(theta), np.cos(theta)
+                sin_2t, cos_2t = np.sin(2 * theta), np.cos(2 * theta)
+                al_c = (t2 + theta * sin_2t / 2 - 2 * sin_t ** 2) / (theta ** 3)
+                be_c = 2 * (theta * (1 + cos_2t) - sin_2t) / (theta ** 3)
+                ga_c = 4 * (sin_t - theta * cos_t) / (theta ** 3)
+            t3 = theta ** 3
+            al_s = 2 * t3 / 45 * (1 - t2 / 7 + t2 ** 2 / 189)
+            be_s = 2 / 3 + 2 * t2 / 15 - 4 * t2 ** 2 / 105 + 2 * t2 ** 3 / 567
+            ga_s = 4 / 3 - 2 * t2 / 15 + 1 * t2 ** 2 / 210 - t2 ** 3 / 11340
+            alpha = np.where(small, al_s, al_c)
+            beta = np.where(small, be_s, be_c)
+            gamma = np.where(small, ga_s, ga_c)
+            e0 = np.exp(1j * (omega * r0 + phase_offset))
+            e1 = np.exp(1j * (omega * r1 + phase_offset))
+            e2 = np.exp(1j * (omega * r2 + phase_offset))
+            tri = (h / 2) * (alpha * (-1j / omega) * (f2 * e2 - f0 * e0) + beta * (f0 * e0 + f2 * e2) + gamma * f1 * e1)
+            # a triple of zero width contributes 0 (filon_segment's h < 1e-15 test)
+            tri = np.where(h < 1e-15, 0j, tri)
+            res = res + tri
+        out[fast] = res
+    for i in np.nonzero(slow)[0]:
+        out[i] = filon_segment(fv[i], nodes[i], omega, phase_offset)
+    return out
 
 
 def chebyshev_diff_matrix(n: int) -> np.ndarray:
@@ -278,9 +335,16 @@ def outer_integral_oscillatory(f_func, phi_func, dphi_func, d2phi_func, r_m, r_m
 
 
 def multipole_envelope(moment_L: float, L: int):
-    """Envelope moment_L / r^(L+1) of radial_ME_all_L (dwba_matrix_elements.py:1170-1175)."""
+    """Envelope moment_L / r^(L+1) of radial_ME_all_L (dwba_matrix_elements.py:1170-1175).
+    env.vec evaluates it on an array (used by the vectorised Filon sweep)."""
     def env(r):
         return moment_L / (r ** (L + 1)) if r > 1e-6 else 0.0
+
+    def vec(r):
+        r = np.asarray(r, dtype=float)
+        with np.errstate(divide="ignore", over="ignore"):
+            return np.where(r > 1e-6, moment_L / (r ** (L + 1)), 0.0)
+    env.vec = vec
     return env
 
 

@@ -334,3 +334,64 @@ def test_grouped_filon_prep_matches_per_pair(ionization):
             tol = 1e-12 * np.abs(c) + 1e-15 * np.max(np.abs(c)) + 1e-300
             assert np.all(np.abs(a - c) <= tol), np.max(np.abs(a - c) / tol)
     assert np.all(np.abs(r1 - r0) <= 1e-12 * np.abs(r0) + 1e-14 * mag)
+
+
+@pytest.mark.parametrize("ionization", [False, True])
+def test_ionic_target_scan_matches_dropin(ionization):
+    """Waves of an ionic target (Coulomb eta != 0, sigma_l): Filon phase
+    nodes stepped on the device (generate_phase_nodes' Coulomb branch) and
+    Coulomb phases in the outer tails.  The batch equals the single-pair
+    drop-in to rounding in the default step, with grouped Filon prep (the
+    group's cell pass reads the node count the stepping wrote) and in the
+    two-half pipelined step, eagerly and as graph replays."""
+    import torch
+    from dwba_amd.batch import ScanBatch
+    if ionization:
+        scan = S.ionization_scan(n_points=2, n_triples=16, n_range=(1500, 2500), l_max=10, seed=13, z_ion=1.0)
+    else:
+        scan = S.excitation_scan(n_energies=3, n_pairs=16, n_range=(1500, 2500), l_max=12, k_range=(0.5, 3.0),
+                                 seed=13, z_ion=1.0)
+    L_max = 8
+    b = ScanBatch(scan, L_max, precision="fp64", oscillatory_method="full_split")
+    assert b.any_coul and b.filon_groups is not None
+    refs = []
+    for en in scan:
+        for pr in en.pairs:
+            d = _dropin(en, pr, L_max, oscillatory_method="full_split")
+            refs.append([[d.I_L_direct[L] for L in range(L_max + 1)], [d.I_L_exchange[L] for L in range(L_max + 1)]])
+    refs = np.asarray(refs)
+    b.run()
+    mag = b.magnitudes()
+    tol = 1e-12 * np.abs(refs) + 1e-14 * mag
+
+    def check(label):
+        got = b.results()
+        bad = np.argwhere(~(np.abs(got - refs) <= tol))
+        assert bad.size == 0, (label, bad[:10].tolist(), got[tuple(bad[0])], refs[tuple(bad[0])])
+
+    b.out.zero_()
+    b.run()
+    check("default")
+    b.prep_grouped = True
+    for st in b.E:
+        for m in (st.ad, st.ae, st.be):
+            m.zero_()
+    b.out.zero_()
+    b.run()
+    check("grouped")
+    b.prep_grouped = False
+    b.semisep_lanes = True
+    b.lane_split = 2
+    b.pipeline = True
+    assert b.pipeline_ok()
+    b.out.zero_()
+    b.run()
+    check("pipelined")
+    b.capture()
+    for rep in range(2):
+        b.out.zero_()
+        torch.cuda.synchronize()
+        b.run()
+        torch.cuda.synchronize()
+        check(f"pipelined replay {rep}")
+        _ = (torch.ones(1000, device="cuda") * rep).sum().item()

@@ -17,9 +17,23 @@ from dwba_amd import grid as PG, synthetic as S
 pytestmark = pytest.mark.gpu
 
 
-def _case(seed, n=1200, r_max=100.0, eta=0.0, ionization=False, with_match=True, k_scale=1.0):
+def _grid(kind, n, r_max):
+    """make_r_grid, or a grid that is not log-spaced: uniform in r, or
+    geometric up to r = 5 a0 and uniform beyond (both fail the 5e-3
+    log-spacing test, so every interpolation is linear in r)."""
+    if kind == "log":
+        return PG.make_r_grid(1e-4, r_max, n)
+    if kind == "linear":
+        return PG._finish(np.linspace(r_max / n, r_max, n))
+    n_in = n // 3
+    inner = 1e-3 * (5.0 / 1e-3) ** np.linspace(0.0, 1.0, n_in)
+    outer = np.linspace(5.0, r_max, n - n_in + 1)[1:]
+    return PG._finish(np.concatenate([inner, outer]))
+
+
+def _case(seed, n=1200, r_max=100.0, eta=0.0, ionization=False, with_match=True, k_scale=1.0, grid="log"):
     rng = np.random.default_rng(seed)
-    g = PG.make_r_grid(1e-4, r_max, n)
+    g = _grid(grid, n, r_max)
     r = g.r
     ki, kf = rng.uniform(0.6, 3.0, 2) * k_scale
     li, lf = rng.integers(0, 12, 2)
@@ -136,3 +150,16 @@ def test_nan_density_sanitised_in_filon_branch():
     ci.chi_of_r = ci.chi_of_r.copy()
     ci.chi_of_r[500] = np.nan
     _compare((g, V, Ui, Uf, bi, bf, ci, cf), dict(), 3)
+
+
+@pytest.mark.parametrize("grid", ["linear", "log_then_linear"])
+@pytest.mark.parametrize("method", ["advanced", "full_split"])
+def test_non_log_grid_linear_interpolation(grid, method):
+    """Grids that fail the log-spacing test take the linear-in-r branch of
+    _interp_on_radial_grid (oscillatory_integrals.py:279-302) for every
+    node interpolation of the Filon direct and exchange quadratures."""
+    from dwba_amd import planner
+    args = _case(12, n=2000, grid=grid)
+    gs = planner.GridStats.of(args[0].r, args[0].w_trapz)
+    assert not gs.log_spaced(len(args[0].r)) and not gs.log_spaced(len(args[0].r) // 2)
+    _compare(args, dict(oscillatory_method=method), 6)

@@ -204,3 +204,44 @@ def test_analytical_tail_against_exact_integral(ki, kf, li, lf, L, rm):
     assert abs(got - (exact + s_m - s_p)) < 1.0 * scale * (n + 1) * (n + 2) / (km * km * rm * rm)
     # and the defect is real: the formula is not the exact tail
     assert abs(got - exact) > 0.5 * abs(s_m - s_p)
+
+
+def test_vectorised_filon_sweep_equals_segment_loop():
+    """The oracle's vectorised Filon sweep (every segment at once) gives the
+    per-segment loop of filon_segment to rounding, on cases that reach the
+    Taylor branch (theta < 0.3), node up-scaling (effective dphi above the
+    cap) and the segment cap."""
+    from oracle import tail as T
+    rng = np.random.default_rng(5)
+    cases = [(float(rng.uniform(0.5, 8)), float(rng.uniform(0.5, 8)), int(rng.integers(0, 41)), int(rng.integers(0, 41)),
+              float(rng.uniform(20, 300)), int(rng.integers(1, 31))) for _ in range(12)]
+    cases += [(0.3, 0.2, 0, 1, 50.0, 2),        # small k_-: theta < 0.3 in the difference phase
+              (8.0, 7.9, 3, 4, 10.0, 5)]        # k_+ ~ 16 over 990 a0: segment cap, up-scaled nodes
+    for ka, kb, la, lb, rm, L in cases:
+        m = float(rng.standard_normal())
+        env = T.multipole_envelope(m, L)
+        vec = T.dwba_outer_integral(env, ka, la, 0.3, 0.0, 0.0, kb, lb, 1.1, 0.0, 0.0, rm, 1000.0)
+        loop = T.dwba_outer_integral(lambda r: env(r), ka, la, 0.3, 0.0, 0.0, kb, lb, 1.1, 0.0, 0.0, rm, 1000.0)
+        assert abs(vec - loop) <= 1e-13 * abs(loop) + 1e-300, (ka, kb, la, lb, rm, L, vec, loop)
+
+
+def test_non_oscillatory_tail_branch_closed_form():
+    """With k_a = k_b, l_a = l_b and no Coulomb phase, Phi_-(r) is the
+    constant c_- = delta_a - delta_b, and the oracle (like the reference)
+    integrates M r^-(L+1) e^{i c_-} over [r_m, r_max] with scipy quad, whose
+    default tolerances (1.49e-8 absolute or relative) bound its distance from
+    the exact M e^{i c_-} (r_m^-L - r_max^-L) / L (a dropped factor, a wrong
+    sign or the wrong end point is far outside that)."""
+    from oracle import tail as T
+    for (k, l, rm, rmax, L, m) in ((1.5, 3, 15.0, 200.0, 1, 0.8), (1.5, 3, 15.0, 200.0, 7, -0.5),
+                                   (0.7, 0, 5.0, 1000.0, 20, 0.02)):
+        pf = T.PhaseFunctions(k, l, 0.3, 0.0, 0.0, k, l, 0.7, 0.0, 0.0)
+        assert abs(pf.dphi(-1, 0.5 * (rm + rmax))) < 1e-10
+        env = T.multipole_envelope(m, L)
+        got = T.outer_integral_oscillatory(env, lambda r: pf.phi(-1, r), lambda r: pf.dphi(-1, r),
+                                           lambda r: pf.d2phi(-1, r), rm, rmax)
+        exact = m * np.exp(1j * (0.3 - 0.7)) * (rm ** -L - rmax ** -L) / L
+        tol = 1.49e-8 * max(1.0, abs(exact))
+        assert abs(got.real - exact.real) <= tol and abs(got.imag - exact.imag) <= tol, (L, got, exact)
+        if abs(exact) > 1e-3:          # well above quad's absolute tolerance: relative agreement too
+            assert abs(got - exact) <= 1e-6 * abs(exact), (L, got, exact)

@@ -105,3 +105,39 @@ def test_legacy_analytic_tail():
     p = dict(CASES["coulomb"], da=0.3, db=0.7, sa=0.1, sb=-0.2)
     moments = np.array([0.0, 0.8, -0.5, 0.3])
     np.testing.assert_allclose(_gpu_tail(p, moments, kind=2), _oracle_tail(p, moments, kind=2), rtol=1e-12)
+
+
+QUAD_EPSABS = 1.49e-8    # scipy.integrate.quad's default absolute (and relative) tolerance
+
+
+def test_non_oscillatory_branch():
+    """Same k, same l, no Coulomb phase: Phi_-' vanishes identically, so the
+    reference integrates Phi_- with scipy quad at its default tolerances
+    (oscillatory_integrals.py:1418-1419) and the library with 512 panels of
+    8-point Gauss-Legendre; Phi_+ goes through Filon.  Phi_- is the constant
+    c = delta_a - delta_b there, so that half is exactly
+    M e^{ic} (r_m^-L - r_max^-L) / L.  The library must equal the oracle with
+    that exact value in place of quad's to 1e-10, and the oracle itself
+    (quad's value) to within quad's stated tolerance."""
+    p = dict(ka=1.5, la=3, kb=1.5, lb=3, rm=15.0, rmax=200.0, ea=0.0, eb=0.0, da=0.3, db=0.7, sa=0.0, sb=0.0)
+    pf = T.PhaseFunctions(p["ka"], p["la"], p["da"], 0.0, 0.0, p["kb"], p["lb"], p["db"], 0.0, 0.0)
+    assert abs(pf.dphi(-1, 0.5 * (p["rm"] + p["rmax"]))) < 1e-10
+    moments = np.array([0.0, 0.8, -0.5, 0.3, 1e-13, 0.05] + [0.02 * (-1) ** k for k in range(20)])
+    got = _gpu_tail(p, moments)
+    ref_quad = _oracle_tail(p, moments)
+    ref_exact = ref_quad.copy()
+    for L in range(1, len(moments)):
+        m = moments[L]
+        if abs(m) <= 1e-12:
+            continue
+        env = T.multipole_envelope(m, L)
+        q = T.outer_integral_oscillatory(env, lambda r: pf.phi(-1, r), lambda r: pf.dphi(-1, r),
+                                         lambda r: pf.d2phi(-1, r), p["rm"], p["rmax"])
+        exact = m * np.exp(1j * (p["da"] - p["db"])) * (p["rm"] ** -L - p["rmax"] ** -L) / L
+        ref_exact[L] += 0.5 * (exact.real - q.real)
+    scale = np.max(np.abs(ref_exact)) + 1e-300
+    assert np.all(np.abs(got - ref_exact) <= 1e-10 * np.abs(ref_exact) + 1e-13 * scale), \
+        (got, ref_exact, np.abs(got - ref_exact))
+    assert np.all(np.abs(got - ref_quad) <= 0.5 * QUAD_EPSABS * np.maximum(1.0, np.abs(ref_quad))), \
+        np.abs(got - ref_quad)
+    assert got[0] == 0.0 and got[4] == 0.0
