@@ -827,8 +827,8 @@ class ScanBatch:
                                            hip.ptr(self.mom_ptr[2]), hip.ptr(self.mom_ptr[3]), hip.ptr(self.mom_n),
                                            Lm, hip.ptr(self.moments), h), "dwba_moments_f64")
             seg, dmax = planner.filon_limits()
-            Dp, _ = hip.f64_array(self.levin[0])
-            Xp, _ = hip.f64_array(self.levin[1])
+            Dp, D_keep = hip.f64_array(self.levin[0])     # the arrays stay bound until the call returns
+            Xp, X_keep = hip.f64_array(self.levin[1])
             hip.check(lib.dwba_outer_tail_f64(self.n_tail, hip.ptr(self.tail_d), Lm, seg, dmax, Dp, Xp,
                                               hip.ptr(self.tail_ws), self.tail_ws_bytes, h), "dwba_outer_tail_f64")
 
@@ -854,8 +854,8 @@ class ScanBatch:
         if fork and self.overlap == "start":
             fork_tails()
         mark("prep", "start")
-        cxp, _ = hip.f64_array(self.cc[0])
-        cwp, _ = hip.f64_array(self.cc[1])
+        cxp, cx_keep = hip.f64_array(self.cc[0])         # bound for the whole launch (prep() reads them)
+        cwp, cw_keep = hip.f64_array(self.cc[1])
         gemv = getattr(self, "direct_gemv", False) and self.prep_d_gemv is not None
         grouped = self.prep_grouped and self.filon_groups is not None
         prep_jobs = self._prep_jobs(gemv, grouped)
@@ -971,8 +971,8 @@ class ScanBatch:
         if (getattr(self, "direct_gemv", False) and self.prep_d_gemv is not None) or self.prep_grouped:
             # the shared-beta path leaves beta_dir of most pairs unwritten: write all
             # (the per-pair prep also recomputes grouped pairs' alpha / beta_ex)
-            cxp, _ = hip.f64_array(self.cc[0])
-            cwp, _ = hip.f64_array(self.cc[1])
+            cxp, cx_keep = hip.f64_array(self.cc[0])
+            cwp, cw_keep = hip.f64_array(self.cc[1])
             hip.check(lib.dwba_fullsplit_prep_f64(n, hip.ptr(self.prep_d), hip.ptr(self.coul_d), int(self.any_coul),
                                                   cxp, cwp, self.CC_nodes, hip.stream_handle()),
                       "dwba_fullsplit_prep_f64")

@@ -9,7 +9,6 @@
 // L = 0 corrections added in the order of radial_ME_all_L
 // (dwba_matrix_elements.py:1112-1298).
 #include "common.hpp"
-#include "batch.hpp"
 
 namespace dwba {
 namespace {

@@ -26,7 +26,6 @@
 // sums (:1249-1253, :1291-1296) and the multipole moments
 // M_L = sum_j w_j r_j^L u_f u_i (:1167).
 #include "common.hpp"
-#include "prep.hpp"
 
 namespace dwba {
 namespace {

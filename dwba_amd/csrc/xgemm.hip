@@ -23,7 +23,6 @@
 // (grid) dimension streams in chunks of 32 through double-buffered LDS.
 // Work items are taken from an atomic counter in cost order (longest first).
 #include "common.hpp"
-#include "batch.hpp"
 
 namespace dwba {
 namespace {

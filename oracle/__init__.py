@@ -1,7 +1,7 @@
 """CPU FP64 oracle for the DWBA radial multipole double integrals I_L.
 
-TEST INFRASTRUCTURE ONLY.  This package restates, in numpy (plus a small C
-file for the O(N) plain contraction), the reference algorithm of
+TEST INFRASTRUCTURE ONLY.  This package restates, in numpy (long double for
+the O(N) contraction, mpmath for the brute force), the reference algorithm of
 StaryPasterz/DWBA's ``radial_ME_all_L`` (dwba_matrix_elements.py:780-1300) and
 the Filon / Clenshaw-Curtis machinery it calls (oscillatory_integrals.py).  It
 is the checker of the HIP product path in ``dwba_amd``: only ``tests/``,
@@ -21,7 +21,10 @@ radial      dwba_matrix_elements.py:780-1300 + oscillatory_integrals.py:2385-278
             and the alpha/beta reduction I_L = alpha^T K_L beta)
 semisep     exact O(N) prefix-sum form of alpha^T K_L beta in long double
 brute       mpmath brute force for tiny N
-synthetic   seeded synthetic inputs of the benchmark workloads
+gradient    d/d rho_a, d/d rho_b of sum_L c_L I_L (K_L x by the same prefix sums)
+
+The seeded synthetic inputs of the benchmark workloads live in the product
+package (dwba_amd/synthetic.py): they are inputs, not a checker.
 
 How each function is pinned (checks live in tests/test_oracle_*.py):
   grid.simpson_weights     scipy.integrate.simpson (odd N, same non-uniform
@@ -40,7 +43,12 @@ How each function is pinned (checks live in tests/test_oracle_*.py):
                            when the supports are disjoint; symmetry
   tail.filon_segment       exactness of the Taylor branch on quadratic
                            envelopes with linear phase (textbook Filon)
-  tail.levin_segment       exactness on f = p' + i w p (Levin collocation)
+  tail.levin_segment       exactness on f = -p' + i w p, p of degree <= 7, linear
+                           phase (the collocation system with the reference's
+                           descending-point Chebyshev matrix on ascending nodes)
+  tail.filon_integral      the vectorised segment sweep equals the loop over
+                           filon_segment; quad branch: closed form within quad's
+                           tolerance
   tail.outer_integral,     golden vectors written by tests/golden/make_golden.py
   radial (composite, H     (which imports the reference's CPU modules where they
   1s->2p at 50 eV)         are importable; the fixtures travel, the reference does not)

@@ -406,9 +406,12 @@ def alpha_beta(s: _Setup):
     w1[:n1] = s.w[:n1]
     clean = lambda v: np.nan_to_num(v, nan=0.0, posinf=0.0, neginf=0.0)
     if nodes is None:
+        # standard weighted dot products of the filon / filon_exchange branches:
+        # rows on r[:N1], columns over the full r2 grid for both families
+        # (oscillatory_integrals.py:2571, :2587-2600, :2676-2691, :2703-2705)
         ad = w1 * clean(s.rho1_dir_uw)
         ae = w1 * clean(s.rho1_ex_uw)
-        be = w1 * clean(s.rho2_ex_uw)
+        be = s.w * clean(s.rho2_ex_uw)
     else:
         ad = filon_alpha(s.rho1_dir_uw, s.r, n1, nodes)
         ae = filon_alpha(s.rho1_ex_uw, s.r, n1, nodes)

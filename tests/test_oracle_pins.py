@@ -245,3 +245,57 @@ def test_non_oscillatory_tail_branch_closed_form():
         assert abs(got.real - exact.real) <= tol and abs(got.imag - exact.imag) <= tol, (L, got, exact)
         if abs(exact) > 1e-3:          # well above quad's absolute tolerance: relative agreement too
             assert abs(got - exact) <= 1e-6 * abs(exact), (L, got, exact)
+
+
+def test_reduced_route_standard_fallback_full_r2_grid():
+    """k_i + k_f < 1e-6 with the oscillatory branch forced (k_threshold = 0):
+    the filon and filon_exchange branches fall back to weighted dot products
+    whose inner sums run over the full r2 grid.  The reduced route (alpha /
+    beta + O(N) contraction) must equal the faithful dense route there too;
+    truncating beta_ex at the split index fails this."""
+    from dwba_amd import grid as PG, synthetic as S
+    from oracle import radial as R
+    g = PG.make_r_grid(1e-4, 60.0, 500)
+    r = g.r
+    ci = S.make_wave(r, 4e-7, 1, 0.3)
+    cf = S.make_wave(r, 5e-7, 2, 1.1)
+    ci.idx_match = cf.idx_match = 200            # split well inside the grid
+    bi, bf = S.Bound(S.u1s(r)), S.Bound(S.u2p(r), l=1)
+    V, U = S.h_potentials(r)
+    kw = dict(k_threshold=0.0, oscillatory_method="full_split", U_f_array=U)
+    red, parts = R.radial_ME_all_L_reduced(g.r, g.w_simpson, g.w_trapz, V, U, bi, bf, ci, cf, 4, return_parts=True,
+                                           **kw)
+    fai = R.radial_ME_all_L_faithful(g.r, g.w_simpson, g.w_trapz, V, U, bi, bf, ci, cf, 4, **kw)
+    s = parts["setup"]
+    assert s.oscillatory and s.k_total < 1e-6 and s.idx_limit < len(r)
+    assert np.count_nonzero(parts["beta_ex"][s.idx_limit:]) > 0
+    for L in range(5):
+        for a, b in ((red.I_L_direct[L], fai.I_L_direct[L]), (red.I_L_exchange[L], fai.I_L_exchange[L])):
+            assert abs(a - b) <= 1e-12 * abs(b) + 1e-300, (L, a, b)
+
+
+def test_levin_segment_exact_on_collocation_solutions():
+    """levin_segment solves the 8-point collocation system of the reference,
+    whose Chebyshev matrix (descending Lobatto points) is applied to values
+    at ascending nodes, i.e. -u' + i Phi' u = f.  For a linear phase
+    Phi = w r + c and f = -p' + i w p with p a polynomial of degree <= 7 the
+    system is solved exactly by u = p, so the segment returns
+    p(b) e^{i Phi(b)} - p(a) e^{i Phi(a)} to rounding (a wrong sign, a
+    transposed matrix or a dropped term fails)."""
+    from oracle import tail as T
+    rng = np.random.default_rng(7)
+    # well-conditioned systems (cond(A) <= 5e5): rounding (~1e-11 here) stays below the bar
+    for deg, w, a, b in ((0, 3.0, 1.0, 2.0), (3, 5.0, 10.0, 12.5), (7, 5.0, 50.0, 50.8)):
+        coef = rng.standard_normal(deg + 1)
+        p = np.polynomial.Polynomial(coef, domain=[a, b], window=[-1, 1])
+        dp = p.deriv()
+        x_ref = np.cos(np.pi * np.arange(8) / 7)[::-1]
+        nodes = 0.5 * (b - a) * (x_ref + 1) + a
+        c = 0.4
+        f = -dp(nodes) + 1j * w * p(nodes)
+        got = T.levin_segment(f, nodes, w * nodes + c, np.full(8, w))
+        exact = p(b) * np.exp(1j * (w * b + c)) - p(a) * np.exp(1j * (w * a + c))
+        assert abs(got - exact) <= 1e-10 * max(1.0, abs(exact)), (deg, got, exact)
+        if deg > 0:     # the textbook sign (f = p' + i w p) is not what the rule integrates
+            f_tb = dp(nodes) + 1j * w * p(nodes)
+            assert abs(T.levin_segment(f_tb, nodes, w * nodes + c, np.full(8, w)) - exact) > 1e-3 * abs(exact)
