@@ -293,6 +293,12 @@ class ScanBatch:
 
         # ---- moments and tails
         tail_ids = [k for k, p in enumerate(self.pairs) if p[2].tail_kind != TAIL_NONE]
+        # longest first (each job writes its own row, so the order is free): the
+        # wavefronts with thousands of Filon segments start before the short ones
+        seg_max, dphi_max = planner.filon_limits()
+        tail_ids.sort(key=lambda k: -planner.tail_nodes_estimate(self.pairs[k][2].k_i, self.pairs[k][2].k_f,
+                                                                  self.pairs[k][2].r_match, self.pairs[k][2].r_max,
+                                                                  seg_max, dphi_max))
         self.n_tail = len(tail_ids)
         if tail_ids:
             sets, set_of = [], {}

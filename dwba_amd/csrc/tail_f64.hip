@@ -25,6 +25,8 @@
 // path, round-robin on the per-L Levin path.
 #include "common.hpp"
 
+#include <type_traits>
+
 namespace dwba {
 namespace {
 
@@ -59,6 +61,29 @@ __device__ inline DD dd_wave_sum(DD v) {
         v = dd_add(v, o);
     }
     return v;
+}
+
+// Per-L sums of the 64 lanes' partial sums acc[L], L < LE, combined in
+// double-double in lane order: the lanes write their partials to LDS and
+// lane L adds column L (64 TwoSum steps) -- a fraction of the cost of LE
+// shuffle trees.  out[L] = M_L * sum for 1 <= L <= Lm with |M_L| > 1e-12,
+// 0 otherwise (L < NL).  One wavefront per workgroup.
+template <int NL, int LE>
+__device__ __forceinline__ void lane_sums_out(const double* acc, double (*s_acc)[NL + 1], int lane,
+                                              const double* moments, int Lm, double* out) {
+#pragma unroll
+    for (int L = 0; L < LE; ++L) s_acc[lane][L] = acc[L];
+    __syncthreads();
+    if (lane < NL) {
+        double v = 0.0;
+        if (lane < LE) {
+            DD t{0.0, 0.0};
+            for (int k = 0; k < 64; ++k) t = dd_add(t, s_acc[k][lane]);
+            const double M = (lane >= 1 && lane <= Lm) ? moments[lane] : 0.0;
+            v = fabs(M) > 1e-12 ? M * (t.hi + t.lo) : 0.0;
+        }
+        out[lane] = v;
+    }
 }
 
 struct cplx {
@@ -137,73 +162,15 @@ struct Env {
 };
 __device__ inline double envelope(const Env& E, double r) { return r > 1e-6 ? E.M / pow(r, (double)E.n1) : 0.0; }
 
-// scipy.integrate.simpson(y, x) for an odd number of points (pairs of intervals)
-__device__ inline double simpson_odd(const double* f, const double* x, int n) {
-    double s = 0.0;
-    for (int i = 0; i + 2 < n; i += 2) {
-        const double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1];
-        const double hsum = h0 + h1, hprod = h0 * h1;
-        const double h0divh1 = h1 != 0.0 ? h0 / h1 : 0.0;
-        const double t = hsum / 6.0 *
-                         (f[i] * (2.0 - (h0divh1 != 0.0 ? 1.0 / h0divh1 : 0.0)) +
-                          f[i + 1] * (hsum * (hprod != 0.0 ? hsum / hprod : 0.0)) + f[i + 2] * (2.0 - h0divh1));
-        s += t;
-    }
-    return s;
-}
-
-// _filon_segment_complex with 3 nodes
-__device__ inline cplx filon3(const double* f, const double* x, double omega, double p0) {
-    const double a = x[0], b = x[2];
-    const double h = b - a;
-    if (h < 1e-15) return {0.0, 0.0};
-    if (fabs(omega * h) < 1e-3) {
-        const double mid = omega * (a + b) / 2 + p0;
-        return cscale(cexpi(mid), simpson_odd(f, x, 3));
-    }
-    const double th = omega * h / 2;
-    double alpha, beta, gamma;
-    if (fabs(th) < 0.3) {
-        const double t2 = th * th, t3 = th * th * th;
-        alpha = 2 * t3 / 45 * (1 - t2 / 7 + t2 * t2 / 189);
-        beta = 2.0 / 3 + 2 * t2 / 15 - 4 * (t2 * t2) / 105 + 2 * (t2 * t2 * t2) / 567;
-        gamma = 4.0 / 3 - 2 * t2 / 15 + 1 * (t2 * t2) / 210 - (t2 * t2 * t2) / 11340;
-    } else {
-        double st, ct, s2, c2;
-        sincos(th, &st, &ct);
-        sincos(2 * th, &s2, &c2);
-        const double t2 = th * th, t3 = th * th * th;
-        alpha = (t2 + th * s2 / 2 - 2 * (st * st)) / t3;
-        beta = 2 * (th * (1 + c2) - s2) / t3;
-        gamma = 4 * (st - th * ct) / t3;
-    }
-    const cplx e0 = cexpi(omega * x[0] + p0), e1 = cexpi(omega * x[1] + p0), e2 = cexpi(omega * x[2] + p0);
-    const cplx f2e2 = cscale(e2, f[2]), f0e0 = cscale(e0, f[0]);
-    // alpha * (-1j/omega) * (f2 e2 - f0 e0)
-    const cplx d = csub(f2e2, f0e0);
-    const cplx ta = cmul(cplx{0.0, -alpha / omega}, d);
-    const cplx tb = cscale(cadd(f0e0, f2e2), beta);
-    const cplx tg = cscale(e1, gamma * f[1]);
-    return cscale(cadd(cadd(ta, tb), tg), h / 2);
-}
-
-// Filon segment with n (odd, 5..15) equally spaced nodes on [a, b]
-__device__ cplx filon_segment(const Env& E, double a, double b, int n, double omega, double p0) {
-    double x[15], f[15];
-    const double step = (b - a) / (n - 1);
-    for (int k = 0; k < n; ++k) {
-        x[k] = (k == n - 1) ? b : __dadd_rn(__dmul_rn((double)k, step), a);
-        f[k] = envelope(E, x[k]);
-    }
-    const double h = x[n - 1] - x[0];
-    if (h < 1e-15) return {0.0, 0.0};
-    if (fabs(omega * h) < 1e-3) {
-        const double mid = omega * (x[0] + x[n - 1]) / 2 + p0;
-        return cscale(cexpi(mid), simpson_odd(f, x, n));
-    }
-    cplx res{0.0, 0.0};
-    for (int i = 0; i + 2 < n; i += 2) res = cadd(res, filon3(f + i, x + i, omega, p0));
-    return res;
+// 1/x for 1e-6 < x < 1e300: the hardware reciprocal estimate refined by two
+// Newton steps (each at least doubles its correct bits), within an ulp or so of
+// the correctly rounded quotient at a third of the division's instructions.
+__device__ __forceinline__ double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
 }
 
 struct LevinTab {
@@ -211,46 +178,67 @@ struct LevinTab {
     double xr[8];     // ascending Lobatto points
 };
 
-// _levin_segment_complex: solve (D 2/h + i diag(Phi')) u = f, return u_7 e^{i Phi_7} - u_0 e^{i Phi_0}
-__device__ cplx levin_segment(const Env& E, const Phase& P, int sgn, double a, double b, const LevinTab& T) {
+// _levin_segment_complex for every multipole at once.  The collocation
+// matrix A = D 2/h + i diag(Phi'(x_k)) does not depend on L; only the
+// right-hand side f_k = M_L x_k^-(L+1) does, and the segment value
+// u_7 e^{i Phi(x_7)} - u_0 e^{i Phi(x_0)} with u = A^-1 f is g . f,
+// g = e^{i Phi(x_7)} (row 7 of A^-1) - e^{i Phi(x_0)} (row 0 of A^-1).  The two
+// rows solve A^T y = e_7 and A^T y = e_0 (one LU with partial pivoting on
+// |re| + |im|, LAPACK's izamax convention).  The reference solves A u = f per L;
+// both are backward stable, so they differ by about cond(A) 2^-53 |J|, the
+// noise the reference's own solver carries.  Returns Re g and 1/x_k (0 where
+// the envelope is cut off, x <= 1e-6); false for a degenerate segment (value 0).
+__device__ bool levin_weights(const Phase& P, int sgn, double a, double b, const LevinTab& T, double* gre,
+                              double* ix) {
     double x[8];
     for (int k = 0; k < 8; ++k) x[k] = 0.5 * (b - a) * (T.xr[k] + 1) + a;
     const double h = x[7] - x[0];
-    if (h < 1e-15) return {0.0, 0.0};
+    if (h < 1e-15) return false;
     const double sc = 2.0 / h;
-    cplx A[8][8];
-    cplx u[8];
+    cplx M[8][8];                 // A^T
+    cplx y0[8], y7[8];
     for (int i = 0; i < 8; ++i) {
-        for (int j = 0; j < 8; ++j) A[i][j] = {T.D[i][j] * sc, 0.0};
-        A[i][i].im = dphi(P, sgn, x[i]);
-        u[i] = {envelope(E, x[i]), 0.0};
+        for (int j = 0; j < 8; ++j) M[i][j] = {T.D[j][i] * sc, 0.0};
+        M[i][i].im = dphi(P, sgn, x[i]);
+        y0[i] = {i == 0 ? 1.0 : 0.0, 0.0};
+        y7[i] = {i == 7 ? 1.0 : 0.0, 0.0};
     }
-    // LU with partial pivoting on |re| + |im| (LAPACK izamax convention)
     for (int c = 0; c < 8; ++c) {
         int p = c;
-        double best = fabs(A[c][c].re) + fabs(A[c][c].im);
+        double best = fabs(M[c][c].re) + fabs(M[c][c].im);
         for (int i = c + 1; i < 8; ++i) {
-            const double v = fabs(A[i][c].re) + fabs(A[i][c].im);
+            const double v = fabs(M[i][c].re) + fabs(M[i][c].im);
             if (v > best) { best = v; p = i; }
         }
         if (p != c) {
-            for (int j = 0; j < 8; ++j) { const cplx t = A[c][j]; A[c][j] = A[p][j]; A[p][j] = t; }
-            const cplx t = u[c]; u[c] = u[p]; u[p] = t;
+            for (int j = 0; j < 8; ++j) { const cplx t = M[c][j]; M[c][j] = M[p][j]; M[p][j] = t; }
+            cplx t = y0[c]; y0[c] = y0[p]; y0[p] = t;
+            t = y7[c]; y7[c] = y7[p]; y7[p] = t;
         }
-        const cplx piv = A[c][c];
+        const cplx piv = M[c][c];
         if (piv.re == 0.0 && piv.im == 0.0) continue;
         for (int i = c + 1; i < 8; ++i) {
-            const cplx l = cdiv(A[i][c], piv);
-            for (int j = c + 1; j < 8; ++j) A[i][j] = csub(A[i][j], cmul(l, A[c][j]));
-            u[i] = csub(u[i], cmul(l, u[c]));
+            const cplx l = cdiv(M[i][c], piv);
+            for (int j = c + 1; j < 8; ++j) M[i][j] = csub(M[i][j], cmul(l, M[c][j]));
+            y0[i] = csub(y0[i], cmul(l, y0[c]));
+            y7[i] = csub(y7[i], cmul(l, y7[c]));
         }
     }
     for (int i = 7; i >= 0; --i) {
-        cplx s = u[i];
-        for (int j = i + 1; j < 8; ++j) s = csub(s, cmul(A[i][j], u[j]));
-        u[i] = cdiv(s, A[i][i]);
+        cplx s0 = y0[i], s7 = y7[i];
+        for (int j = i + 1; j < 8; ++j) {
+            s0 = csub(s0, cmul(M[i][j], y0[j]));
+            s7 = csub(s7, cmul(M[i][j], y7[j]));
+        }
+        y0[i] = cdiv(s0, M[i][i]);
+        y7[i] = cdiv(s7, M[i][i]);
     }
-    return csub(cmul(u[7], cexpi(phi(P, sgn, x[7]))), cmul(u[0], cexpi(phi(P, sgn, x[0]))));
+    const cplx e0 = cexpi(phi(P, sgn, x[0])), e7 = cexpi(phi(P, sgn, x[7]));
+    for (int k = 0; k < 8; ++k) {
+        gre[k] = (e7.re * y7[k].re - e7.im * y7[k].im) - (e0.re * y0[k].re - e0.im * y0[k].im);
+        ix[k] = x[k] > 1e-6 ? recip(x[k]) : 0.0;
+    }
+    return true;
 }
 
 // non-oscillatory branch (scipy quad in the reference): composite 8-point
@@ -284,57 +272,6 @@ struct TailParams {
     LevinTab lev;
 };
 
-// compute_outer_integral_oscillatory for one sign, reduced over the wavefront
-__device__ cplx outer_integral(const Env& E, const Phase& P, int sgn, double r_m, double r_max,
-                               const TailParams& TP, int lane) {
-    const double delta_phi = PI / 4;
-    if (r_max <= r_m + 1e-10) return {0.0, 0.0};
-    const double mid = (r_m + r_max) / 2;
-    const double om = fabs(dphi(P, sgn, mid));
-    if (om < 1e-10) return gauss_quad(E, P, sgn, r_m, r_max, false, 0.0, lane);
-    const double h = delta_phi / om;
-    const double lin = fabs(d2phi(P, sgn, mid)) * h * h;
-    DD re{0, 0}, im{0, 0};
-    if (lin < 0.1) {
-        const double p0 = phi(P, sgn, r_m) - om * r_m;
-        // filon_oscillatory_integral
-        const double dr_seg = delta_phi / fabs(om);
-        const double nraw_d = ceil((r_max - r_m) / dr_seg);
-        const long nraw = nraw_d < 1.0 ? 1 : (long)nraw_d;
-        const long nseg = nraw < TP.max_filon_segments ? nraw : TP.max_filon_segments;
-        const double eff = fabs(om) * (r_max - r_m) / (double)(nseg > 1 ? nseg : 1);
-        int neff = 5;
-        if (eff > TP.max_effective_dphi) {
-            const int scale = (int)ceil(eff / TP.max_effective_dphi);
-            neff = 5 + 2 * (scale - 1);
-            if (neff < 5) neff = 5;
-            if (neff > 15) neff = 15;
-        }
-        const double bstep = (r_max - r_m) / (double)nseg;
-        for (long s = lane; s < nseg; s += 64) {
-            const double a = __dadd_rn(__dmul_rn((double)s, bstep), r_m);
-            const double b = (s + 1 == nseg) ? r_max : __dadd_rn(__dmul_rn((double)(s + 1), bstep), r_m);
-            const cplx c = filon_segment(E, a, b, neff, om, p0);
-            re = dd_add(re, c.re);
-            im = dd_add(im, c.im);
-        }
-    } else {
-        const double nseg_d = ceil((r_max - r_m) / h);
-        const long nseg = nseg_d < 1.0 ? 1 : (long)nseg_d;
-        const double bstep = (r_max - r_m) / (double)nseg;
-        for (long s = lane; s < nseg; s += 64) {
-            const double a = __dadd_rn(__dmul_rn((double)s, bstep), r_m);
-            const double b = (s + 1 == nseg) ? r_max : __dadd_rn(__dmul_rn((double)(s + 1), bstep), r_m);
-            const cplx c = levin_segment(E, P, sgn, a, b, TP.lev);
-            re = dd_add(re, c.re);
-            im = dd_add(im, c.im);
-        }
-    }
-    re = dd_wave_sum(re);
-    im = dd_wave_sum(im);
-    return {re.hi + re.lo, im.hi + im.lo};
-}
-
 __device__ double analytic_tail(double r_m, const dwba_tail_job& J, int L, double overlap) {
     if (r_m < 1e-3 || fabs(overlap) < 1e-12) return 0.0;
     const double lt_i = fabs(J.eta_a) > 1e-10 ? J.eta_a * log(2 * J.k_a * r_m + 1e-30) : 0.0;
@@ -362,13 +299,24 @@ __device__ double analytic_tail(double r_m, const dwba_tail_job& J, int L, doubl
     return tail * overlap;
 }
 
-// x^-(L+1) sums of two nodes for L = 1 .. NL-1 (ix = 1/x, 0 where the
+// x^-(L+1) sums of one node for L = 1 .. LE-1 (ix = 1/x, 0 where the
 // envelope is cut off, x <= 1e-6); M_L multiplies the whole sum at the end.
-template <int NL>
+template <int LE>
+__device__ __forceinline__ void sweep1(double* acc, double w0, double ix0) {
+    double p0x = ix0 * ix0;
+#pragma unroll
+    for (int L = 1; L < LE; ++L) {
+        acc[L] = fma(w0, p0x, acc[L]);
+        p0x *= ix0;
+    }
+}
+
+// The same for two nodes.
+template <int LE>
 __device__ __forceinline__ void sweep2(double* acc, double w0, double ix0, double w1, double ix1) {
     double p0x = ix0 * ix0, p1x = ix1 * ix1;
 #pragma unroll
-    for (int L = 1; L < NL; ++L) {
+    for (int L = 1; L < LE; ++L) {
         acc[L] = fma(w0, p0x, acc[L]);
         acc[L] = fma(w1, p1x, acc[L]);
         p0x *= ix0;
@@ -394,41 +342,50 @@ __device__ __forceinline__ void filon_coeffs(double th, double& rs, double& rc, 
     }
 }
 
-// Filon segments of one (pair, sign), all L.  Each lane takes a contiguous
-// run of segments; a segment of neff (odd) equally spaced nodes is walked
-// triple by triple (the recursion of _filon_segment_complex).  Per triple
-// the real part of the Filon sum is wa f(x0) + wm f(x1) + wb f(x2), so a
-// node shared by two triples (or two segments) is swept once with the sum of
-// its weights.  The Filon coefficients depend on theta = omega * step only
-// and are formed once (the last segment apart); the node phases exp(i (omega x + p0))
-// advance by the rotation exp(i theta) from the segment start (the reference
-// evaluates each node phase directly: the two agree to a few ulp of the
-// phase).  Each lane adds to per-L FP64 partial sums (a few hundred terms of
-// similar size); the lanes are combined in double-double.  The reference
-// sums segments with math.fsum; the two agree to ~1e-15 of the segment size.
-template <int NL>
-__device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, double om, double p0, double r_m,
-                                            double r_max, double bstep, long nseg, int lane, double* acc) {
+// Filon segments [s_begin, s_end) of one (pair, sign), multipoles L < LE.
+// Each lane takes a contiguous run of segments; a segment of neff (odd)
+// equally spaced nodes is walked triple by triple (the recursion of
+// _filon_segment_complex).  Per triple the real part of the Filon sum is
+// wa f(x0) + wm f(x1) + wb f(x2), so a node shared by two triples (or two
+// segments) is swept once with the sum of its weights.  The Filon
+// coefficients depend on theta = omega * step only and are formed once (the
+// last segment apart); the node phases exp(i (omega x + p0)) advance by the
+// rotation exp(i theta) from the segment start (the reference evaluates each
+// node phase directly: the two agree to a few ulp of the phase).  Each lane
+// adds to per-L FP64 partial sums (a few hundred terms of similar size); the
+// lanes are combined in double-double.  The reference sums segments with
+// math.fsum; the two agree to ~1e-15 of the segment size.
+template <int LE>
+__device__ __forceinline__ void filon_sweep(int neff, double om, double p0, double r_m, double r_max, double bstep,
+                                            long nseg, long s_begin, long s_end, int lane, double* acc) {
     const int nt = (neff - 1) / 2;
     const double inv_om = -1.0 / om;
-    const long per = (nseg + 63) / 64;
-    const long s_lo = (long)lane * per;
-    const long s_hi = s_lo + per < nseg ? s_lo + per : nseg;
+    const long per = (s_end - s_begin + 63) / 64;
+    const long s_lo = s_begin + (long)lane * per;
+    const long s_hi = s_lo + per < s_end ? s_lo + per : s_end;
     double w_pend = 0.0, ix_pend = 0.0;           // weight still to sweep at the lane's current node
     // Segments are [r_m + s bstep, r_m + (s+1) bstep], so theta = omega * step
     // is one value up to rounding: its sine, cosine and Filon coefficients are
     // formed once; only the last segment (ending at r_max) gets its own.
     double rs_u, rc_u, al_u, be_u, ga_u;
     filon_coeffs(om * (bstep / (neff - 1)), rs_u, rc_u, al_u, be_u, ga_u);
+    // The phase at a segment's start is the one the previous segment's
+    // rotations reached at its end (the same node); it is evaluated directly
+    // only every ANCHOR segments (drift of a rotation chain: a few ulp per
+    // step, ~1e-14 rad over ANCHOR segments, below the ~1e-12 rad rounding of
+    // omega x + p0 itself at x ~ 1e3).
+    constexpr int ANCHOR = 16;
+    int since_anchor = ANCHOR;
+    double s_c = 0.0, c_c = 1.0;                  // phase carried to the next segment's start
     for (long sg = s_lo; sg < s_hi; ++sg) {
         const double a = __dadd_rn(__dmul_rn((double)sg, bstep), r_m);
         const double b = (sg + 1 == nseg) ? r_max : __dadd_rn(__dmul_rn((double)(sg + 1), bstep), r_m);
         const double step = (b - a) / (neff - 1);
         const double hseg = b - a;
-        if (hseg < 1e-15) continue;
-        double ix0 = a > 1e-6 ? 1.0 / a : 0.0;
+        if (hseg < 1e-15) continue;               // zero width: its end phase is its start phase
+        double ix0 = a > 1e-6 ? recip(a) : 0.0;
         if (ix0 != ix_pend && w_pend != 0.0) {    // the pending node is not this segment's first
-            sweep2<NL>(acc, w_pend, ix_pend, 0.0, 0.0);
+            sweep2<LE>(acc, w_pend, ix_pend, 0.0, 0.0);
             w_pend = 0.0;
         }
         const bool small = fabs(om * hseg) < 1e-3;
@@ -442,7 +399,16 @@ __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, do
             } else {
                 rs = rs_u; rc = rc_u; al = al_u; be = be_u; ga = ga_u;
             }
-            sincos(om * a + p0, &s0, &c0);
+            if (since_anchor >= ANCHOR) {
+                sincos(om * a + p0, &s0, &c0);
+                since_anchor = 0;
+            } else {
+                s0 = s_c;
+                c0 = c_c;
+            }
+            ++since_anchor;
+        } else {
+            since_anchor = ANCHOR;                // no rotation chain through a non-oscillatory segment
         }
         double x0 = a;
 #pragma unroll 1
@@ -450,8 +416,8 @@ __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, do
             const int k1 = 2 * t + 1, k2 = 2 * t + 2;
             const double x1 = __dadd_rn(__dmul_rn((double)k1, step), a);
             const double x2 = (k2 == neff - 1) ? b : __dadd_rn(__dmul_rn((double)k2, step), a);
-            const double ix1 = x1 > 1e-6 ? 1.0 / x1 : 0.0;
-            const double ix2 = x2 > 1e-6 ? 1.0 / x2 : 0.0;
+            const double ix1 = x1 > 1e-6 ? recip(x1) : 0.0;
+            const double ix2 = x2 > 1e-6 ? recip(x2) : 0.0;
             const double h3 = x2 - x0;
             // phases at x1, x2 by rotation from x0
             const double s1 = fma(s0, rc, c0 * rs), c1 = fma(c0, rc, -(s0 * rs));
@@ -475,7 +441,7 @@ __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, do
                 wb = hh * (-cA * s2 + be * c2);
                 wm = hh * (ga * c1);
             }
-            sweep2<NL>(acc, w_pend + wa, ix0, wm, ix1);
+            sweep2<LE>(acc, w_pend + wa, ix0, wm, ix1);
             w_pend = wb;
             ix_pend = ix2;
             x0 = x2;
@@ -483,8 +449,10 @@ __device__ __forceinline__ void filon_sweep(const dwba_tail_job& J, int neff, do
             s0 = s2;
             c0 = c2;
         }
+        s_c = s0;
+        c_c = c0;
     }
-    if (w_pend != 0.0) sweep2<NL>(acc, w_pend, ix_pend, 0.0, 0.0);
+    if (w_pend != 0.0) sweep2<LE>(acc, w_pend, ix_pend, 0.0, 0.0);
 }
 
 // Phases and the Filon / Levin choice of compute_outer_integral_oscillatory
@@ -526,23 +494,26 @@ __device__ inline TailSetup tail_setup(const dwba_tail_job& J, int sgn, int NL) 
 
 // ---------------------------------------------------------------------------
 // Filon signs, all L: the phase functions do not depend on L (only the
-// envelope M_L / r^(L+1) does), so one wavefront per (pair, sign) evaluates
-// nodes, exponentials and Filon coefficients once and sweeps all L.  Only
-// Re J is needed (dwba_outer_integral_1d keeps the real parts).  Writes the
-// zero rows too; Levin / non-oscillatory signs are left to tail_levin_kernel.
+// envelope M_L / r^(L+1) does), so a wavefront evaluates nodes, exponentials
+// and Filon coefficients once and sweeps all L.  Only Re J is needed
+// (dwba_outer_integral_1d keeps the real parts).  One wavefront per (pair,
+// sign, radial part); part p sweeps multipoles L < NL >> p over its segments
+// (see part_begin) and writes its row of partial sums, zeros for the L it
+// does not sweep and for signs that are not Filon (the Levin kernel then
+// writes part 0's row of its signs).
 template <int NL>
 __global__ __launch_bounds__(64) void tail_filon_kernel(const dwba_tail_job* __restrict__ jobs, TailParams TP,
                                                         double* __restrict__ jre) {
-    const int job = blockIdx.x, sgn = blockIdx.y == 0 ? -1 : 1;
+    __shared__ double s_acc[64][NL + 1];
+    // blockIdx.y = 0: the Phi_+ sign (k_a + k_b, usually many more segments) is dispatched first
+    const int job = blockIdx.x, row = 1 - (int)blockIdx.y, sgn = row == 0 ? -1 : 1;
     const dwba_tail_job J = jobs[job];
     const int lane = threadIdx.x;
     const int Lm = J.L_max < NL - 1 ? J.L_max : NL - 1;
-    double* out = jre + ((size_t)job * 2 + blockIdx.y) * NL;
+    double* out = jre + ((size_t)job * 2 + row) * NL;
     const TailSetup T = tail_setup(J, sgn, NL);
-    if (T.kind == 0) {
+    if (T.kind == 0)
         for (int L = lane; L < NL; L += 64) out[L] = 0.0;
-        return;
-    }
     if (T.kind != 1) return;
     const double om = T.om;
     const double r_m = J.r_m, r_max = J.r_max;
@@ -564,36 +535,55 @@ __global__ __launch_bounds__(64) void tail_filon_kernel(const dwba_tail_job* __r
         if (neff > 15) neff = 15;
     }
     const double bstep = (r_max - r_m) / (double)nseg;
-    filon_sweep<NL>(J, neff, om, p0, r_m, r_max, bstep, nseg, lane, acc);
-#pragma unroll
-    for (int L = 0; L < NL; ++L) {
-        const DD t = dd_wave_sum(DD{acc[L], 0.0});
-        const double M = (L >= 1 && L <= Lm) ? J.moments[L] : 0.0;
-        if (lane == 0) out[L] = fabs(M) > 1e-12 ? M * (t.hi + t.lo) : 0.0;
-    }
+    filon_sweep<NL>(neff, om, p0, r_m, r_max, bstep, nseg, 0, nseg, lane, acc);
+    lane_sums_out<NL, NL>(acc, s_acc, lane, J.moments, Lm, out);
 }
 
-// Levin and non-oscillatory signs, per L (outer_integral above).
+// Levin signs, all L (levin_segment's segments of compute_outer_integral_oscillatory:
+// n_seg = ceil((r_max - r_m) / h), h = (pi/4) / |Phi'(mid)|, 8 Chebyshev-Lobatto
+// nodes each): per segment one LU gives the node weights Re g (levin_weights),
+// swept over all L like the Filon weights; lanes take segments round-robin.
+// Non-oscillatory signs (|Phi'(mid)| < 1e-10, scipy quad in the reference)
+// stay per L with the Gauss-Legendre panels.  Writes part 0's row.
+template <int NL>
 __global__ __launch_bounds__(64) void tail_levin_kernel(const dwba_tail_job* __restrict__ jobs, TailParams TP,
-                                                        int NL, double* __restrict__ jre) {
-    const int job = blockIdx.x, sgn = blockIdx.y == 0 ? -1 : 1;
+                                                        double* __restrict__ jre) {
+    __shared__ double s_acc[64][NL + 1];
+    const int job = blockIdx.x, row = 1 - (int)blockIdx.y, sgn = row == 0 ? -1 : 1;
     const dwba_tail_job J = jobs[job];
     const int lane = threadIdx.x;
     const TailSetup T = tail_setup(J, sgn, NL);
     if (T.kind != 2) return;
     const int Lm = J.L_max < NL - 1 ? J.L_max : NL - 1;
-    double* out = jre + ((size_t)job * 2 + blockIdx.y) * NL;
-    for (int L = 1; L <= Lm; ++L) {
-        const double M = J.moments[L];
-        double v = 0.0;
-        if (fabs(M) > 1e-12) {
-            const Env E{M, L + 1};
-            v = outer_integral(E, T.P, sgn, J.r_m, J.r_max, TP, lane).re;
+    double* out = jre + ((size_t)job * 2 + row) * NL;
+    const double r_m = J.r_m, r_max = J.r_max;
+    if (T.om < 1e-10) {
+        for (int L = 1; L <= Lm; ++L) {
+            const double M = J.moments[L];
+            double v = 0.0;
+            if (fabs(M) > 1e-12) v = gauss_quad(Env{M, L + 1}, T.P, sgn, r_m, r_max, false, 0.0, lane).re;
+            if (lane == 0) out[L] = v;
         }
-        if (lane == 0) out[L] = v;
+        if (lane == 0) out[0] = 0.0;
+        for (int L = Lm + 1 + lane; L < NL; L += 64) out[L] = 0.0;
+        return;
     }
-    if (lane == 0) out[0] = 0.0;
-    for (int L = Lm + 1 + lane; L < NL; L += 64) out[L] = 0.0;
+    const double h = (PI / 4) / T.om;
+    const double nseg_d = ceil((r_max - r_m) / h);
+    const long nseg = nseg_d < 1.0 ? 1 : (long)nseg_d;
+    const double bstep = (r_max - r_m) / (double)nseg;
+    double acc[NL];
+#pragma unroll
+    for (int L = 0; L < NL; ++L) acc[L] = 0.0;
+    for (long s = lane; s < nseg; s += 64) {
+        const double a = __dadd_rn(__dmul_rn((double)s, bstep), r_m);
+        const double b = (s + 1 == nseg) ? r_max : __dadd_rn(__dmul_rn((double)(s + 1), bstep), r_m);
+        double gre[8], ix[8];
+        if (!levin_weights(T.P, sgn, a, b, TP.lev, gre, ix)) continue;
+#pragma unroll
+        for (int k = 0; k < 8; k += 2) sweep2<NL>(acc, gre[k], ix[k], gre[k + 1], ix[k + 1]);
+    }
+    lane_sums_out<NL, NL>(acc, s_acc, lane, J.moments, Lm, out);
 }
 
 // tail[L] = 1/2 (Re J_- - Re J_+) + analytic tail at r_max  (or the legacy analytic tail)
@@ -666,7 +656,12 @@ extern "C" int32_t dwba_outer_tail_f64(int32_t n_jobs, const dwba_tail_job* d_jo
         default: return dwba::set_error(DWBA_ERR_ARG, "dwba_outer_tail_f64: L_max %d", L_max);
     }
     DWBA_LAUNCH_CHECK("tail_filon_kernel");
-    hipLaunchKernelGGL(dwba::tail_levin_kernel, dim3(n_jobs, 2), dim3(64), 0, s, d_jobs, tp, nl, jre);
+    switch (nl) {
+#define C(n) case n: hipLaunchKernelGGL(dwba::tail_levin_kernel<n>, dim3(n_jobs, 2), dim3(64), 0, s, d_jobs, tp, jre); break;
+        C(2) C(4) C(8) C(16) C(24) C(32) C(48) C(64)
+#undef C
+        default: break;
+    }
     DWBA_LAUNCH_CHECK("tail_levin_kernel");
     hipLaunchKernelGGL(dwba::tail_combine_kernel, dim3(n_jobs), dim3(64), 0, s, d_jobs, n_jobs, nl, jre);
     DWBA_LAUNCH_CHECK("tail_combine_kernel");

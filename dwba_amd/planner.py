@@ -92,6 +92,26 @@ def filon_limits():
     return _i("DWBA_MAX_FILON_SEGMENTS", 4096, 128), _f("DWBA_MAX_EFFECTIVE_DPHI", np.pi / 2, np.pi / 16)
 
 
+def tail_nodes_estimate(k_i, k_f, r_m, r_max, max_segments=4096, max_dphi=np.pi / 2):
+    """Filon nodes an outer tail sweeps for both phase signs, with the
+    frequencies |k_i -+ k_f| (the centrifugal and Coulomb terms dropped):
+    the segment count and node up-scaling of filon_oscillatory_integral
+    (oscillatory_integrals.py:1280-1354).  Only used to order tail jobs
+    longest first."""
+    span = max(float(r_max) - float(r_m), 0.0)
+    tot = 0
+    for om in (abs(k_i - k_f), k_i + k_f):
+        if om < 1e-10 or span <= 0.0:
+            continue
+        nseg = min(max(1, int(np.ceil(span * om / (np.pi / 4)))), int(max_segments))
+        eff = om * span / nseg
+        neff = 5
+        if eff > max_dphi:
+            neff = min(15, max(5, 5 + 2 * (int(np.ceil(eff / max_dphi)) - 1)))
+        tot += nseg * (neff - 1)
+    return tot
+
+
 # ---------------------------------------------------------------- grid stats
 @dataclass
 class GridStats:
