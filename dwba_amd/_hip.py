@@ -90,7 +90,7 @@ _SIGS = {
     "dwba_ddot_mfma_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_vp, c_vp]),
     "dwba_semisep_lane_table_stride": (c_i32, [c_i32, c_i32]),
     "dwba_filon_cells_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp]),
-    "dwba_filon_apply_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_vp]),
+    "dwba_filon_fused_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_vp]),
     "dwba_semisep_lane_tables_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_i32, c_vp]),
     "dwba_il_semisep_lanes_tab_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp]),
     "dwba_il_semisep_lanes_split_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),

@@ -35,7 +35,7 @@ from .planner import GridStats, PREP_PLAIN, PREP_FILON, PREP_STANDARD, PREP_ZERO
 
 PREP_NO_ZERO_FILL = 0x10   # include/dwba_hip.h: the batch's per-pair vectors are zero-initialised once
 PREP_FILON_GROUPED = 0x20  # include/dwba_hip.h: Filon alpha/beta from the group cell sums instead
-GROUP_SLICE = 16           # members per workgroup of the grouped Filon apply pass
+GROUP_SLICE = 4            # members per workgroup of the fused grouped Filon pass (its FUSE_MAX)
 from .dwba_matrix_elements import (PairPlan, RadialDWBAIntegrals, TAIL_NONE, TAIL_OUTER, TAIL_LEGACY, plan_pair,
                                    _fit)
 
@@ -618,7 +618,7 @@ class ScanBatch:
     def _build_filon_groups(self):
         """Filon pairs of one energy with the same split index, phase layout
         and interpolation kinds share their cell sums: groups for
-        dwba_filon_cells_f64 / dwba_filon_apply_f64 (prep_grouped)."""
+        dwba_filon_cells_f64 / dwba_filon_fused_f64 (prep_grouped)."""
         torch, hip = self.torch, self.hip
         self.prep_grouped = False
         self.filon_groups = None
@@ -636,10 +636,10 @@ class ScanBatch:
             members += g
             off += 7 * ld
             first += len(g)
-        # the apply pass takes slices of <= GROUP_SLICE members of a group
-        # (same cell sums): more workgroups, shorter per-thread member loops
+        # the fused pass takes slices of <= GROUP_SLICE members of a group (same
+        # cell sums), one workgroup each over its grid, longest grids first
         sl = []
-        for i in range(len(gl)):
+        for i in sorted(range(len(gl)), key=lambda i: -self.pairs[gl[i][0]][2].N):
             G = arr[i]
             for q in range(0, G.count, GROUP_SLICE):
                 sl.append(FilonGroup(G.coef, G.ld_coef, G.rep, G.first + q, min(GROUP_SLICE, G.count - q)))
@@ -652,8 +652,8 @@ class ScanBatch:
 
     def _prep_jobs(self, gemv, grouped):
         """Device prep jobs: with gemv, beta_dir only where the shared-beta
-        path reads it; with grouped, the grouped Filon pairs skip the per-pair
-        cell pass (dwba_filon_apply_f64 writes their alpha / beta_ex)."""
+        path reads it; with grouped, the per-pair prep leaves the grouped Filon
+        pairs to dwba_filon_fused_f64 (alpha / beta, beta_dir, L = 0 sums)."""
         key = (bool(gemv), bool(grouped))
         if key == (False, False):
             return self.prep_d
@@ -877,8 +877,8 @@ class ScanBatch:
                 gd, ng, mem, max_n1, _, sd, ns = self.filon_groups
                 hip.check(lib.dwba_filon_cells_f64(ng, hip.ptr(gd), hip.ptr(prep_jobs), cxp, cwp, self.CC_nodes, h),
                           "dwba_filon_cells_f64")
-                hip.check(lib.dwba_filon_apply_f64(ns, hip.ptr(sd), hip.ptr(prep_jobs), hip.ptr(mem), max_n1, h),
-                          "dwba_filon_apply_f64")
+                hip.check(lib.dwba_filon_fused_f64(ns, hip.ptr(sd), hip.ptr(prep_jobs), hip.ptr(mem), h),
+                          "dwba_filon_fused_f64")
 
         # Pipelined step (no stage events): prep of the second half of the
         # energies runs on the launch stream while the first half is

@@ -272,6 +272,7 @@ __device__ void filon_alpha_beta(const dwba_prep_job& J, const CCTable& cc) {
 __global__ __launch_bounds__(PREP_THREADS) void prep_f64_kernel(const dwba_prep_job* __restrict__ jobs,
                                                                 CCTable cc) {
     const dwba_prep_job J = jobs[blockIdx.x];
+    if (J.mode & DWBA_PREP_FILON_GROUPED) return;   // dwba_filon_cells_f64 + dwba_filon_fused_f64 do it all
     const int N = J.N, N1 = J.N1;
     __shared__ double s_red[PREP_THREADS / 64][4];
 
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(PREP_THREADS) void prep_f64_kernel(const dwba_prep_
         for (int wv = 0; wv < PREP_THREADS / 64; ++wv) sum += s_red[wv][threadIdx.x];
         J.scalars[threadIdx.x] = sum;
     }
-    if (mode != DWBA_PREP_FILON || (J.mode & DWBA_PREP_FILON_GROUPED)) return;
+    if (mode != DWBA_PREP_FILON) return;
 
     // FILON
     if (J.phase_nodes) filon_alpha_beta<true>(J, cc);
@@ -382,46 +383,134 @@ __global__ __launch_bounds__(PREP_THREADS) void filon_cells_kernel(const dwba_fi
     else filon_cells<false>(G, J, cc);
 }
 
-// alpha_dir, alpha_ex, beta_ex of every member of a group: one thread per
-// point j < N1, the group's cell sums loaded once and applied to each member
-// (point j: cell j's part c00 rho_j + c01 rho_j+1 plus cell j-1's part
-// c01 rho_j-1 + c11 rho_j, the same sums filon_alpha_beta forms).
-__global__ __launch_bounds__(PREP_THREADS) void filon_apply_kernel(const dwba_filon_group* __restrict__ groups,
+// Grouped Filon prep of one slice of <= FUSE_MAX members of a group (same
+// grid, split index, phase layout and interpolation kinds), fused with the
+// per-pair streaming pass: one workgroup per slice sweeps the grid [0, N)
+// in chunks of 256 points, a thread per point.  The group's cell sums of
+// cell j and (from the neighbouring lane) cell j - 1, and the grid data
+// shared by the members (u_i, w, V_diff) are loaded once per point; each
+// member's densities once per member and point, the values at j - 1 and
+// j + 1 coming from the neighbouring lanes (lanes 0 and 63 load theirs).
+// Per member it writes, for j < N1,
+//   alpha_dir[j] = (c00 d_j + c01 d_j+1) + (c01' d_j-1 + c11' d_j),
+//   alpha_ex[j]  the same with e,  beta_ex[j] = (b00 f_j + b01 f_j+1) + (b10' f_j-1 + b11' f_j)
+// (primed: cell j - 1; d = chi_f chi_i, e = u_f chi_i, f = chi_f u_i, non-finite
+// values cleared) -- the per-pair cell pass's sums -- and beta_dir = (u_f u_i) w
+// on [0, N) where the job has a beta_dir, and reduces the four L = 0
+// correction sums of the streaming pass over the whole grid with w_lim = 0
+// beyond N1 (a non-finite density there still propagates, as in the
+// reference's dot products).
+constexpr int FUSE_MAX = 4;
+
+__global__ __launch_bounds__(PREP_THREADS) void filon_fused_kernel(const dwba_filon_group* __restrict__ slices,
                                                                    const dwba_prep_job* __restrict__ jobs,
                                                                    const int* __restrict__ members) {
-    const dwba_filon_group G = groups[blockIdx.y];
-    const int N1 = jobs[G.rep].N1;
-    const int j = blockIdx.x * PREP_THREADS + (int)threadIdx.x;
-    if (j >= N1) return;
+    __shared__ double s_red[PREP_THREADS / 64][FUSE_MAX * 4];
+    const dwba_filon_group G = slices[blockIdx.x];
+    const dwba_prep_job& R = jobs[G.rep];
+    const int N = R.N, N1 = R.N1;
+    const int cnt = G.count < FUSE_MAX ? G.count : FUSE_MAX;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t ld = (size_t)G.ld_coef;
     const double* __restrict__ co = G.coef;
-    const bool here = j < N1 - 1, prev = j >= 1;
-    const double c00 = here ? co[j] : 0.0, c01 = here ? co[ld + j] : 0.0;
-    const double p01 = prev ? co[ld + j - 1] : 0.0, p11 = prev ? co[2 * ld + j - 1] : 0.0;
-    const double b00 = here ? co[3 * ld + j] : 0.0, b01 = here ? co[4 * ld + j] : 0.0;
-    const double q10 = prev ? co[5 * ld + j - 1] : 0.0, q11 = prev ? co[6 * ld + j - 1] : 0.0;
-    for (int m = 0; m < G.count; ++m) {
-        const dwba_prep_job J = jobs[members[G.first + m]];
-        const double ci = J.chi_i[j], cf = J.chi_f[j], uf = J.u_f[j], ui = J.u_i[j];
-        const double d = clean(cf * ci), e = clean(uf * ci), f = clean(cf * ui);
-        double s0d = 0.0, s0e = 0.0, s0b = 0.0, s1d = 0.0, s1e = 0.0, s1b = 0.0;
+    const double* __restrict__ ui_p = R.u_i;
+    const double* __restrict__ w_p = R.w;
+    const double* __restrict__ v_p = R.V_diff;
+    const dwba_prep_job* Jm[FUSE_MAX];
+#pragma unroll
+    for (int m = 0; m < FUSE_MAX; ++m) Jm[m] = &jobs[members[G.first + (m < cnt ? m : 0)]];
+    double acc[FUSE_MAX][4];
+#pragma unroll
+    for (int m = 0; m < FUSE_MAX; ++m)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[m][q] = 0.0;
+    for (int base = 0; base < N; base += PREP_THREADS) {
+        const int j = base + (int)threadIdx.x;
+        const bool in = j < N;
+        const bool here = j < N1 - 1;                   // cell j exists
+        const bool prev = j >= 1 && j < N1;             // cell j - 1 exists
+        // ---- every load of the chunk first (the stores below could alias them)
+        double c00 = 0.0, c01 = 0.0, c11 = 0.0, b00 = 0.0, b01 = 0.0, b10 = 0.0, b11 = 0.0;
         if (here) {
-            const double ci1 = J.chi_i[j + 1], cf1 = J.chi_f[j + 1];
-            const double d1 = clean(cf1 * ci1), e1 = clean(J.u_f[j + 1] * ci1), f1 = clean(cf1 * J.u_i[j + 1]);
-            s0d = c00 * d + c01 * d1;
-            s0e = c00 * e + c01 * e1;
-            s0b = b00 * f + b01 * f1;
+            c00 = co[j]; c01 = co[ld + j]; c11 = co[2 * ld + j];
+            b00 = co[3 * ld + j]; b01 = co[4 * ld + j]; b10 = co[5 * ld + j]; b11 = co[6 * ld + j];
         }
-        if (prev) {
-            const double ci0 = J.chi_i[j - 1], cf0 = J.chi_f[j - 1];
-            const double d0 = clean(cf0 * ci0), e0 = clean(J.u_f[j - 1] * ci0), f0 = clean(cf0 * J.u_i[j - 1]);
-            s1d = p01 * d0 + p11 * d;
-            s1e = p01 * e0 + p11 * e;
-            s1b = q10 * f0 + q11 * f;
+        double p01 = 0.0, p11 = 0.0, q10 = 0.0, q11 = 0.0;   // cell j - 1 (lane 0: loaded)
+        if (lane == 0 && prev) { p01 = co[ld + j - 1]; p11 = co[2 * ld + j - 1]; q10 = co[5 * ld + j - 1]; q11 = co[6 * ld + j - 1]; }
+        const double ui = in ? ui_p[j] : 0.0;
+        const double wf = in ? w_p[j] : 0.0;
+        const double vd = (in && v_p) ? v_p[j] : 0.0;
+        // the neighbours a wave's end lanes need: lane 63 at j + 1, lane 0 at j - 1
+        const int jh = lane == 63 ? j + 1 : j - 1;
+        const bool halo = (lane == 63 && here) || (lane == 0 && prev);
+        const double uih = halo ? ui_p[jh] : 0.0;
+        double ci[FUSE_MAX], cf[FUSE_MAX], uf[FUSE_MAX], cih[FUSE_MAX], cfh[FUSE_MAX], ufh[FUSE_MAX];
+#pragma unroll
+        for (int m = 0; m < FUSE_MAX; ++m) {
+            const bool live = in && m < cnt;
+            ci[m] = live ? Jm[m]->chi_i[j] : 0.0;
+            cf[m] = live ? Jm[m]->chi_f[j] : 0.0;
+            uf[m] = live ? Jm[m]->u_f[j] : 0.0;
+            const bool hl = halo && m < cnt;
+            cih[m] = hl ? Jm[m]->chi_i[jh] : 0.0;
+            cfh[m] = hl ? Jm[m]->chi_f[jh] : 0.0;
+            ufh[m] = hl ? Jm[m]->u_f[jh] : 0.0;
         }
-        J.alpha_dir[j] = s0d + s1d;
-        J.alpha_ex[j] = s0e + s1e;
-        J.beta_ex[j] = s0b + s1b;
+        {
+            const double t01 = __shfl_up(c01, 1, 64), t11 = __shfl_up(c11, 1, 64);
+            const double t10 = __shfl_up(b10, 1, 64), u11 = __shfl_up(b11, 1, 64);
+            if (lane != 0) { p01 = t01; p11 = t11; q10 = t10; q11 = u11; }
+        }
+        const double w = j < N1 ? wf : 0.0;
+        // ---- compute and store
+#pragma unroll
+        for (int m = 0; m < FUSE_MAX; ++m) {
+            if (m >= cnt) break;
+            const dwba_prep_job& J = *Jm[m];
+            acc[m][0] += (w * uf[m]) * ui;            // rho2_dir_w
+            acc[m][1] += ((w * cf[m]) * ci[m]) * vd;  // rho1_dir_w . V_diff
+            acc[m][2] += (w * cf[m]) * ui;            // rho2_ex_w
+            acc[m][3] += ((w * uf[m]) * ci[m]) * vd;  // rho1_ex_w . V_diff
+            if (in && J.beta_dir) J.beta_dir[j] = clean(uf[m] * ui) * wf;
+            const double d = clean(cf[m] * ci[m]), e = clean(uf[m] * ci[m]), f = clean(cf[m] * ui);
+            double d1 = __shfl_down(d, 1, 64), e1 = __shfl_down(e, 1, 64), f1 = __shfl_down(f, 1, 64);
+            double d0 = __shfl_up(d, 1, 64), e0 = __shfl_up(e, 1, 64), f0 = __shfl_up(f, 1, 64);
+            const double dh = clean(cfh[m] * cih[m]), eh = clean(ufh[m] * cih[m]), fh = clean(cfh[m] * uih);
+            if (lane == 63) { d1 = dh; e1 = eh; f1 = fh; }
+            if (lane == 0) { d0 = dh; e0 = eh; f0 = fh; }
+            if (j < N1) {
+                double s0d = 0.0, s0e = 0.0, s0b = 0.0, s1d = 0.0, s1e = 0.0, s1b = 0.0;
+                if (here) {
+                    s0d = c00 * d + c01 * d1;
+                    s0e = c00 * e + c01 * e1;
+                    s0b = b00 * f + b01 * f1;
+                }
+                if (prev) {
+                    s1d = p01 * d0 + p11 * d;
+                    s1e = p01 * e0 + p11 * e;
+                    s1b = q10 * f0 + q11 * f;
+                }
+                J.alpha_dir[j] = s0d + s1d;
+                J.alpha_ex[j] = s0e + s1e;
+                J.beta_ex[j] = s0b + s1b;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < FUSE_MAX; ++m) {
+        if (m >= cnt) break;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double v = wave_sum(acc[m][q]);
+            if (lane == 0) s_red[wave][m * 4 + q] = v;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < cnt * 4) {
+        const int m = threadIdx.x >> 2, q = threadIdx.x & 3;
+        double sum = 0.0;
+        for (int wv = 0; wv < PREP_THREADS / 64; ++wv) sum += s_red[wv][m * 4 + q];
+        jobs[members[G.first + m]].scalars[q] = sum;
     }
 }
 
@@ -545,15 +634,13 @@ extern "C" int32_t dwba_filon_cells_f64(int32_t n_groups, const dwba_filon_group
     return DWBA_OK;
 }
 
-extern "C" int32_t dwba_filon_apply_f64(int32_t n_groups, const dwba_filon_group* d_groups,
-                                       const dwba_prep_job* d_jobs, const int32_t* d_members, int32_t max_n1,
-                                       void* stream) {
-    DWBA_REQUIRE(dwba::initialised(), "dwba_filon_apply_f64: call dwba_init first");
-    DWBA_REQUIRE(n_groups > 0 && n_groups <= 65535 && d_groups && d_jobs && d_members && max_n1 > 0,
-                 "dwba_filon_apply_f64: bad arguments (n_groups=%d, max_n1=%d)", n_groups, max_n1);
-    hipLaunchKernelGGL(dwba::filon_apply_kernel, dim3((max_n1 + dwba::PREP_THREADS - 1) / dwba::PREP_THREADS, n_groups),
-                       dim3(dwba::PREP_THREADS), 0, reinterpret_cast<hipStream_t>(stream), d_groups, d_jobs,
-                       d_members);
-    DWBA_LAUNCH_CHECK("filon_apply_kernel");
+extern "C" int32_t dwba_filon_fused_f64(int32_t n_slices, const dwba_filon_group* d_slices,
+                                       const dwba_prep_job* d_jobs, const int32_t* d_members, void* stream) {
+    DWBA_REQUIRE(dwba::initialised(), "dwba_filon_fused_f64: call dwba_init first");
+    DWBA_REQUIRE(n_slices > 0 && d_slices && d_jobs && d_members, "dwba_filon_fused_f64: bad arguments (n_slices=%d)",
+                 n_slices);
+    hipLaunchKernelGGL(dwba::filon_fused_kernel, dim3(n_slices), dim3(dwba::PREP_THREADS), 0,
+                       reinterpret_cast<hipStream_t>(stream), d_slices, d_jobs, d_members);
+    DWBA_LAUNCH_CHECK("filon_fused_kernel");
     return DWBA_OK;
 }

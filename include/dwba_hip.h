@@ -196,8 +196,9 @@ enum {
  * and prepare the same pair again (a captured scan): the untouched points are
  * the same on every call and keep their zeros. */
 #define DWBA_PREP_NO_ZERO_FILL 0x10
-#define DWBA_PREP_FILON_GROUPED 0x20   /* FILON: streaming pass only; alpha_dir, alpha_ex, beta_ex come from
-                                          dwba_filon_cells_f64 + dwba_filon_apply_f64 */
+#define DWBA_PREP_FILON_GROUPED 0x20   /* FILON: nothing in this launch (Coulomb nodes aside); alpha/beta,
+                                          beta_dir and the scalars come from dwba_filon_cells_f64 +
+                                          dwba_filon_fused_f64 */
 #define DWBA_PREP_MODE_MASK 0x0f
 
 /* One channel pair.  Device pointers; outputs are the alpha and beta vectors (N doubles
@@ -266,15 +267,19 @@ typedef struct dwba_filon_group {
 } dwba_filon_group;
 int32_t dwba_filon_cells_f64(int32_t n_groups, const dwba_filon_group* d_groups, const dwba_prep_job* d_jobs,
                              const double* h_cc_x, const double* h_cc_w, int32_t n_cc, void* stream);
-/* alpha_dir, alpha_ex, beta_ex of every member job from its group's cells
- * (d_groups may hold slices of groups: entries with the same coef and rep
- * and disjoint member ranges, one workgroup row each):
- * point j takes cell j's part (c00 rho_j + c01 rho_j+1) and cell j-1's part
- * (c01 rho_j-1 + c11 rho_j); max_n1 bounds every group's N1.  With
- * dwba_filon_cells_f64 it replaces the same reference code (the density
- * interpolations at the nodes, oscillatory_integrals.py:2578-2644, :2667-2770). */
-int32_t dwba_filon_apply_f64(int32_t n_groups, const dwba_filon_group* d_groups, const dwba_prep_job* d_jobs,
-                             const int32_t* d_members, int32_t max_n1, void* stream);
+/* Everything dwba_fullsplit_prep_f64 writes for the member jobs of each
+ * slice (<= 4 members of one group: entries with the same coef and rep and
+ * disjoint member ranges), from the group's cells: alpha_dir, alpha_ex,
+ * beta_ex on [0, N1) (point j takes cell j's part c00 rho_j + c01 rho_j+1 and
+ * cell j-1's part c01 rho_j-1 + c11 rho_j), beta_dir = (u_f u_i) w on [0, N)
+ * where the job has one, and the four L = 0 correction scalars; the jobs'
+ * densities are read once.  The members share the representative's grid, N,
+ * N1, u_i and V_diff.  With dwba_filon_cells_f64 it replaces the density
+ * interpolations at the nodes (oscillatory_integrals.py:2578-2644,
+ * :2667-2770) and the density preparation of radial_ME_all_L
+ * (dwba_matrix_elements.py:1025-1049). */
+int32_t dwba_filon_fused_f64(int32_t n_slices, const dwba_filon_group* d_slices, const dwba_prep_job* d_jobs,
+                             const int32_t* d_members, void* stream);
 
 /* Multipole moments M_L = sum_j ((w_j r_j^L) u_f,j) u_i,j for L = 0..L_max of
  * n_sets density pairs (device pointer arrays d_r/d_w/d_u_i/d_u_f of length

@@ -307,9 +307,10 @@ def test_pipelined_step_bit_identical(gemv):
 @pytest.mark.parametrize("ionization", [False, True])
 def test_grouped_filon_prep_matches_per_pair(ionization):
     """Filon cell sums formed once per group of pairs (same energy, split
-    index and layout) and applied to each pair give the per-pair prep's
-    alpha / beta to rounding, hence the same integrals; the stacked vectors
-    are cleared first so every value must be rewritten."""
+    index and layout) and applied to each pair by the fused pass give the
+    per-pair prep's alpha / beta to rounding, hence the same integrals; the
+    stacked vectors, the shared direct beta and the L = 0 scalars are cleared
+    first so every value must be rewritten."""
     from dwba_amd.batch import ScanBatch
     scan = _small_scan(10, n_e=3, n_p=30, ionization=ionization)
     b = ScanBatch(scan, 12, precision="fp64", oscillatory_method="full_split")
@@ -321,13 +322,17 @@ def test_grouped_filon_prep_matches_per_pair(ionization):
     r0 = b.results()
     mag = b.magnitudes()
     vecs0 = [[m.clone() for m in (st.ad, st.ae, st.be)] for st in b.E]
+    sc0 = b.scalars.clone()
     b.prep_grouped = True
     for st in b.E:
-        for m in (st.ad, st.ae, st.be):
+        for m in (st.ad, st.ae, st.be, st.bd):
             m.zero_()
+    b.scalars.zero_()
     b.out.zero_()
     b.run()
     r1 = b.results()
+    a, c = b.scalars.cpu().numpy(), sc0.cpu().numpy()
+    assert np.all(np.abs(a - c) <= 1e-12 * np.abs(c) + 1e-15 * np.max(np.abs(c), axis=0) + 1e-300)
     for st, v0 in zip(b.E, vecs0):
         for m, m0 in zip((st.ad, st.ae, st.be), v0):
             a, c = m.cpu().numpy(), m0.cpu().numpy()
