@@ -40,6 +40,7 @@ if ROOT not in sys.path:
 METRIC = "I_L integrals/sec (all L, dir+exch) at N=15000, FP32-MFMA vs FP64; rel err"
 PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix) FLOP/s
 PEAK_FP32_MFMA = 157.3e12
+PEAK_HBM = 8.0e12        # MI355X HBM3E bytes/s
 
 
 def cpu_info():
@@ -140,7 +141,7 @@ def bench_kernel_only(args):
         "config": {"workload": "kernel-only single channel pair, exp grid N=15000 on [1e-5,1000] a0, "
                                "L=0..30, direct+exchange, plain FP64 (Simpson weights)",
                    "N": N, "L_max": L_max, "integrals_per_step": n_int, "seed": args.seed, "draws": draws},
-        "roofline": {"bound": "mfma", "achieved": W / (ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+        "roofline": {"bound": "fp64_valu", "achieved": W / (ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": W / (ms * 1e-3) / PEAK_FP64, "traffic": None,
                      "kernel": "plain_f64_kernel", "work_flop_per_launch": W,
                      "note": "FP64 VALU path; MI355X FP64 vector peak = FP64 matrix peak = 78.6 TF"},
@@ -189,7 +190,7 @@ def bench_backward(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "backward check: exp grid N=15000 on [1e-5,1000] a0, 32 pairs, L=0..20, "
                                "c_L ~ N(0,1)", "N": N, "pairs": P, "L_max": L_max},
-        "roofline": {"bound": "valu", "achieved": W / (ms * 1e-3) / 1e12, "peak": peak_fp32_valu / 1e12,
+        "roofline": {"bound": "fp32_valu", "achieved": W / (ms * 1e-3) / 1e12, "peak": peak_fp32_valu / 1e12,
                      "unit": "TFLOP/s", "frac": W / (ms * 1e-3) / peak_fp32_valu, "traffic": None,
                      "kernel": "il_backward_kernel<21,float>", "work_flop_per_launch": W},
         "fp64": {"ms": legs["fp64"], "tflops": W / (legs["fp64"] * 1e-3) / 1e12,
@@ -271,6 +272,62 @@ def semisep_work_flop(b):
             "lanes": bool(getattr(b, "semisep_lanes", False))}
 
 
+def tail_work_flop(b):
+    """Algorithmic FP64 work of the outer-tail stage: 3 flop (one FMA into the
+    sum, one multiply of the power chain x^-(L+1)) per swept node and multipole
+    L = 1..L_max, over the Filon and Levin nodes each (pair, sign) sweeps
+    (planner.tail_sweep_nodes: the reference's branch choice, segment counts
+    and node up-scaling).  Returns (flop, nodes)."""
+    from dwba_amd.dwba_matrix_elements import TAIL_OUTER
+    nodes = sum(sum(v) for k, v in getattr(b, "tail_nodes", {}).items() if b.pairs[k][2].tail_kind == TAIL_OUTER)
+    return 3.0 * nodes * b.L_max, nodes
+
+
+def prep_bytes(b):
+    """Algorithmic HBM bytes of the prep stage: every input read once and every
+    output written once, FP64 -- per pair chi_i, chi_f on [0, N) and alpha_dir,
+    alpha_ex, beta_ex on [0, N1); per energy u_i, u_f, w, V_core - U_i read and
+    the shared beta_dir written on [0, N)."""
+    tot = 0.0
+    for (ei, _, plan, *_rest) in b.pairs:
+        tot += 8.0 * (2 * plan.N + 3 * plan.N1)
+    for st in b.E:
+        tot += 8.0 * 5 * st.N
+    return tot
+
+
+def tail_entry_ms(b, steps, stream):
+    """Average duration (HIP events on the launch stream) of the outer-tail
+    entry point alone (dwba_outer_tail_f64: Filon, Levin and combine kernels)
+    on the batch's tail jobs, moments already in place."""
+    from dwba_amd import planner
+    hip, lib = b.hip, b.lib
+    sh = hip.stream_handle(stream)
+    seg, dmax = planner.filon_limits()
+    Dp, D_keep = hip.f64_array(b.levin[0])
+    Xp, X_keep = hip.f64_array(b.levin[1])
+    fn = lambda: hip.check(lib.dwba_outer_tail_f64(b.n_tail, hip.ptr(b.tail_d), b.L_max, seg, dmax, Dp, Xp,
+                                                   hip.ptr(b.tail_ws), b.tail_ws_bytes, sh), "dwba_outer_tail_f64")
+    fn()
+    return _events_ms(fn, steps, stream)
+
+
+def stage_times(b, reps=5):
+    """Stage durations (ms) of eager one-stream launches with events, averaged."""
+    import torch
+    acc = {}
+    ov = b.overlap
+    b.overlap = None
+    for _ in range(reps):
+        ev = {}
+        b.launch(events=ev)
+        torch.cuda.synchronize()
+        for k, v in b.stage_ms(ev).items():
+            acc[k] = acc.get(k, 0.0) + v / reps
+    b.overlap = ov
+    return acc
+
+
 def scan_kernel_ms(b, steps, stream):
     """Average duration (HIP events on the launch stream) of the batch's
     per-pair recurrence kernel alone -- the lane-serial kernel (table-driven
@@ -312,16 +369,30 @@ def scan_kernel_ms(b, steps, stream):
     return name, ms
 
 
-def _committed_traffic(stage, workload):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
-    scaled by the copy-kernel calibration) and recorded in profiles/traffic.json."""
+def _source_sha16(name):
+    import hashlib
+    with open(os.path.join(ROOT, "dwba_amd", "csrc", name), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
+def _committed_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and
+    recorded in profiles/traffic.json together with the SHA-256 of the source
+    file the kernel was built from.  Returns (bytes, provenance); bytes is None
+    when there is no record or the source has changed since it was measured."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
-            d = json.load(f)
-        return d.get(workload, {}).get(stage)
+            rec = json.load(f).get(workload, {}).get("kernels", {}).get(kernel)
     except (OSError, ValueError):
-        return None
+        rec = None
+    if not rec or "source" not in rec:
+        return None, "no counter record for this kernel source"
+    sha = _source_sha16(rec["source"])
+    if sha != rec.get("source_sha16"):
+        return None, f"stale: {rec['source']} changed since the counters were taken ({rec.get('source_sha16')})"
+    return rec["fetch_bytes"] + rec["write_bytes"], f"{rec['source']} sha256 {sha}, rocprofv3 --pmc (profiles/)"
 
 
 def _scan_leg(scan, L_max, precision, contraction, steps, warmup, graph, stream, tune=True):
@@ -376,9 +447,13 @@ def bench_batched(args, ionization=False):
     t_gen = time.perf_counter() - t0
     n_pairs = sum(len(e.pairs) for e in scan)
     n_int = n_pairs * (L_max + 1) * 2
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
+    # the CPU baseline's worker pool forks: run it before anything touches the GPU
+    cpu = None if args.no_cpu_baseline else cpu_baseline_batched(scan, L_max)
+    if cpu:
+        log(f"cpu baseline: {cpu['value']:.3g} integrals/s ({cpu['sample']})")
     stream = torch.cuda.current_stream()
     graph = not args.no_graph
-    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
     log(f"scan generated: {len(scan)} energies, {n_pairs} pairs ({t_gen:.1f} s)")
 
     # the product path: every contraction in FP64 by the O(N) semiseparable kernel
@@ -419,35 +494,74 @@ def bench_batched(args, ionization=False):
                         if getattr(b64s, "tuning_step", None) else None),
         "host_note": "host planning, uploads and autotuning precede the timed region",
     }
+    # stage rooflines: each stage's kernels timed on one stream, on the same batch
+    stages_alone = stage_times(b64s)
+    tail_ms = tail_entry_ms(b64s, max(args.steps, 10), stream) if b64s.n_tail else None
+    W_tail, tail_nodes = tail_work_flop(b64s)
+    p_bytes = prep_bytes(b64s)
+    grouped = bool(b64s.prep_grouped and b64s.filon_groups is not None)
+    wl = "ionization_48x400" if ionization else "batched_64x400"
+    kname, k_ms = kernel_timing
+    Wk = ss_work["scan_flop"]
+    cands = {}
+    t_p = stages_alone.get("prep")
+    if t_p:
+        tr, prov = _committed_traffic("filon_fused_kernel" if grouped else "prep_f64_kernel", wl)
+        cands["prep"] = {
+            "bound": "hbm", "achieved": p_bytes / (t_p * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+            "frac": p_bytes / (t_p * 1e-3) / PEAK_HBM, "traffic": tr, "traffic_provenance": prov,
+            "kernel": ("filon_fused_kernel + filon_cells_kernel + prep_f64_kernel (grouped Filon prep)" if grouped
+                       else "prep_f64_kernel (per-pair Filon prep)"),
+            "kernel_ms": t_p, "algorithmic_bytes_per_launch": p_bytes,
+            "note": "prep stage alone on one stream; algorithmic bytes: chi_i, chi_f read once, alpha_dir, "
+                    "alpha_ex, beta_ex written once per pair, shared grid data once per energy"}
+    if tail_ms:
+        tr, prov = _committed_traffic("tail_filon_kernel<32>", wl)
+        cands["tails"] = {
+            "bound": "fp64_valu", "achieved": W_tail / (tail_ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+            "unit": "TFLOP/s", "frac": W_tail / (tail_ms * 1e-3) / PEAK_FP64, "traffic": tr,
+            "traffic_provenance": prov,
+            "kernel": "tail_filon_kernel<32> + tail_levin_kernel<32> + tail_combine_kernel (dwba_outer_tail_f64)",
+            "kernel_ms": tail_ms, "work_flop_per_launch": W_tail, "nodes_per_launch": tail_nodes,
+            "note": "3 flop per swept node and multipole L = 1..L_max (one FMA into the sum, one multiply of "
+                    "the x^-(L+1) chain); phases, weights and reciprocals are overhead"}
+    if k_ms:
+        tr, prov = _committed_traffic(kname, wl)
+        cands["contraction"] = {
+            "bound": "fp64_valu", "achieved": Wk / (k_ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+            "unit": "TFLOP/s", "frac": Wk / (k_ms * 1e-3) / PEAK_FP64, "traffic": tr, "traffic_provenance": prov,
+            "kernel": kname, "kernel_ms": k_ms, "work_flop_per_launch": Wk,
+            "algorithmic_bytes_per_launch": ss_work["scan_bytes"],
+            "hbm_gbs_algorithmic": ss_work["scan_bytes"] / (k_ms * 1e-3) / 1e9,
+            "note": "dominant contraction kernel timed alone; 9 flop per (grid point, L, family) of the "
+                    "recurrences it runs; algorithmic bytes = the alpha and beta it reads, FP64, once"}
+        t_c = stages_alone.get("plain")
+        if t_c:
+            W = ss_work["flop"]
+            cands["contraction"]["contraction_stage"] = {
+                "ms": t_c, "achieved_tflops": W / (t_c * 1e-3) / 1e12, "frac": W / (t_c * 1e-3) / PEAK_FP64,
+                "work_flop": W,
+                "kernels": kname + (" + semisep_gemv_scan_kernel + ddot_" + dot_kind + "_kernel"
+                                    if ss_work["gemv"] else ""),
+                "note": "all contraction kernels of a step (recurrences, shared-beta GEMV and per-pair dots at "
+                        "2 flop per point and L)"}
+    if cands:
+        dom = max(cands, key=lambda k: cands[k]["kernel_ms"])
+        result["roofline"] = dict(cands[dom], stage=dom,
+                                  note_dominant="the stage with the longest kernel time alone; the others are in "
+                                                "stage_rooflines")
+        result["stage_rooflines"] = cands
+    result["stage_ms_alone"] = stages_alone
     del b64s
-    t_c = st64s.get("plain")
-    if t_c:
-        W = ss_work["flop"]
-        kname, k_ms = kernel_timing
-        Wk = ss_work["scan_flop"]
-        wl = "ionization_48x400" if ionization else "batched_64x400"
-        result["roofline"] = {"bound": "mfma", "achieved": Wk / (k_ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
-                              "unit": "TFLOP/s", "frac": Wk / (k_ms * 1e-3) / PEAK_FP64,
-                              "traffic": _committed_traffic("scan_kernel", wl),
-                              "kernel": kname, "kernel_ms": k_ms, "work_flop_per_launch": Wk,
-                              "algorithmic_bytes_per_launch": ss_work["scan_bytes"],
-                              "hbm_gbs_algorithmic": ss_work["scan_bytes"] / (k_ms * 1e-3) / 1e9,
-                              "note": "dominant contraction kernel, timed alone with HIP events on its launch "
-                                      "stream (average over the timed launches); FP64 arithmetic on the vector "
-                                      "ALUs (MI355X FP64 vector peak = FP64 matrix peak = 78.6 TF/s); 9 flop per "
-                                      "(grid point, L, family) of the recurrences it runs; algorithmic bytes = "
-                                      "the alpha and beta it reads, FP64, once",
-                              "contraction_stage": {
-                                  "ms": t_c, "achieved_tflops": W / (t_c * 1e-3) / 1e12,
-                                  "frac": W / (t_c * 1e-3) / PEAK_FP64, "work_flop": W,
-                                  "kernels": kname + (" + semisep_gemv_scan_kernel + ddot_" + dot_kind + "_kernel"
-                                                      if ss_work["gemv"] else ""),
-                                  "note": "all contraction kernels of a step (recurrences, shared-beta GEMV and "
-                                          "per-pair dots at 2 flop per point and L)"}}
+    result["one_shot"] = one_shot_scan(scan, L_max)
+    log(f"one-shot scan: {result['one_shot']}")
+    if not ionization:
+        result["dropin_latency"] = dropin_latency()
+        log(f"drop-in: {result['dropin_latency']}")
 
     if args.no_fp32:
-        if not args.no_cpu_baseline:
-            result["cpu_baseline"] = cpu_baseline_batched(scan, L_max)
+        if cpu:
+            result["cpu_baseline"] = cpu
         return result
     b32, ms32, st32, res32 = _scan_leg(scan, L_max, "fp32", "dense", args.steps, args.warmup, graph, stream)
     wd, we = b32.work_flop()
@@ -486,41 +600,180 @@ def bench_batched(args, ionization=False):
         result["speedups_vs_fp64_dense"] = {"fp64_semiseparable": msd / ms64s, "fp32_mfma": msd / ms32}
         if "bf16x3" in result:
             result["speedups_vs_fp64_dense"]["bf16x3"] = msd / result["bf16x3"]["ms_per_step"]
-    if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline_batched(scan, L_max)
+    if cpu:
+        result["cpu_baseline"] = cpu
     return result
 
 
-def cpu_baseline_batched(scan, L_max, max_pairs=8, sample_L=(0, 1, 2, 3, 4), budget_s=10.0):
-    """Oracle's faithful restatement of radial_ME_all_L (materialises N x N
-    K_L, Filon / filon_exchange node interpolations with N1 x n_cc gathers,
-    per-L outer tails with math.fsum) on a sample: pairs of the smallest-N
-    energy, one call each for L in sample_L, until about budget_s seconds of
-    CPU work (at least one pair, at most max_pairs); the per-(pair, L) cost is
-    extrapolated to the full batch (sum over energies scaled by (N_e / N)^2)."""
-    from oracle import radial
-    model, ncpu, blas = cpu_info()
-    e = min(range(len(scan)), key=lambda i: len(scan[i].grid.r))
-    en = scan[e]
-    g = en.grid
-    Lm = max(sample_L)
+def one_shot_scan(scan, L_max):
+    """What a scan that runs once pays, wall clock: host planning, uploads and
+    device tables (ScanBatch construction), the launch choice (tune(): the
+    per-device cache written by the timed leg, or library defaults), one eager
+    launch with its synchronisation, and the copy of every I_L back into
+    RadialDWBAIntegrals objects."""
+    import torch
+    from dwba_amd.batch import ScanBatch
     t0 = time.perf_counter()
-    done = 0
-    for pr in en.pairs[:max_pairs]:
-        radial.radial_ME_all_L_faithful(g.r, g.w_simpson, g.w_trapz, en.V_core, en.U_i, en.bound_i, en.bound_f,
-                                        pr[0], pr[1], Lm, oscillatory_method="full_split", U_f_array=en.U_f)
-        done += 1
-        if time.perf_counter() - t0 >= budget_s:
-            break
-    t = time.perf_counter() - t0
-    n_s = len(g.r)
-    per_pair_L = t / (done * len(sample_L))
-    total = sum(len(x.pairs) * (L_max + 1) * per_pair_L * (len(x.grid.r) / n_s) ** 2 for x in scan)
+    b = ScanBatch(scan, L_max, precision="fp64", oscillatory_method="full_split")
+    t1 = time.perf_counter()
+    b.tune()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    b.launch()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    res = b.integrals()
+    t4 = time.perf_counter()
+    n = sum(len(r) for r in res)
+    del b, res
+    return {"construct_s": t1 - t0, "tune_s": t2 - t1, "launch_s": t3 - t2, "copy_back_s": t4 - t3,
+            "total_s": t4 - t0, "pairs": n,
+            "note": "fresh ScanBatch on the same scan: planning + uploads + tables, tune() (cached choice), "
+                    "one eager launch, results to Python dicts"}
+
+
+def dropin_latency():
+    """Per-call latency of the drop-in radial_ME_all_L_gpu over the 220 channel
+    pairs of H 1s -> 2p at 50 eV (the reference's own inputs, tests/golden
+    fixture: make_r_grid(1e-4, 200, 3000), waves l <= 20 from its solver,
+    L = 0..4, full_split, CC 5, one GPUCache for the energy) -- the call
+    sequence of driver.py:1016-1029 -- with the agreement against the
+    reference's CPU values, and the same 220 pairs as one ScanBatch."""
+    import torch
+    from dwba_amd.batch import EnergyInput, ScanBatch
+    from dwba_amd.dwba_matrix_elements import GPUCache, radial_ME_all_L_gpu
+    from dwba_amd.synthetic import Bound, Wave
+    path = os.path.join(ROOT, "tests", "golden", "c1_h1s2p_50eV.npz")
+    if not os.path.exists(path):
+        return None
+    C1 = np.load(path)
+
+    class _Grid:
+        r, w_simpson, w_trapz = C1["r"], C1["w_simpson"], C1["w_trapz"]
+
+    def wave(meta, chi):
+        k, l, ps, im, eta, sig = meta
+        return Wave(l=int(l), k_au=float(k), chi_of_r=chi, phase_shift=float(ps), idx_match=int(im),
+                    eta=float(eta), sigma_l=float(sig))
+    g = _Grid()
+    bi = Bound(C1["u_i"], l=0, energy_au=float(C1["e_i"]))
+    bf = Bound(C1["u_f"], l=1, energy_au=float(C1["e_f"]))
+    wi = [wave(C1["meta_i"][l], C1["chi_i"][l]) for l in range(C1["chi_i"].shape[0])]
+    wf = [wave(C1["meta_f"][l], C1["chi_f"][l]) for l in range(C1["chi_f"].shape[0])]
+    pairs = [tuple(int(x) for x in p) for p in C1["pairs"]]
+    Lm = int(C1["L_max"])
+    gold = C1["I_full_split_cc5"]
+    cache = GPUCache.from_grid(g)
+    kw = dict(oscillatory_method="full_split", CC_nodes=5, gpu_cache=cache, U_f_array=C1["U_f"])
+    radial_ME_all_L_gpu(g, C1["V_core"], C1["U_i"], bi, bf, wi[pairs[0][0]], wf[pairs[0][1]], Lm, **kw)   # warm
+    times, worst = [], 0.0
+    for q, (li, lf) in enumerate(pairs):
+        t0 = time.perf_counter()
+        res = radial_ME_all_L_gpu(g, C1["V_core"], C1["U_i"], bi, bf, wi[li], wf[lf], Lm, **kw)
+        times.append(time.perf_counter() - t0)
+        for fam, d in ((0, res.I_L_direct), (1, res.I_L_exchange)):
+            for L in range(Lm + 1):
+                ref = gold[q, fam, L]
+                worst = max(worst, abs(d[L] - ref) / max(abs(ref), 1e-300))
+    t = np.asarray(times) * 1e3
+    en = EnergyInput(grid=g, V_core=C1["V_core"], U_i=C1["U_i"], bound_i=bi, bound_f=bf,
+                     pairs=[(wi[li], wf[lf]) for (li, lf) in pairs], U_f=C1["U_f"])
+    t0 = time.perf_counter()
+    b = ScanBatch([en], Lm, precision="fp64", oscillatory_method="full_split", CC_nodes=5)
+    b.launch()
+    resb = b.results()
+    t_once = time.perf_counter() - t0
+    b.capture()
+    b.run()
+    ms_replay = _events_ms(b.run, 20, torch.cuda.current_stream())
+    worst_b = float(np.max(np.abs(resb - gold) / np.maximum(np.abs(gold), 1e-300)))
+    return {"calls": len(pairs), "mean_ms": float(t.mean()), "median_ms": float(np.median(t)),
+            "p90_ms": float(np.percentile(t, 90)), "total_s": float(t.sum() / 1e3),
+            "max_rel_vs_reference_cpu": worst, "scan_batch_one_shot_s": t_once, "scan_batch_replay_ms": ms_replay,
+            "scan_batch_max_rel_vs_reference_cpu": worst_b,
+            "workload": "H 1s->2p 50 eV, make_r_grid(1e-4,200,3000), 220 pairs l<=20, L=0..4, full_split CC5 "
+                        "(reference inputs and CPU values from the tests/golden fixture)"}
+
+
+_CPU_SCAN = None      # the scan the CPU-baseline workers read (set before the pool forks)
+
+
+def _cpu_worker_init(threads):
+    try:
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(threads)
+    except Exception:
+        pass
+
+
+def _cpu_faithful_call(task):
+    """One faithful radial_ME_all_L restatement call (L = 0..Lm) on pair pi of
+    energy ei of _CPU_SCAN; returns its wall time."""
+    from oracle import radial
+    ei, pi, Lm = task
+    en = _CPU_SCAN[ei]
+    pr = en.pairs[pi]
+    bf = pr[2] if len(pr) > 2 else en.bound_f
+    g = en.grid
+    t0 = time.perf_counter()
+    radial.radial_ME_all_L_faithful(g.r, g.w_simpson, g.w_trapz, en.V_core, en.U_i, en.bound_i, bf, pr[0], pr[1],
+                                    Lm, oscillatory_method="full_split", U_f_array=en.U_f)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline_batched(scan, L_max, workers=None):
+    """The oracle's faithful restatement of radial_ME_all_L (materialises the
+    N x N K_L of every call, Filon / filon_exchange node interpolations with
+    N1 x n_cc gathers, per-L outer tails with math.fsum) in min(cpu_count, 8)
+    worker processes, the reference's own CPU parallelism (driver.py:181-216,
+    :1159-1247; one pair per task).  Sample: on the smallest-N and the
+    largest-N energy, half the workers run one pair each with L = 0 and the
+    other half the same pairs with L = 0..2, all at once; a call's cost is setup S + per-L P (from
+    the two call lengths), modelled as a power of N between the two energies,
+    and the whole batch is (sum over energies of pairs x (S + (L_max+1) P)) /
+    workers.  Must run before the GPU is initialised (the pool forks)."""
+    import multiprocessing as mp
+    global _CPU_SCAN
+    model, ncpu, blas = cpu_info()
+    W = workers or min(ncpu or 1, 8)
+    thr = max(1, (ncpu or 1) // W)
+    _CPU_SCAN = scan
+    Ns = [len(e.grid.r) for e in scan]
+    picks = sorted({int(np.argmin(Ns)), int(np.argmax(Ns))})
+    cost, walls = {}, {}
+    ctx = mp.get_context("fork")
+    with ctx.Pool(W, initializer=_cpu_worker_init, initargs=(thr,)) as pool:
+        for e in picks:
+            half = max(1, W // 2)         # the same pairs with L = 0 and with L = 0..2
+            tasks = [(e, q % len(scan[e].pairs), 0) for q in range(half)] + \
+                    [(e, q % len(scan[e].pairs), 2) for q in range(W - half)]
+            t0 = time.perf_counter()
+            ts = pool.map(_cpu_faithful_call, tasks, chunksize=1)
+            walls[e] = time.perf_counter() - t0
+            t_l0 = float(np.mean(ts[:half]))
+            t_l2 = float(np.mean(ts[half:])) if W > half else t_l0
+            P = max((t_l2 - t_l0) / 2.0, 1e-9)
+            S = max(t_l0 - P, 0.0)
+            cost[e] = (Ns[e], S, P)
+    _CPU_SCAN = None
+    (n_a, S_a, P_a), (n_b, S_b, P_b) = cost[picks[0]], cost[picks[-1]]
+    expo = (lambda x, y: np.log(y / x) / np.log(n_b / n_a) if n_b != n_a else 2.0)
+    pS, pP = expo(max(S_a, 1e-9), max(S_b, 1e-9)), expo(P_a, P_b)
+    total = 0.0
+    for e, x in enumerate(scan):
+        S_e = S_a * (Ns[e] / n_a) ** pS
+        P_e = P_a * (Ns[e] / n_a) ** pP
+        total += len(x.pairs) * (S_e + (L_max + 1) * P_e)
+    total /= W
     n_int = sum(len(x.pairs) for x in scan) * (L_max + 1) * 2
-    return {"value": n_int / total, "unit": "integrals/s", "cores": int(blas or 1), "kind": "port",
-            "sample": f"faithful radial_ME_all_L (full_split) restatement, {done} pairs x L={list(sample_L)} "
-                      f"at N={n_s} ({t:.1f} s), extrapolated to the whole batch by (N_e/N)^2 and pair/L counts; "
-                      f"BLAS threads={blas}; host {model}, os.cpu_count={ncpu}"}
+    return {"value": float(n_int / total), "unit": "integrals/s", "cores": W * thr, "kind": "port", "workers": W,
+            "threads_per_worker": thr,
+            "sample": (f"faithful radial_ME_all_L (full_split) restatement in {W} worker processes x {thr} BLAS "
+                       f"threads: {W // 2} pairs with L=0 and the same with L=0..2, at once, on the N={n_a} "
+                       f"and N={n_b} energies ({sum(walls.values()):.1f} s wall); per-call setup S and per-L cost "
+                       f"P fitted as N^{pS:.2f} and N^{pP:.2f}; batch = sum pairs x (S + {L_max + 1} P) / {W}; "
+                       f"host {model}, os.cpu_count={ncpu}"),
+            "fit": {"N": [n_a, n_b], "setup_s": [S_a, S_b], "per_L_s": [P_a, P_b]}}
 
 
 def main():

@@ -296,9 +296,12 @@ class ScanBatch:
         # longest first (each job writes its own row, so the order is free): the
         # wavefronts with thousands of Filon segments start before the short ones
         seg_max, dphi_max = planner.filon_limits()
-        tail_ids.sort(key=lambda k: -planner.tail_nodes_estimate(self.pairs[k][2].k_i, self.pairs[k][2].k_f,
-                                                                  self.pairs[k][2].r_match, self.pairs[k][2].r_max,
-                                                                  seg_max, dphi_max))
+        self.tail_nodes = {}
+        for k in tail_ids:
+            pl = self.pairs[k][2]
+            self.tail_nodes[k] = planner.tail_sweep_nodes(pl.k_i, pl.l_i, pl.eta_i, pl.k_f, pl.l_f, pl.eta_f,
+                                                          pl.r_match, pl.r_max, seg_max, dphi_max)
+        tail_ids.sort(key=lambda k: -sum(self.tail_nodes[k]))
         self.n_tail = len(tail_ids)
         if tail_ids:
             sets, set_of = [], {}

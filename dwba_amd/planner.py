@@ -92,24 +92,49 @@ def filon_limits():
     return _i("DWBA_MAX_FILON_SEGMENTS", 4096, 128), _f("DWBA_MAX_EFFECTIVE_DPHI", np.pi / 2, np.pi / 16)
 
 
-def tail_nodes_estimate(k_i, k_f, r_m, r_max, max_segments=4096, max_dphi=np.pi / 2):
-    """Filon nodes an outer tail sweeps for both phase signs, with the
-    frequencies |k_i -+ k_f| (the centrifugal and Coulomb terms dropped):
-    the segment count and node up-scaling of filon_oscillatory_integral
-    (oscillatory_integrals.py:1280-1354).  Only used to order tail jobs
-    longest first."""
-    span = max(float(r_max) - float(r_m), 0.0)
-    tot = 0
-    for om in (abs(k_i - k_f), k_i + k_f):
-        if om < 1e-10 or span <= 0.0:
+def tail_sweep_nodes(k_i, l_i, eta_i, k_f, l_f, eta_f, r_m, r_max, max_segments=4096, max_dphi=np.pi / 2):
+    """Nodes the outer tail of one pair sweeps over all L, per phase sign, as
+    compute_outer_integral_oscillatory chooses them (oscillatory_integrals.py:
+    1361-1451) with the sinA*sinB phases of :350-456: omega = |Phi'(mid)|;
+    Filon when |Phi''(mid)| h^2 < 0.1 (h = (pi/4)/omega): n_seg segments of
+    neff nodes (segment cap and node up-scaling of :1280-1354), of which a lane
+    sweeps neff - 1 per segment (shared end nodes once) plus one; else Levin,
+    8 nodes per segment; nothing when |Phi'(mid)| < 1e-10 (the per-L
+    Gauss-Legendre branch) or r_max <= r_m.  Returns (filon_nodes, levin_nodes)."""
+    filon = levin = 0
+    if r_max <= r_m + 1e-10:
+        return 0, 0
+    mid = 0.5 * (r_m + r_max)
+    for sgn in (-1.0, 1.0):
+        d1 = (k_i - k_f) if sgn < 0 else (k_i + k_f)
+        d2 = 0.0
+        if abs(eta_i) > 1e-15:
+            d1 += eta_i / mid
+            d2 -= eta_i / (mid * mid)
+        if abs(eta_f) > 1e-15:
+            d1 += sgn * eta_f / mid
+            d2 -= sgn * eta_f / (mid * mid)
+        if l_i > 0:
+            d1 += l_i * (l_i + 1) / (2.0 * k_i * mid * mid)
+            d2 -= l_i * (l_i + 1) / (k_i * mid ** 3)
+        if l_f > 0:
+            d1 += sgn * l_f * (l_f + 1) / (2.0 * k_f * mid * mid)
+            d2 -= sgn * l_f * (l_f + 1) / (k_f * mid ** 3)
+        om = abs(d1)
+        if om < 1e-10:
             continue
-        nseg = min(max(1, int(np.ceil(span * om / (np.pi / 4)))), int(max_segments))
-        eff = om * span / nseg
-        neff = 5
-        if eff > max_dphi:
-            neff = min(15, max(5, 5 + 2 * (int(np.ceil(eff / max_dphi)) - 1)))
-        tot += nseg * (neff - 1)
-    return tot
+        h = (np.pi / 4) / om
+        span = r_max - r_m
+        if abs(d2) * h * h < 0.1:
+            nseg = min(max(1, int(np.ceil(span / h))), int(max_segments))
+            eff = om * span / nseg
+            neff = 5
+            if eff > max_dphi:
+                neff = min(15, max(5, 5 + 2 * (int(np.ceil(eff / max_dphi)) - 1)))
+            filon += nseg * (neff - 1) + 1
+        else:
+            levin += 8 * max(1, int(np.ceil(span / h)))
+    return filon, levin
 
 
 # ---------------------------------------------------------------- grid stats
