@@ -158,7 +158,9 @@ def bench_kernel_only(args):
 
 def bench_backward(args):
     """Gradient workload: N = 15000, 32 pairs, L = 0..20; d/d rho_a and d/d rho_b
-    of sum_{p,L} c_L I_L(p) by the backward kernels (FP64 and FP32)."""
+    of sum_{p,L} c_L I_L(p).  value: the FP64 O(N) path (K_L x by the
+    semiseparable recurrences for every L, dwba_il_backward_semisep_f64);
+    the dense FP64 and FP32 Keff kernels are timed beside it."""
     import torch
     from dwba_amd import synthetic as S
     from dwba_amd.autograd import density_gradients
@@ -168,34 +170,41 @@ def bench_backward(args):
     t = lambda x: torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)
     r_d, w_d, ra_d, rb_d, c_d = map(t, (grid.r, grid.w_simpson, ra, rb, c))
     stream = torch.cuda.current_stream()
-    # algorithmic work per element and pair: Horner for Keff (L_max FMAs + 1 mul), two products
-    W = float(P) * N * N * (2 * L_max + 1 + 4)
+    # dense: per element and pair, Horner for Keff (L_max FMAs + 1 mul) and two products
+    W_dense = float(P) * N * N * (2 * L_max + 1 + 4)
+    # O(N): per point, multipole, pair and side, the forward and backward recurrence steps of the
+    # GEMV (SS_FLOP) and one FMA of the combination
+    W_ss = float(P) * 2 * (L_max + 1) * N * (SS_FLOP + 2)
     legs = {}
-    for prec in ("fp64", "fp32"):
-        fn = lambda: density_gradients(ra_d, rb_d, r_d, w_d, c_d, L_max, prec)
-        for _ in range(args.warmup):
+    for prec, con in (("fp64", "semiseparable"), ("fp64", "dense"), ("fp32", "dense")):
+        fn = lambda: density_gradients(ra_d, rb_d, r_d, w_d, c_d, L_max, prec, con)
+        for _ in range(max(args.warmup, 1)):
             fn()
-        legs[prec] = _events_ms(fn, args.steps, stream)
+        legs[(prec, con)] = _events_ms(fn, args.steps, stream)
     from oracle import gradient as OG
-    ga, gb = density_gradients(ra_d, rb_d, r_d, w_d, c_d, L_max, "fp32")
     sub = slice(0, 4)   # error check on 4 pairs (oracle cost)
     ref_a, _ = OG.density_gradients(grid.r, grid.w_simpson, ra[sub], rb[sub], c[sub], L_max)
     mag_a, _ = OG.density_gradients(grid.r, grid.w_simpson, ra[sub], rb[sub], c[sub], L_max, absolute=True)
-    err32 = float(np.max(np.abs(ga[sub].cpu().numpy() - ref_a) / np.maximum(mag_a, 1e-300)))
-    ms = legs["fp32"]
-    peak_fp32_valu = 157.3e12
+    errs = {}
+    for prec, con in legs:
+        ga, gb = density_gradients(ra_d, rb_d, r_d, w_d, c_d, L_max, prec, con)
+        errs[(prec, con)] = float(np.max(np.abs(ga[sub].cpu().numpy() - ref_a) / np.maximum(mag_a, 1e-300)))
+    ms = legs[("fp64", "semiseparable")]
     return {
         "metric": "gradient of sum_L c_L I_L w.r.t. rho_a and rho_b, pairs/s", "value": P / (ms * 1e-3),
         "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "backward check: exp grid N=15000 on [1e-5,1000] a0, 32 pairs, L=0..20, "
-                               "c_L ~ N(0,1)", "N": N, "pairs": P, "L_max": L_max},
-        "roofline": {"bound": "fp32_valu", "achieved": W / (ms * 1e-3) / 1e12, "peak": peak_fp32_valu / 1e12,
-                     "unit": "TFLOP/s", "frac": W / (ms * 1e-3) / peak_fp32_valu, "traffic": None,
-                     "kernel": "il_backward_kernel<21,float>", "work_flop_per_launch": W},
-        "fp64": {"ms": legs["fp64"], "tflops": W / (legs["fp64"] * 1e-3) / 1e12,
-                 "frac_of_fp64_peak": W / (legs["fp64"] * 1e-3) / PEAK_FP64},
-        "fp32_max_err_over_magnitude": err32,
+                               "c_L ~ N(0,1); value = FP64 O(N) gradients", "N": N, "pairs": P, "L_max": L_max},
+        "roofline": {"bound": "fp64_valu", "achieved": W_ss / (ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": W_ss / (ms * 1e-3) / PEAK_FP64, "traffic": None,
+                     "kernel": "backward_setup_kernel + semisep_gemv_scan_kernel + backward_combine_kernel",
+                     "work_flop_per_launch": W_ss},
+        "max_err_over_magnitude": {f"{p}_{c}": e for (p, c), e in errs.items()},
+        "dense_fp64": {"ms": legs[("fp64", "dense")],
+                       "frac_of_fp64_peak": W_dense / (legs[("fp64", "dense")] * 1e-3) / PEAK_FP64},
+        "dense_fp32": {"ms": legs[("fp32", "dense")],
+                       "frac_of_fp32_valu_peak": W_dense / (legs[("fp32", "dense")] * 1e-3) / 157.3e12},
     }
 
 

@@ -941,6 +941,60 @@ __global__ __launch_bounds__(GT) void semisep_gemv_scan_kernel(const dwba_dgemv_
     }
 }
 
+// ---------------------------------------------------------------------------
+// O(N) gradients of sum_{p,L} g[p][L] I_L(p), I_L(p) = (w rho_a[p])^T K_L (w rho_b[p]):
+//     grad_a[p] = w (sum_L g[p][L] K_L (w rho_b[p])),  grad_b[p] = w (sum_L g[p][L] K_L (w rho_a[p]))
+// (K_L symmetric).  Row k < P of the work is K_L (w rho_b[k]), row P + k is
+// K_L (w rho_a[k]); each is one shared-beta GEMV of semisep_gemv_scan_kernel.
+__global__ __launch_bounds__(GT) void backward_setup_kernel(const double* __restrict__ r, const double* __restrict__ w,
+                                                            int n, int P, int ld_in, const double* __restrict__ rho_a,
+                                                            const double* __restrict__ rho_b, int L_max, int ld_t,
+                                                            double* __restrict__ xw, double* __restrict__ t,
+                                                            dwba_dgemv_energy* __restrict__ desc) {
+    const int k = blockIdx.y;                                 // 0 .. 2P-1
+    const double* __restrict__ src = k < P ? rho_b + (size_t)k * ld_in : rho_a + (size_t)(k - P) * ld_in;
+    double* __restrict__ dst = xw + (size_t)k * ld_t;
+    for (int i = blockIdx.x * GT + threadIdx.x; i < n; i += gridDim.x * GT) dst[i] = w[i] * src[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        desc[k] = dwba_dgemv_energy{r, dst, t + (size_t)k * (L_max + 1) * ld_t, n, n, ld_t, L_max};
+}
+
+__global__ __launch_bounds__(GT) void backward_combine_kernel(const double* __restrict__ w, int n, int P, int ld_out,
+                                                              const double* __restrict__ g, int L_max, int ld_t,
+                                                              const double* __restrict__ t, double* __restrict__ ga,
+                                                              double* __restrict__ gb) {
+    const int k = blockIdx.y;
+    const int p = k < P ? k : k - P;
+    double* __restrict__ out = k < P ? ga : gb;
+    if (out == nullptr) return;
+    out += (size_t)p * ld_out;
+    const double* __restrict__ tk = t + (size_t)k * (L_max + 1) * ld_t;
+    const double* __restrict__ gp = g + (size_t)p * (L_max + 1);
+    for (int i = blockIdx.x * GT + threadIdx.x; i < ld_out; i += gridDim.x * GT) {
+        double v = 0.0;
+        if (i < n) {
+            double s = 0.0;
+            for (int L = 0; L <= L_max; ++L) s = fma(gp[L], tk[(size_t)L * ld_t + i], s);
+            v = w[i] * s;
+        }
+        out[i] = v;
+    }
+}
+
+struct BackwardLayout {
+    int ld_t;
+    size_t desc_off, xw_off, t_off, total;
+};
+BackwardLayout backward_layout(int n, int P, int L_max) {
+    BackwardLayout b;
+    b.ld_t = (n + 31) / 32 * 32;
+    b.desc_off = 0;
+    b.xw_off = align_up(sizeof(dwba_dgemv_energy) * 2 * (size_t)P);
+    b.t_off = b.xw_off + align_up(sizeof(double) * 2 * (size_t)P * b.ld_t);
+    b.total = b.t_off + align_up(sizeof(double) * 2 * (size_t)P * (L_max + 1) * b.ld_t);
+    return b;
+}
+
 int host_tasks(int32_t n_jobs, const int32_t* h_n_cols, int seg) {
     int64_t tot = 0;
     for (int k = 0; k < n_jobs; ++k) {
@@ -1137,6 +1191,43 @@ int32_t dwba_il_semisep_lanes_split_f64(int32_t n_jobs, const dwba_plain_job* d_
         default: return set_error(DWBA_ERR_ARG, "dwba_il_semisep_lanes_split_f64: L_max %d", L_max);
     }
     DWBA_LAUNCH_CHECK("semisep_lane_split_kernel");
+    return DWBA_OK;
+}
+
+size_t dwba_il_backward_semisep_f64_workspace_bytes(int32_t n, int32_t n_pairs, int32_t L_max) {
+    if (n < 2 || n_pairs < 1 || L_max < 0 || L_max > 63) return 0;
+    return backward_layout(n, n_pairs, L_max).total;
+}
+
+int32_t dwba_il_backward_semisep_f64(const double* d_r, const double* d_w, int32_t n, int32_t n_pairs, int32_t ld,
+                                     const double* d_rho_a, const double* d_rho_b, const double* d_g, int32_t L_max,
+                                     double* d_grad_a, double* d_grad_b, void* d_workspace, size_t workspace_bytes,
+                                     void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_il_backward_semisep_f64: call dwba_init first");
+    DWBA_REQUIRE(n >= 2 && n_pairs >= 1 && ld >= n && L_max >= 0 && L_max <= 63 && n_pairs <= 32767,
+                 "dwba_il_backward_semisep_f64: bad shape (n=%d, pairs=%d, ld=%d, L_max=%d)", n, n_pairs, ld, L_max);
+    DWBA_REQUIRE(d_r && d_w && d_rho_a && d_rho_b && d_g && d_workspace,
+                 "dwba_il_backward_semisep_f64: null pointer");
+    const BackwardLayout b = backward_layout(n, n_pairs, L_max);
+    if (workspace_bytes < b.total)
+        return set_error(DWBA_ERR_WORKSPACE, "dwba_il_backward_semisep_f64: workspace %zu < %zu bytes",
+                         workspace_bytes, b.total);
+    if (!d_grad_a && !d_grad_b) return DWBA_OK;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    char* ws = reinterpret_cast<char*>(d_workspace);
+    auto* desc = reinterpret_cast<dwba_dgemv_energy*>(ws + b.desc_off);
+    auto* xw = reinterpret_cast<double*>(ws + b.xw_off);
+    auto* t = reinterpret_cast<double*>(ws + b.t_off);
+    const int bx = std::min((n + GT - 1) / GT, 64);
+    hipLaunchKernelGGL(backward_setup_kernel, dim3(bx, 2 * n_pairs), dim3(GT), 0, s, d_r, d_w, n, n_pairs, ld, d_rho_a,
+                       d_rho_b, L_max, b.ld_t, xw, t, desc);
+    DWBA_LAUNCH_CHECK("backward_setup_kernel");
+    hipLaunchKernelGGL(semisep_gemv_scan_kernel, dim3(2 * n_pairs, L_max + 1), dim3(GT), 0, s, desc);
+    DWBA_LAUNCH_CHECK("semisep_gemv_scan_kernel");
+    const int cx = std::min((ld + GT - 1) / GT, 64);
+    hipLaunchKernelGGL(backward_combine_kernel, dim3(cx, 2 * n_pairs), dim3(GT), 0, s, d_w, n, n_pairs, ld, d_g,
+                       L_max, b.ld_t, t, d_grad_a, d_grad_b);
+    DWBA_LAUNCH_CHECK("backward_combine_kernel");
     return DWBA_OK;
 }
 

@@ -462,6 +462,21 @@ int32_t dwba_il_backward_f32(const double* d_r, const double* d_w, int32_t n, in
                              const double* d_rho_a, const double* d_rho_b, const double* d_g, int32_t L_max,
                              double* d_grad_a, double* d_grad_b, void* stream);
 
+/* The same gradients (FP64, same arguments and output layout as
+ * dwba_il_backward_f64) in O(N) per pair and multipole: K_L x is formed by
+ * the semiseparable recurrences of dwba_semisep_gemv_f64 for x = w rho_b[p]
+ * and x = w rho_a[p], every L, then grad = w sum_L g[p][L] (K_L x).  Nothing of
+ * size N x N is formed; the workspace (>= the _workspace_bytes value) holds
+ * the weighted rows and the 2 n_pairs (L_max + 1) vectors K_L x.  The forward
+ * quantity is the plain branch of radial_ME_all_L
+ * (dwba_matrix_elements.py:1243-1245, :1288-1289); the reference has no
+ * gradient. */
+size_t dwba_il_backward_semisep_f64_workspace_bytes(int32_t n, int32_t n_pairs, int32_t L_max);
+int32_t dwba_il_backward_semisep_f64(const double* d_r, const double* d_w, int32_t n, int32_t n_pairs, int32_t ld,
+                                     const double* d_rho_a, const double* d_rho_b, const double* d_g, int32_t L_max,
+                                     double* d_grad_a, double* d_grad_b, void* d_workspace, size_t workspace_bytes,
+                                     void* stream);
+
 /* d_dst[k*dst_ld + i] = d_w[i] * d_src[k*src_ld + i] for i < n, 0 for
  * n <= i < dst_ld, k < n_rows (the quadrature-weighted densities
  * alpha = w rho_a, beta = w rho_b of the plain branch,

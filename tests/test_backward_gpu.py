@@ -1,5 +1,6 @@
 """Gradient of the plain multipole integrals on the GPU (N = 15000, 32 pairs,
-L = 0..20, FP64 and FP32): against the long-double oracle gradient, against
+L = 0..20; FP64 by the O(N) semiseparable recurrences and by the dense Keff
+kernel, FP32 dense): against the long-double oracle gradient, against
 central finite differences (step 1e-6) of the GPU forward at 64 seeded grid
 points, and through torch.autograd.
 
@@ -29,12 +30,16 @@ def case():
                 r_d=t(grid.r), w_d=t(grid.w_simpson), ra_d=t(ra), rb_d=t(rb), c_d=t(c))
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
-def test_gradient_against_oracle(case, precision, tol):
+MODES = [("fp64", "semiseparable", 1e-12), ("fp64", "dense", 1e-12), ("fp32", "dense", 2e-6)]
+
+
+@pytest.mark.parametrize("precision,contraction,tol", MODES)
+def test_gradient_against_oracle(case, precision, contraction, tol):
     import torch
     from dwba_amd.autograd import density_gradients
-    print(f"\n[backward] N={N} P={P} L_max={LMAX} precision={precision}", flush=True)
-    ga, gb = density_gradients(case["ra_d"], case["rb_d"], case["r_d"], case["w_d"], case["c_d"], LMAX, precision)
+    print(f"\n[backward] N={N} P={P} L_max={LMAX} precision={precision} contraction={contraction}", flush=True)
+    ga, gb = density_gradients(case["ra_d"], case["rb_d"], case["r_d"], case["w_d"], case["c_d"], LMAX, precision,
+                               contraction)
     torch.cuda.synchronize()
     ga, gb = ga.cpu().numpy(), gb.cpu().numpy()
     ea = np.abs(ga - case["ga"]) / np.maximum(case["ma"], 1e-300)
@@ -44,18 +49,19 @@ def test_gradient_against_oracle(case, precision, tol):
     assert np.all(np.abs(gb - case["gb"]) <= tol * case["mb"] + 1e-300)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_autograd_function(case, precision):
+@pytest.mark.parametrize("precision,contraction,tol", MODES)
+def test_autograd_function(case, precision, contraction, tol):
     import torch
     from dwba_amd.autograd import multipole_integrals, density_gradients
-    print(f"\n[autograd] N={N} P={P} L_max={LMAX} precision={precision}", flush=True)
+    print(f"\n[autograd] N={N} P={P} L_max={LMAX} precision={precision} contraction={contraction}", flush=True)
     ra = case["ra_d"].clone().requires_grad_(True)
     rb = case["rb_d"].clone().requires_grad_(True)
-    I = multipole_integrals(ra, rb, case["r_d"], case["w_d"], LMAX, precision)
+    I = multipole_integrals(ra, rb, case["r_d"], case["w_d"], LMAX, precision, contraction)
     assert I.shape == (P, LMAX + 1)
     loss = (case["c_d"] * I).sum()
     loss.backward()
-    ga, gb = density_gradients(case["ra_d"], case["rb_d"], case["r_d"], case["w_d"], case["c_d"], LMAX, precision)
+    ga, gb = density_gradients(case["ra_d"], case["rb_d"], case["r_d"], case["w_d"], case["c_d"], LMAX, precision,
+                               contraction)
     torch.cuda.synchronize()
     assert torch.equal(ra.grad, ga) and torch.equal(rb.grad, gb)
     # forward values against the oracle
@@ -70,11 +76,13 @@ def test_autograd_function(case, precision):
         assert np.all(np.abs(got - ref) <= 1e-6 * mag)
 
 
-def test_gradient_against_finite_differences(case):
+@pytest.mark.parametrize("contraction", ["semiseparable", "dense"])
+def test_gradient_against_finite_differences(case, contraction):
     import torch
     from dwba_amd.autograd import multipole_integrals, density_gradients
-    print(f"\n[finite differences] N={N} 64 points, step 1e-6, FP64 forward", flush=True)
-    ga, gb = density_gradients(case["ra_d"], case["rb_d"], case["r_d"], case["w_d"], case["c_d"], LMAX, "fp64")
+    print(f"\n[finite differences] N={N} 64 points, step 1e-6, FP64 forward, {contraction}", flush=True)
+    ga, gb = density_gradients(case["ra_d"], case["rb_d"], case["r_d"], case["w_d"], case["c_d"], LMAX, "fp64",
+                               contraction)
     ga, gb = ga.cpu().numpy(), gb.cpu().numpy()
     r = case["grid"].r
     lo, hi = np.searchsorted(r, 0.1), np.searchsorted(r, 200.0)
@@ -92,7 +100,7 @@ def test_gradient_against_finite_differences(case):
         for s in (+1.0, -1.0):
             a, b = base_a.clone(), base_b.clone()
             (a if which == 0 else b)[0, i] += s * h
-            I = multipole_integrals(a, b, case["r_d"], case["w_d"], LMAX, "fp64")
+            I = multipole_integrals(a, b, case["r_d"], case["w_d"], LMAX, "fp64", contraction)
             vals.append(float((torch.as_tensor(c[p], device=I.device) * I[0]).sum()))
         fd = (vals[0] - vals[1]) / (2 * h)
         grad = (ga if which == 0 else gb)[p, i]
@@ -114,3 +122,5 @@ def test_backward_rejects_bad_arguments(case):
         density_gradients(case["ra_d"], case["rb_d"], case["r_d"], case["w_d"], case["c_d"], 64)
     with pytest.raises(ValueError):
         density_gradients(case["ra_d"], case["rb_d"], case["r_d"], case["w_d"], case["c_d"][:, :3], LMAX)
+    with pytest.raises(ValueError):
+        density_gradients(case["ra_d"], case["rb_d"], case["r_d"], case["w_d"], case["c_d"], LMAX, "fp64", "fast")

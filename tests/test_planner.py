@@ -122,3 +122,36 @@ def test_filon_groups_share_geometry():
     for pos, (e, p) in enumerate(plans):
         assert (pos in seen) == (p.mode == P.PREP_FILON and p.N1 >= 2)
     assert 0 < len(groups) < len(seen) // 2
+
+
+def test_tail_sweep_nodes_follows_the_oracle_branch_choice():
+    """planner.tail_sweep_nodes (tail job ordering and the tail roofline's work)
+    takes the Filon / Levin / quad branch, the segment count and the node
+    up-scaling the oracle's compute_outer_integral_oscillatory takes."""
+    import math
+    from oracle import tail as T
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        ka, kb = rng.uniform(0.2, 8.0, 2)
+        la, lb = (int(x) for x in rng.integers(0, 41, 2))
+        ea, eb = (rng.uniform(-2, 0, 2) if rng.random() < 0.3 else (0.0, 0.0))
+        rm = float(rng.uniform(5, 200))
+        f, lv = planner.tail_sweep_nodes(ka, la, ea, kb, lb, eb, rm, 1000.0)
+        pf = T.PhaseFunctions(ka, la, 0.0, ea, 0.0, kb, lb, 0.0, eb, 0.0)
+        ef = el = 0
+        mid = 0.5 * (rm + 1000.0)
+        for sg in (-1, 1):
+            om = abs(pf.dphi(sg, mid))
+            if om < 1e-10:
+                continue
+            h = (np.pi / 4) / om
+            if abs(pf.d2phi(sg, mid)) * h * h < 0.1:
+                nseg = min(max(1, int(np.ceil((1000.0 - rm) / h))), T.filon_max_segments())
+                eff = om * (1000.0 - rm) / nseg
+                neff = 5
+                if eff > T.filon_max_effective_dphi():
+                    neff = min(15, max(5, 5 + 2 * (math.ceil(eff / T.filon_max_effective_dphi()) - 1)))
+                ef += nseg * (neff - 1) + 1
+            else:
+                el += 8 * max(1, int(np.ceil((1000.0 - rm) / h)))
+        assert (f, lv) == (ef, el), (ka, la, ea, kb, lb, eb, rm)
