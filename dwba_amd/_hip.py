@@ -49,6 +49,11 @@ class PrepJob(ctypes.Structure):
                 ("log2", c_i32)]
 
 
+class AmpPair(ctypes.Structure):
+    """dwba_amp_pair of include/dwba_hip.h."""
+    _fields_ = [("row", c_i32), ("type", c_i32), ("l_f", c_i32), ("pad_", c_i32), ("scale", ctypes.c_double)]
+
+
 class TailJob(ctypes.Structure):
     """dwba_tail_job of include/dwba_hip.h."""
     _fields_ = [("k_a", ctypes.c_double), ("delta_a", ctypes.c_double), ("eta_a", ctypes.c_double),
@@ -101,6 +106,12 @@ _SIGS = {
     "dwba_il_backward_semisep_f64": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp,
                                              c_vp, c_sz, c_vp]),
     "dwba_weight_rows": (c_i32, [c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_i32, c_vp]),
+    "dwba_ylm_f64": (c_i32, [c_i32, c_i32, c_i32, c_vp, c_vp, c_vp]),
+    "dwba_amp_coeffs_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "dwba_amp_sum_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_i32p, c_i32p, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp,
+                                 c_vp, c_vp]),
+    "dwba_dcs_f64": (c_i32, [c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "dwba_sphere_trapz_f64": (c_i32, [c_i32, c_i32, c_vp, c_vp, c_vp, c_vp]),
 }
 
 

@@ -477,6 +477,71 @@ int32_t dwba_il_backward_semisep_f64(const double* d_r, const double* d_w, int32
                                      double* d_grad_a, double* d_grad_b, void* d_workspace, size_t workspace_bytes,
                                      void* stream);
 
+/* ---- amplitude assembly -------------------------------------------------
+ * The per-(M_i, M_f) sums that turn I_L into scattering amplitudes:
+ * calculate_amplitude_contribution (dwba_coupling.py:285-504) accumulated
+ * over the channel pairs of an energy as the driver does
+ * (driver.py:1041-1052), calculate_ionization_coefficients
+ * (dwba_coupling.py:507-634), dcs_dwba and integrate_dcs_over_angles
+ * (sigma_total.py:147-217).  The angular coefficients are formed on the
+ * host; a "type" is one (l_i, l_f, L_i, L_f) with its list of n_combos
+ * projection pairs (M_i, M_f):
+ *   d_coef  [type][combo][2][L_max+1]: Cd_L then Cx_L (0 where a term is cut);
+ *   d_phase [type][combo][4]: complex constants (re, im) of the direct and
+ *           exchange terms: i^(l_i+l_f) Y*_{l_i 0} and
+ *           i^(l_i-l_f) (-1)^(L_f+M_f) (-1)^nu Y*_{l_i 0} (0 for a family
+ *           whose Y_{l_f,m} does not exist).
+ * d_integrals is [row][2][L_max+1] (direct, exchange) -- the layout of the
+ * batched scan's output. */
+typedef struct dwba_amp_pair {
+    int32_t row;      /* the pair's row of d_integrals */
+    int32_t type;     /* its coefficient table */
+    int32_t l_f;      /* projectile final partial wave: F uses Y_{l_f, M_i - M_f}, G Y_{l_f, M_f - M_i} */
+    int32_t pad_;
+    double scale;     /* (2/pi)/(k_i k_f), times sqrt(2/pi)/k_ej for ionization */
+} dwba_amp_pair;
+
+/* d_ylm[(l*(2 m_max+1) + m + m_max)*n_theta + j] = Y_{l,m}(theta_j, phi = 0)
+ * for l <= l_max, |m| <= m_max (Condon-Shortley phase, 0 where |m| > l), by the
+ * normalised three-term recurrence in l.  The reference evaluates
+ * scipy.special.sph_harm per call (dwba_coupling.py:430-470). */
+int32_t dwba_ylm_f64(int32_t l_max, int32_t m_max, int32_t n_theta, const double* d_theta, double* d_ylm,
+                     void* stream);
+
+/* d_a[pair][combo][4] = scale * (phase_d S_d, phase_x S_x) (complex, re/im),
+ * S_d = sum_L Cd_L I_dir(L), S_x = sum_L Cx_L I_ex(L) in ascending L, a term
+ * with |I| < 1e-20 skipped (dwba_coupling.py:372-430).  For ionization these
+ * are f_coeff and g_coeff.  n_combos <= 4096, L_max <= 63. */
+int32_t dwba_amp_coeffs_f64(int32_t n_pairs, const dwba_amp_pair* d_pairs, int32_t n_combos, int32_t L_max,
+                            const double* d_integrals, const double* d_coef, const double* d_phase, double* d_a,
+                            void* stream);
+
+/* F[slot][combo][j] = sum_p a_d[p][combo] Y_{l_f(p), m_dir[combo]}(theta_j),
+ * G likewise with a_x and m_ex, over the pairs p in [slot_first[slot],
+ * slot_first[slot+1]) in index order (complex, re/im interleaved).  m_dir,
+ * m_ex: host arrays of n_combos values (M_i - M_f and M_f - M_i); d_ylm from
+ * dwba_ylm_f64 with the same l_max, m_max, n_theta.  A pair whose l_f > l_max
+ * contributes 0.  d_ylm NULL (with n_theta == 1): Y = 1, the coherent sums of
+ * the coefficients over the slot's pairs (the ionization SDCS amplitudes,
+ * ionization.py:631-642).  n_slots <= 65535, n_combos <= 64. */
+int32_t dwba_amp_sum_f64(int32_t n_slots, const int32_t* d_slot_first, const dwba_amp_pair* d_pairs,
+                         int32_t n_combos, const int32_t* h_m_dir, const int32_t* h_m_ex, int32_t l_max,
+                         int32_t m_max, int32_t n_theta, const double* d_ylm, const double* d_a, double* d_F,
+                         double* d_G, void* stream);
+
+/* d_dcs[slot][j] = sum_combo max(0, prefac[slot] (|F-G|^2 3/4 + |F+G|^2 / 4) (2 pi)^4),
+ * prefac = (k_f/k_i) N_eq/(2 L_i + 1) (dcs_dwba, sigma_total.py:147-177);
+ * d_tcs[slot] = 2 pi trapezoid(sin(theta) dcs, theta)
+ * (integrate_dcs_over_angles, sigma_total.py:179-217), atomic units. */
+int32_t dwba_dcs_f64(int32_t n_slots, int32_t n_combos, int32_t n_theta, const double* d_theta, const double* d_F,
+                     const double* d_G, const double* d_prefac, double* d_dcs, double* d_tcs, void* stream);
+
+/* d_tcs[k] = 2 pi trapezoid(sin(theta) d_dcs[k][:], theta) for k < n_rows: the
+ * integral alone, for a DCS formed elsewhere (integrate_dcs_over_angles,
+ * sigma_total.py:179-217). */
+int32_t dwba_sphere_trapz_f64(int32_t n_rows, int32_t n_theta, const double* d_theta, const double* d_dcs,
+                              double* d_tcs, void* stream);
+
 /* d_dst[k*dst_ld + i] = d_w[i] * d_src[k*src_ld + i] for i < n, 0 for
  * n <= i < dst_ld, k < n_rows (the quadrature-weighted densities
  * alpha = w rho_a, beta = w rho_b of the plain branch,
