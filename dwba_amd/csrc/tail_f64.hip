@@ -64,25 +64,37 @@ __device__ inline DD dd_wave_sum(DD v) {
 }
 
 // Per-L sums of the 64 lanes' partial sums acc[L], L < LE, combined in
-// double-double in lane order: the lanes write their partials to LDS and
-// lane L adds column L (64 TwoSum steps) -- a fraction of the cost of LE
-// shuffle trees.  out[L] = M_L * sum for 1 <= L <= Lm with |M_L| > 1e-12,
-// 0 otherwise (L < NL).  One wavefront per workgroup.
+// double-double: 8 multipoles at a time pass through a 64 x 9 LDS tile (a
+// small tile keeps the LDS per wavefront low, so it does not bound the
+// number of resident waves); lane (g, c) adds rows 8g..8g+7 of column c, and
+// the 8 row groups are joined by xor shuffles (TwoSum's error term is exact,
+// so both lanes of a shuffle step hold the same sum).  out[L] = M_L * sum for
+// 1 <= L <= Lm with |M_L| > 1e-12, 0 otherwise (L < NL).  One wavefront per
+// workgroup.
+constexpr int SUM_COLS = 8;
 template <int NL, int LE>
-__device__ __forceinline__ void lane_sums_out(const double* acc, double (*s_acc)[NL + 1], int lane,
+__device__ __forceinline__ void lane_sums_out(const double* acc, double (*s_acc)[SUM_COLS + 1], int lane,
                                               const double* moments, int Lm, double* out) {
+    const int c = lane & (SUM_COLS - 1), g = lane / SUM_COLS;
 #pragma unroll
-    for (int L = 0; L < LE; ++L) s_acc[lane][L] = acc[L];
-    __syncthreads();
-    if (lane < NL) {
-        double v = 0.0;
-        if (lane < LE) {
-            DD t{0.0, 0.0};
-            for (int k = 0; k < 64; ++k) t = dd_add(t, s_acc[k][lane]);
-            const double M = (lane >= 1 && lane <= Lm) ? moments[lane] : 0.0;
-            v = fabs(M) > 1e-12 ? M * (t.hi + t.lo) : 0.0;
+    for (int L0 = 0; L0 < NL; L0 += SUM_COLS) {
+        if (L0 > 0) __syncthreads();              // the previous tile has been read
+#pragma unroll
+        for (int q = 0; q < SUM_COLS; ++q) s_acc[lane][q] = (L0 + q < LE) ? acc[L0 + q] : 0.0;
+        __syncthreads();
+        DD t{0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < 64 / SUM_COLS; ++k) t = dd_add(t, s_acc[g * (64 / SUM_COLS) + k][c]);
+#pragma unroll
+        for (int m = SUM_COLS; m < 64; m <<= 1) {
+            DD o{__shfl_xor(t.hi, m, 64), __shfl_xor(t.lo, m, 64)};
+            t = dd_add(t, o);
         }
-        out[lane] = v;
+        const int L = L0 + c;
+        if (g == 0 && L < NL) {
+            const double M = (L >= 1 && L <= Lm && L < LE) ? moments[L] : 0.0;
+            out[L] = fabs(M) > 1e-12 ? M * (t.hi + t.lo) : 0.0;
+        }
     }
 }
 
@@ -324,6 +336,26 @@ __device__ __forceinline__ void sweep2(double* acc, double w0, double ix0, doubl
     }
 }
 
+// The same for four nodes, added to each acc[L] in node order (so the sums
+// equal four single-node sweeps); four independent power chains give the
+// scheduler room between each product and the FMA that uses it.
+template <int LE>
+__device__ __forceinline__ void sweep4(double* acc, double w0, double ix0, double w1, double ix1, double w2,
+                                       double ix2, double w3, double ix3) {
+    double p0x = ix0 * ix0, p1x = ix1 * ix1, p2x = ix2 * ix2, p3x = ix3 * ix3;
+#pragma unroll
+    for (int L = 1; L < LE; ++L) {
+        acc[L] = fma(w0, p0x, acc[L]);
+        acc[L] = fma(w1, p1x, acc[L]);
+        acc[L] = fma(w2, p2x, acc[L]);
+        acc[L] = fma(w3, p3x, acc[L]);
+        p0x *= ix0;
+        p1x *= ix1;
+        p2x *= ix2;
+        p3x *= ix3;
+    }
+}
+
 // sin, cos of theta and the Filon coefficients alpha, beta, gamma at theta
 // (_filon_segment_complex's formulas, Taylor series below |theta| = 0.3).
 __device__ __forceinline__ void filon_coeffs(double th, double& rs, double& rc, double& al, double& be, double& ga) {
@@ -339,6 +371,33 @@ __device__ __forceinline__ void filon_coeffs(double th, double& rs, double& rc, 
         al = (t2 + th * sd / 2 - 2 * (rs * rs)) * it3;
         be = 2 * (th * (1 + cd) - sd) * it3;
         ga = 4 * (rs - th * rc) * it3;
+    }
+}
+
+// Node weights (wa, wm, wb) of one triple x0 < x1 < x2 with the phases
+// (s_k, c_k) = (sin, cos)(omega x_k + p0): the real part of the Filon sum, or
+// simpson_odd times the cosine of the mid phase where the triple (or, with
+// `small`, its whole segment, mid phase cosine cmid) does not oscillate.
+__device__ __forceinline__ void triple_weights(double x0, double x1, double x2, double s0, double c0, double c1,
+                                               double s2, double c2, bool small, double cmid, double om, double p0,
+                                               double inv_om, double al, double be, double ga, double& wa,
+                                               double& wm, double& wb) {
+    const double h3 = x2 - x0;
+    if (small || fabs(om * h3) < 1e-3 || h3 < 1e-15) {
+        const double wscale = small ? cmid : (h3 < 1e-15 ? 0.0 : cos(om * (x0 + x2) / 2 + p0));
+        const double h0 = x1 - x0, h1 = x2 - x1;
+        const double hsum = h0 + h1, hprod = h0 * h1;
+        const double h0divh1 = h1 != 0.0 ? h0 / h1 : 0.0;
+        const double q = hsum / 6.0;
+        wa = q * (2.0 - (h0divh1 != 0.0 ? 1.0 / h0divh1 : 0.0)) * wscale;
+        wm = q * (hsum * (hprod != 0.0 ? hsum / hprod : 0.0)) * wscale;
+        wb = q * (2.0 - h0divh1) * wscale;
+    } else {
+        // Re[(h/2)(al (-i/om)(f2 e2 - f0 e0) + be (f0 e0 + f2 e2) + ga f1 e1)]
+        const double hh = h3 / 2, cA = al * inv_om;
+        wa = hh * (cA * s0 + be * c0);
+        wb = hh * (-cA * s2 + be * c2);
+        wm = hh * (ga * c1);
     }
 }
 
@@ -376,14 +435,20 @@ __device__ __forceinline__ void filon_sweep(int neff, double om, double p0, doub
     // omega x + p0 itself at x ~ 1e3).
     constexpr int ANCHOR = 16;
     int since_anchor = ANCHOR;
+    const bool pow2_div = ((neff - 1) & (neff - 2)) == 0;
+    const double inv_div = 1.0 / (double)(neff - 1);
+    double sg_prev_end = 0.0;
     double s_c = 0.0, c_c = 1.0;                  // phase carried to the next segment's start
     for (long sg = s_lo; sg < s_hi; ++sg) {
         const double a = __dadd_rn(__dmul_rn((double)sg, bstep), r_m);
         const double b = (sg + 1 == nseg) ? r_max : __dadd_rn(__dmul_rn((double)(sg + 1), bstep), r_m);
-        const double step = (b - a) / (neff - 1);
+        // (b - a) / (neff - 1); a power-of-two divisor (neff = 5, 9) is an exact scaling
+        const double step = pow2_div ? (b - a) * inv_div : (b - a) / (neff - 1);
         const double hseg = b - a;
         if (hseg < 1e-15) continue;               // zero width: its end phase is its start phase
-        double ix0 = a > 1e-6 ? recip(a) : 0.0;
+        // a segment's start is the previous one's end node (the same expression), whose reciprocal is pending
+        double ix0 = (sg > s_lo && sg_prev_end == a) ? ix_pend : (a > 1e-6 ? recip(a) : 0.0);
+        sg_prev_end = b;
         if (ix0 != ix_pend && w_pend != 0.0) {    // the pending node is not this segment's first
             sweep2<LE>(acc, w_pend, ix_pend, 0.0, 0.0);
             w_pend = 0.0;
@@ -411,36 +476,39 @@ __device__ __forceinline__ void filon_sweep(int neff, double om, double p0, doub
             since_anchor = ANCHOR;                // no rotation chain through a non-oscillatory segment
         }
         double x0 = a;
+        const auto inner = [&](int k) { return __dadd_rn(__dmul_rn((double)k, step), a); };
+        const auto node = [&](int k) { return (k == neff - 1) ? b : inner(k); };
+        const auto inv = [](double x) { return x > 1e-6 ? recip(x) : 0.0; };
+        int t = 0;
+        // two triples (x0 x1 x2), (x2 x3 x4) per pass: four nodes swept together
 #pragma unroll 1
-        for (int t = 0; t < nt; ++t) {
-            const int k1 = 2 * t + 1, k2 = 2 * t + 2;
-            const double x1 = __dadd_rn(__dmul_rn((double)k1, step), a);
-            const double x2 = (k2 == neff - 1) ? b : __dadd_rn(__dmul_rn((double)k2, step), a);
-            const double ix1 = x1 > 1e-6 ? recip(x1) : 0.0;
-            const double ix2 = x2 > 1e-6 ? recip(x2) : 0.0;
-            const double h3 = x2 - x0;
-            // phases at x1, x2 by rotation from x0
+        for (; t + 1 < nt; t += 2) {
+            // only the pass's last node can be the segment's end b
+            const double x1 = inner(2 * t + 1), x2 = inner(2 * t + 2), x3 = inner(2 * t + 3), x4 = node(2 * t + 4);
+            const double ix1 = inv(x1), ix2 = inv(x2), ix3 = inv(x3), ix4 = inv(x4);
+            // phases at x1..x4 by rotation from x0
+            const double s1 = fma(s0, rc, c0 * rs), c1 = fma(c0, rc, -(s0 * rs));
+            const double s2 = fma(s1, rc, c1 * rs), c2 = fma(c1, rc, -(s1 * rs));
+            const double s3 = fma(s2, rc, c2 * rs), c3 = fma(c2, rc, -(s2 * rs));
+            const double s4 = fma(s3, rc, c3 * rs), c4 = fma(c3, rc, -(s3 * rs));
+            double wa0, wm0, wb0, wa1, wm1, wb1;
+            triple_weights(x0, x1, x2, s0, c0, c1, s2, c2, small, cmid, om, p0, inv_om, al, be, ga, wa0, wm0, wb0);
+            triple_weights(x2, x3, x4, s2, c2, c3, s4, c4, small, cmid, om, p0, inv_om, al, be, ga, wa1, wm1, wb1);
+            sweep4<LE>(acc, w_pend + wa0, ix0, wm0, ix1, wb0 + wa1, ix2, wm1, ix3);
+            w_pend = wb1;
+            ix_pend = ix4;
+            x0 = x4;
+            ix0 = ix4;
+            s0 = s4;
+            c0 = c4;
+        }
+        if (t < nt) {                             // an odd number of triples: the last one alone
+            const double x1 = inner(2 * t + 1), x2 = node(2 * t + 2);
+            const double ix1 = inv(x1), ix2 = inv(x2);
             const double s1 = fma(s0, rc, c0 * rs), c1 = fma(c0, rc, -(s0 * rs));
             const double s2 = fma(s1, rc, c1 * rs), c2 = fma(c1, rc, -(s1 * rs));
             double wa, wm, wb;
-            if (small || fabs(om * h3) < 1e-3 || h3 < 1e-15) {
-                // simpson_odd on the triple times cos of the mid phase (of the
-                // segment when the whole segment is non-oscillatory)
-                const double wscale = small ? cmid : (h3 < 1e-15 ? 0.0 : cos(om * (x0 + x2) / 2 + p0));
-                const double h0 = x1 - x0, h1 = x2 - x1;
-                const double hsum = h0 + h1, hprod = h0 * h1;
-                const double h0divh1 = h1 != 0.0 ? h0 / h1 : 0.0;
-                const double q = hsum / 6.0;
-                wa = q * (2.0 - (h0divh1 != 0.0 ? 1.0 / h0divh1 : 0.0)) * wscale;
-                wm = q * (hsum * (hprod != 0.0 ? hsum / hprod : 0.0)) * wscale;
-                wb = q * (2.0 - h0divh1) * wscale;
-            } else {
-                // Re[(h/2)(al (-i/om)(f2 e2 - f0 e0) + be (f0 e0 + f2 e2) + ga f1 e1)]
-                const double hh = h3 / 2, cA = al * inv_om;
-                wa = hh * (cA * s0 + be * c0);
-                wb = hh * (-cA * s2 + be * c2);
-                wm = hh * (ga * c1);
-            }
+            triple_weights(x0, x1, x2, s0, c0, c1, s2, c2, small, cmid, om, p0, inv_om, al, be, ga, wa, wm, wb);
             sweep2<LE>(acc, w_pend + wa, ix0, wm, ix1);
             w_pend = wb;
             ix_pend = ix2;
@@ -504,7 +572,7 @@ __device__ inline TailSetup tail_setup(const dwba_tail_job& J, int sgn, int NL) 
 template <int NL>
 __global__ __launch_bounds__(64) void tail_filon_kernel(const dwba_tail_job* __restrict__ jobs, TailParams TP,
                                                         double* __restrict__ jre) {
-    __shared__ double s_acc[64][NL + 1];
+    __shared__ double s_acc[64][SUM_COLS + 1];
     // blockIdx.y = 0: the Phi_+ sign (k_a + k_b, usually many more segments) is dispatched first
     const int job = blockIdx.x, row = 1 - (int)blockIdx.y, sgn = row == 0 ? -1 : 1;
     const dwba_tail_job J = jobs[job];
@@ -548,7 +616,7 @@ __global__ __launch_bounds__(64) void tail_filon_kernel(const dwba_tail_job* __r
 template <int NL>
 __global__ __launch_bounds__(64) void tail_levin_kernel(const dwba_tail_job* __restrict__ jobs, TailParams TP,
                                                         double* __restrict__ jre) {
-    __shared__ double s_acc[64][NL + 1];
+    __shared__ double s_acc[64][SUM_COLS + 1];
     const int job = blockIdx.x, row = 1 - (int)blockIdx.y, sgn = row == 0 ? -1 : 1;
     const dwba_tail_job J = jobs[job];
     const int lane = threadIdx.x;
