@@ -112,6 +112,9 @@ _SIGS = {
                                  c_vp, c_vp]),
     "dwba_dcs_f64": (c_i32, [c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "dwba_sphere_trapz_f64": (c_i32, [c_i32, c_i32, c_vp, c_vp, c_vp, c_vp]),
+    "dwba_pw_cumulate_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "dwba_ion_spin_terms_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
+    "dwba_slot_sums_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
 }
 
 

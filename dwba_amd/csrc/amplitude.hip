@@ -185,6 +185,96 @@ __global__ __launch_bounds__(AT) void dcs_kernel(int n_combos, int n_theta, cons
     block_sphere_sum(acc, s_red, tcs + slot);
 }
 
+// Partial-wave sums of an energy: slot k of an energy holds the amplitudes of
+// one projectile wave l_i (its F, G); the running totals after each slot are
+// the snapshots the partial-wave loop tests for convergence.  One thread per
+// (energy, projection pair, angle) runs over the energy's slots in order;
+// flags[slot] |= 1 when the slot's own F or G has a non-finite value, 2 when
+// the running total after it has.
+__global__ __launch_bounds__(AT) void pw_cumulate_kernel(const int* __restrict__ energy_first, int n_combos,
+                                                         int n_theta, const double* __restrict__ F,
+                                                         const double* __restrict__ G, double* __restrict__ cF,
+                                                         double* __restrict__ cG, int* __restrict__ flags) {
+    const int j = blockIdx.x * AT + threadIdx.x;
+    const int e = blockIdx.y, c = blockIdx.z;
+    if (j >= n_theta) return;
+    double fr = 0.0, fi = 0.0, gr = 0.0, gi = 0.0;
+    const int s1 = energy_first[e + 1];
+    for (int s = energy_first[e]; s < s1; ++s) {
+        const size_t o = (((size_t)s * n_combos + c) * n_theta + j) * 2;
+        const double a = F[o], b = F[o + 1], x = G[o], y = G[o + 1];
+        fr += a;
+        fi += b;
+        gr += x;
+        gi += y;
+        cF[o] = fr;
+        cF[o + 1] = fi;
+        cG[o] = gr;
+        cG[o + 1] = gi;
+        int fl = 0;
+        if (!isfinite(a) || !isfinite(b) || !isfinite(x) || !isfinite(y)) fl |= 1;
+        if (!isfinite(fr) || !isfinite(fi) || !isfinite(gr) || !isfinite(gi)) fl |= 2;
+        if (fl) atomicOr(flags + s, fl);
+    }
+}
+
+__device__ inline double spin_combo(double fr, double fi, double gr, double gi) {
+    const double pr = fr + gr, pi = fi + gi, mr = fr - gr, mi = fi - gi;
+    return 0.25 * (pr * pr + pi * pi) + 0.75 * (mr * mr + mi * mi);
+}
+
+// Ionization partial waves.  A run is the pairs of one (point, l_ej, l_f) in
+// ascending l_i; per projection pair its coefficients (f, g) accumulate over
+// the run (the coherent SDCS amplitude of that key).  For every pair:
+//   dcoh = spin(running sum after it) - spin(running sum before it),
+//   inc  = spin(its own coefficients),       spin(f, g) = |f+g|^2/4 + 3|f-g|^2/4.
+// One thread per (run, projection pair).
+__global__ __launch_bounds__(AT) void ion_spin_terms_kernel(int n_runs, const int* __restrict__ run_first,
+                                                            const int* __restrict__ run_pairs, int n_combos,
+                                                            const double* __restrict__ a, double* __restrict__ dcoh,
+                                                            double* __restrict__ inc) {
+    const long x = (long)blockIdx.x * AT + threadIdx.x;
+    if (x >= (long)n_runs * n_combos) return;
+    const int run = (int)(x / n_combos), c = (int)(x - (long)run * n_combos);
+    double fr = 0.0, fi = 0.0, gr = 0.0, gi = 0.0, prev = 0.0;
+    for (int q = run_first[run]; q < run_first[run + 1]; ++q) {
+        const size_t o = (size_t)run_pairs[q] * n_combos + c;
+        const double* __restrict__ ap = a + o * 4;
+        fr += ap[0];
+        fi += ap[1];
+        gr += ap[2];
+        gi += ap[3];
+        const double now = spin_combo(fr, fi, gr, gi);
+        dcoh[o] = now - prev;
+        inc[o] = spin_combo(ap[0], ap[1], ap[2], ap[3]);
+        prev = now;
+    }
+}
+
+// out1[slot] = sum over the slot's pairs (in the listed order) and projection
+// pairs of x1[pair][combo]; the same for x2.  One wavefront per slot.
+__global__ __launch_bounds__(64) void slot_sums_kernel(const int* __restrict__ slot_first,
+                                                       const int* __restrict__ slot_pairs, int n_combos,
+                                                       const double* __restrict__ x1, const double* __restrict__ x2,
+                                                       double* __restrict__ out1, double* __restrict__ out2) {
+    const int slot = blockIdx.x, lane = threadIdx.x;
+    const int q0 = slot_first[slot], q1 = slot_first[slot + 1];
+    double s1 = 0.0, s2 = 0.0;
+    for (int q = q0; q < q1; ++q) {
+        const size_t base = (size_t)slot_pairs[q] * n_combos;
+        for (int c = lane; c < n_combos; c += 64) {
+            s1 += x1[base + c];
+            s2 += x2[base + c];
+        }
+    }
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if (lane == 0) {
+        out1[slot] = s1;
+        out2[slot] = s2;
+    }
+}
+
 }  // namespace
 }  // namespace dwba
 
@@ -245,6 +335,44 @@ int32_t dwba_dcs_f64(int32_t n_slots, int32_t n_combos, int32_t n_theta, const d
     hipLaunchKernelGGL(dcs_kernel, dim3(n_slots), dim3(AT), 0, reinterpret_cast<hipStream_t>(stream), n_combos,
                        n_theta, d_theta, d_F, d_G, d_prefac, d_dcs, d_tcs);
     DWBA_LAUNCH_CHECK("dcs_kernel");
+    return DWBA_OK;
+}
+
+int32_t dwba_pw_cumulate_f64(int32_t n_energies, const int32_t* d_energy_first, int32_t n_combos, int32_t n_theta,
+                             const double* d_F, const double* d_G, double* d_cumF, double* d_cumG, int32_t* d_flags,
+                             void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_pw_cumulate_f64: call dwba_init first");
+    DWBA_REQUIRE(n_energies >= 1 && n_energies <= 65535 && n_combos >= 1 && n_combos <= 1024 && n_theta >= 1 &&
+                 d_energy_first && d_F && d_G && d_cumF && d_cumG && d_flags, "dwba_pw_cumulate_f64: bad arguments");
+    hipLaunchKernelGGL(pw_cumulate_kernel, dim3((n_theta + AT - 1) / AT, n_energies, n_combos), dim3(AT), 0,
+                       reinterpret_cast<hipStream_t>(stream), d_energy_first, n_combos, n_theta, d_F, d_G, d_cumF,
+                       d_cumG, d_flags);
+    DWBA_LAUNCH_CHECK("pw_cumulate_kernel");
+    return DWBA_OK;
+}
+
+int32_t dwba_ion_spin_terms_f64(int32_t n_runs, const int32_t* d_run_first, const int32_t* d_run_pairs,
+                                int32_t n_combos, const double* d_a, double* d_dcoh, double* d_inc, void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_ion_spin_terms_f64: call dwba_init first");
+    DWBA_REQUIRE(n_runs >= 1 && n_combos >= 1 && n_combos <= 4096 && d_run_first && d_run_pairs && d_a && d_dcoh &&
+                 d_inc, "dwba_ion_spin_terms_f64: bad arguments");
+    const long n = (long)n_runs * n_combos;
+    hipLaunchKernelGGL(ion_spin_terms_kernel, dim3((unsigned)((n + AT - 1) / AT)), dim3(AT), 0,
+                       reinterpret_cast<hipStream_t>(stream), n_runs, d_run_first, d_run_pairs, n_combos, d_a, d_dcoh,
+                       d_inc);
+    DWBA_LAUNCH_CHECK("ion_spin_terms_kernel");
+    return DWBA_OK;
+}
+
+int32_t dwba_slot_sums_f64(int32_t n_slots, const int32_t* d_slot_first, const int32_t* d_slot_pairs,
+                           int32_t n_combos, const double* d_x1, const double* d_x2, double* d_out1, double* d_out2,
+                           void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_slot_sums_f64: call dwba_init first");
+    DWBA_REQUIRE(n_slots >= 1 && n_combos >= 1 && d_slot_first && d_slot_pairs && d_x1 && d_x2 && d_out1 && d_out2,
+                 "dwba_slot_sums_f64: bad arguments");
+    hipLaunchKernelGGL(slot_sums_kernel, dim3(n_slots), dim3(64), 0, reinterpret_cast<hipStream_t>(stream),
+                       d_slot_first, d_slot_pairs, n_combos, d_x1, d_x2, d_out1, d_out2);
+    DWBA_LAUNCH_CHECK("slot_sums_kernel");
     return DWBA_OK;
 }
 

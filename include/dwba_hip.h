@@ -542,6 +542,43 @@ int32_t dwba_dcs_f64(int32_t n_slots, int32_t n_combos, int32_t n_theta, const d
 int32_t dwba_sphere_trapz_f64(int32_t n_rows, int32_t n_theta, const double* d_theta, const double* d_dcs,
                               double* d_tcs, void* stream);
 
+/* ---- partial-wave sums --------------------------------------------------
+ * The projectile partial-wave loops around the integrals, batched: the
+ * excitation loop over l_i with its post-add convergence snapshots
+ * (driver.py:877-1152) and the ionization SDCS loop with its coherent sums
+ * over l_i (ionization.py:526-705).  The convergence decisions themselves are
+ * a few comparisons per l_i, made on the host from these outputs. */
+
+/* Running sums over the slots of each energy: slots
+ * [energy_first[e], energy_first[e+1]) are the energy's l_i in order, with
+ * F, G laid out as dwba_amp_sum_f64 writes them; cumF/cumG[slot] = the sum
+ * over the energy's slots up to and including it (the reference's
+ * total_amplitudes after adding that l_i).  flags[slot] (caller-zeroed)
+ * gets bit 0 when the slot's own F or G holds a non-finite value and bit 1
+ * when the running total does (the loop's numerical-safety checks).
+ * n_combos <= 1024. */
+int32_t dwba_pw_cumulate_f64(int32_t n_energies, const int32_t* d_energy_first, int32_t n_combos, int32_t n_theta,
+                             const double* d_F, const double* d_G, double* d_cumF, double* d_cumG, int32_t* d_flags,
+                             void* stream);
+
+/* Ionization spin sums: a run is the pairs (indices into d_a, whose layout
+ * is dwba_amp_coeffs_f64's) of one (point, l_ej, l_f) in ascending l_i,
+ * listed at d_run_pairs[run_first[run] .. run_first[run+1]).  Per pair and
+ * projection pair, with spin(f, g) = |f+g|^2/4 + 3|f-g|^2/4:
+ *   dcoh = spin(running sum over the run after it) - spin(before it)
+ *   inc  = spin(its own coefficients)
+ * (the incremental coherent sum and the incoherent per-l_i sum of
+ * ionization.py:627-642); d_dcoh, d_inc: [pair][combo]. */
+int32_t dwba_ion_spin_terms_f64(int32_t n_runs, const int32_t* d_run_first, const int32_t* d_run_pairs,
+                                int32_t n_combos, const double* d_a, double* d_dcoh, double* d_inc, void* stream);
+
+/* out1[slot] = sum over the pairs d_slot_pairs[slot_first[slot] ..
+ * slot_first[slot+1]) and all projection pairs of x1[pair][combo] (same for
+ * x2/out2), in a fixed order: the per-l_i totals of the ionization loop. */
+int32_t dwba_slot_sums_f64(int32_t n_slots, const int32_t* d_slot_first, const int32_t* d_slot_pairs,
+                           int32_t n_combos, const double* d_x1, const double* d_x2, double* d_out1, double* d_out2,
+                           void* stream);
+
 /* d_dst[k*dst_ld + i] = d_w[i] * d_src[k*src_ld + i] for i < n, 0 for
  * n <= i < dst_ld, k < n_rows (the quadrature-weighted densities
  * alpha = w rho_a, beta = w rho_b of the plain branch,

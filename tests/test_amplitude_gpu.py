@@ -155,7 +155,8 @@ def test_scan_output_to_cross_sections():
     asm = AmplitudeAssembler(theta, 0, 1, L_max)
     k_i = [en.pairs[0][0].k_au for en in scan]
     k_f = [en.pairs[0][1].k_au for en in scan]
-    rows = [(k, pr[3].l, pr[4].l, pr[0]) for k, pr in enumerate(b.pairs)]
+    # pair k of the scan is stored at row out_index[k] of the device output
+    rows = [(int(b.out_index[k]), pr[3].l, pr[4].l, pr[0]) for k, pr in enumerate(b.pairs)]
     res = asm.excitation(out, rows, k_i, k_f)
     # the oracle on the same integrals, energy by energy
     for e in range(len(scan)):
