@@ -163,3 +163,26 @@ def test_ionization_scan_end_to_end():
             c = res.coherent[(le, lf)][res.combos.index((mi, mf))]
             sc = max(abs(f), abs(g))
             assert abs(c[0] - f) <= 1e-11 * sc and abs(c[1] - g) <= 1e-11 * sc
+
+
+def test_c1_record_in_reference_format(tmp_path):
+    """From the device sums to a stored record: top-up on the per-l_i sums,
+    sigma in both units, DCS on theta in degrees, merged into results/."""
+    import json
+    from dwba_amd import results as R
+    torch, PW, c, g, I, pairs, idx = _c1_setup()
+    chan = PW.ExcitationChannel(0, 1, 4, 1.0)
+    en = _energy(PW, 20, float(g["k_i"]), float(g["k_f"]), lambda l: l <= 20, lambda l: l <= 20)
+    triples, has = PW.excitation_pairs([en], chan)
+    rows = np.array([idx[(li, lf)] for (_, li, lf) in triples])
+    res, = PW.excitation_sums_from_integrals(torch.as_tensor(I, device="cuda").contiguous(), rows, triples, has,
+                                             [en], chan, g["theta"])
+    entry = R.excitation_result_entry(50.0, res, g["theta"], 0, 1, 4, 20, run_meta={"model": "static"})
+    want_sig, want_pw, top = R.excitation_topup(res.partial_waves, R.sigma_au_to_cm2(float(g["tcs"])), 0, 1)
+    assert abs(entry["sigma_cm2"] - want_sig) <= 1e-12 * want_sig
+    assert abs(entry["sigma_au"] - R.sigma_cm2_to_au(entry["sigma_cm2"])) <= 1e-12 * entry["sigma_au"]
+    assert entry["meta"]["runtime"]["n_projectile_partial_waves_summed"] == 21
+    assert np.allclose(entry["theta_deg"], np.degrees(g["theta"]), rtol=0, atol=1e-12)
+    p = R.save_results("results_H2p_exc.json", {"H_1s-2p": [entry]}, root=tmp_path)
+    back = json.loads(p.read_text())["H_1s-2p"][0]
+    assert back["partial_waves"] == entry["partial_waves"] and back["dcs_au_raw"] == entry["dcs_au_raw"]
