@@ -54,6 +54,25 @@ class AmpPair(ctypes.Structure):
     _fields_ = [("row", c_i32), ("type", c_i32), ("l_f", c_i32), ("pad_", c_i32), ("scale", ctypes.c_double)]
 
 
+class CwWave(ctypes.Structure):
+    """dwba_cw_wave of include/dwba_hip.h."""
+    _fields_ = [("r", c_vp), ("U", c_vp), ("k", ctypes.c_double), ("r1_ratio", ctypes.c_double),
+                ("chi0", ctypes.c_double), ("chi1", ctypes.c_double), ("n", c_i32), ("l", c_i32), ("start", c_i32),
+                ("method", c_i32), ("renorm", c_i32), ("pad_", c_i32)]
+
+
+class CwFinish(ctypes.Structure):
+    """dwba_cw_finish of include/dwba_hip.h."""
+    _fields_ = [("r", c_vp), ("w_trapz", c_vp), ("k", ctypes.c_double), ("delta", ctypes.c_double),
+                ("eta", ctypes.c_double), ("sigma", ctypes.c_double), ("n", c_i32), ("l", c_i32),
+                ("idx_match", c_i32), ("tail", c_i32), ("coulomb", c_i32), ("pad_", c_i32)]
+
+
+class CwBypass(ctypes.Structure):
+    """dwba_cw_bypass of include/dwba_hip.h."""
+    _fields_ = [("r", c_vp), ("w_trapz", c_vp), ("U", c_vp), ("k", ctypes.c_double), ("n", c_i32), ("l", c_i32)]
+
+
 class TailJob(ctypes.Structure):
     """dwba_tail_job of include/dwba_hip.h."""
     _fields_ = [("k_a", ctypes.c_double), ("delta_a", ctypes.c_double), ("eta_a", ctypes.c_double),
@@ -115,6 +134,11 @@ _SIGS = {
     "dwba_pw_cumulate_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "dwba_ion_spin_terms_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
     "dwba_slot_sums_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "dwba_cw_workspace_bytes": (c_sz, [c_i32, c_i32]),
+    "dwba_cw_propagate_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp, c_vp, c_sz, c_vp, c_vp]),
+    "dwba_cw_tail_sums_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp, c_vp]),
+    "dwba_cw_stitch_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "dwba_cw_bypass_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp, c_vp]),
 }
 
 

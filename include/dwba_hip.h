@@ -579,6 +579,85 @@ int32_t dwba_slot_sums_f64(int32_t n_slots, const int32_t* d_slot_first, const i
                            int32_t n_combos, const double* d_x1, const double* d_x2, double* d_out1, double* d_out2,
                            void* stream);
 
+/* ---- continuum waves ----------------------------------------------------
+ * Distorted waves chi_l(k, r) for many partial waves at once: the
+ * deterministic paths of solve_continuum_wave (continuum.py:1439-2200),
+ * which the drivers call once per (energy, channel, l) before the integrals
+ * (precompute_continuum_waves, driver.py:486-540).  The host chooses, per
+ * wave, what the reference chooses from r, U, k and l alone (analytic bypass,
+ * first propagated index, match point, start values); these entry points do
+ * every grid-sized step.  Arrays are [wave][ld], ld >= every wave's n. */
+#define DWBA_CW_JOHNSON 0
+#define DWBA_CW_NUMEROV 1
+#define DWBA_CW_MAX_EVENTS 512
+
+typedef struct dwba_cw_wave {
+    const double* r;       /* grid, n points */
+    const double* U;       /* channel potential on the grid */
+    double k;              /* wave number (a.u.) */
+    double r1_ratio;       /* Johnson: R_1 = psi_0 / psi_1 of the propagated range (clipped to +-1e10) */
+    double chi0, chi1;     /* Numerov: the first two values */
+    int32_t n, l;
+    int32_t start;         /* first propagated index; chi = 0 below it */
+    int32_t method;        /* DWBA_CW_JOHNSON or DWBA_CW_NUMEROV */
+    int32_t renorm;        /* Numerov: rescale check every `renorm` steps */
+    int32_t pad_;
+} dwba_cw_wave;
+
+/* Propagation: Johnson's ratio recursion with the backward product and the
+ * division by the maximum (_johnson_log_derivative_solve,
+ * continuum.py:1180-1332), or local-step Numerov (_numerov_propagate,
+ * :985-1069), each with its periodic rescaling reproduced exactly; then
+ * chi' by the 5-point Fornberg stencil on the propagated range
+ * (_derivative_5point, :924-983).  d_status[w] = 1 if a wave rescaled more
+ * than DWBA_CW_MAX_EVENTS times (its values are then not valid).
+ * workspace >= dwba_cw_workspace_bytes(n_waves, ld). */
+size_t dwba_cw_workspace_bytes(int32_t n_waves, int32_t ld);
+int32_t dwba_cw_propagate_f64(int32_t n_waves, const dwba_cw_wave* d_waves, int32_t ld, double* d_chi, double* d_dchi,
+                              void* d_work, size_t work_bytes, int32_t* d_status, void* stream);
+
+typedef struct dwba_cw_finish {
+    const double* r;
+    const double* w_trapz;
+    double k, delta;       /* phase shift of the asymptotic form */
+    double eta, sigma;     /* Coulomb parameter and phase (ionic waves) */
+    int32_t n, l;
+    int32_t idx_match;     /* match point */
+    int32_t tail;          /* first index of the least-squares tail */
+    int32_t coulomb;       /* 1: Coulomb asymptotic form (|z_ion| >= 1e-3) */
+    int32_t pad_;
+} dwba_cw_finish;
+
+/* d_sums[w][0..5] = sums over j in [tail, n) of s^2, s c, c^2, chi s, chi c
+ * and max |chi|, with (s, c) = (sin, cos) of k r - l pi/2 (neutral) or of the
+ * Coulomb phase rho + eta ln 2 rho - l pi/2 + sigma: the normal equations of
+ * the tail fit chi ~ a sin + b cos (_fit_asymptotic_phase_neutral /
+ * _coulomb, continuum.py:656-767). */
+int32_t dwba_cw_tail_sums_f64(int32_t n_waves, const dwba_cw_finish* d_fin, int32_t ld, const double* d_chi,
+                              double* d_sums, void* stream);
+
+/* The stitched wave (continuum.py:2030-2126): d_out[w][j] = s chi[j] for
+ * j <= idx_match, the asymptotic wave sqrt(2/pi) [j^_l cos d - n^_l sin d]
+ * (or sin(th - d) for ions) beyond, s from chi, chi' and the asymptotic
+ * wave's value and 5-point derivative at the match point; divided by its
+ * RMS amplitude when that is off 1 by more than 5%.  d_scratch: [wave][ld]. */
+int32_t dwba_cw_stitch_f64(int32_t n_waves, const dwba_cw_finish* d_fin, int32_t ld, const double* d_chi,
+                           const double* d_dchi, double* d_scratch, double* d_out, void* stream);
+
+typedef struct dwba_cw_bypass {
+    const double* r;
+    const double* w_trapz;
+    const double* U;
+    double k;
+    int32_t n, l;
+} dwba_cw_bypass;
+
+/* The analytic bypass of a weak neutral channel (continuum.py:1620-1668):
+ * d_out[w][j] = sqrt(2/pi) r_j j_l(k r_j) k and d_delta[w] = the Born phase
+ * -k sum_j w_j U_j j_l(k r_j)^2 r_j^2 clipped to [-pi/2, pi/2]. */
+int32_t dwba_cw_bypass_f64(int32_t n_waves, const dwba_cw_bypass* d_byp, int32_t ld, double* d_out, double* d_delta,
+                           void* stream);
+
 /* d_dst[k*dst_ld + i] = d_w[i] * d_src[k*src_ld + i] for i < n, 0 for
  * n <= i < dst_ld, k < n_rows (the quadrature-weighted densities
  * alpha = w rho_a, beta = w rho_b of the plain branch,
