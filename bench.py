@@ -567,6 +567,11 @@ def bench_batched(args, ionization=False):
     if not ionization:
         result["dropin_latency"] = dropin_latency()
         log(f"drop-in: {result['dropin_latency']}")
+        if not args.no_rows:
+            result["continuum_waves"] = bench_continuum()
+            log(f"continuum waves: {result['continuum_waves']}")
+            result["partial_wave_sums"] = bench_partial_waves()
+            log(f"partial-wave sums: {result['partial_wave_sums']}")
 
     if args.no_fp32:
         if cpu:
@@ -612,6 +617,110 @@ def bench_batched(args, ionization=False):
     if cpu:
         result["cpu_baseline"] = cpu
     return result
+
+
+def bench_continuum():
+    """Continuum waves of one energy, both channels, as a scan computes them
+    before its integrals (precompute_continuum_waves, driver.py:486-540):
+    chi_i for l = 0..80 at 100 eV and chi_f for l = 0..84 at 89.8 eV on the
+    15000-point exp grid of the batched scan ([1e-5, 1000] a0), H static
+    potential, Johnson chain, hybrid phase.  Wall clock per solve call (host
+    set-up, launches, copies back), with the analytic bypass on (the
+    reference's default) and off (every wave propagated).  CPU baseline: the
+    oracle's restatement of the same solver (one process) on a sample of
+    propagated waves, scaled to the set."""
+    import torch
+    from dwba_amd.continuum import solve_continuum_waves
+    from dwba_amd.grid import exp_grid
+    from dwba_amd.synthetic import h_potentials
+    gr = exp_grid(15000, 1e-5, 1e3)
+    _, U = h_potentials(gr.r)
+    sets = [(list(range(81)), 100.0), (list(range(85)), 89.8)]
+    out = {}
+    for byp in (True, False):
+        for ls, E in sets:
+            solve_continuum_waves(gr, U, ls, E, 0.0, "hybrid", "johnson", byp)       # warm
+        torch.cuda.synchronize()
+        reps = 3
+        t0 = time.perf_counter()
+        n_ok = 0
+        for _ in range(reps):
+            for ls, E in sets:
+                ws = solve_continuum_waves(gr, U, ls, E, 0.0, "hybrid", "johnson", byp)
+                n_ok += sum(w is not None for w in ws)
+        dt = (time.perf_counter() - t0) / reps
+        n = sum(len(ls) for ls, _ in sets)
+        out["bypass_on" if byp else "bypass_off"] = {"waves": n, "s_per_energy": dt, "waves_per_s": n / dt,
+                                                     "solved": n_ok // reps}
+    from oracle import continuum as OC
+    sample = [(0, 100.0), (40, 100.0), (80, 89.8)]
+    t0 = time.perf_counter()
+    for l, E in sample:
+        OC.solve(gr.r, gr.w_trapz, U, l, E, 0.0, "hybrid", False)
+    per = (time.perf_counter() - t0) / len(sample)
+    n = out["bypass_off"]["waves"]
+    out["cpu_baseline"] = {"value": 1.0 / per, "unit": "waves/s", "cores": 1, "kind": "port",
+                           "sample": f"{len(sample)} propagated waves (l = 0, 40, 80), N = 15000, "
+                                     f"{per:.2f} s each; the set of {n} would take {per * n:.0f} s"}
+    out["speedup_vs_cpu_bypass_off"] = out["bypass_off"]["waves_per_s"] * per
+    out["note"] = ("serial recursions, one lane per wave: latency-bound (no roofline applies); "
+                   "waves/s of whole solve calls")
+    return out
+
+
+def bench_partial_waves():
+    """Amplitudes, per-l_i sums, running totals and cross sections of 64
+    energies on integrals already on the device (the partial-wave loop of
+    driver.py:877-1152 after its integral calls): H 1s -> 2p (3 projection
+    pairs), l_i <= 40, l_f <= l_i + 8 of odd l_i + l_f, L = 0..8, 200 angles;
+    seeded synthetic integrals decaying with l.  Wall clock of
+    excitation_sums_from_integrals.  CPU baseline: the oracle's restatement of
+    the loop (reference amplitude routines) on one energy, scaled to 64."""
+    import torch
+    from dwba_amd import partial_waves as PW
+    rng = np.random.default_rng(7)
+    chan = PW.ExcitationChannel(0, 1, 8, 1.0)
+
+    class _W:
+        pass
+    w = _W()
+    ens = [PW.ExcitationEnergy(None, None, None, None, None, None, float(ki), float(kf),
+                               {l: w for l in range(41)}, {l: w for l in range(49)}, 40)
+           for ki, kf in rng.uniform(1.0, 4.0, (64, 2))]
+    triples, has = PW.excitation_pairs(ens, chan)
+    n = len(triples)
+    I = np.zeros((n, 2, 9))
+    for q, (_, li, lf) in enumerate(triples):
+        I[q] = rng.standard_normal((2, 9)) * 0.7 ** (0.5 * (li + lf))
+    dev = torch.as_tensor(I, device="cuda").contiguous()
+    rows = np.arange(n)
+    theta = np.linspace(0.0, np.pi, 200)
+    PW.excitation_sums_from_integrals(dev, rows, triples, has, ens, chan, theta)      # warm
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = PW.excitation_sums_from_integrals(dev, rows, triples, has, ens, chan, theta)
+    dt = (time.perf_counter() - t0) / reps
+    from oracle import partial_waves as OP
+    q0 = [q for q, t in enumerate(triples) if t[0] == 0]
+    look = {(triples[q][1], triples[q][2]): q for q in q0}
+
+    def integ(li, lf):
+        q = look[(li, lf)]
+        return {L: float(I[q, 0, L]) for L in range(9)}, {L: float(I[q, 1, L]) for L in range(9)}
+    t0 = time.perf_counter()
+    ref = OP.excitation_loop(theta, integ, lambda l: True, lambda l: l <= 48, ens[0].k_i, ens[0].k_f, 0, 1, 8, 40)
+    tc = time.perf_counter() - t0
+    agree = abs(res[0].sigma_au - ref["sigma_au"]) / max(abs(ref["sigma_au"]), 1e-300)
+    return {"energies": 64, "pairs": n, "s_per_call": dt, "pairs_per_s": n / dt,
+            "cpu_baseline": {"value": len(q0) / tc, "unit": "pairs/s", "cores": 1, "kind": "port",
+                             "sample": f"one energy ({len(q0)} pairs, 3 projection pairs, 200 angles) "
+                                       f"in {tc:.2f} s"},
+            "speedup_vs_cpu": (n / dt) / (len(q0) / tc),
+            "sigma_rel_diff_vs_oracle_energy0": agree,
+            "note": "host pair tables, amplitude/sum/cumulate/DCS launches and the copy of the snapshots "
+                    "back, per call"}
 
 
 def one_shot_scan(scan, L_max):
@@ -800,6 +909,7 @@ def main():
     ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-MFMA (and BF16x3, FP64 dense) legs")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-rows", action="store_true", help="skip the continuum-wave and partial-wave-sum legs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="library defaults instead of autotuned launch parameters")
     ap.add_argument("--copy-probe", action="store_true",

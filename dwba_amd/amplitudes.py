@@ -97,12 +97,22 @@ class _TypeTable:
         return out
 
 
-def _pairs_array(rows, types, l_f, scale):
-    n = len(rows)
-    arr = (_hip.AmpPair * n)()
-    for q in range(n):
-        arr[q].row, arr[q].type, arr[q].l_f, arr[q].scale = int(rows[q]), int(types[q]), int(l_f[q]), float(scale[q])
-    return arr
+_PAIR_DTYPE = np.dtype([("row", "<i4"), ("type", "<i4"), ("l_f", "<i4"), ("pad_", "<i4"), ("scale", "<f8")])
+
+
+def _pairs_device(rows, types, l_f, scale, device):
+    """dwba_amp_pair records (the struct's layout: four int32 and a double) on the device."""
+    a = np.zeros(len(rows), dtype=_PAIR_DTYPE)
+    a["row"], a["type"], a["l_f"], a["scale"] = rows, types, l_f, scale
+    return torch.from_numpy(a.view(np.uint8).copy()).to(device)
+
+
+def _type_ids(table, keys2, device):
+    """Channel-type indices of the (a, b) key rows, building tables only for the distinct keys."""
+    k = np.asarray(keys2, dtype=np.int64)
+    uniq, inv = np.unique(k, axis=0, return_inverse=True)
+    ids = table.ids([tuple(int(x) for x in u) for u in uniq], device)
+    return ids[np.asarray(inv).reshape(-1)]
 
 
 def _check_integrals(integrals, L_max):
@@ -177,7 +187,7 @@ class AmplitudeAssembler:
         k_f = np.broadcast_to(np.asarray(k_f, dtype=np.float64), (ns,))
         order = np.argsort(slot, kind="stable")
         rows, li, lf, slot = rows[order], li[order], lf[order], slot[order]
-        types = self.types.ids([(int(a), int(b)) for a, b in zip(li, lf)], self.device)
+        types = _type_ids(self.types, np.stack([li, lf], axis=1), self.device)
         scale = (2.0 / math.pi) / (k_i[slot] * k_f[slot])
         first = np.zeros(ns + 1, dtype=np.int32)
         np.add.at(first, slot + 1, 1)
@@ -189,7 +199,7 @@ class AmplitudeAssembler:
         sh = _hip.stream_handle()
         dev = self.device
         n, nc, nt = len(rows), len(self.combos), self.theta.numel()
-        d_pairs = _hip.struct_array_to_device(_pairs_array(rows, types, lf, scale), dev)
+        d_pairs = _pairs_device(rows, types, lf, scale, dev)
         d_first = torch.as_tensor(first, device=dev)
         a = torch.empty((n, nc, 4), dtype=torch.float64, device=dev)
         _hip.check(lib.dwba_amp_coeffs_f64(n, _hip.ptr(d_pairs), nc, self.L_max, _hip.ptr(integrals),
@@ -264,9 +274,9 @@ class IonizationAssembler:
         scale = np.where(ok, (2.0 / math.pi) / np.where(ok, k_i * k_f, 1.0), 0.0)
         if self.norm:
             scale = scale * np.where(ok, math.sqrt(2.0 / math.pi) / np.where(ok, k_ej, 1.0), 0.0)
-        types = self.types.ids([tuple(int(x) for x in p[1:]) for p in P], self.device)
+        types = _type_ids(self.types, P[:, 1:], self.device)
         lib = _hip.load_library()
-        d_pairs = _hip.struct_array_to_device(_pairs_array(P[:, 0], types, P[:, 2], scale), self.device)
+        d_pairs = _pairs_device(P[:, 0], types, P[:, 2], scale, self.device)
         a = torch.empty((n, len(self.combos), 4), dtype=torch.float64, device=self.device)
         _hip.check(lib.dwba_amp_coeffs_f64(n, _hip.ptr(d_pairs), len(self.combos), self.L_max, _hip.ptr(integrals),
                                            _hip.ptr(self.types.d_coef), _hip.ptr(self.types.d_phase), _hip.ptr(a),

@@ -33,6 +33,7 @@ namespace dwba {
 namespace {
 
 constexpr int CT = 256;
+constexpr int CW_B = 16;         // grid points per register block of the serial recursions
 constexpr double PI_C = 3.14159265358979323846;
 
 struct CwWork {       // per-wave workspace (see dwba_cw_workspace_bytes)
@@ -89,38 +90,68 @@ __global__ __launch_bounds__(64) void cw_propagate_kernel(int n_waves, const dwb
         R[1] = W.r1_ratio;
         double Ri = W.r1_ratio;
         double rm1 = r[0], r0 = r[1];
-        for (int i = 1; i < m - 1; ++i) {
-            const double rp1 = r[i + 1];
-            const double he = sqrt(__dmul_rn(__dsub_rn(rp1, r0), __dsub_rn(r0, rm1)));
-            const double Q = q_of(ell, r0, U[i], k2);
-            double d = __dsub_rn(__dsub_rn(2.0, __dmul_rn(__dmul_rn(he, he), Q)), Ri);
-            if (fabs(d) < 1e-12) d = d != 0.0 ? copysign(1e-12, d) : 1e-12;
-            double Rn = __ddiv_rn(1.0, d);
-            Rn = fmin(fmax(Rn, -1e10), 1e10);
-            R[i + 1] = Rn;
-            Ri = Rn;
-            rm1 = r0;
-            r0 = rp1;
+        // steps i = 1 .. m-2 in blocks of CW_B: the block's grid values are
+        // loaded before its serial steps, so the loads are not on the chain
+        for (int i0 = 1; i0 < m - 1; i0 += CW_B) {
+            double rb[CW_B], ub[CW_B], Rb[CW_B];
+#pragma unroll
+            for (int b = 0; b < CW_B; ++b) {
+                const int i = i0 + b;
+                rb[b] = i + 1 < m ? r[i + 1] : 0.0;
+                ub[b] = i < m ? U[i] : 0.0;
+            }
+#pragma unroll
+            for (int b = 0; b < CW_B; ++b) {
+                const int i = i0 + b;
+                if (i < m - 1) {
+                    const double rp1 = rb[b];
+                    const double he = sqrt(__dmul_rn(__dsub_rn(rp1, r0), __dsub_rn(r0, rm1)));
+                    const double Q = q_of(ell, r0, ub[b], k2);
+                    double d = __dsub_rn(__dsub_rn(2.0, __dmul_rn(__dmul_rn(he, he), Q)), Ri);
+                    if (fabs(d) < 1e-12) d = d != 0.0 ? copysign(1e-12, d) : 1e-12;
+                    double Rn = __ddiv_rn(1.0, d);
+                    Rn = fmin(fmax(Rn, -1e10), 1e10);
+                    Rb[b] = Rn;
+                    Ri = Rn;
+                    rm1 = r0;
+                    r0 = rp1;
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < CW_B; ++b)
+                if (i0 + b < m - 1) R[i0 + b + 1] = Rb[b];
         }
-        // backward product; chi_{m-1} = 1
+        // backward product; chi_{m-1} = 1, blocks of CW_B ratios loaded ahead
         double* __restrict__ raw = S.raw;
         raw[m - 1] = 1.0;
         double cur = 1.0, tmax = 1.0;
-        for (int i = m - 1; i >= 1; --i) {
-            const double v = __dmul_rn(R[i], cur);
-            raw[i - 1] = v;
-            cur = v;
-            tmax = fmax(tmax, fabs(v));
-            if (i % 100 == 0 && (tmax > 1e30 || (tmax < 1e-30 && tmax > 0.0))) {
-                const double s = __ddiv_rn(1.0, tmax);
-                if (nev < DWBA_CW_MAX_EVENTS) {
-                    S.ev_pos[nev] = i - 1;          // applies to positions >= i-1
-                    S.ev_scale[nev] = s;
+        for (int i0 = m - 1; i0 >= 1; i0 -= CW_B) {
+            double Rb[CW_B], vb[CW_B];
+#pragma unroll
+            for (int b = 0; b < CW_B; ++b) Rb[b] = i0 - b >= 1 ? R[i0 - b] : 0.0;
+#pragma unroll
+            for (int b = 0; b < CW_B; ++b) {
+                const int i = i0 - b;
+                if (i >= 1) {
+                    const double v = __dmul_rn(Rb[b], cur);
+                    vb[b] = v;
+                    cur = v;
+                    tmax = fmax(tmax, fabs(v));
+                    if (i % 100 == 0 && (tmax > 1e30 || (tmax < 1e-30 && tmax > 0.0))) {
+                        const double s = __ddiv_rn(1.0, tmax);
+                        if (nev < DWBA_CW_MAX_EVENTS) {
+                            S.ev_pos[nev] = i - 1;  // applies to positions >= i-1
+                            S.ev_scale[nev] = s;
+                        }
+                        ++nev;
+                        cur = __dmul_rn(cur, s);
+                        tmax = __dmul_rn(tmax, s);
+                    }
                 }
-                ++nev;
-                cur = __dmul_rn(cur, s);
-                tmax = __dmul_rn(tmax, s);
             }
+#pragma unroll
+            for (int b = 0; b < CW_B; ++b)
+                if (i0 - b >= 1) raw[i0 - b - 1] = vb[b];
         }
         *S.maxv = tmax;
     } else {
@@ -129,19 +160,34 @@ __global__ __launch_bounds__(64) void cw_propagate_kernel(int n_waves, const dwb
         raw[1] = W.chi1;
         double cm = W.chi0, cc = W.chi1;
         double Qm = q_of(ell, r[0], U[0], k2), Qc = q_of(ell, r[1], U[1], k2);
-        for (int i = 1; i < m - 1; ++i) {
-            const double h1 = __dsub_rn(r[i], r[i - 1]), h2 = __dsub_rn(r[i + 1], r[i]);
-            const double Qn = q_of(ell, r[i + 1], U[i + 1], k2);
+        double rp = r[0], rc = r[1];
+        for (int i0 = 1; i0 < m - 1; i0 += CW_B) {
+          double rb[CW_B], ub[CW_B], vout[CW_B];
+#pragma unroll
+          for (int b = 0; b < CW_B; ++b) {
+              const int i = i0 + b;
+              rb[b] = i + 1 < m ? r[i + 1] : 0.0;
+              ub[b] = i + 1 < m ? U[i + 1] : 0.0;
+          }
+#pragma unroll
+          for (int b = 0; b < CW_B; ++b) {
+            const int i = i0 + b;
+            if (i < m - 1) {
+            const double rn = rb[b];
+            const double h1 = __dsub_rn(rc, rp), h2 = __dsub_rn(rn, rc);
+            const double Qn = q_of(ell, rn, ub[b], k2);
             const double a_prev = __dsub_rn(1.0, __dmul_rn(__ddiv_rn(__dmul_rn(h1, h1), 12.0), Qm));
             const double b_curr = __dadd_rn(2.0, __dmul_rn(__ddiv_rn(__dmul_rn(5.0, __dmul_rn(h1, h2)), 6.0), Qc));
             double a_next = __dsub_rn(1.0, __dmul_rn(__ddiv_rn(__dmul_rn(h2, h2), 12.0), Qn));
             if (fabs(a_next) < 1e-15) a_next = a_next >= 0.0 ? 1e-15 : -1e-15;
             double v = __ddiv_rn(__dsub_rn(__dmul_rn(b_curr, cc), __dmul_rn(a_prev, cm)), a_next);
-            raw[i + 1] = v;
+            vout[b] = v;
             cm = cc;
             cc = v;
             Qm = Qc;
             Qc = Qn;
+            rp = rc;
+            rc = rn;
             if ((i + 1) % W.renorm == 0) {
                 const double mv = fmax(fabs(cm), fabs(cc));
                 double s = 0.0;
@@ -157,6 +203,11 @@ __global__ __launch_bounds__(64) void cw_propagate_kernel(int n_waves, const dwb
                     cc = __dmul_rn(cc, s);
                 }
             }
+                      }
+          }
+#pragma unroll
+          for (int b = 0; b < CW_B; ++b)
+              if (i0 + b < m - 1) raw[i0 + b + 1] = vout[b];
         }
         *S.maxv = 0.0;
     }

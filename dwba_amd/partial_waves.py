@@ -26,10 +26,9 @@ as the reference skips it.
 """
 from __future__ import annotations
 
-import ctypes
 import math
 from collections import deque
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import Dict, List, Optional, Sequence
 
 import numpy as np
@@ -140,8 +139,8 @@ def excitation_partial_wave_sums(energies: Sequence[ExcitationEnergy], chan: Exc
     return excitation_sums_from_integrals(b.out, rows, triples, has, energies, chan, theta_grid)
 
 
-def excitation_sums_from_integrals(integrals: torch.Tensor, rows, triples, has_chi_i, energies, chan, theta_grid,
-                                   _keep=None) -> List[ExcitationResult]:
+def excitation_sums_from_integrals(integrals: torch.Tensor, rows, triples, has_chi_i, energies, chan,
+                                   theta_grid) -> List[ExcitationResult]:
     """The amplitude and convergence part of the loop on integrals already
     on the device: integrals[rows[q]] holds the (direct, exchange) I_L of
     triples[q] = (energy, l_i, l_f)."""
@@ -158,7 +157,8 @@ def excitation_sums_from_integrals(integrals: torch.Tensor, rows, triples, has_c
     ns = int(first[-1])
     if ns > 65535:
         raise ValueError("more than 65535 (energy, l_i) slots in one call")
-    pairs = [(int(rows[q]), li, lf, int(first[e]) + li) for q, (e, li, lf) in enumerate(triples)]
+    T = np.asarray(triples, dtype=np.int64).reshape(-1, 3)
+    pairs = np.stack([np.asarray(rows, dtype=np.int64), T[:, 1], T[:, 2], first[T[:, 0]] + T[:, 1]], axis=1)
     k_i = np.concatenate([np.full(en.L_max_proj + 1, en.k_i) for en in energies])
     k_f = np.concatenate([np.full(en.L_max_proj + 1, en.k_f) for en in energies])
     res = asm.excitation(integrals, pairs, k_i, k_f, n_slots=ns)
@@ -185,8 +185,7 @@ def excitation_sums_from_integrals(integrals: torch.Tensor, rows, triples, has_c
     slot_tcs_h = res.tcs.cpu().numpy()
     flags_h = flags.cpu().numpy()
     valid = np.zeros(ns, dtype=bool)
-    for (_, _, _, s) in pairs:
-        valid[s] = True
+    valid[pairs[:, 3]] = True
     out = []
     for e, en in enumerate(energies):
         s0 = int(first[e])
@@ -233,7 +232,7 @@ def excitation_sums_from_integrals(integrals: torch.Tensor, rows, triples, has_c
             dcs = np.zeros(nt)
             sig = 0.0
         out.append(ExcitationResult(asm.combos, F, G, dcs, sig, float(sigma_au_to_cm2(sig)), pw, stop_reason,
-                                    stop_L, skipped, sum(1 for t in triples if t[0] == e)))
+                                    stop_L, skipped, int(np.count_nonzero(T[:, 0] == e))))
     return out
 
 
