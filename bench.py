@@ -384,24 +384,31 @@ def _source_sha16(name):
         return hashlib.sha256(f.read()).hexdigest()[:16]
 
 
-def _committed_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+def _committed_traffic(kernels, workload):
+    """HBM bytes per launch of a stage's kernels measured with rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and
     recorded in profiles/traffic.json together with the SHA-256 of the source
-    file the kernel was built from.  Returns (bytes, provenance); bytes is None
-    when there is no record or the source has changed since it was measured."""
+    file each kernel was built from.  Returns (bytes summed over `kernels`,
+    provenance); bytes is None when a kernel has no record or its source has
+    changed since it was measured."""
+    names = [kernels] if isinstance(kernels, str) else list(kernels)
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
-            rec = json.load(f).get(workload, {}).get("kernels", {}).get(kernel)
+            recs = json.load(f).get(workload, {}).get("kernels", {})
     except (OSError, ValueError):
-        rec = None
-    if not rec or "source" not in rec:
-        return None, "no counter record for this kernel source"
-    sha = _source_sha16(rec["source"])
-    if sha != rec.get("source_sha16"):
-        return None, f"stale: {rec['source']} changed since the counters were taken ({rec.get('source_sha16')})"
-    return rec["fetch_bytes"] + rec["write_bytes"], f"{rec['source']} sha256 {sha}, rocprofv3 --pmc (profiles/)"
+        recs = {}
+    total, prov = 0.0, []
+    for k in names:
+        rec = recs.get(k)
+        if not rec or "source" not in rec:
+            return None, f"no counter record for {k}"
+        sha = _source_sha16(rec["source"])
+        if sha != rec.get("source_sha16"):
+            return None, f"stale: {rec['source']} changed since the counters were taken ({rec.get('source_sha16')})"
+        total += rec["fetch_bytes"] + rec["write_bytes"]
+        prov.append(f"{k}: {rec['source']} sha256 {sha}")
+    return total, "; ".join(prov) + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; profiles/)"
 
 
 def _scan_leg(scan, L_max, precision, contraction, steps, warmup, graph, stream, tune=True):
@@ -515,7 +522,8 @@ def bench_batched(args, ionization=False):
     cands = {}
     t_p = stages_alone.get("prep")
     if t_p:
-        tr, prov = _committed_traffic("filon_fused_kernel" if grouped else "prep_f64_kernel", wl)
+        tr, prov = _committed_traffic(["filon_fused_kernel", "filon_cells_kernel"] if grouped
+                                      else ["prep_f64_kernel"], wl)
         cands["prep"] = {
             "bound": "hbm", "achieved": p_bytes / (t_p * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
             "frac": p_bytes / (t_p * 1e-3) / PEAK_HBM, "traffic": tr, "traffic_provenance": prov,
@@ -525,7 +533,7 @@ def bench_batched(args, ionization=False):
             "note": "prep stage alone on one stream; algorithmic bytes: chi_i, chi_f read once, alpha_dir, "
                     "alpha_ex, beta_ex written once per pair, shared grid data once per energy"}
     if tail_ms:
-        tr, prov = _committed_traffic("tail_filon_kernel<32>", wl)
+        tr, prov = _committed_traffic(["tail_filon_kernel<32>", "tail_levin_kernel<32>", "tail_combine_kernel"], wl)
         cands["tails"] = {
             "bound": "fp64_valu", "achieved": W_tail / (tail_ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
             "unit": "TFLOP/s", "frac": W_tail / (tail_ms * 1e-3) / PEAK_FP64, "traffic": tr,
