@@ -82,6 +82,42 @@ class TailJob(ctypes.Structure):
                 ("L_max", c_i32), ("kind", c_i32)]
 
 
+class ScanEnergy(ctypes.Structure):
+    """dwba_scan_energy of include/dwba_hip.h (host pointers)."""
+    _fields_ = [("r", c_vp), ("w_simpson", c_vp), ("w_trapz", c_vp), ("V_core", c_vp), ("U_i", c_vp),
+                ("U_f", c_vp), ("u_i", c_vp), ("n", c_i32), ("pad_", c_i32)]
+
+
+class ScanWave(ctypes.Structure):
+    """dwba_scan_wave of include/dwba_hip.h (host pointer)."""
+    _fields_ = [("chi", c_vp), ("k", ctypes.c_double), ("phase_shift", ctypes.c_double),
+                ("eta", ctypes.c_double), ("sigma_l", ctypes.c_double), ("l", c_i32), ("idx_match", c_i32),
+                ("n", c_i32), ("pad_", c_i32)]
+
+
+class ScanPair(ctypes.Structure):
+    """dwba_scan_pair of include/dwba_hip.h (host pointer)."""
+    _fields_ = [("u_f", c_vp), ("n_u_f", c_i32), ("energy", c_i32), ("wave_i", c_i32), ("wave_f", c_i32),
+                ("excitation", c_i32), ("pad_", c_i32)]
+
+
+class ScanOptions(ctypes.Structure):
+    """dwba_scan_options of include/dwba_hip.h."""
+    _fields_ = [("L_max", c_i32), ("use_oscillatory_quadrature", c_i32), ("method", c_i32), ("CC_nodes", c_i32),
+                ("phase_increment", ctypes.c_double), ("min_grid_fraction", ctypes.c_double),
+                ("k_threshold", ctypes.c_double)]
+
+
+class ScanPairPlan(ctypes.Structure):
+    """dwba_scan_pair_plan of include/dwba_hip.h."""
+    _fields_ = [("p_start", ctypes.c_double), ("p_step", ctypes.c_double), ("p_end", ctypes.c_double),
+                ("r_match", ctypes.c_double), ("N1", c_i32), ("mode", c_i32), ("tail", c_i32), ("rows", c_i32),
+                ("cols", c_i32), ("log1", c_i32), ("log2", c_i32), ("n_pnodes", c_i32), ("coulomb", c_i32),
+                ("oscillatory", c_i32)]
+
+
+SCAN_METHODS = {"legacy": 0, "advanced": 1, "full_split": 2}   # DWBA_METHOD_* of include/dwba_hip.h
+
 c_dp = ctypes.POINTER(ctypes.c_double)
 
 # name -> (restype, argtypes)
@@ -139,6 +175,15 @@ _SIGS = {
     "dwba_cw_tail_sums_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp, c_vp]),
     "dwba_cw_stitch_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "dwba_cw_bypass_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp, c_vp]),
+    "dwba_scan_plan": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp]),
+    "dwba_scan_plan_pairs": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp]),
+    "dwba_scan_launch": (c_i32, [c_vp, c_vp]),
+    "dwba_scan_capture": (c_i32, [c_vp, c_vp]),
+    "dwba_scan_replay": (c_i32, [c_vp, c_vp]),
+    "dwba_scan_results": (c_vp, [c_vp]),
+    "dwba_scan_read_results": (c_i32, [c_vp, c_vp, c_vp]),
+    "dwba_scan_info": (c_i32, [c_vp, c_vp, c_i32]),
+    "dwba_scan_destroy": (c_i32, [c_vp]),
 }
 
 

@@ -35,6 +35,7 @@ from .planner import GridStats, PREP_PLAIN, PREP_FILON, PREP_STANDARD, PREP_ZERO
 
 PREP_NO_ZERO_FILL = 0x10   # include/dwba_hip.h: the batch's per-pair vectors are zero-initialised once
 PREP_FILON_GROUPED = 0x20  # include/dwba_hip.h: Filon alpha/beta from the group cell sums instead
+PREP_NODES_SHARED = 0x40   # include/dwba_hip.h: Coulomb phase nodes filled by another job of the launch
 GROUP_SLICE = 4            # members per workgroup of the fused grouped Filon pass (its FUSE_MAX)
 from .dwba_matrix_elements import (PairPlan, RadialDWBAIntegrals, TAIL_NONE, TAIL_OUTER, TAIL_LEGACY, plan_pair,
                                    _fit)
@@ -253,6 +254,7 @@ class ScanBatch:
         coul_par = np.zeros((n_pairs, 4))
         any_coul = False
         node_bufs = []
+        node_owner = {}
         self.mom_sets = {}   # (energy, id(u_f)) -> index
         for k, (ei, pi, plan, ci, cf, bf) in enumerate(self.pairs):
             st = self.E[ei]
@@ -268,17 +270,25 @@ class ScanBatch:
                 outs = (st.ad[c], st.bd[c], st.ae[c], st.be[c])
             lay = plan.layout
             nodes = None
+            flags = PREP_NO_ZERO_FILL if plan.mode == PREP_FILON else 0
             if plan.mode == PREP_FILON and lay.coulomb:
-                nodes = torch.empty(10002, dtype=f64, device=dev)
-                node_bufs.append(nodes)
+                # pairs of one energy with the same stepping share one node
+                # buffer: the first fills it, the others only count their nodes
+                key = (ei, lay.p_start, lay.p_end, lay.p_step, lay.k_total, lay.eta_total)
+                nodes = node_owner.get(key)
+                if nodes is None:
+                    nodes = torch.empty(10002, dtype=f64, device=dev)
+                    node_owner[key] = nodes
+                    node_bufs.append(nodes)
+                else:
+                    flags |= PREP_NODES_SHARED
                 coul_par[k] = (lay.p_start, lay.p_end, lay.k_total, lay.eta_total)
                 any_coul = True
             prep[k] = hip.PrepJob(hip.ptr(st.r_d), hip.ptr(st.logr_d), hip.ptr(st.w_d), hip.ptr(ui_d),
                                   hip.ptr(uf_d), hip.ptr(ci_d), hip.ptr(cf_d), hip.ptr(st.vdiff_d), hip.ptr(nodes),
                                   hip.ptr(outs[0]), hip.ptr(outs[1]), hip.ptr(outs[2]), hip.ptr(outs[3]),
                                   hip.ptr(self.scalars[k]), lay.p_start, lay.p_step, lay.p_end, st.N, plan.N1,
-                                  lay.n_pnodes, plan.mode | (PREP_NO_ZERO_FILL if plan.mode == PREP_FILON else 0),
-                                  plan.log1, plan.log2)
+                                  lay.n_pnodes, plan.mode | flags, plan.log1, plan.log2)
         self.prep_d = hip.struct_array_to_device(prep, dev)
         self._prep_host = prep
         self.prep_d_gemv = None           # prep jobs for the shared-beta direct path (_build_fp64)

@@ -542,7 +542,9 @@ __global__ __launch_bounds__(PREP_THREADS) void moments_f64_kernel(const double*
 
 // Coulomb-phase node stepping of generate_phase_nodes (oscillatory_integrals.py:1685-1713):
 // r_{n+1} = r_n + clip(dphi / |k + eta / max(r_n, 1e-10)|, 1e-6, (b - a)/10), capped at b,
-// at most 10001 nodes.  One thread per job; node count written to the job.
+// at most 10001 nodes.  One thread per job; node count written to the job.  A
+// job flagged DWBA_PREP_NODES_SHARED reads a buffer another job of the launch
+// fills with the same stepping: it steps only to count its nodes.
 __global__ void coulomb_nodes_kernel(dwba_prep_job* __restrict__ jobs, int n_jobs, const double* __restrict__ par) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_jobs) return;
@@ -550,9 +552,10 @@ __global__ void coulomb_nodes_kernel(dwba_prep_job* __restrict__ jobs, int n_job
     if ((J.mode & DWBA_PREP_MODE_MASK) != DWBA_PREP_FILON || J.phase_nodes == nullptr) return;
     const double a = par[4 * k + 0], b = par[4 * k + 1], kt = par[4 * k + 2], eta = par[4 * k + 3];
     const double dphi = J.p_step;  // phase increment for Coulomb jobs
-    double* nodes = const_cast<double*>(J.phase_nodes);
+    double* nodes = (J.mode & DWBA_PREP_NODES_SHARED) ? nullptr : const_cast<double*>(J.phase_nodes);
     int n = 0;
-    nodes[n++] = a;
+    if (nodes) nodes[n] = a;
+    ++n;
     double r = a;
     const double cap = (b - a) / 10;
     while (r < b) {
@@ -562,7 +565,8 @@ __global__ void coulomb_nodes_kernel(dwba_prep_job* __restrict__ jobs, int n_job
         dr = fmax(dr, 1e-6);
         double rn = r + dr;
         if (rn >= b) rn = b;
-        nodes[n++] = rn;
+        if (nodes) nodes[n] = rn;
+        ++n;
         r = rn;
         if (n > 10000) break;
     }

@@ -7,7 +7,8 @@
  * Conventions
  *  - Every pointer argument named d_* is DEVICE memory owned by the caller;
  *    h_* is host memory.  The library allocates nothing behind the caller's
- *    back except the per-process state created by dwba_init.
+ *    back except the per-process state created by dwba_init and the one
+ *    device allocation a scan handle owns (dwba_scan_plan .. dwba_scan_destroy).
  *  - `stream` is a hipStream_t (NULL = default stream).  Calls only enqueue
  *    work; results are valid after the caller synchronises the stream.
  *  - Every function returns DWBA_OK (0) or an error code; the message of the
@@ -199,6 +200,9 @@ enum {
 #define DWBA_PREP_FILON_GROUPED 0x20   /* FILON: nothing in this launch (Coulomb nodes aside); alpha/beta,
                                           beta_dir and the scalars come from dwba_filon_cells_f64 +
                                           dwba_filon_fused_f64 */
+#define DWBA_PREP_NODES_SHARED 0x40    /* FILON with Coulomb stepping: phase_nodes is the buffer of another job of
+                                          the same launch with the same d_coulomb_par entry and p_step (pairs of one
+                                          Filon group); that job fills it, this one only counts its nodes */
 #define DWBA_PREP_MODE_MASK 0x0f
 
 /* One channel pair.  Device pointers; outputs are the alpha and beta vectors (N doubles
@@ -441,6 +445,117 @@ typedef struct dwba_finalize_pair {
 } dwba_finalize_pair;
 int32_t dwba_batch_finalize(int32_t n_pairs, const dwba_finalize_pair* d_pairs, int32_t L_max, double* d_out,
                             void* stream);
+
+/* ------------------------------------------------------------------ */
+/* A whole scan behind one handle: plan, launch, capture.                */
+/* ------------------------------------------------------------------ */
+/* The reference computes a scan's integrals one channel pair at a time:
+ * radial_ME_all_L_gpu inside the l_i / l_f loops of compute_total_excitation_cs
+ * (driver.py:907-1029), and for ionization one _compute_sdcs_at_energy call
+ * per ejected energy (ionization.py:991-1002), each with its own pair loop
+ * (:526-608).  dwba_scan_plan takes every pair of a scan at once, as HOST
+ * arrays (the arrays those call sites hold), makes radial_ME_all_L's per-pair
+ * decisions on the host (split index, Filon phase layout, tail kind;
+ * dwba_matrix_elements.py:884-1108), and uploads grids, potentials and
+ * waves into one device allocation together with every launch table.
+ * dwba_scan_launch then enqueues the whole scan on a stream and allocates
+ * nothing, so it can be captured (dwba_scan_capture, then dwba_scan_replay).
+ * Results (device, valid once the stream reaches them): I_L of pair p at
+ * dwba_scan_results(scan)[p*2*(L_max+1) + L] (direct) and
+ * [p*2*(L_max+1) + L_max+1 + L] (exchange), pairs in the order given.
+ * Arguments and array-length errors follow radial_ME_all_L_gpu
+ * (dwba_matrix_elements.py:1637-1672: arrays longer than the grid are
+ * truncated, shorter ones rejected).  The tail stage runs on a stream of the
+ * scan's own, forked from and joined back into the launch stream.  One
+ * scan's launches share its buffers: do not run two at once. */
+typedef struct dwba_scan_energy {
+    const double* r;          /* grid points (grid.r), n, increasing */
+    const double* w_simpson;  /* integration weights (grid.w_simpson) */
+    const double* w_trapz;    /* trapezoid weights (grid.w_trapz): the 99% extent of the bound states */
+    const double* V_core;     /* core potential */
+    const double* U_i;        /* distorting potential of the incident channel (U_i.U_of_r) */
+    const double* U_f;        /* of the final channel (U_f_array), or NULL */
+    const double* u_i;        /* initial bound state (bound_i.u_of_r) */
+    int32_t n;                /* grid points, >= 2 */
+    int32_t pad_;
+} dwba_scan_energy;
+
+typedef struct dwba_scan_wave {   /* one continuum wave (continuum.ContinuumWave) */
+    const double* chi;        /* chi_of_r on the grid of the energy whose pairs use it */
+    double k;                 /* k_au */
+    double phase_shift, eta, sigma_l;
+    int32_t l;
+    int32_t idx_match;        /* match index; <= 0 or >= n: none */
+    int32_t n;                /* points in chi (>= the grid's n) */
+    int32_t pad_;
+} dwba_scan_wave;
+
+typedef struct dwba_scan_pair {
+    const double* u_f;        /* bound_f.u_of_r (excitation) or the ejected wave's chi_of_r (ionization);
+                                 pairs of an energy that pass the same pointer share its upload and moments */
+    int32_t n_u_f;            /* points in u_f (>= the grid's n) */
+    int32_t energy;           /* index into the energies */
+    int32_t wave_i, wave_f;   /* indices into the waves: incident and scattered projectile */
+    int32_t excitation;       /* 1: bound final state (outer tails apply); 0: ionization */
+    int32_t pad_;
+} dwba_scan_pair;
+
+enum {                        /* oscillatory_method */
+    DWBA_METHOD_LEGACY = 0,
+    DWBA_METHOD_ADVANCED = 1,
+    DWBA_METHOD_FULL_SPLIT = 2
+};
+
+typedef struct dwba_scan_options {   /* radial_ME_all_L's keyword arguments */
+    int32_t L_max;                       /* 0..63 */
+    int32_t use_oscillatory_quadrature;  /* reference default 1 */
+    int32_t method;                      /* DWBA_METHOD_* (reference default "advanced") */
+    int32_t CC_nodes;                    /* 2..16 (reference default 5) */
+    double phase_increment;              /* reference default 1.5708 */
+    double min_grid_fraction;            /* reference default 0.10 */
+    double k_threshold;                  /* reference default 0.5 */
+} dwba_scan_options;
+
+typedef struct dwba_scan dwba_scan;   /* opaque */
+
+/* Plans a scan (host decisions, one device allocation, uploads; synchronous).
+ * On success *out_scan is the handle; on failure it is NULL. */
+int32_t dwba_scan_plan(int32_t n_energies, const dwba_scan_energy* h_energies, int32_t n_waves,
+                       const dwba_scan_wave* h_waves, int32_t n_pairs, const dwba_scan_pair* h_pairs,
+                       const dwba_scan_options* h_options, dwba_scan** out_scan);
+/* The per-pair decisions dwba_scan_plan makes, without a device (host only,
+ * no dwba_init needed): for pair k, h_out[k] holds the split index N1
+ * (idx_limit), the prep mode (DWBA_PREP_*), the tail kind (DWBA_TAIL_*), the
+ * contraction extent (rows x cols), the log-spacing tests of r[:N1] and r[:N],
+ * and the Filon phase layout (uniform: n_pnodes nodes from p_start to p_end
+ * in steps p_step; Coulomb stepping: p_step is the phase increment and the
+ * nodes are made on the device). */
+typedef struct dwba_scan_pair_plan {
+    double p_start, p_step, p_end;
+    double r_match;           /* r[N1-1] */
+    int32_t N1, mode, tail, rows, cols, log1, log2, n_pnodes, coulomb, oscillatory;
+} dwba_scan_pair_plan;
+int32_t dwba_scan_plan_pairs(int32_t n_energies, const dwba_scan_energy* h_energies, int32_t n_waves,
+                             const dwba_scan_wave* h_waves, int32_t n_pairs, const dwba_scan_pair* h_pairs,
+                             const dwba_scan_options* h_options, dwba_scan_pair_plan* h_out);
+/* Enqueues the whole scan on stream (NULL = the null stream). */
+int32_t dwba_scan_launch(dwba_scan* scan, void* stream);
+/* Records dwba_scan_launch on stream (a created stream) into a HIP graph
+ * owned by the scan, replacing an earlier one; nothing runs. */
+int32_t dwba_scan_capture(dwba_scan* scan, void* stream);
+/* Launches the captured graph on stream. */
+int32_t dwba_scan_replay(dwba_scan* scan, void* stream);
+/* Device pointer to the results (n_pairs * 2 * (L_max+1) doubles). */
+const double* dwba_scan_results(const dwba_scan* scan);
+/* Copies the results to host memory h_out (same layout) after the work
+ * enqueued on stream so far; returns when they are there. */
+int32_t dwba_scan_read_results(const dwba_scan* scan, double* h_out, void* stream);
+/* h_info[0..n_info): pairs, L_max, device bytes, pairs with a tail, Filon
+ * groups, energies with a shared direct beta, Coulomb node buffers, moment
+ * sets (further entries 0). */
+int32_t dwba_scan_info(const dwba_scan* scan, int64_t* h_info, int32_t n_info);
+/* Waits for the device, then frees the scan (NULL is a no-op). */
+int32_t dwba_scan_destroy(dwba_scan* scan);
 
 /* ---- differentiation of the plain contraction -------------------------
  * Forward quantity (the plain branch of radial_ME_all_L,

@@ -53,6 +53,10 @@ def test_struct_layouts_match_header(tmp_path):
         "dwba_xgemm_energy": batch.XgemmEnergy, "dwba_dgemv_energy": batch.DgemvEnergy,
         "dwba_ddot_pair": batch.DdotPair, "dwba_finalize_pair": batch.FinalizePair,
         "dwba_lane_grid": batch.LaneGrid, "dwba_filon_group": batch.FilonGroup,
+        "dwba_amp_pair": _hip.AmpPair, "dwba_cw_wave": _hip.CwWave, "dwba_cw_finish": _hip.CwFinish,
+        "dwba_cw_bypass": _hip.CwBypass, "dwba_scan_energy": _hip.ScanEnergy, "dwba_scan_wave": _hip.ScanWave,
+        "dwba_scan_pair": _hip.ScanPair, "dwba_scan_options": _hip.ScanOptions,
+        "dwba_scan_pair_plan": _hip.ScanPairPlan,
     }
     src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, cls in structs.items():
