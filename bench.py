@@ -570,6 +570,8 @@ def bench_batched(args, ionization=False):
         result["stage_rooflines"] = cands
     result["stage_ms_alone"] = stages_alone
     del b64s
+    result["c_abi_scan"] = c_abi_scan(scan, L_max, args.steps, res64s, mag)
+    log(f"C-ABI scan: {result['c_abi_scan']}")
     result["one_shot"] = one_shot_scan(scan, L_max)
     log(f"one-shot scan: {result['one_shot']}")
     if not ionization:
@@ -756,6 +758,42 @@ def one_shot_scan(scan, L_max):
             "total_s": t4 - t0, "pairs": n,
             "note": "fresh ScanBatch on the same scan: planning + uploads + tables, tune() (cached choice), "
                     "one eager launch, results to Python dicts"}
+
+
+def c_abi_scan(scan, L_max, steps, ref=None, mag=None):
+    """The same FP64 scan through the whole-scan C entry points (dwba_scan_plan
+    / _launch / _capture / _replay, C++ planning and launch sequence), timed
+    like the headline leg: graph replays on one stream between HIP events,
+    inputs resident.  Also its host planning + upload time and the agreement
+    with the ScanBatch result (|dI| / sum|a K b|)."""
+    import torch
+    from dwba_amd.cscan import CScan
+    t0 = time.perf_counter()
+    cs = CScan(scan, L_max, oscillatory_method="full_split")
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter() - t0
+    stream = torch.cuda.Stream()
+    cs.launch(stream)
+    stream.synchronize()
+    eager_ms = _events_ms(lambda: cs.launch(stream), steps, stream)
+    cs.capture(stream)
+    cs.replay(stream)
+    stream.synchronize()
+    replay_ms = _events_ms(lambda: cs.replay(stream), steps, stream)
+    res = cs.results(stream)
+    info = cs.info()
+    cs.close()
+    n_int = 2 * res.shape[0] * res.shape[2]
+    out = {"ms_per_step": replay_ms, "eager_ms_per_step": eager_ms, "value": n_int / (replay_ms * 1e-3),
+           "unit": "integrals/s", "plan_s": t_plan, "info": info,
+           "note": "include/dwba_hip.h dwba_scan_*: C++ host planning, one device allocation, fixed FP64 step "
+                   "(tail fork, shared direct beta, split-lane contraction, grouped Filon prep)"}
+    if ref is not None:
+        d = np.abs(res - ref)
+        out["max_abs_diff_vs_scanbatch"] = float(np.max(d))
+        if mag is not None:
+            out["max_diff_over_sum_abs_aKb"] = float(np.max(d / np.maximum(mag, 1e-300)))
+    return out
 
 
 def dropin_latency():

@@ -283,3 +283,38 @@ def test_c_scan_matches_oracle_and_dropin(case):
         assert 0 < info[6] < n_pairs             # Coulomb node buffers shared within groups
     drop, orc = _references(points, L_max, method)
     _check(got, drop, orc)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("z_ion", [0.0, 1.0])
+def test_cscan_binding_matches_scanbatch_step(z_ion):
+    """The Python binding of the C engine (dwba_amd.cscan) on a
+    multi-energy scan agrees with ScanBatch set to the same FP64 step (tail
+    fork, shared direct beta with matrix-core dots, split-lane contraction in
+    energy order, grouped Filon prep) to FP64 rounding."""
+    from dwba_amd.batch import ScanBatch
+    from dwba_amd.cscan import CScan
+    scan = S.excitation_scan(n_energies=3, n_pairs=24, n_range=(1500, 3000), l_max=14, k_range=(0.5, 4.0), seed=31,
+                             z_ion=z_ion)
+    L_max = 12
+    b = ScanBatch(scan, L_max, precision="fp64", oscillatory_method="full_split")
+    b.overlap = "start"
+    b.direct_gemv = True
+    b.dot_kind = "mfma"
+    b.semisep_lanes = True
+    b.lane_tables = True
+    b.lane_split = 2
+    b.semisep_config = (0, 8)
+    b.lane_order = "energy"
+    b.prep_grouped = True
+    b.run()
+    ref = b.results()
+    mag = b.magnitudes()
+    cs = CScan(scan, L_max, oscillatory_method="full_split")
+    cs.launch()
+    got = cs.results()
+    info = cs.info()
+    cs.close()
+    assert info["pairs"] == ref.shape[0] and info["shared_beta_energies"] == len(scan)
+    bad = ~(np.abs(got - ref) <= 1e-12 * np.abs(ref) + 1e-14 * mag)
+    assert not bad.any(), (np.argwhere(bad)[:5].tolist(), got[bad][:3], ref[bad][:3])
