@@ -650,12 +650,21 @@ class ScanBatch:
             off += 7 * ld
             first += len(g)
         # the fused pass takes slices of <= GROUP_SLICE members of a group (same
-        # cell sums), one workgroup each over its grid, longest grids first
-        sl = []
-        for i in sorted(range(len(gl)), key=lambda i: -self.pairs[gl[i][0]][2].N):
+        # cell sums), one workgroup each over its grid; each energy's slices go
+        # to one XCD (its grid arrays, u_f and cell tables stay in that L2),
+        # energies balanced over the XCDs, longest first on each
+        per_e = {}
+        for i, g in enumerate(gl):
             G = arr[i]
+            ei = self.pairs[g[0]][0]
             for q in range(0, G.count, GROUP_SLICE):
-                sl.append(FilonGroup(G.coef, G.ld_coef, G.rep, G.first + q, min(GROUP_SLICE, G.count - q)))
+                c = min(GROUP_SLICE, G.count - q)
+                per_e.setdefault(ei, []).append((FilonGroup(G.coef, G.ld_coef, G.rep, G.first + q, c),
+                                                 self.E[ei].N * (10 + 6 * c)))
+        units = [(sum(c for _, c in v), [sl for sl, _ in v]) for v in per_e.values()]
+        sl = planner.xcd_runs(units, pad=FilonGroup(arr[0].coef, arr[0].ld_coef, arr[0].rep, 0, 0))
+        if len(sl) > 65535:
+            return
         slices = (FilonGroup * len(sl))(*sl)
         self.filon_groups = (hip.struct_array_to_device(arr, self.dev), len(gl),
                              torch.as_tensor(np.asarray(members, dtype=np.int32), device=self.dev),

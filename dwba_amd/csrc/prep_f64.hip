@@ -399,14 +399,23 @@ __global__ __launch_bounds__(PREP_THREADS) void filon_cells_kernel(const dwba_fi
 // on [0, N) where the job has a beta_dir, and reduces the four L = 0
 // correction sums of the streaming pass over the whole grid with w_lim = 0
 // beyond N1 (a non-finite density there still propagates, as in the
-// reference's dot products).
+// reference's dot products).  Workgroups are dealt to the 8 XCDs
+// round-robin; slice s runs on XCD s / ceil(n / 8), so a caller that lists
+// its slices in eight consecutive runs (one energy's slices in one run,
+// count 0 for padding) keeps each energy's shared arrays and cell tables in
+// one XCD's L2.
 constexpr int FUSE_MAX = 4;
 
 __global__ __launch_bounds__(PREP_THREADS) void filon_fused_kernel(const dwba_filon_group* __restrict__ slices,
                                                                    const dwba_prep_job* __restrict__ jobs,
                                                                    const int* __restrict__ members) {
     __shared__ double s_red[PREP_THREADS / 64][FUSE_MAX * 4];
-    const dwba_filon_group G = slices[blockIdx.x];
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = nb / 8, rem = nb % 8, xcd = b % 8;
+    const int sidx = (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + b / 8;
+    if (sidx >= nb) return;
+    const dwba_filon_group G = slices[sidx];
+    if (G.count <= 0) return;
     const dwba_prep_job& R = jobs[G.rep];
     const int N = R.N, N1 = R.N1;
     const int cnt = G.count < FUSE_MAX ? G.count : FUSE_MAX;

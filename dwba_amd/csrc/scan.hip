@@ -652,8 +652,45 @@ int plan_scan(dwba_scan* s, int32_t n_energies, const dwba_scan_energy* E, int32
         }
         groups[it->second].push_back(k);
     }
-    long n_slices = 0;
-    for (auto& g : groups) n_slices += (g.size() + GROUP_SLICE - 1) / GROUP_SLICE;
+    // the fused pass's slices (<= GROUP_SLICE members of a group): each
+    // energy's slices in one XCD run of dwba_filon_fused_f64 (its grid
+    // arrays, u_f and cell tables stay in one L2), energies spread over the
+    // 8 runs by longest-processing-time, heaviest first in each, runs padded
+    // to one length with empty slices (g < 0)
+    struct SliceRef { int g, q, c; };
+    std::vector<SliceRef> slice_list;
+    {
+        std::vector<int> e_order;
+        std::map<int, std::vector<SliceRef>> per_e;
+        std::map<int, double> e_cost;
+        for (size_t g = 0; g < groups.size(); ++g) {
+            const int e = plan[groups[g][0]].energy;
+            if (!per_e.count(e)) e_order.push_back(e);
+            const int cnt = (int)groups[g].size();
+            for (int q = 0; q < cnt; q += GROUP_SLICE) {
+                const int c = std::min(GROUP_SLICE, cnt - q);
+                per_e[e].push_back({(int)g, q, c});
+                e_cost[e] += (double)plan[groups[g][0]].N * (10 + 6 * c);
+            }
+        }
+        std::stable_sort(e_order.begin(), e_order.end(), [&](int a, int b) { return e_cost[a] > e_cost[b]; });
+        std::vector<SliceRef> bins[8];
+        double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int e : e_order) {
+            int b = 0;
+            for (int i = 1; i < 8; ++i)
+                if (load[i] < load[b]) b = i;
+            bins[b].insert(bins[b].end(), per_e[e].begin(), per_e[e].end());
+            load[b] += e_cost[e];
+        }
+        size_t len = 0;
+        for (auto& b : bins) len = std::max(len, b.size());
+        for (auto& b : bins) {
+            slice_list.insert(slice_list.end(), b.begin(), b.end());
+            for (size_t i = b.size(); i < len; ++i) slice_list.push_back({-1, 0, 0});
+        }
+    }
+    const long n_slices = (long)slice_list.size();
     const bool grouped = !groups.empty() && n_slices <= 65535;
     std::vector<char> in_group(n_pairs, 0);
     if (grouped)
@@ -846,7 +883,7 @@ int plan_scan(dwba_scan* s, int32_t n_energies, const dwba_scan_energy* E, int32
         s->any_coul = 1;
     }
 
-    // ---- Filon groups and the fused pass's slices (longest grids first)
+    // ---- Filon groups and the fused pass's slices
     if (grouped) {
         std::vector<dwba_filon_group> ga(groups.size());
         std::vector<int32_t> mem;
@@ -860,19 +897,12 @@ int plan_scan(dwba_scan* s, int32_t n_energies, const dwba_scan_energy* E, int32
             for (int k : groups[g]) mem.push_back(k);
             first += (int)groups[g].size();
         }
-        std::vector<int> order(groups.size());
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int a, int b) { return plan[groups[a][0]].N > plan[groups[b][0]].N; });
         std::vector<dwba_filon_group> sl;
-        for (int g : order) {
-            const dwba_filon_group& G = ga[g];
-            for (int q = 0; q < G.count; q += GROUP_SLICE) {
-                dwba_filon_group x = G;
-                x.first = G.first + q;
-                x.count = std::min(GROUP_SLICE, G.count - q);
-                sl.push_back(x);
-            }
+        for (const SliceRef& r : slice_list) {
+            dwba_filon_group x = ga[r.g >= 0 ? r.g : 0];
+            x.first = r.g >= 0 ? x.first + r.q : 0;
+            x.count = r.c;
+            sl.push_back(x);
         }
         DWBA_HIP_CHECK(up(o_groups, ga.data(), sizeof(dwba_filon_group) * ga.size()));
         DWBA_HIP_CHECK(up(o_slices, sl.data(), sizeof(dwba_filon_group) * sl.size()));

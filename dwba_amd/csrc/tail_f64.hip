@@ -336,23 +336,48 @@ __device__ __forceinline__ void sweep2(double* acc, double w0, double ix0, doubl
     }
 }
 
-// The same for four nodes, added to each acc[L] in node order (so the sums
-// equal four single-node sweeps); four independent power chains give the
-// scheduler room between each product and the FMA that uses it.
+// The same for four nodes, added to each acc[L] in node order.  The powers
+// x^-(L+1) = x^-(2+c) (x^-4)^a (L = 1 + 4a + c) come in blocks of four: per
+// node four products w x^-(2+c) and one power of x^-4 per block feed four
+// FMAs, about 1.4 instructions per node and multipole instead of an FMA and
+// a multiply each (the products round differently from a running power, at
+// the 1e-16 level of each term).
 template <int LE>
 __device__ __forceinline__ void sweep4(double* acc, double w0, double ix0, double w1, double ix1, double w2,
                                        double ix2, double w3, double ix3) {
-    double p0x = ix0 * ix0, p1x = ix1 * ix1, p2x = ix2 * ix2, p3x = ix3 * ix3;
+    const double ws[4] = {w0, w1, w2, w3}, ys[4] = {ix0, ix1, ix2, ix3};
+    double u[4][4], z[4], za[4];
 #pragma unroll
-    for (int L = 1; L < LE; ++L) {
-        acc[L] = fma(w0, p0x, acc[L]);
-        acc[L] = fma(w1, p1x, acc[L]);
-        acc[L] = fma(w2, p2x, acc[L]);
-        acc[L] = fma(w3, p3x, acc[L]);
-        p0x *= ix0;
-        p1x *= ix1;
-        p2x *= ix2;
-        p3x *= ix3;
+    for (int n = 0; n < 4; ++n) {
+        const double y2 = ys[n] * ys[n];
+        u[n][0] = ws[n] * y2;
+        u[n][1] = u[n][0] * ys[n];
+        u[n][2] = u[n][1] * ys[n];
+        u[n][3] = u[n][2] * ys[n];
+        z[n] = y2 * y2;
+        za[n] = z[n];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int L = 1 + c;
+        if (L < LE) acc[L] = (((acc[L] + u[0][c]) + u[1][c]) + u[2][c]) + u[3][c];
+    }
+#pragma unroll
+    for (int a = 1; 1 + 4 * a < LE; ++a) {
+        if (a > 1) {
+#pragma unroll
+            for (int n = 0; n < 4; ++n) za[n] *= z[n];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int L = 1 + 4 * a + c;
+            if (L < LE) {
+                acc[L] = fma(u[0][c], za[0], acc[L]);
+                acc[L] = fma(u[1][c], za[1], acc[L]);
+                acc[L] = fma(u[2][c], za[2], acc[L]);
+                acc[L] = fma(u[3][c], za[3], acc[L]);
+            }
+        }
     }
 }
 

@@ -322,6 +322,26 @@ def filon_layout(gs: GridStats, n1: int, k_total: float, phase_inc: float, eta_t
     return FilonLayout(PREP_FILON, a, dphi, b, 0, True, k_total, eta_total)
 
 
+def xcd_runs(units, n_xcd: int = 8, pad=None) -> list:
+    """Order work for a kernel that deals its workgroups to the XCDs in
+    consecutive runs (item q of n runs on XCD q // ceil(n / n_xcd)): units are
+    (cost, [items]) that should share one XCD's L2 (one energy's slices);
+    they are spread over n_xcd bins by longest-processing-time (heaviest
+    unit to the lightest bin), each bin keeps its units heaviest first, and
+    the bins are padded with `pad` to one length and concatenated."""
+    bins = [[] for _ in range(n_xcd)]
+    load = [0.0] * n_xcd
+    for cost, items in sorted(units, key=lambda u: -u[0]):
+        b = min(range(n_xcd), key=lambda i: load[i])
+        bins[b].extend(items)
+        load[b] += cost
+    n = max((len(b) for b in bins), default=0)
+    out = []
+    for b in bins:
+        out.extend(b + [pad] * (n - len(b)))
+    return out
+
+
 def filon_groups(plans) -> list:
     """Groups of FILON pairs whose node geometry is the same: one energy
     (same grid), split index N1, phase layout and interpolation kinds.

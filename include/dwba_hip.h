@@ -281,7 +281,11 @@ int32_t dwba_filon_cells_f64(int32_t n_groups, const dwba_filon_group* d_groups,
  * N1, u_i and V_diff.  With dwba_filon_cells_f64 it replaces the density
  * interpolations at the nodes (oscillatory_integrals.py:2578-2644,
  * :2667-2770) and the density preparation of radial_ME_all_L
- * (dwba_matrix_elements.py:1025-1049). */
+ * (dwba_matrix_elements.py:1025-1049).  Slices with count 0 are skipped.
+ * Placement: the slices are cut into 8 consecutive runs of
+ * ceil(n_slices/8) (the first n_slices % 8 runs one longer), and run q goes
+ * to XCD q, in order; listing each energy's slices inside one run keeps its
+ * shared arrays and cell tables in one L2 (results do not depend on it). */
 int32_t dwba_filon_fused_f64(int32_t n_slices, const dwba_filon_group* d_slices, const dwba_prep_job* d_jobs,
                              const int32_t* d_members, void* stream);
 
