@@ -36,6 +36,7 @@ from .planner import GridStats, PREP_PLAIN, PREP_FILON, PREP_STANDARD, PREP_ZERO
 PREP_NO_ZERO_FILL = 0x10   # include/dwba_hip.h: the batch's per-pair vectors are zero-initialised once
 PREP_FILON_GROUPED = 0x20  # include/dwba_hip.h: Filon alpha/beta from the group cell sums instead
 PREP_NODES_SHARED = 0x40   # include/dwba_hip.h: Coulomb phase nodes filled by another job of the launch
+PREP_FINITE_TAIL = 0x80    # include/dwba_hip.h: densities finite beyond N1 (the fused pass may stop there)
 GROUP_SLICE = 4            # members per workgroup of the fused grouped Filon pass (its FUSE_MAX)
 from .dwba_matrix_elements import (PairPlan, RadialDWBAIntegrals, TAIL_NONE, TAIL_OUTER, TAIL_LEGACY, plan_pair,
                                    _fit)
@@ -92,6 +93,11 @@ class EnergyInput:
     bound_f: object
     pairs: list
     U_f: Optional[np.ndarray] = None
+
+
+def _last_nonfinite(a) -> int:
+    bad = np.flatnonzero(~np.isfinite(a))
+    return int(bad[-1]) if bad.size else -1
 
 
 def _round_up(x, m):
@@ -166,6 +172,7 @@ class ScanBatch:
         self.E = []
         self.pairs = []        # (energy index, pair index in energy, PairPlan, inputs)
         self._uploads = {}
+        self._nf = {}
         self._bound_objs = [en.bound_i for en in energies]
         for ei, en in enumerate(energies):
             r = np.ascontiguousarray(en.grid.r, dtype=np.float64)
@@ -180,6 +187,7 @@ class ScanBatch:
             Uf = _fit("U_f", en.U_f, N) if en.U_f is not None else None
             st.vdiff_d = T(V - Ui)
             st.ui = _fit("bound_i.u", en.bound_i.u_of_r, N)
+            st.grid_nf = max(_last_nonfinite(V - Ui), _last_nonfinite(st.ui))
             ext = {}
             for pi, pr in enumerate(en.pairs):
                 ci, cf = pr[0], pr[1]
@@ -190,6 +198,16 @@ class ScanBatch:
                                  k_thr, is_exc, ext)
                 self.pairs.append((ei, pi, plan, ci, cf, bf))
             self.E.append(st)
+
+    def _nonfinite(self, obj, attr, N):
+        """Last index < N where obj.attr is not finite (-1: none), once per object."""
+        key = (id(obj), attr, N)
+        hit = self._nf.get(key)
+        if hit is not None and hit[0] is obj:
+            return hit[1]
+        v = _last_nonfinite(_fit(attr, getattr(obj, attr), N))
+        self._nf[key] = (obj, v)
+        return v
 
     def _dev_array(self, obj, attr, N):
         """Upload a wave / bound density once per object."""
@@ -271,6 +289,10 @@ class ScanBatch:
             lay = plan.layout
             nodes = None
             flags = PREP_NO_ZERO_FILL if plan.mode == PREP_FILON else 0
+            if plan.mode == PREP_FILON and max(st.grid_nf, self._nonfinite(ci, "chi_of_r", st.N),
+                                               self._nonfinite(cf, "chi_of_r", st.N),
+                                               self._nonfinite(bf, "u_of_r", st.N)) < plan.N1:
+                flags |= PREP_FINITE_TAIL
             if plan.mode == PREP_FILON and lay.coulomb:
                 # pairs of one energy with the same stepping share one node
                 # buffer: the first fills it, the others only count their nodes
@@ -625,6 +647,10 @@ class ScanBatch:
             tabs.append(t)
             grids[ei] = LaneGrid(hip.ptr(st.r_d), hip.ptr(t), st.N, 0)
         ent = (hip.struct_array_to_device(grids, self.dev), len(self.E), max(st.N for st in self.E), tabs)
+        # the tables depend on the grids only: filled once here, read by every launch
+        hip.check(self.lib.dwba_semisep_lane_tables_f64(ent[1], hip.ptr(ent[0]), ent[2], self.L_max, lb,
+                                                        hip.stream_handle()), "dwba_semisep_lane_tables_f64")
+        torch.cuda.current_stream().synchronize()
         self._lane_tabs[lb] = ent
         return ent
 
@@ -761,10 +787,7 @@ class ScanBatch:
         jobs = self.plain_d_split if direct_gemv else self.plain_d
         if lanes and self.lane_tables:
             lbv = lb or 8
-            gd, ng, max_n, _ = self._lane_table_set(lbv)
             sj, ptrs, oidx, _ = self._lane_job_set(lbv, direct_gemv)
-            hip.check(lib.dwba_semisep_lane_tables_f64(ng, hip.ptr(gd), max_n, Lm, lbv, sh),
-                      "dwba_semisep_lane_tables_f64")
             split = self.lane_split if split is None else split     # 0: whole ranges, 2 or 3: waves per SIMD
             if split:
                 hip.check(lib.dwba_il_semisep_lanes_split_f64(self.plain_count, hip.ptr(sj), hip.ptr(ptrs),
@@ -920,10 +943,6 @@ class ScanBatch:
             ev_b = torch.cuda.Event()
             ev_b.record(strm)
             s2.wait_event(ev_a)
-            lbv = self.semisep_config[1] or 8
-            gd, ng, max_n, _ = self._lane_table_set(lbv)
-            hip.check(lib.dwba_semisep_lane_tables_f64(ng, hip.ptr(gd), max_n, Lm, lbv, sh2),
-                      "dwba_semisep_lane_tables_f64")
             self._contract_half(0, self.out, sh2)
             s2.wait_event(ev_b)
             self._contract_half(1, self.out, sh2)

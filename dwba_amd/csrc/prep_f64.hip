@@ -419,6 +419,14 @@ __global__ __launch_bounds__(PREP_THREADS) void filon_fused_kernel(const dwba_fi
     const dwba_prep_job& R = jobs[G.rep];
     const int N = R.N, N1 = R.N1;
     const int cnt = G.count < FUSE_MAX ? G.count : FUSE_MAX;
+    // points at and beyond N1 matter only for a member that writes beta_dir
+    // there or whose densities may be non-finite there (w_lim = 0 makes every
+    // L = 0 term an exact zero otherwise)
+    int nend = N1;
+    for (int m = 0; m < cnt; ++m) {
+        const dwba_prep_job& J = jobs[members[G.first + m]];
+        if (J.beta_dir || !(J.mode & DWBA_PREP_FINITE_TAIL)) nend = N;
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t ld = (size_t)G.ld_coef;
     const double* __restrict__ co = G.coef;
@@ -433,7 +441,7 @@ __global__ __launch_bounds__(PREP_THREADS) void filon_fused_kernel(const dwba_fi
     for (int m = 0; m < FUSE_MAX; ++m)
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[m][q] = 0.0;
-    for (int base = 0; base < N; base += PREP_THREADS) {
+    for (int base = 0; base < nend; base += PREP_THREADS) {
         const int j = base + (int)threadIdx.x;
         const bool in = j < N;
         const bool here = j < N1 - 1;                   // cell j exists

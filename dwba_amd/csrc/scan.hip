@@ -192,6 +192,13 @@ struct GridStats {
     }
 };
 
+// Last index < n where a is not finite, -1 if none.
+int last_nonfinite(const double* a, int n) {
+    for (int i = n - 1; i >= 0; --i)
+        if (!std::isfinite(a[i])) return i;
+    return -1;
+}
+
 // First index i with a[i] >= v (numpy searchsorted, side "left").
 int search_left(const double* a, int n, double v) { return (int)(std::lower_bound(a, a + n, v) - a); }
 
@@ -832,7 +839,19 @@ int plan_scan(dwba_scan* s, int32_t n_energies, const dwba_scan_energy* E, int32
         if (wave_energy[w] >= 0) DWBA_HIP_CHECK(up(o_chi[w], W[w].chi, sizeof(double) * E[wave_energy[w]].n));
     for (auto& kv : o_uf) DWBA_HIP_CHECK(up(kv.second, kv.first.second, sizeof(double) * E[kv.first.first].n));
 
-    // ---- prep jobs
+    // ---- prep jobs (FILON pairs whose densities are finite beyond N1 let the
+    // fused pass stop there)
+    std::vector<int> nf_grid(n_energies), nf_wave(n_waves, -1);
+    for (int e = 0; e < n_energies; ++e) {
+        int v = last_nonfinite(E[e].u_i, E[e].n);
+        for (int i = E[e].n - 1; i > v; --i)
+            if (!std::isfinite(E[e].V_core[i] - E[e].U_i[i])) { v = i; break; }
+        nf_grid[e] = v;
+    }
+    for (int w = 0; w < n_waves; ++w)
+        if (wave_energy[w] >= 0) nf_wave[w] = last_nonfinite(W[w].chi, E[wave_energy[w]].n);
+    std::map<std::pair<int, const double*>, int> nf_uf;
+    for (auto& kv : o_uf) nf_uf[kv.first] = last_nonfinite(kv.first.second, E[kv.first.first].n);
     std::vector<dwba_prep_job> pj(n_pairs);
     std::vector<double> coul(any_coul ? 4 * n_pairs : 0, 0.0);
     for (int k = 0; k < n_pairs; ++k) {
@@ -864,6 +883,10 @@ int plan_scan(dwba_scan* s, int32_t n_energies, const dwba_scan_energy* E, int32
         int mode = q.mode;
         if (q.mode == DWBA_PREP_FILON) mode |= DWBA_PREP_NO_ZERO_FILL;
         if (nodes_shared[k]) mode |= DWBA_PREP_NODES_SHARED;
+        if (q.mode == DWBA_PREP_FILON &&
+            std::max(std::max(nf_grid[e], nf_uf[{e, P[k].u_f}]), std::max(nf_wave[P[k].wave_i], nf_wave[P[k].wave_f])) <
+                q.N1)
+            mode |= DWBA_PREP_FINITE_TAIL;
         if (in_group[k]) mode |= DWBA_PREP_FILON_GROUPED;
         J.mode = mode;
         J.log1 = q.log1;

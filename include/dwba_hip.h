@@ -203,6 +203,9 @@ enum {
 #define DWBA_PREP_NODES_SHARED 0x40    /* FILON with Coulomb stepping: phase_nodes is the buffer of another job of
                                           the same launch with the same d_coulomb_par entry and p_step (pairs of one
                                           Filon group); that job fills it, this one only counts its nodes */
+#define DWBA_PREP_FINITE_TAIL 0x80     /* FILON grouped: chi_i, chi_f, u_f, u_i and V_diff are finite on [N1, N), so
+                                          the L = 0 sums gain exact zeros there (w_lim = 0) and dwba_filon_fused_f64
+                                          stops at N1 for a slice whose members all carry it and write no beta_dir */
 #define DWBA_PREP_MODE_MASK 0x0f
 
 /* One channel pair.  Device pointers; outputs are the alpha and beta vectors (N doubles

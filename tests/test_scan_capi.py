@@ -318,3 +318,55 @@ def test_cscan_binding_matches_scanbatch_step(z_ion):
     assert info["pairs"] == ref.shape[0] and info["shared_beta_energies"] == len(scan)
     bad = ~(np.abs(got - ref) <= 1e-12 * np.abs(ref) + 1e-14 * mag)
     assert not bad.any(), (np.argwhere(bad)[:5].tolist(), got[bad][:3], ref[bad][:3])
+
+
+@pytest.mark.gpu
+def test_nonfinite_density_beyond_split_index():
+    """A density that is not finite beyond idx_limit (where w_lim = 0) turns
+    the L = 0 correction sums into NaN, as the reference's dot products do,
+    so the correction is skipped for the pairs that use it; pairs whose
+    densities are finite there may stop their prep at idx_limit.  The C scan
+    and ScanBatch (grouped Filon prep, shared direct beta) agree with the
+    drop-in on both kinds of pairs."""
+    from dwba_amd.batch import ScanBatch, EnergyInput
+    from dwba_amd.cscan import CScan
+    from dwba_amd.dwba_matrix_elements import radial_ME_all_L_gpu
+    points = make_points(kind="excitation", n_energies=1, n_pairs=10, seed=41)
+    p = points[0]
+    ci0 = p.pairs[0][0]
+    bad = np.array(ci0.chi_of_r, dtype=np.float64)
+    bad[len(bad) - 3] = np.inf                       # beyond every pair's split index
+    ci0.chi_of_r = bad
+    L_max = 6
+    scan = [EnergyInput(grid=p.grid, V_core=p.V_core, U_i=p.U_i, bound_i=p.bound_i, bound_f=p.pairs[0][2],
+                        pairs=[(ci, cf) for ci, cf, _ in p.pairs], U_f=p.U_f)]
+    ref = []
+    for ci, cf, bf in p.pairs:
+        d = radial_ME_all_L_gpu(p.grid, p.V_core, p.U_i, p.bound_i, bf, ci, cf, L_max, oscillatory_method="full_split",
+                                U_f_array=p.U_f)
+        ref.append([[d.I_L_direct[L] for L in range(L_max + 1)], [d.I_L_exchange[L] for L in range(L_max + 1)]])
+    ref = np.asarray(ref)
+    b = ScanBatch(scan, L_max, precision="fp64", oscillatory_method="full_split")
+    b.direct_gemv = b.ss_gemv is not None
+    b.semisep_lanes = True
+    b.lane_split = 2
+    b.prep_grouped = True
+    b.run()
+    got_b = b.results()
+    mag = b.magnitudes()
+    cs = CScan(scan, L_max, oscillatory_method="full_split")
+    cs.launch()
+    got_c = cs.results()
+    cs.close()
+    uses_bad = np.array([ci is ci0 for ci, _, _ in p.pairs])
+    assert uses_bad.any() and not uses_bad.all()
+    # the 1s -> 2p overlap is far above 1e-12, so the NaN dot product reaches
+    # I_0 (direct, and exchange where its overlap sum passes the same test) of
+    # exactly the pairs that use the planted wave; every L >= 1 stays finite
+    assert np.array_equal(np.isnan(ref[:, 0, 0]), uses_bad)
+    assert not np.isnan(ref[~uses_bad]).any()
+    assert np.all(np.isfinite(ref[:, :, 1:]))
+    for got in (got_b, got_c):
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        ok = np.isnan(ref) | (np.abs(got - ref) <= 1e-12 * np.abs(ref) + 1e-14 * mag)
+        assert ok.all(), np.argwhere(~ok)[:5].tolist()
