@@ -602,6 +602,34 @@ def bench_batched(args, ionization=False):
     result["fp32_mfma"] = leg32
     del res32
 
+    # block form of the same MFMA contraction: rank-one moments between blocks of
+    # 8 points and each block's own 36 entries on the matrix cores
+    for prec, key in (("fp32", "fp32_mfma_block"), ("bf16x3", "bf16x3_block")):
+        if prec == "bf16x3" and args.no_bf16:
+            continue
+        bb, msb, stb, resb = _scan_leg(scan, L_max, prec, "block", args.steps, args.warmup, graph, stream)
+        pp, fl = bb.block_work()
+        del bb
+        log(f"{prec} MFMA block form: {msb:.1f} ms/step, stages {stb}")
+        leg = {"value": n_int / (msb * 1e-3), "unit": "integrals/s", "ms_per_step": msb, "stage_ms": stb,
+               "rel_err_vs_fp64": _err_stats(resb, res64s, mag)}
+        if stb.get("xgemm"):
+            k_s = stb["xgemm"] * 1e-3
+            peak = PEAK_FP32_MFMA if prec == "fp32" else 2.5e15
+            work = fl if prec == "fp32" else 3 * fl
+            leg["roofline"] = {
+                "bound": "mfma", "achieved": work / k_s / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                "frac": work / k_s / peak,
+                "kernel": ("xblock_f32_kernel (v_mfma_f32_16x16x4_f32)" if prec == "fp32" else
+                           "xblock_bf16x3_kernel (v_mfma_f32_16x16x16_bf16, 3 products)") + " + xblock_join_kernel",
+                "kernel_ms": stb["xgemm"], "work_flop": work, "pair_points": pp,
+                "hbm_gbs_alpha_beta": 16.0 * pp / k_s / 1e9,
+                "note": "MFMA work per (grid point, pair): four point moments (4 x 32 multipoles x 2 flop) and the "
+                        "block's 36 entries / 8 points (32 x 36 x 2 / 8), 544 flop; split-BF16x3: 3 products of it; "
+                        "hbm: FP64 alpha and beta read once"}
+        result[key] = leg
+        del resb
+
     if not args.no_bf16:
         bbf, msbf, stbf, resbf = _scan_leg(scan, L_max, "bf16x3", "dense", args.steps, args.warmup, graph, stream)
         del bbf
@@ -622,8 +650,9 @@ def bench_batched(args, ionization=False):
                                 "max_abs_diff_vs_semiseparable_over_sum_abs_terms":
                                     float(np.max(np.abs(resd - res64s) / np.maximum(mag, 1e-300)))}
         result["speedups_vs_fp64_dense"] = {"fp64_semiseparable": msd / ms64s, "fp32_mfma": msd / ms32}
-        if "bf16x3" in result:
-            result["speedups_vs_fp64_dense"]["bf16x3"] = msd / result["bf16x3"]["ms_per_step"]
+        for key in ("bf16x3", "fp32_mfma_block", "bf16x3_block"):
+            if key in result:
+                result["speedups_vs_fp64_dense"][key] = msd / result[key]["ms_per_step"]
     if cpu:
         result["cpu_baseline"] = cpu
     return result

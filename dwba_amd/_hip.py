@@ -142,6 +142,10 @@ _SIGS = {
     "dwba_xgemm_f32": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp, c_i32, c_vp, c_i32, c_vp]),
     "dwba_xgemm_bf16x3": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp, c_i32, c_vp, c_i32, c_vp]),
     "dwba_split_bf16": (c_i32, [c_vp, c_vp, c_vp, ctypes.c_int64, c_vp]),
+    "dwba_xblock_table_bytes": (c_sz, [c_i32, c_i32]),
+    "dwba_xblock_partial_doubles": (c_sz, [c_i32]),
+    "dwba_xblock_tables": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_vp]),
+    "dwba_xblock_contract": (c_i32, [c_i32, c_i32, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp]),
     "dwba_dgemv_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_vp]),
     "dwba_semisep_gemv_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp]),
     "dwba_il_semisep_lanes_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_vp, c_vp]),

@@ -30,7 +30,7 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from . import planner
+from . import planner, xblock
 from .planner import GridStats, PREP_PLAIN, PREP_FILON, PREP_STANDARD, PREP_ZERO
 
 PREP_NO_ZERO_FILL = 0x10   # include/dwba_hip.h: the batch's per-pair vectors are zero-initialised once
@@ -119,21 +119,32 @@ class _EnergyState:
 class ScanBatch:
     """Plans and runs all I_L of a scan.  precision: "fp64" (every
     contraction in FP64), "fp32" (stacked contractions on FP32 MFMA) or
-    "bf16x3" (split-BF16x3 MFMA with FP32 accumulation).  contraction (FP64
-    only): "semiseparable" (O(N) per L, dwba_il_semisep_f64) or "dense"
-    (O(N^2) per L, dwba_il_plain_f64_batch); the MFMA modes are dense."""
+    "bf16x3" (split-BF16x3 MFMA with FP32 accumulation).  contraction: for
+    FP64 "semiseparable" (O(N) per L, dwba_il_semisep_f64, default) or
+    "dense" (O(N^2) per L, dwba_il_plain_f64_batch); for the MFMA modes
+    "block" (dwba_xblock_contract: rank-one moments between blocks of 8
+    points and each block's own entries on the matrix cores, L_max <= 31,
+    the default there) or "dense" (dwba_xgemm_*: every K_L tile on the
+    matrix cores, the default above L_max = 31).
+    blocks_per_range: blocks of 8 points per workgroup of the block form."""
 
     def __init__(self, energies: Sequence[EnergyInput], L_max: int, precision: str = "fp64",
                  use_oscillatory_quadrature=True, oscillatory_method="advanced", CC_nodes=5,
                  phase_increment=1.5708, min_grid_fraction=0.10, k_threshold=0.5,
-                 contraction: str = "semiseparable"):
+                 contraction: Optional[str] = None, blocks_per_range: int = 64):
         if precision not in ("fp32", "bf16x3", "fp64"):
             raise ValueError(f"precision must be 'fp32', 'bf16x3' or 'fp64', got {precision!r}")
-        if contraction not in ("semiseparable", "dense"):
-            raise ValueError(f"contraction must be 'semiseparable' or 'dense', got {contraction!r}")
-        self.contraction = contraction if precision == "fp64" else "dense"
+        allowed = ("semiseparable", "dense") if precision == "fp64" else ("block", "dense")
+        if contraction is None:             # the block form takes L_max <= 31
+            contraction = allowed[0] if (precision == "fp64" or L_max <= 31) else "dense"
+        if contraction not in allowed:
+            raise ValueError(f"contraction for precision {precision!r} must be one of {allowed}, got {contraction!r}")
+        self.contraction = contraction
         if L_max < 0 or L_max > 63:
             raise ValueError("L_max must be in [0, 63]")
+        if contraction == "block" and L_max > 31:
+            raise ValueError("the block-form MFMA contraction takes L_max <= 31; use contraction='dense'")
+        self.blocks_per_range = int(blocks_per_range)
         import torch
         from . import _hip
         self.torch, self.hip = torch, _hip
@@ -523,6 +534,8 @@ class ScanBatch:
             self.plain_ws = torch.empty(self.plain_ws_bytes, dtype=torch.uint8, device=dev)
         # direct family: shared beta per energy -> dgemv + ddot; else stacked GEMM
         dg, dd, xg_fams = [], [], []
+        dd_counts = []                    # dots per shared-beta energy (consecutive in dd)
+        owners, shared_pairs = set(), set()
         max_rows = 0
         self.t_bufs = []
         for ei, st in enumerate(self.E):
@@ -540,10 +553,14 @@ class ScanBatch:
                     self.t_bufs.append(t)
                     dg.append(DgemvEnergy(hip.ptr(st.r_d), hip.ptr(st.bd[src]), hip.ptr(t), st.N, rows_d, st.ld, Lm))
                     max_rows = max(max_rows, rows_d)
+                    n0 = len(dd)
+                    owners.add(st.order[src])
                     for c, k in enumerate(st.order):
                         p = self.pairs[k][2]
                         if p.mode in (PREP_FILON, PREP_STANDARD):
                             dd.append(DdotPair(hip.ptr(st.ad[c]), hip.ptr(t), p.N1, st.ld, base + c, 0))
+                            shared_pairs.add(k)
+                    dd_counts.append(len(dd) - n0)
             else:
                 sup_d = [p.N if p.mode in (PREP_FILON, PREP_STANDARD) else 0 for p in plans]
                 rows = [p.N1 if p.mode in (PREP_FILON, PREP_STANDARD) else 0 for p in plans]
@@ -559,10 +576,43 @@ class ScanBatch:
         if dd:
             arr = (DdotPair * len(dd))(*dd)
             self.ddot = (hip.struct_array_to_device(arr, dev), len(dd))
+        self.xg_count = 0
+        self.xb_plan = None
+        self.b32 = []
+        if self.contraction == "block":
+            # the FP64 parts of the step as the FP64 path runs them: grouped Filon
+            # prep, beta_dir written once per shared-beta energy, t_L by the O(N)
+            # GEMV and the per-pair dots on the FP64 matrix cores
+            self.prep_grouped = self.filon_groups is not None
+            if dg:
+                self._gemv_nobeta = shared_pairs - owners
+                self.prep_d_gemv = self._prep_jobs(True, False)
+                self.direct_gemv = True
+            if dd and Lm <= 31:
+                items16, first = [], 0
+                for cnt in dd_counts:
+                    for q in range(0, cnt, 16):
+                        items16.append((first + q, min(16, cnt - q)))
+                    first += cnt
+                self.items16 = (torch.as_tensor(np.asarray(items16, dtype=np.int32).reshape(-1, 2), device=dev),
+                                len(items16))
+            # per-grid factor tables (grid only: written once here) and the
+            # groups of 64 pairs of every stacked family
+            tabs, grids_d = xblock.build_tables(hip, lib, torch, dev, [(st.r_d, st.N) for st in self.E],
+                                                self.precision)
+            fams = []
+            for (ei, Amat, Bmat, rows, sup, base, slot) in xg_fams:
+                ext = [max(int(a), int(b)) for a, b in zip(rows, sup)]
+                fams.append((ei, Amat, Bmat, self.E[ei].ld, ext, base, slot))
+            self.xb_plan = xblock.XBlockPlan(hip, lib, torch, dev, fams, tabs, self.precision,
+                                             blocks_per_range=self.blocks_per_range)
+            self.xb_partial = self.xb_plan.partial(lib, torch, dev)
+            self._keep += [tabs, grids_d]
+            torch.cuda.current_stream().synchronize()
+            return
         # stacked GEMM descriptors and work list
         descs, items, costs = [], [], []
         part_off = 0
-        self.b32 = []
         max_cb = 1
         for fi, (ei, Amat, Bmat, rows, sup, base, slot) in enumerate(xg_fams):
             st = self.E[ei]
@@ -974,11 +1024,23 @@ class ScanBatch:
             mark("direct", "start")
             if self.dgemv is not None:
                 d, n, mr = self.dgemv
-                hip.check(lib.dwba_dgemv_f64(n, hip.ptr(d), mr, Lm, sh), "dwba_dgemv_f64")
+                if self.contraction == "block":     # t_L = K_L beta in O(N) per L, FP64
+                    hip.check(lib.dwba_semisep_gemv_f64(n, hip.ptr(d), Lm, sh), "dwba_semisep_gemv_f64")
+                else:
+                    hip.check(lib.dwba_dgemv_f64(n, hip.ptr(d), mr, Lm, sh), "dwba_dgemv_f64")
             if self.ddot is not None:
                 d, n = self.ddot
-                hip.check(lib.dwba_ddot_f64(n, hip.ptr(d), Lm, hip.ptr(self.out), sh), "dwba_ddot_f64")
+                if self.contraction == "block" and getattr(self, "items16", None) is not None:
+                    it16, n16 = self.items16
+                    hip.check(lib.dwba_ddot_mfma_f64(n16, hip.ptr(it16), hip.ptr(d), Lm, hip.ptr(self.out), sh),
+                              "dwba_ddot_mfma_f64")
+                else:
+                    hip.check(lib.dwba_ddot_f64(n, hip.ptr(d), Lm, hip.ptr(self.out), sh), "dwba_ddot_f64")
             mark("direct", "end")
+            if self.xb_plan is not None:
+                mark("xgemm", "start")
+                self.xb_plan.launch(hip, lib, self.precision, Lm, self.xb_partial, self.out, strm)
+                mark("xgemm", "end")
             if self.xg_count:
                 for Bm, Bx in self.b32:
                     if self.precision == "fp32":
@@ -1075,6 +1137,17 @@ class ScanBatch:
                     wd += nl * (2.0 * p.N1 * p.N1 + 2.0 * p.N1)
                     we += nl * (2.0 * p.N1 * p.N1 + 2.0 * p.N1)
         return wd, we
+
+    def block_work(self):
+        """(pair grid points, MFMA flop) of one launch of the block-form MFMA
+        contraction: every group sweeps n_blocks blocks of 8 points for its
+        pairs; per (point, pair) the matrix cores do the four point moments
+        (4 x 32 multipoles x 2 flop) and 1/8 of the block's 36 entries
+        (32 x 36 x 2 flop): 544 flop."""
+        if self.xb_plan is None:
+            return 0, 0.0
+        pp = sum(8 * g.n_blocks * g.n_pairs for g in self.xb_plan._groups_host)
+        return pp, 544.0 * pp
 
     def tune(self, tuner=None, steps: int = 3):
         """Choose how the FP64 step runs, with the autotuner (cached per device

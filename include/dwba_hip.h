@@ -392,6 +392,57 @@ int32_t dwba_xgemm_f32(int32_t n_energies, const dwba_xgemm_energy* d_energies, 
                        const int32_t* d_items, int32_t L_max, double* d_partial, int32_t* d_counter,
                        int32_t max_cb, double* d_out, int32_t n_workgroups, void* stream);
 
+/* Block form of the stacked MFMA contraction (same results as dwba_xgemm_f32
+ * / _bf16x3 up to their rounding, same reference code replaced:
+ * dwba_matrix_elements.py:1281-1289, :2508-2509).  K_L between two different
+ * blocks of 8 grid points is rank one (r_j^L r_i^-(L+1)), so the contraction is
+ * per block: four point moments (32 multipoles x 8 points, MFMA), the block's
+ * own 36 (i <= j) entries (32 x 36, MFMA) and an FP64 scan over blocks.
+ * Operands FP32 (v_mfma_f32_16x16x4_f32) or split-BF16x3
+ * (v_mfma_f32_16x16x16_bf16, hi*hi + hi*lo + lo*hi); FP32 accumulation within
+ * a block, FP64 between blocks.  L_max <= 31. */
+enum {
+    DWBA_XBLOCK_F32 = 0,
+    DWBA_XBLOCK_BF16X3 = 1,
+};
+typedef struct dwba_xblock_grid {
+    const double* r;          /* N radii, strictly increasing, > 0 */
+    void* tab;                /* dwba_xblock_table_bytes(N, mode) bytes, written by dwba_xblock_tables */
+    int32_t N;
+    int32_t pad_;
+} dwba_xblock_grid;
+size_t dwba_xblock_table_bytes(int32_t N, int32_t mode);
+/* The per-block factor tables of every grid (depend on the grid only). */
+int32_t dwba_xblock_tables(int32_t n_grids, const dwba_xblock_grid* d_grids, int32_t max_n, int32_t mode,
+                           void* stream);
+/* Up to 64 consecutive pairs of one stacked matrix: alpha of pair c at
+ * A + c*ld, beta at B + c*ld, both zero beyond their supports;
+ * n_blocks = ceil(extent / 8) with extent >= every alpha and beta support of
+ * the group and 8 n_blocks <= ld; the blocks are cut into n_ranges ranges
+ * (one workgroup each) whose partials sit at range0 .. range0 + n_ranges - 1
+ * of d_partial.  Writes d_out[(out_row0 + c)*2*(L_max+1) + slot*(L_max+1) + L]. */
+typedef struct dwba_xblock_group {
+    const double* A;
+    const double* B;
+    const void* tab;          /* the grid's table */
+    int32_t ld;
+    int32_t n_pairs;          /* 1..64 */
+    int32_t n_blocks;
+    int32_t n_ranges;
+    int32_t range0;
+    int32_t out_row0;
+    int32_t slot;             /* 0: direct, 1: exchange */
+    int32_t pad_;
+} dwba_xblock_group;
+/* d_partial: dwba_xblock_partial_doubles(total ranges) doubles of scratch. */
+size_t dwba_xblock_partial_doubles(int32_t n_ranges_total);
+/* d_items: n_items int32 pairs (group, range) in dispatch order (workgroup i
+ * runs item i; a group index < 0 is an empty item); every range of every
+ * group appears once. */
+int32_t dwba_xblock_contract(int32_t mode, int32_t n_groups, const dwba_xblock_group* d_groups, int32_t n_items,
+                             const int32_t* d_items, int32_t L_max, double* d_partial, double* d_out,
+                             void* stream);
+
 /* Direct family (FP64 VALU): t[L*ld + i] = sum_j K_L(i, j) beta[j] for
  * i < rows, j < N, L = 0..L_max, for each energy (one shared beta per
  * energy).  max_rows = max over energies of rows. */

@@ -43,7 +43,7 @@ def test_struct_layouts_match_header(tmp_path):
     """ctypes mirrors of the C structs have the header's size and field offsets."""
     import shutil
     import subprocess
-    from dwba_amd import batch
+    from dwba_amd import batch, xblock
     gcc = shutil.which("gcc")
     if gcc is None:
         import pytest
@@ -56,7 +56,8 @@ def test_struct_layouts_match_header(tmp_path):
         "dwba_amp_pair": _hip.AmpPair, "dwba_cw_wave": _hip.CwWave, "dwba_cw_finish": _hip.CwFinish,
         "dwba_cw_bypass": _hip.CwBypass, "dwba_scan_energy": _hip.ScanEnergy, "dwba_scan_wave": _hip.ScanWave,
         "dwba_scan_pair": _hip.ScanPair, "dwba_scan_options": _hip.ScanOptions,
-        "dwba_scan_pair_plan": _hip.ScanPairPlan,
+        "dwba_scan_pair_plan": _hip.ScanPairPlan, "dwba_xblock_grid": xblock.XBlockGrid,
+        "dwba_xblock_group": xblock.XBlockGroup,
     }
     src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, cls in structs.items():

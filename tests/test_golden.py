@@ -153,14 +153,18 @@ def test_dropin_matches_reference_variants(v):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision,rtol,atol_rel", [("fp64", 1e-10, 1e-13), ("fp32", 0.0, 1e-6),
-                                                     ("bf16x3", 0.0, 1e-6)])
-def test_batch_matches_reference(precision, rtol, atol_rel):
+@pytest.mark.parametrize("precision,contraction,rtol,atol_rel", [("fp64", None, 1e-10, 1e-13),
+                                                                 ("fp32", "block", 0.0, 1e-6),
+                                                                 ("fp32", "dense", 0.0, 1e-6),
+                                                                 ("bf16x3", "block", 0.0, 3e-6),
+                                                                 ("bf16x3", "dense", 0.0, 1e-6)])
+def test_batch_matches_reference(precision, contraction, rtol, atol_rel):
     from dwba_amd.batch import ScanBatch, EnergyInput
-    print(f"\n[golden] batch precision={precision}", flush=True)
+    print(f"\n[golden] batch precision={precision} contraction={contraction}", flush=True)
     en = EnergyInput(grid=GRID, V_core=C1["V_core"], U_i=C1["U_i"], bound_i=BI, bound_f=BF,
                      pairs=[(WI[li], WF[lf]) for (li, lf) in PAIRS], U_f=C1["U_f"])
-    b = ScanBatch([en], LMAX, precision=precision, oscillatory_method="full_split", CC_nodes=5)
+    b = ScanBatch([en], LMAX, precision=precision, oscillatory_method="full_split", CC_nodes=5,
+                  contraction=contraction)
     b.run()
     res = b.results()
     for q in range(len(PAIRS)):
