@@ -620,8 +620,8 @@ def bench_batched(args, ionization=False):
             leg["roofline"] = {
                 "bound": "mfma", "achieved": work / k_s / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                 "frac": work / k_s / peak,
-                "kernel": ("xblock_f32_kernel (v_mfma_f32_16x16x4_f32)" if prec == "fp32" else
-                           "xblock_bf16x3_kernel (v_mfma_f32_16x16x16_bf16, 3 products)") + " + xblock_join_kernel",
+                "kernel": ("xblock_kernel<false> (v_mfma_f32_16x16x4_f32)" if prec == "fp32" else
+                           "xblock_kernel<true> (v_mfma_f32_16x16x16_bf16, 3 products)") + " + xblock_join_kernel",
                 "kernel_ms": stb["xgemm"], "work_flop": work, "pair_points": pp,
                 "hbm_gbs_alpha_beta": 16.0 * pp / k_s / 1e9,
                 "note": "MFMA work per (grid point, pair): four point moments (4 x 32 multipoles x 2 flop) and the "
