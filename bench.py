@@ -624,9 +624,15 @@ def bench_batched(args, ionization=False):
                            "xblock_kernel<true> (v_mfma_f32_16x16x16_bf16, 3 products)") + " + xblock_join_kernel",
                 "kernel_ms": stb["xgemm"], "work_flop": work, "pair_points": pp,
                 "hbm_gbs_alpha_beta": 16.0 * pp / k_s / 1e9,
+                "traffic": None, "traffic_provenance": None,
                 "note": "MFMA work per (grid point, pair): four point moments (4 x 32 multipoles x 2 flop) and the "
                         "block's 36 entries / 8 points (32 x 36 x 2 / 8), 544 flop; split-BF16x3: 3 products of it; "
                         "hbm: FP64 alpha and beta read once"}
+        if "roofline" in leg:
+            kn = "xblock_kernel<false>" if prec == "fp32" else "xblock_kernel<true>"
+            tr, prov = _committed_traffic([kn, "xblock_join_kernel"], "batched_64x400" if not ionization else
+                                          "ionization_48x400")
+            leg["roofline"]["traffic"], leg["roofline"]["traffic_provenance"] = tr, prov
         result[key] = leg
         del resb
 

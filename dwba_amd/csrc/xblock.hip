@@ -31,7 +31,8 @@
 // the rotation (diagonal, distance 1, 2, 3 twice each, distance 4 once).
 //
 // Work: a workgroup = 4 wavefronts = 64 consecutive pairs of one stacked
-// matrix over one contiguous range of blocks; ranges start from zero scans
+// matrix over one range of blocks (the same block boundaries for every group
+// of a grid); ranges start from zero scans
 // and also accumulate Ca = sum_b Na_b P_b, Cb = sum_b Nb_b P_b,
 // P_b = (f_start / f_b)^L, so that a join kernel adds the true incoming scans
 // range by range (the scans are linear).
@@ -393,8 +394,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x
     const int pc = lane & 15, g = lane >> 4;
     const int p = 16 * w + pc;                     // pair within the group
     const bool active = 16 * w < G.n_pairs;        // this wavefront has pairs
-    const int b0 = (int)((long)G.n_blocks * it.y / G.n_ranges);
-    const int b1 = (int)((long)G.n_blocks * (it.y + 1) / G.n_ranges);
+    // range r is blocks [r bpr, (r + 1) bpr) of the grid for every group, so the
+    // groups of one grid read the same table blocks in the same ranges
+    const int b0 = min(G.n_blocks, it.y * G.blocks_per_range);
+    const int b1 = min(G.n_blocks, b0 + G.blocks_per_range);
     const Tab* __restrict__ tab = reinterpret_cast<const Tab*>(G.tab);
     XbScan X;
     xb_scan_init(X);

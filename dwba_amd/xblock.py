@@ -3,12 +3,14 @@ include/dwba_hip.h): per-grid factor tables, groups of up to 64 consecutive
 pairs of a stacked alpha / beta matrix, their block ranges and the dispatch
 order of the ranges.
 
-The kernel cuts each grid into blocks of 8 points; a group's block range is
-split into ranges of about `blocks_per_range` blocks, one workgroup each, and
-a join adds the ranges in grid order.  Ranges of one grid are dispatched to
-one XCD (workgroup i runs on XCD i mod 8), so that grid's factor table stays
-in one L2: grids are assigned to the eight dispatch queues longest first
-(LPT on blocks x pairs) and the queues interleaved.
+The kernel cuts each grid into blocks of 8 points; a group's blocks are
+split into ranges of `blocks_per_range` blocks at the same boundaries for
+every group of a grid, one workgroup each, and a join adds the ranges in grid
+order.  Ranges of one grid are dispatched to one XCD (workgroup i runs on XCD
+i mod 8), range by range with the grid's groups side by side, so the groups
+read each block of the grid's factor table from that XCD's L2 at about the
+same time: grids are assigned to the eight dispatch queues longest first (LPT
+on blocks x pairs) and the queues interleaved.
 """
 from __future__ import annotations
 
@@ -33,7 +35,7 @@ class XBlockGroup(ctypes.Structure):
     _fields_ = [("A", ctypes.c_void_p), ("B", ctypes.c_void_p), ("tab", ctypes.c_void_p), ("ld", ctypes.c_int32),
                 ("n_pairs", ctypes.c_int32), ("n_blocks", ctypes.c_int32), ("n_ranges", ctypes.c_int32),
                 ("range0", ctypes.c_int32), ("out_row0", ctypes.c_int32), ("slot", ctypes.c_int32),
-                ("pad_", ctypes.c_int32)]
+                ("blocks_per_range", ctypes.c_int32)]
 
 
 def build_tables(hip, lib, torch, dev, grids: Sequence, mode: str, stream=None) -> List:
@@ -85,7 +87,8 @@ class XBlockPlan:
                 nr = max(1, -(-nb // self.blocks_per_range))
                 es = A.element_size()
                 groups.append(XBlockGroup(hip.ptr(A) + c0 * ld * es, hip.ptr(B) + c0 * ld * es, hip.ptr(tabs[gi]),
-                                          int(ld), n, nb, nr, range0, int(row0) + c0, int(slot), 0))
+                                          int(ld), n, nb, nr, range0, int(row0) + c0, int(slot),
+                                          self.blocks_per_range))
                 grid_of_group.append(gi)
                 cost_of_grid[gi] = cost_of_grid.get(gi, 0) + nb * ((n + 15) // 16)
                 range0 += nr
@@ -97,9 +100,9 @@ class XBlockPlan:
         for gi in sorted(cost_of_grid, key=lambda g: -cost_of_grid[g]):
             x = int(np.argmin(load))
             load[x] += cost_of_grid[gi]
-            for k, G in enumerate(groups):
-                if grid_of_group[k] == gi:
-                    queues[x] += [(k, rg) for rg in range(G.n_ranges)]
+            mine = [k for k in range(len(groups)) if grid_of_group[k] == gi]
+            for rg in range(max(groups[k].n_ranges for k in mine)):
+                queues[x] += [(k, rg) for k in mine if rg < groups[k].n_ranges]
         depth = max((len(q) for q in queues), default=0)
         items = np.full((depth * N_XCD, 2), -1, dtype=np.int32)
         items[:, 1] = 0

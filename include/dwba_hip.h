@@ -418,9 +418,10 @@ int32_t dwba_xblock_tables(int32_t n_grids, const dwba_xblock_grid* d_grids, int
 /* Up to 64 consecutive pairs of one stacked matrix: alpha of pair c at
  * A + c*ld, beta at B + c*ld, both zero beyond their supports;
  * n_blocks = ceil(extent / 8) with extent >= every alpha and beta support of
- * the group and 8 n_blocks <= ld; the blocks are cut into n_ranges ranges
- * (one workgroup each) whose partials sit at range0 .. range0 + n_ranges - 1
- * of d_partial.  Writes d_out[(out_row0 + c)*2*(L_max+1) + slot*(L_max+1) + L]. */
+ * the group and 8 n_blocks <= ld; range r (one workgroup) is blocks
+ * [r blocks_per_range, (r + 1) blocks_per_range) of them, n_ranges =
+ * ceil(n_blocks / blocks_per_range), and its partials sit at range0 + r of
+ * d_partial.  Writes d_out[(out_row0 + c)*2*(L_max+1) + slot*(L_max+1) + L]. */
 typedef struct dwba_xblock_group {
     const double* A;
     const double* B;
@@ -432,7 +433,7 @@ typedef struct dwba_xblock_group {
     int32_t range0;
     int32_t out_row0;
     int32_t slot;             /* 0: direct, 1: exchange */
-    int32_t pad_;
+    int32_t blocks_per_range;
 } dwba_xblock_group;
 /* d_partial: dwba_xblock_partial_doubles(total ranges) doubles of scratch. */
 size_t dwba_xblock_partial_doubles(int32_t n_ranges_total);

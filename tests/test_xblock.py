@@ -52,7 +52,7 @@ def block_form(r, alpha, beta, L_max, blocks_per_range, drop_upper=False):
     n_ranges = max(1, -(-nb // blocks_per_range))
     parts = []
     for rg in range(n_ranges):
-        b0, b1 = nb * rg // n_ranges, nb * (rg + 1) // n_ranges
+        b0, b1 = min(nb, rg * blocks_per_range), min(nb, (rg + 1) * blocks_per_range)
         S = np.zeros(L_max + 1); T = np.zeros(L_max + 1); tot = np.zeros(L_max + 1)
         Ca = np.zeros(L_max + 1); Cb = np.zeros(L_max + 1); P = np.ones(L_max + 1)
         for b in range(b0, b1):
