@@ -64,8 +64,9 @@ struct XbTabF32 {
     float kd[XB_DS][2][64];      // K_L of entry (s, lane / 16), L = 16 t + lane % 16
     float u[2][2][64];           // (r_k / g_b)^L,  k = 4 s + lane / 16
     float v[2][2][64];           // (f_b / r_k)^(L+1) / f_b
+    float pad_[256];             // to 2 x 256 chunks of 16 bytes: every thread stages two
 };
-static_assert(sizeof(XbTabF32) == 7168, "table layout");
+static_assert(sizeof(XbTabF32) == 8192, "table layout");
 
 // Per-block table, split-BF16 operands for v_mfma_f32_16x16x16_bf16: lane l
 // holds A[l % 16][4 (l / 16) + e], e = 0..3, as 4 bf16.  The 9 entry steps
@@ -79,8 +80,9 @@ struct XbTabBF16 {
     v4s kd_lo[3][2][64];
     v4s u_hi[2][64], u_lo[2][64];
     v4s v_hi[2][64], v_lo[2][64];
+    double pad_[192];            // to 3 x 256 chunks of 16 bytes
 };
-static_assert(sizeof(XbTabBF16) == 512 + 2 * 3 * 2 * 64 * 8 + 4 * 2 * 64 * 8, "table layout");
+static_assert(sizeof(XbTabBF16) == 12288, "table layout");
 
 __device__ __forceinline__ unsigned short bf16_bits(float x) {
     return __builtin_bit_cast(unsigned short, (__bf16)x);
@@ -230,36 +232,43 @@ struct XbBuf {
 };
 
 // One block's loads in flight in registers, issued by all 256 threads of the
-// workgroup as 16-byte loads: NT chunks of the table and 2 of alpha / beta
-// (64 pairs x 8 chunks).  Pairs beyond the group load pair 0 (never used).
+// workgroup as 16-byte loads: NT chunks of the table (padded to whole
+// multiples of 256 chunks) and 2 of alpha / beta (64 pairs x 8 chunks).  The
+// thread's source pointers are formed once (XbSrc); a block only adds its
+// offset.  Pairs beyond the group load pair 0 (never used).
+typedef const v4f __attribute__((address_space(1)))* gv4f;
 template <typename Tab>
-struct XbStage {
+struct XbSrc {
     static constexpr int NCH = (int)(sizeof(Tab) / 16);
-    static constexpr int NT = (NCH + 255) / 256;
-    v4f t[NT], ab[2];
-    __device__ __forceinline__ void load(const Tab* __restrict__ tab, const dwba_xblock_group& G, int b, int tid) {
-        const v4f* __restrict__ s4 = reinterpret_cast<const v4f*>(tab + b);
-#pragma unroll
-        for (int q = 0; q < NT; ++q) {
-            const int i = tid + 256 * q;
-            if (i < NCH) t[q] = s4[i];
-        }
+    static constexpr int NT = NCH / 256;
+    static_assert(NCH % 256 == 0, "table padded to whole rounds of 256 chunks");
+    gv4f tab;                    // chunk tid of block 0
+    gv4f ab[2];                  // chunk of block 0 of slot tid, tid + 256
+    __device__ __forceinline__ XbSrc(const Tab* t, const dwba_xblock_group& G, int tid) {
+        tab = (gv4f)(reinterpret_cast<const v4f*>(t) + tid);
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const int sl = tid + 256 * q;
             const int p = sl >> 3, cc = (sl & 7) ^ (p & 7);
             const int pp = p < G.n_pairs ? p : 0;
-            const double* __restrict__ src = ((cc >> 2) ? G.B : G.A) + (size_t)pp * G.ld + XBP * b + 2 * (cc & 3);
-            ab[q] = *reinterpret_cast<const v4f*>(src);
+            ab[q] = (gv4f)reinterpret_cast<const v4f*>(((cc >> 2) ? G.B : G.A) + (size_t)pp * G.ld + 2 * (cc & 3));
         }
+    }
+};
+template <typename Tab>
+struct XbStage {
+    static constexpr int NT = XbSrc<Tab>::NT;
+    v4f t[NT], ab[2];
+    __device__ __forceinline__ void load(const XbSrc<Tab>& S, int b) {
+#pragma unroll
+        for (int q = 0; q < NT; ++q) t[q] = S.tab[(size_t)b * XbSrc<Tab>::NCH + 256 * q];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) ab[q] = S.ab[q][(size_t)b * (XBP / 2)];   // 8 doubles = 4 chunks per block
     }
     __device__ __forceinline__ void store(XbBuf<Tab>& buf, int tid) const {
         v4f* d4 = reinterpret_cast<v4f*>(&buf.tab);
 #pragma unroll
-        for (int q = 0; q < NT; ++q) {
-            const int i = tid + 256 * q;
-            if (i < NCH) d4[i] = t[q];
-        }
+        for (int q = 0; q < NT; ++q) d4[tid + 256 * q] = t[q];
         v4f* a4 = reinterpret_cast<v4f*>(buf.ab);
         a4[tid] = ab[0];
         a4[tid + 256] = ab[1];
@@ -401,26 +410,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x
     const Tab* __restrict__ tab = reinterpret_cast<const Tab*>(G.tab);
     XbScan X;
     xb_scan_init(X);
+    const XbSrc<Tab> src(tab, G, tid);
     XbStage<Tab> sA, sB;
     auto block = [&](int buf) {
         if constexpr (BF16) xb_block_bf16(s_buf[buf], lane, g, p, X);
         else xb_block_f32(s_buf[buf], lane, g, p, X);
     };
     if (b0 < b1) {
-        sA.load(tab, G, b0, tid);
-        if (b0 + 1 < b1) sB.load(tab, G, b0 + 1, tid);
+        sA.load(src, b0);
+        if (b0 + 1 < b1) sB.load(src, b0 + 1);
         sA.store(s_buf[0], tid);
     }
     __syncthreads();
     for (int b = b0; b < b1; b += 2) {
         // block b in buffer 0; block b + 1 in flight in sB
-        if (b + 2 < b1) sA.load(tab, G, b + 2, tid);
+        if (b + 2 < b1) sA.load(src, b + 2);
         if (active) block(0);
         if (b + 1 < b1) sB.store(s_buf[1], tid);  // buffer 1 was last read before the previous barrier
         __syncthreads();
         if (b + 1 >= b1) break;
         // block b + 1 in buffer 1; block b + 2 in flight in sA
-        if (b + 3 < b1) sB.load(tab, G, b + 3, tid);
+        if (b + 3 < b1) sB.load(src, b + 3);
         if (active) block(1);
         if (b + 2 < b1) sA.store(s_buf[0], tid);
         __syncthreads();
