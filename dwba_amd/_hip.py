@@ -158,6 +158,8 @@ _SIGS = {
     "dwba_semisep_lane_tables_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_i32, c_vp]),
     "dwba_il_semisep_lanes_tab_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp]),
     "dwba_il_semisep_lanes_split_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
+    "dwba_semisep_lane_stage_group": (c_i32, [c_i32]),
+    "dwba_il_semisep_lanes_stage_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]),
     "dwba_batch_finalize": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
     "dwba_il_backward_f64": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
     "dwba_il_backward_f32": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
@@ -369,6 +371,73 @@ def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None, lb: int = 
     check(lib.dwba_il_semisep_f64(n, ptr(d_jobs), nr, nc, L_max, seg, lb, ptr(out), ptr(ws), ws_bytes,
                                   stream_handle(stream)), "dwba_il_semisep_f64")
     out._dwba_keep = (ws, d_jobs)
+    return out
+
+
+def il_semisep_stage_f64_batch(jobs, L_max: int, stream=None, pad: bool = True):
+    """dwba_il_semisep_lanes_stage_f64 on the job tuples of il_semisep_f64_batch
+    (16 <= L_max <= 63).  Jobs on the same grid tensor share one grid table;
+    pad: each grid's jobs (in reverse order) are padded to whole groups of
+    dwba_semisep_lane_stage_group(L_max) with empty jobs that d_out_index
+    marks unwritten.  pad=False passes the jobs as given (no padding), so a
+    group whose streams are on different grids reads back NaN.  Returns a
+    (n_jobs, 2, L_max+1) float64 CUDA tensor."""
+    import numpy as np
+    from .batch import LaneGrid
+    ensure_init()
+    lib = load_library()
+    n = len(jobs)
+    if n == 0:
+        raise ValueError("il_semisep_stage_f64_batch: no jobs")
+    G = int(lib.dwba_semisep_lane_stage_group(L_max))
+    if G <= 0:
+        raise ValueError(f"il_semisep_stage_f64_batch: L_max {L_max} outside [16, 63]")
+    dev = jobs[0][0].device
+    arr = []
+    for k, (r, ad, bd, ae, be, n_rows, n_cols) in enumerate(jobs):
+        for nm, t, m in (("r", r, n_cols), ("alpha_dir", ad, n_rows), ("beta_dir", bd, n_cols),
+                         ("alpha_ex", ae, n_rows), ("beta_ex", be, n_cols)):
+            _require_f64_cuda(f"job {k} {nm}", t, m)
+        arr.append(PlainJob(ptr(r), ptr(ad), ptr(bd), ptr(ae), ptr(be), int(n_rows), int(n_cols)))
+    stride = lib.dwba_semisep_lane_table_stride(L_max, 8)
+    # one table per grid tensor, covering the longest job on it
+    groups = {}
+    for k in range(n):
+        groups.setdefault(jobs[k][0].data_ptr(), []).append(k)
+    tabs, gl, tab_of = [], [], {}
+    for g, ks in groups.items():
+        npts = max(int(jobs[k][6]) for k in ks)
+        t = torch.empty(max(npts, 1) * stride, dtype=torch.float64, device=dev)
+        tabs.append(t)
+        tab_of[g] = ptr(t)
+        if npts > 0:
+            gl.append(LaneGrid(ptr(jobs[ks[0]][0]), ptr(t), npts, 0))
+    if gl:
+        d_grids = struct_array_to_device((LaneGrid * len(gl))(*gl), dev)
+        check(lib.dwba_semisep_lane_tables_f64(len(gl), ptr(d_grids), max(g.n for g in gl), L_max, 8,
+                                               stream_handle(stream)), "dwba_semisep_lane_tables_f64")
+    else:
+        d_grids = None
+    order, tp = [], []
+    if pad:
+        for g, ks in groups.items():
+            body = ks[::-1] + [-1] * ((-len(ks)) % G)
+            order += body
+            tp += [tab_of[g]] * len(body)
+    else:
+        order = list(range(n))
+        tp = [tab_of[jobs[k][0].data_ptr()] for k in order]
+    sj = (PlainJob * len(order))()
+    for p, k in enumerate(order):
+        sj[p] = arr[k] if k >= 0 else PlainJob(arr[0].r, arr[0].alpha_dir, arr[0].beta_dir, arr[0].alpha_ex,
+                                               arr[0].beta_ex, 0, 0)
+    d_sj = struct_array_to_device(sj, dev)
+    d_tabs = torch.as_tensor(np.asarray(tp, dtype=np.int64), device=dev)
+    d_oidx = torch.as_tensor(np.asarray(order, dtype=np.int32), device=dev)
+    out = torch.full((n, 2, L_max + 1), float("nan"), dtype=torch.float64, device=dev)
+    check(lib.dwba_il_semisep_lanes_stage_f64(len(order), ptr(d_sj), ptr(d_tabs), ptr(d_oidx), L_max, ptr(out),
+                                              stream_handle(stream)), "dwba_il_semisep_lanes_stage_f64")
+    out._dwba_keep = (d_sj, d_tabs, d_oidx, tabs, d_grids)
     return out
 
 

@@ -166,16 +166,18 @@ SEMISEP_SEGS = (0, 16, 32, 64, 128, 256)
 SEMISEP_LBS = (4, 8)
 
 
-def semisep_candidates() -> List[Dict[str, int]]:
+def semisep_candidates(L_max: int = None) -> List[Dict[str, int]]:
     """(seg, lb) of the wavefront-scan kernel, plus the lane-serial kernel
     (one lane per job, family and block of lb multipoles; seg unused), whole
     range per lane or split between two wavefronts (split = the occupancy,
-    2 or 3 waves per SIMD, that kernel is compiled for); order 1: the lane
-    kernel's jobs grouped by energy instead of sorted by length alone."""
+    2 or 3 waves per SIMD, that kernel is compiled for; 4 = the LDS-staged
+    split kernel, for L_max >= 16); order 1: the lane kernel's jobs grouped
+    by energy instead of sorted by length alone."""
+    stage = [{"seg": 0, "lb": 8, "lanes": 1, "split": 4, "order": 1}] if L_max is None or 16 <= L_max <= 63 else []
     return [{"seg": s, "lb": b} for b in SEMISEP_LBS for s in SEMISEP_SEGS] + [
         {"seg": 0, "lb": b, "lanes": 1} for b in SEMISEP_LBS] + [
         {"seg": 0, "lb": b, "lanes": 1, "split": w} for b in SEMISEP_LBS for w in (2, 3)] + [
-        {"seg": 0, "lb": 8, "lanes": 1, "split": w, "order": 1} for w in (0, 2)]
+        {"seg": 0, "lb": 8, "lanes": 1, "split": w, "order": 1} for w in (0, 2)] + stage
 
 
 def device_id(dev) -> str:

@@ -503,6 +503,7 @@ class ScanBatch:
             self.lane_tables = True           # that kernel reads per-grid 1/r, q, q^(lb b) tables
             self.lane_order = "length"        # job order of that kernel: "length" or "energy" (tune())
             self.lane_split = 0               # and splits each range between two wavefronts (tune()): 0 no,
+                                              # 4 the LDS-staged split kernel (energy order, padded groups),
                                               # 2 or 3 the occupancy (waves per SIMD) the split kernel targets
             self.plain_ws_bytes = self._semisep_ws_bytes(0)
         else:
@@ -789,15 +790,17 @@ class ScanBatch:
         itA = sum(1 for e in getattr(self, "_item16_energy", []) if e < eA)
         return {"eA": eA, "kA": kA, "dgA": dgA, "itA": itA}
 
-    def _lane_job_set(self, lb, split, half=None):
+    def _lane_job_set(self, lb, split, half=None, group=0):
         """Jobs of the table-driven lane kernel sorted by length, longest
         first (wavefronts of equal-length pairs, long ones dispatched first),
         with their table pointers and output rows; half 0 / 1: only the jobs
         of that half of the energies (pipelined step).  lane_order "energy":
         pairs of one energy kept together (energies with the longest pairs
         first, longest first within each), so a wavefront's grid-table reads
-        hit one table."""
-        key = (lb, bool(split), half, self.lane_order)
+        hit one table.  group > 0 (the LDS-staged kernel): energy order with
+        each energy's jobs padded to whole groups of that many by empty jobs
+        whose output index is -1, so a group never spans two grid tables."""
+        key = (lb, bool(split), half, "energy" if group else self.lane_order, group)
         hit = self._lane_sets.get(key)
         if hit is not None:
             return hit
@@ -808,7 +811,7 @@ class ScanBatch:
         if half is not None:
             eA = self._pipe["eA"]
             jobs = [g for g in jobs if (self._job_energy[g] >= eA) == (half == 1)]
-        if self.lane_order == "energy":
+        if self.lane_order == "energy" or group:
             emax = {}
             for g in jobs:
                 e = self._job_energy[g]
@@ -816,13 +819,27 @@ class ScanBatch:
             perm = sorted(jobs, key=lambda g: (-emax[self._job_energy[g]], self._job_energy[g], -int(src[g].n_cols)))
         else:
             perm = sorted(jobs, key=lambda g: -int(src[g].n_cols))
+        energy = [self._job_energy[g] for g in perm]
+        if group and perm:
+            padded, pe = [], []
+            for p, g in enumerate(perm):
+                padded.append(g)
+                pe.append(energy[p])
+                if p + 1 == len(perm) or energy[p + 1] != energy[p]:
+                    while len(padded) % group:
+                        padded.append(-1)
+                        pe.append(energy[p])
+            perm, energy = padded, pe
         n = len(perm)
-        arr = (hip.PlainJob * n)()
+        arr = (hip.PlainJob * max(n, 1))()
         for p, g in enumerate(perm):
-            arr[p] = src[g]
+            if g >= 0:
+                arr[p] = src[g]
+            else:
+                j = src[next(q for q in perm if q >= 0)]
+                arr[p] = hip.PlainJob(j.r, j.alpha_dir, j.beta_dir, j.alpha_ex, j.beta_ex, 0, 0)
         jobs_d = hip.struct_array_to_device(arr, self.dev)
-        ptrs = torch.as_tensor(np.asarray([hip.ptr(tabs[self._job_energy[g]]) for g in perm], dtype=np.int64),
-                               device=self.dev)
+        ptrs = torch.as_tensor(np.asarray([hip.ptr(tabs[e]) for e in energy], dtype=np.int64), device=self.dev)
         oidx = torch.as_tensor(np.asarray(perm, dtype=np.int32), device=self.dev)
         ent = (jobs_d, ptrs, oidx, n)
         self._lane_sets[key] = ent
@@ -837,8 +854,13 @@ class ScanBatch:
         jobs = self.plain_d_split if direct_gemv else self.plain_d
         if lanes and self.lane_tables:
             lbv = lb or 8
+            split = self.lane_split if split is None else split     # 0: whole ranges, 2 or 3: waves per SIMD,
+            if split == 4:                                          # 4: LDS-staged
+                sj, ptrs, oidx, n = self._lane_job_set(8, direct_gemv, group=self._stage_group())
+                hip.check(lib.dwba_il_semisep_lanes_stage_f64(n, hip.ptr(sj), hip.ptr(ptrs), hip.ptr(oidx), Lm,
+                                                              hip.ptr(out), sh), "dwba_il_semisep_lanes_stage_f64")
+                return self._contract_direct(direct_gemv, out, sh)
             sj, ptrs, oidx, _ = self._lane_job_set(lbv, direct_gemv)
-            split = self.lane_split if split is None else split     # 0: whole ranges, 2 or 3: waves per SIMD
             if split:
                 hip.check(lib.dwba_il_semisep_lanes_split_f64(self.plain_count, hip.ptr(sj), hip.ptr(ptrs),
                                                               hip.ptr(oidx), Lm, lbv, 3 if int(split) == 3 else 2,
@@ -853,6 +875,16 @@ class ScanBatch:
         else:
             hip.check(lib.dwba_il_semisep_f64(self.plain_count, hip.ptr(jobs), nra, nca, Lm, seg, lb, hip.ptr(out),
                                               hip.ptr(ws), ws_bytes, sh), "dwba_il_semisep_f64")
+        self._contract_direct(direct_gemv, out, sh)
+
+    def _stage_group(self):
+        """Jobs per group of the LDS-staged lane kernel (its wavefront pair's
+        pairs), 0 where it does not apply (L_max below 16)."""
+        return max(0, int(self.lib.dwba_semisep_lane_stage_group(self.L_max)))
+
+    def _contract_direct(self, direct_gemv, out, sh):
+        """The shared-beta direct family: t_L = K_L beta per energy, then the dots."""
+        hip, lib, Lm = self.hip, self.lib, self.L_max
         if direct_gemv:
             gd, ng, dd, nd, items_d, ni = self.ss_gemv
             hip.check(lib.dwba_semisep_gemv_f64(ng, hip.ptr(gd), Lm, sh), "dwba_semisep_gemv_f64")
@@ -893,6 +925,12 @@ class ScanBatch:
             if i_n > 0:
                 hip.check(lib.dwba_ddot_mfma_f64(i_n, hip.ptr(it16) + i0 * 8, hip.ptr(dd), Lm, hip.ptr(out), sh),
                           "dwba_ddot_mfma_f64")
+        if int(self.lane_split) == 4:
+            sj, ptrs, oidx, n = self._lane_job_set(8, self.direct_gemv, half, group=self._stage_group())
+            if n:
+                hip.check(lib.dwba_il_semisep_lanes_stage_f64(n, hip.ptr(sj), hip.ptr(ptrs), hip.ptr(oidx), Lm,
+                                                              hip.ptr(out), sh), "dwba_il_semisep_lanes_stage_f64")
+            return
         sj, ptrs, oidx, n = self._lane_job_set(lbv, self.direct_gemv, half)
         if n == 0:
             return
@@ -1172,7 +1210,7 @@ class ScanBatch:
         Lm = self.L_max
         has_gemv = self.ss_gemv is not None
         key = AT.signature("semisep_f64", AT.device_id(self.dev), jobs=AT.bucket(self.plain_count),
-                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm, gemv=int(has_gemv), v=6)
+                           n=AT.bucket(float(np.mean(list(nca)))), L=Lm, gemv=int(has_gemv), v=7)
         self.launch()                       # alpha/beta in place for the contraction
         torch.cuda.synchronize()
         out = torch.empty_like(self.out)
@@ -1188,7 +1226,7 @@ class ScanBatch:
             self._contract_semisep(seg, int(c["lb"]), bool(c.get("gemv", 0)), out, ws, nb, hip.stream_handle(),
                                    bool(c.get("lanes", 0)), int(c.get("split", 0)))
 
-        cands = AT.semisep_candidates()
+        cands = AT.semisep_candidates(Lm)
         if has_gemv:
             cands = cands + [dict(c, gemv=1) for c in cands]
         choice = tuner.tune(key, cands, run, steps=steps)

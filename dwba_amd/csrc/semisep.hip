@@ -831,6 +831,289 @@ void semisep_lane_split_kernel(const dwba_plain_job* __restrict__ jobs,
 }
 
 // ---------------------------------------------------------------------------
+// LDS-staged split form.  The 64 / NBP (job, family) streams of a wavefront
+// pair run at the same grid positions: the first wavefront covers [0, m),
+// the second [m, end) with the carry sums of the split kernel above, m a
+// multiple of the chunk that balances the two over the streams' longest
+// alpha rows and grid.  Each wavefront copies its streams' alpha / beta and
+// the grid-table rows of a chunk of SSC points from HBM straight into LDS
+// (global_load_lds_dword: no registers held while the copy is in flight),
+// two chunks ahead of the one it computes, so the recurrences read LDS only.
+// Points past a stream's own rows are fetched from a zero array, so one
+// instruction stream serves every lane: a step with alpha_i = beta_i = 0 adds
+// nothing to a sum, and chunks past the longest alpha rows take the
+// beta-only steps.  The streams of a wavefront pair with points must share
+// one grid table (the host pads each grid's jobs to whole pairs: job k with
+// d_out_index[k] < 0 is not written); a pair that does not writes NaN.
+//
+// LDS layout of a chunk buffer (dwords): alpha copies k = 0 .. KA-1 at 64 k,
+// beta at 64 (KA + k), table rows after them.  Copy k of lane l = NBP s + b
+// (stream s, multipole block b) brings dword NBP k + b of stream s's chunk to
+// dword 64 k + l, so point u of stream s sits at double 32 (2u / NBP) +
+// (2u % NBP) / 2 past NBP s / 2: the 64 / NBP streams of a read hit distinct
+// banks.
+constexpr int SSC = 16;          // grid points per staged chunk
+constexpr int SNBUF = 3;         // chunk buffers per wavefront (two copies in flight)
+
+__device__ double g_stage_zeros[64];   // read in place of points past a stream's rows
+
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+
+template <int NBP>
+struct StageShape {
+    static constexpr int LB = 8;
+    static constexpr int LS = 2 + NBP;                              // table row (doubles)
+    static constexpr int KA = (2 * (SSC + 1) + NBP - 1) / NBP;      // copies per array and chunk
+    static constexpr int PA = KA * NBP / 2;                         // points copied per stream (>= SSC + 1)
+    static constexpr int TW = SSC * 2 * LS;                         // table dwords per chunk
+    static constexpr int KT = (TW + 63) / 64;
+    static constexpr int NI = 2 * KA + KT;                          // copies per chunk
+    static constexpr int CW = 64 * NI;                              // dwords per chunk buffer
+    static constexpr int OFF_B = 64 * KA, OFF_T = 128 * KA;
+    static_assert(TW % 64 == 0, "table copies fill whole wavefronts");
+    static_assert(2 * SNBUF * CW * 4 >= (3 * LB + 1) * 64 * 8, "join buffer fits the pair's chunk buffers");
+    static_assert(2 * NI <= 63, "two chunks of copies fit the vmcnt counter");
+};
+
+// double index of point u of a stream's chunk, past the stream's own offset
+__device__ __forceinline__ constexpr int stage_at(int u, int nbp) { return 32 * ((2 * u) / nbp) + ((2 * u) % nbp) / 2; }
+
+template <int N>
+__device__ __forceinline__ void stage_wait() { asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory"); }
+
+__device__ __forceinline__ void stage_dma(const void* g, float* l) {
+    __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)l, 4, 0, 0);
+}
+
+// Copies of chunk [s0, s0 + SSC) into buf.  al / be: the lane's stream
+// (the zero array for lanes without points, with s0l = 0); la / lbn: its
+// alpha and beta lengths; clean_*: every stream's copy lies inside its rows
+// (wave-uniform; otherwise each dword past them is fetched from the zeros).
+template <int NBP>
+__device__ __forceinline__ void stage_copy(float* buf, const double* al, const double* be, const double* T, int s0,
+                                           int s0l, int la, int lbn, int t_rows, bool clean_a, bool clean_b,
+                                           int lane) {
+    using S = StageShape<NBP>;
+    const int b = lane % NBP;
+    const char* sa = reinterpret_cast<const char*>(al + s0l) + 4 * b;
+    const char* sb = reinterpret_cast<const char*>(be + s0l) + 4 * b;
+    if (clean_a) {
+#pragma unroll
+        for (int k = 0; k < S::KA; ++k) stage_dma(sa + 4 * NBP * k, buf + 64 * k);
+    } else {
+#pragma unroll
+        for (int k = 0; k < S::KA; ++k) {
+            const bool in = s0l + (NBP * k + b) / 2 < la;
+            stage_dma(in ? (const void*)(sa + 4 * NBP * k) : (const void*)g_stage_zeros, buf + 64 * k);
+        }
+    }
+    if (clean_b) {
+#pragma unroll
+        for (int k = 0; k < S::KA; ++k) stage_dma(sb + 4 * NBP * k, buf + S::OFF_B + 64 * k);
+    } else {
+#pragma unroll
+        for (int k = 0; k < S::KA; ++k) {
+            const bool in = s0l + (NBP * k + b) / 2 < lbn;
+            stage_dma(in ? (const void*)(sb + 4 * NBP * k) : (const void*)g_stage_zeros, buf + S::OFF_B + 64 * k);
+        }
+    }
+    // table rows s0 .. s0 + SSC - 1, row-major, rows past the grid clamped to its last
+    const float* tf = reinterpret_cast<const float*>(T);
+    const long d0 = (long)s0 * 2 * S::LS + lane;
+    if (s0 + SSC <= t_rows) {
+#pragma unroll
+        for (int j = 0; j < S::KT; ++j) stage_dma(tf + d0 + 64 * j, buf + S::OFF_T + 64 * j);
+    } else {
+        const long dmax = (long)t_rows * 2 * S::LS - 1;
+#pragma unroll
+        for (int j = 0; j < S::KT; ++j) stage_dma(tf + min(d0 + 64 * j, dmax), buf + S::OFF_T + 64 * j);
+    }
+}
+
+// One chunk of points with alpha (the arithmetic of split_point_a).
+template <int NBP, bool WC>
+__device__ __forceinline__ void stage_rows(const double* pa, const double* pb, const double* pt, const double* pq,
+                                           double& diag, double* Ap, double* Zp, double* acc, double* Ca, double* Cb,
+                                           double* P) {
+    constexpr int LB = StageShape<NBP>::LB, LS = StageShape<NBP>::LS;
+    double a_u = pa[0], b_u = pb[0];
+#pragma unroll
+    for (int u = 0; u < SSC; ++u) {
+        const double a_n = pa[stage_at(u + 1, NBP)];
+        const double b_n = pb[stage_at(u + 1, NBP)];
+        split_point_a<LB, WC>(pt[LS * u], pt[LS * u + 1], pq[LS * u], a_u, b_u, a_n, b_n, diag, Ap, Zp, acc, Ca, Cb,
+                              P);
+        a_u = a_n;
+        b_u = b_n;
+    }
+}
+
+// One chunk of beta-only points (lane_point_bt; with carries split_point_b).
+template <int NBP, bool WC>
+__device__ __forceinline__ void stage_tail(const double* pb, const double* pt, const double* pq, double* Zp,
+                                           double* acc, double* Cb, double* P) {
+    constexpr int LB = StageShape<NBP>::LB, LS = StageShape<NBP>::LS;
+#pragma unroll
+    for (int u = 0; u < SSC; ++u) {
+        const double b_u = pb[stage_at(u, NBP)];
+        if (WC) split_point_b<LB>(pt[LS * u], pt[LS * u + 1], pq[LS * u], b_u, Zp, acc, Cb, P);
+        else lane_point_bt<LB>(pt[LS * u], pt[LS * u + 1], pq[LS * u], b_u, Zp, acc);
+    }
+}
+
+// Chunks [c0, c1) of one wavefront, alpha steps below chunk ca.
+template <int NBP, bool WC>
+__device__ __forceinline__ void stage_run(float* mine, int c0, int c1, int ca, const double* al, const double* be,
+                                          const double* T, bool has, int la, int lbn, int la_min, int lb_min,
+                                          int t_rows, int lane, double& diag, double* Ap, double* Zp, double* acc,
+                                          double* Ca, double* Cb, double* P) {
+    using S = StageShape<NBP>;
+    auto copy = [&](int c) {
+        const int s0 = c * SSC;
+        stage_copy<NBP>(mine + (c % SNBUF) * S::CW, al, be, T, s0, has ? s0 : 0, la, lbn, t_rows,
+                        s0 + S::PA <= la_min, s0 + S::PA <= lb_min, lane);
+    };
+    if (c0 < c1) copy(c0);
+    if (c0 + 1 < c1) copy(c0 + 1);
+#pragma unroll 1
+    for (int c = c0; c < c1; ++c) {
+        if (c + 2 < c1) {
+            copy(c + 2);
+            stage_wait<2 * S::NI>();
+        } else if (c + 1 < c1) {
+            stage_wait<S::NI>();
+        } else {
+            stage_wait<0>();
+        }
+        const float* buf = mine + (c % SNBUF) * S::CW;
+        const double* pa = reinterpret_cast<const double*>(buf) + (lane / NBP) * (NBP / 2);
+        const double* pb = pa + S::OFF_B / 2;
+        const double* pt = reinterpret_cast<const double*>(buf + S::OFF_T);
+        const double* pq = pt + 2 + lane % NBP;
+        if (c < ca) stage_rows<NBP, WC>(pa, pb, pt, pq, diag, Ap, Zp, acc, Ca, Cb, P);
+        else stage_tail<NBP, WC>(pb, pt, pq, Zp, acc, Cb, P);
+    }
+    stage_wait<0>();
+}
+
+__device__ __forceinline__ int wave_max(int x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x = max(x, __shfl_xor(x, d, 64));
+    return x;
+}
+__device__ __forceinline__ int wave_min(int x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x = min(x, __shfl_xor(x, d, 64));
+    return x;
+}
+
+template <int NBP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void semisep_lane_stage_kernel(const dwba_plain_job* __restrict__ jobs, const double* const* __restrict__ tabs,
+                               const int* __restrict__ out_index, int n_jobs, int L_max, double* __restrict__ out) {
+    using S = StageShape<NBP>;
+    constexpr int LB = S::LB;
+    __shared__ __attribute__((aligned(16))) float s_stage[4][SNBUF][S::CW];
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int grp = w >> 1, h = w & 1;              // wave pair grp, part h of the range
+    const long gt = (long)blockIdx.x * 128 + grp * 64 + lane;
+    const long stream = gt / NBP;
+    const int blk = lane % NBP;
+    const int NL = L_max + 1;
+    const int L0 = blk * LB;
+    bool ok = stream < 2L * n_jobs;
+    int fam = 0, k = 0, n = 0, nr = 0;
+    const double* al = g_stage_zeros;
+    const double* be = g_stage_zeros;
+    const double* T = nullptr;
+    if (ok) {
+        fam = stream >= n_jobs ? 1 : 0;
+        k = (int)(fam ? stream - n_jobs : stream);
+        const dwba_plain_job J = jobs[k];
+        const double* a = fam ? J.alpha_ex : J.alpha_dir;
+        if (a == nullptr) {
+            ok = false;                                // direct family computed elsewhere
+        } else if (J.n_cols > 0) {
+            al = a;
+            be = fam ? J.beta_ex : J.beta_dir;
+            n = J.n_cols;
+            nr = max(0, min(J.n_rows, J.n_cols));
+            T = tabs[k];
+        }
+    }
+    const bool has = n > 0;                             // a stream with points (its lanes copy and compute)
+    // pair-uniform extents: longest grid and alpha rows, shortest of each for the copies' bounds
+    const int e_w = __builtin_amdgcn_readfirstlane(wave_max(n)), r_w = __builtin_amdgcn_readfirstlane(wave_max(nr));
+    const int la_min = __builtin_amdgcn_readfirstlane(wave_min(has ? nr : INT_MAX));
+    const int lb_min = __builtin_amdgcn_readfirstlane(wave_min(has ? n : INT_MAX));
+    const unsigned long long hm = __ballot(has);
+    const double* T0 = nullptr;
+    bool mixed = false;
+    if (hm) {
+        const int first = __ffsll((long long)hm) - 1;
+        const unsigned long long tp = reinterpret_cast<unsigned long long>(T);
+        const unsigned lo = __builtin_amdgcn_readlane((unsigned)tp, first);
+        const unsigned hi = __builtin_amdgcn_readlane((unsigned)(tp >> 32), first);
+        T0 = reinterpret_cast<const double*>(((unsigned long long)hi << 32) | lo);
+        mixed = __ballot(has && T != T0) != 0;
+    }
+    int m_w = 0, c_end = 0, c_alpha = 0;
+    if (e_w > 0 && !mixed) {
+        c_end = (e_w + SSC - 1) / SSC;
+        c_alpha = (r_w + SSC - 1) / SSC;
+        // balance 5 (first) against 8 (second) operations per point and multipole on the alpha rows, 5
+        // against 3 past them (beta-only steps, with and without the carry sums)
+        const long c0 = 5 * LB + 8, c1 = 8 * LB + 8, cb = 5 * LB + 8, c0b = 3 * LB + 8;
+        long m = (c1 * r_w + cb * (long)(e_w - r_w)) / (c0 + c1);
+        if (m > r_w) m = ((c0b - c0) * (long)r_w + cb * (long)e_w) / (c0b + cb);
+        m_w = (int)min((long)c_end, max(0L, (m + SSC / 2) / SSC)) * SSC;
+    }
+    const int i0 = h ? m_w : 0;
+    const double a0 = (has && i0 < nr) ? al[i0] : 0.0, b0 = (has && i0 < n) ? be[i0] : 0.0;
+    double Ap[LB], Zp[LB], acc[LB], Ca[LB], Cb[LB], P[LB];
+    double diag = 0.0;
+#pragma unroll
+    for (int l = 0; l < LB; ++l) { Ap[l] = b0; Zp[l] = a0; acc[l] = 0.0; Ca[l] = 0.0; Cb[l] = 0.0; P[l] = 1.0; }
+    const int la = has ? nr : INT_MAX, lbn = has ? n : INT_MAX;
+    float* mine = &s_stage[w][0][0];
+    if (h == 0)
+        stage_run<NBP, false>(mine, 0, m_w / SSC, c_alpha, al, be, T0, has, la, lbn, la_min, lb_min, e_w, lane, diag,
+                              Ap, Zp, acc, Ca, Cb, P);
+    else
+        stage_run<NBP, true>(mine, m_w / SSC, c_end, c_alpha, al, be, T0, has, la, lbn, la_min, lb_min, e_w, lane,
+                             diag, Ap, Zp, acc, Ca, Cb, P);
+    __syncthreads();                                    // every copy landed and read: the buffers are free
+    double* s_x = reinterpret_cast<double*>(&s_stage[2 * grp][0][0]);   // [3 LB + 1][64]
+    if (h == 0) {
+#pragma unroll
+        for (int l = 0; l < LB; ++l) {
+            s_x[l * 64 + lane] = Ap[l];
+            s_x[(LB + l) * 64 + lane] = Zp[l];
+            s_x[(2 * LB + l) * 64 + lane] = acc[l];
+        }
+        s_x[3 * LB * 64 + lane] = diag;
+    }
+    __syncthreads();
+    if (h == 1 && ok && L0 < NL) {
+        const int ko = out_index ? out_index[k] : k;
+        if (ko >= 0) {
+            double* __restrict__ o = out + (size_t)ko * 2 * NL + fam * NL;
+            const double dg = diag + s_x[3 * LB * 64 + lane];
+            const double ldl = log(1.0 - 1e-12);
+#pragma unroll
+            for (int l = 0; l < LB; ++l) {
+                const double A_m = s_x[l * 64 + lane] - b0;
+                const double Z_m = s_x[(LB + l) * 64 + lane] - a0;
+                const double tot = fma(A_m, Ca[l], fma(Z_m, Cb[l], acc[l] + s_x[(2 * LB + l) * 64 + lane]));
+                const int L = L0 + l;
+                const double dL = L == 0 ? 1.0 : exp((double)L * ldl);
+                if (L < NL) o[L] = mixed ? __builtin_nan("") : (nr > 0 ? fma(dL - 2.0, dg, tot) : 0.0);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Shared-beta form (the direct family of an excitation scan, where every
 // pair of an energy has the same beta = u_f u_i w): t_L(i) = sum_j K_L(i, j) beta_j
 // for i < rows, all L, once per energy, in O(N) per L:
@@ -1191,6 +1474,31 @@ int32_t dwba_il_semisep_lanes_split_f64(int32_t n_jobs, const dwba_plain_job* d_
         default: return set_error(DWBA_ERR_ARG, "dwba_il_semisep_lanes_split_f64: L_max %d", L_max);
     }
     DWBA_LAUNCH_CHECK("semisep_lane_split_kernel");
+    return DWBA_OK;
+}
+
+int32_t dwba_semisep_lane_stage_group(int32_t L_max) {
+    if (L_max < 16 || L_max > 63) return -1;
+    return 64 / lane_nbp(L_max, 8);
+}
+
+int32_t dwba_il_semisep_lanes_stage_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const double* const* d_tabs,
+                                        const int32_t* d_out_index, int32_t L_max, double* d_out, void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_il_semisep_lanes_stage_f64: call dwba_init first");
+    DWBA_REQUIRE(n_jobs > 0 && d_jobs && d_tabs && d_out && L_max >= 16 && L_max <= 63,
+                 "dwba_il_semisep_lanes_stage_f64: bad arguments (n_jobs=%d, L_max=%d; needs 16 <= L_max <= 63)",
+                 n_jobs, L_max);
+    const int nbp = lane_nbp(L_max, 8);
+    const long blocks = (2L * n_jobs * nbp + 127) / 128;
+    DWBA_REQUIRE(blocks <= INT32_MAX, "dwba_il_semisep_lanes_stage_f64: too many jobs (%d)", n_jobs);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (nbp == 4)
+        hipLaunchKernelGGL(semisep_lane_stage_kernel<4>, dim3((unsigned)blocks), dim3(256), 0, s, d_jobs, d_tabs,
+                           d_out_index, n_jobs, L_max, d_out);
+    else
+        hipLaunchKernelGGL(semisep_lane_stage_kernel<8>, dim3((unsigned)blocks), dim3(256), 0, s, d_jobs, d_tabs,
+                           d_out_index, n_jobs, L_max, d_out);
+    DWBA_LAUNCH_CHECK("semisep_lane_stage_kernel");
     return DWBA_OK;
 }
 

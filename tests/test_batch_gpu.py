@@ -124,7 +124,7 @@ def test_autotuned_batch_matches_default(tmp_path, monkeypatch):
     ch = b.tune(tuner)
     print(f"\n[autotune] choice {ch.params} ({ch.ms:.3f} ms) of {len(ch.timings)} candidates", flush=True)
     assert not ch.cached
-    assert len(ch.timings) == len(AT.semisep_candidates()) * (2 if b.ss_gemv is not None else 1)
+    assert len(ch.timings) == len(AT.semisep_candidates(b.L_max)) * (2 if b.ss_gemv is not None else 1)
     for _ in range(2):
         b.run()
     r1 = b.results()
@@ -228,8 +228,9 @@ def test_lane_tables_bit_identical():
             b.run()
             assert np.array_equal(r0, b.results()), (lb, gemv, "energy order")
             b.lane_order = "length"
-            # the range split between two wavefronts: same sums, other order
-            for occ in (2, 3):
+            # the range split between two wavefronts (4: LDS-staged, pairs of an energy padded to whole
+            # groups): same sums, other order
+            for occ in (2, 3, 4):
                 b.lane_split = occ
                 b.out.zero_()
                 b.run()

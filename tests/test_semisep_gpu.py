@@ -108,6 +108,68 @@ def test_semisep_lane_tables_random_jobs(L_max, lb, split):
     print(f"  worst |d|/sum|aKb| {w:.2e}", flush=True)
 
 
+def _shared_jobs(grids, seed=0):
+    """grids: (n, grid kind, [(n_cols, n_rows), ...]) -> jobs sharing each
+    grid tensor (the LDS-staged kernel's one table per grid)."""
+    rng = np.random.default_rng(seed)
+    host, dev = [], []
+    for (n, kind, ext) in grids:
+        g = G.exp_grid(n, 1e-5, 1e3) if kind == "exp" else G.make_r_grid(1e-4, 200.0, n)
+        r_d = _t(g.r)
+        for (nc, nrow) in ext:
+            r, w = g.r[:nc], g.w_simpson[:nc]
+            vecs = [rng.standard_normal(nc) * w for _ in range(4)]
+            vecs[0][nrow:] = 0.0
+            vecs[2][nrow:] = 0.0
+            host.append((r, vecs, nrow, nc))
+            dev.append((r_d, *[_t(v) for v in vecs], nrow, nc))
+    return host, dev
+
+
+# ragged extents on shared grids: alpha rows shorter than, equal to and one
+# short of the grid, chunk-boundary lengths (15, 16, 17, 33), one- and
+# two-point grids, no alpha rows at all, a group of 16 that fills exactly
+STAGE_GRIDS = [
+    (1000, "exp", [(1000, 1000), (1000, 999), (1000, 0), (999, 500), (500, 500), (17, 17), (16, 3), (15, 15),
+                   (33, 1), (2, 2), (1, 1), (1000, 640)]),
+    (3001, "ref", [(3001 - 37 * k, 2900 - 61 * k) for k in range(16)]),
+    (300, "exp", [(300, 300 - 13 * k) for k in range(21)]),
+    (5, "exp", [(5, 5), (4, 2)]),
+]
+
+
+@pytest.mark.parametrize("L_max", [16, 20, 30, 31, 32, 63])
+def test_semisep_lane_stage_random_jobs(L_max):
+    """LDS-staged split lane kernel (dwba_il_semisep_lanes_stage_f64) against
+    the oracle: jobs of several grids padded to whole groups per grid (in
+    reverse order through the output index), ragged extents inside a group,
+    multipole blocks past L_max (L_max 16, 20), 8 pairs per group (L_max > 31)."""
+    from dwba_amd import _hip
+    host, dev = _shared_jobs(STAGE_GRIDS, seed=300 + L_max)
+    print(f"\n[semisep lane stage] {len(host)} jobs L_max={L_max}", flush=True)
+    got = _hip.il_semisep_stage_f64_batch(dev, L_max).cpu().numpy()
+    w = _check(host, got, L_max)
+    print(f"  worst |d|/sum|aKb| {w:.2e}", flush=True)
+
+
+def test_semisep_lane_stage_mixed_tables_read_nan():
+    """A group whose streams are on different grid tables is not computed: its
+    outputs read NaN (no silent mixing), the other groups are exact.  17 jobs
+    unpadded, 16 streams per group: group 0 = direct family of jobs 0-15 (one
+    grid), group 1 = direct of job 16 (another grid) + exchange of jobs 0-14,
+    group 2 = exchange of jobs 15 and 16."""
+    from dwba_amd import _hip
+    host, dev = _shared_jobs([(400, "exp", [(400, 400 - 9 * k) for k in range(16)])], seed=3)
+    h2, d2 = _jobs([(200, 150, "exp")], seed=4)
+    print("\n[semisep lane stage] mixed tables in one group", flush=True)
+    got = _hip.il_semisep_stage_f64_batch(dev + d2, 30, pad=False).cpu().numpy()
+    for k, (r, vecs, n_rows, n) in enumerate(host):
+        ref = semisep.il_all_L(r, vecs[0], vecs[1], 30)
+        mag = semisep.abs_bound_all_L(r, vecs[0], vecs[1], 30)
+        assert np.all(np.abs(got[k, 0] - ref) <= 1e-10 * np.abs(ref) + 1e-13 * mag), k
+    assert np.all(np.isnan(got[16, 0])) and np.all(np.isnan(got[:, 1]))
+
+
 def test_semisep_lanes_matches_scan_kernel():
     from dwba_amd import _hip
     host, dev = _jobs(SPECS, seed=11)
