@@ -158,8 +158,8 @@ _SIGS = {
     "dwba_semisep_lane_tables_f64": (c_i32, [c_i32, c_vp, c_i32, c_i32, c_i32, c_vp]),
     "dwba_il_semisep_lanes_tab_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp]),
     "dwba_il_semisep_lanes_split_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
-    "dwba_semisep_lane_stage_group": (c_i32, [c_i32]),
-    "dwba_il_semisep_lanes_stage_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]),
+    "dwba_semisep_lane_stage_group": (c_i32, [c_i32, c_i32]),
+    "dwba_il_semisep_lanes_stage_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp]),
     "dwba_batch_finalize": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
     "dwba_il_backward_f64": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
     "dwba_il_backward_f32": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
@@ -374,11 +374,11 @@ def il_semisep_f64_batch(jobs, L_max: int, seg: int = 0, stream=None, lb: int = 
     return out
 
 
-def il_semisep_stage_f64_batch(jobs, L_max: int, stream=None, pad: bool = True):
+def il_semisep_stage_f64_batch(jobs, L_max: int, stream=None, pad: bool = True, lb: int = 0):
     """dwba_il_semisep_lanes_stage_f64 on the job tuples of il_semisep_f64_batch
-    (16 <= L_max <= 63).  Jobs on the same grid tensor share one grid table;
-    pad: each grid's jobs (in reverse order) are padded to whole groups of
-    dwba_semisep_lane_stage_group(L_max) with empty jobs that d_out_index
+    (lb = multipoles per lane, 4 or 8; 0 = 4).  Jobs on the same grid tensor
+    share one grid table; pad: each grid's jobs (in reverse order) are padded
+    to whole groups of dwba_semisep_lane_stage_group(L_max, lb) with empty jobs that d_out_index
     marks unwritten.  pad=False passes the jobs as given (no padding), so a
     group whose streams are on different grids reads back NaN.  Returns a
     (n_jobs, 2, L_max+1) float64 CUDA tensor."""
@@ -389,9 +389,10 @@ def il_semisep_stage_f64_batch(jobs, L_max: int, stream=None, pad: bool = True):
     n = len(jobs)
     if n == 0:
         raise ValueError("il_semisep_stage_f64_batch: no jobs")
-    G = int(lib.dwba_semisep_lane_stage_group(L_max))
+    lbv = lb or 4
+    G = int(lib.dwba_semisep_lane_stage_group(L_max, lbv))
     if G <= 0:
-        raise ValueError(f"il_semisep_stage_f64_batch: L_max {L_max} outside [16, 63]")
+        raise ValueError(f"il_semisep_stage_f64_batch: (L_max, lb) = ({L_max}, {lbv}) not supported")
     dev = jobs[0][0].device
     arr = []
     for k, (r, ad, bd, ae, be, n_rows, n_cols) in enumerate(jobs):
@@ -399,7 +400,7 @@ def il_semisep_stage_f64_batch(jobs, L_max: int, stream=None, pad: bool = True):
                          ("alpha_ex", ae, n_rows), ("beta_ex", be, n_cols)):
             _require_f64_cuda(f"job {k} {nm}", t, m)
         arr.append(PlainJob(ptr(r), ptr(ad), ptr(bd), ptr(ae), ptr(be), int(n_rows), int(n_cols)))
-    stride = lib.dwba_semisep_lane_table_stride(L_max, 8)
+    stride = lib.dwba_semisep_lane_table_stride(L_max, lbv)
     # one table per grid tensor, covering the longest job on it
     groups = {}
     for k in range(n):
@@ -414,7 +415,7 @@ def il_semisep_stage_f64_batch(jobs, L_max: int, stream=None, pad: bool = True):
             gl.append(LaneGrid(ptr(jobs[ks[0]][0]), ptr(t), npts, 0))
     if gl:
         d_grids = struct_array_to_device((LaneGrid * len(gl))(*gl), dev)
-        check(lib.dwba_semisep_lane_tables_f64(len(gl), ptr(d_grids), max(g.n for g in gl), L_max, 8,
+        check(lib.dwba_semisep_lane_tables_f64(len(gl), ptr(d_grids), max(g.n for g in gl), L_max, lbv,
                                                stream_handle(stream)), "dwba_semisep_lane_tables_f64")
     else:
         d_grids = None
@@ -435,8 +436,8 @@ def il_semisep_stage_f64_batch(jobs, L_max: int, stream=None, pad: bool = True):
     d_tabs = torch.as_tensor(np.asarray(tp, dtype=np.int64), device=dev)
     d_oidx = torch.as_tensor(np.asarray(order, dtype=np.int32), device=dev)
     out = torch.full((n, 2, L_max + 1), float("nan"), dtype=torch.float64, device=dev)
-    check(lib.dwba_il_semisep_lanes_stage_f64(len(order), ptr(d_sj), ptr(d_tabs), ptr(d_oidx), L_max, ptr(out),
-                                              stream_handle(stream)), "dwba_il_semisep_lanes_stage_f64")
+    check(lib.dwba_il_semisep_lanes_stage_f64(len(order), ptr(d_sj), ptr(d_tabs), ptr(d_oidx), L_max, lbv,
+                                              ptr(out), stream_handle(stream)), "dwba_il_semisep_lanes_stage_f64")
     out._dwba_keep = (d_sj, d_tabs, d_oidx, tabs, d_grids)
     return out
 

@@ -171,9 +171,10 @@ def semisep_candidates(L_max: int = None) -> List[Dict[str, int]]:
     (one lane per job, family and block of lb multipoles; seg unused), whole
     range per lane or split between two wavefronts (split = the occupancy,
     2 or 3 waves per SIMD, that kernel is compiled for; 4 = the LDS-staged
-    split kernel, for L_max >= 16); order 1: the lane kernel's jobs grouped
+    split kernel, lb 4 from L_max 4 and lb 8 from L_max 16); order 1: the lane kernel's jobs grouped
     by energy instead of sorted by length alone."""
-    stage = [{"seg": 0, "lb": 8, "lanes": 1, "split": 4, "order": 1}] if L_max is None or 16 <= L_max <= 63 else []
+    stage = [{"seg": 0, "lb": b, "lanes": 1, "split": 4, "order": 1} for b in SEMISEP_LBS
+             if L_max is None or (16 if b == 8 else 4) <= L_max <= 63]
     return [{"seg": s, "lb": b} for b in SEMISEP_LBS for s in SEMISEP_SEGS] + [
         {"seg": 0, "lb": b, "lanes": 1} for b in SEMISEP_LBS] + [
         {"seg": 0, "lb": b, "lanes": 1, "split": w} for b in SEMISEP_LBS for w in (2, 3)] + [

@@ -829,7 +829,10 @@ class ScanBatch:
                     while len(padded) % group:
                         padded.append(-1)
                         pe.append(energy[p])
-            perm, energy = padded, pe
+            # whole groups dispatched longest first across energies (a group's first job is its longest)
+            starts = sorted(range(0, len(padded), group), key=lambda s: -int(src[padded[s]].n_cols))
+            perm = [padded[s + q] for s in starts for q in range(group)]
+            energy = [pe[s + q] for s in starts for q in range(group)]
         n = len(perm)
         arr = (hip.PlainJob * max(n, 1))()
         for p, g in enumerate(perm):
@@ -856,8 +859,8 @@ class ScanBatch:
             lbv = lb or 8
             split = self.lane_split if split is None else split     # 0: whole ranges, 2 or 3: waves per SIMD,
             if split == 4:                                          # 4: LDS-staged
-                sj, ptrs, oidx, n = self._lane_job_set(8, direct_gemv, group=self._stage_group())
-                hip.check(lib.dwba_il_semisep_lanes_stage_f64(n, hip.ptr(sj), hip.ptr(ptrs), hip.ptr(oidx), Lm,
+                sj, ptrs, oidx, n = self._lane_job_set(lbv, direct_gemv, group=self._stage_group(lbv))
+                hip.check(lib.dwba_il_semisep_lanes_stage_f64(n, hip.ptr(sj), hip.ptr(ptrs), hip.ptr(oidx), Lm, lbv,
                                                               hip.ptr(out), sh), "dwba_il_semisep_lanes_stage_f64")
                 return self._contract_direct(direct_gemv, out, sh)
             sj, ptrs, oidx, _ = self._lane_job_set(lbv, direct_gemv)
@@ -877,10 +880,10 @@ class ScanBatch:
                                               hip.ptr(ws), ws_bytes, sh), "dwba_il_semisep_f64")
         self._contract_direct(direct_gemv, out, sh)
 
-    def _stage_group(self):
+    def _stage_group(self, lb):
         """Jobs per group of the LDS-staged lane kernel (its wavefront pair's
-        pairs), 0 where it does not apply (L_max below 16)."""
-        return max(0, int(self.lib.dwba_semisep_lane_stage_group(self.L_max)))
+        pairs) at lb multipoles per lane; 0 where it does not apply."""
+        return max(0, int(self.lib.dwba_semisep_lane_stage_group(self.L_max, lb)))
 
     def _contract_direct(self, direct_gemv, out, sh):
         """The shared-beta direct family: t_L = K_L beta per energy, then the dots."""
@@ -926,9 +929,9 @@ class ScanBatch:
                 hip.check(lib.dwba_ddot_mfma_f64(i_n, hip.ptr(it16) + i0 * 8, hip.ptr(dd), Lm, hip.ptr(out), sh),
                           "dwba_ddot_mfma_f64")
         if int(self.lane_split) == 4:
-            sj, ptrs, oidx, n = self._lane_job_set(8, self.direct_gemv, half, group=self._stage_group())
+            sj, ptrs, oidx, n = self._lane_job_set(lbv, self.direct_gemv, half, group=self._stage_group(lbv))
             if n:
-                hip.check(lib.dwba_il_semisep_lanes_stage_f64(n, hip.ptr(sj), hip.ptr(ptrs), hip.ptr(oidx), Lm,
+                hip.check(lib.dwba_il_semisep_lanes_stage_f64(n, hip.ptr(sj), hip.ptr(ptrs), hip.ptr(oidx), Lm, lbv,
                                                               hip.ptr(out), sh), "dwba_il_semisep_lanes_stage_f64")
             return
         sj, ptrs, oidx, n = self._lane_job_set(lbv, self.direct_gemv, half)

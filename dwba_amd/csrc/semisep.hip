@@ -853,26 +853,29 @@ void semisep_lane_split_kernel(const dwba_plain_job* __restrict__ jobs,
 // (2u % NBP) / 2 past NBP s / 2: the 64 / NBP streams of a read hit distinct
 // banks.
 constexpr int SSC = 16;          // grid points per staged chunk
-constexpr int SNBUF = 3;         // chunk buffers per wavefront (two copies in flight)
 
 __device__ double g_stage_zeros[64];   // read in place of points past a stream's rows
 
 typedef __attribute__((address_space(3))) void* lds_void_ptr;
 
-template <int NBP>
+// LB multipoles per lane, NBP lanes per stream, NBUF chunk buffers per
+// wavefront (NBUF - 1 chunks in flight).
+template <int LB_, int NBP_, int NBUF_>
 struct StageShape {
-    static constexpr int LB = 8;
+    static constexpr int LB = LB_, NBP = NBP_, NBUF = NBUF_;
     static constexpr int LS = 2 + NBP;                              // table row (doubles)
     static constexpr int KA = (2 * (SSC + 1) + NBP - 1) / NBP;      // copies per array and chunk
     static constexpr int PA = KA * NBP / 2;                         // points copied per stream (>= SSC + 1)
     static constexpr int TW = SSC * 2 * LS;                         // table dwords per chunk
-    static constexpr int KT = (TW + 63) / 64;
+    static constexpr int KT = TW / 64;
     static constexpr int NI = 2 * KA + KT;                          // copies per chunk
     static constexpr int CW = 64 * NI;                              // dwords per chunk buffer
     static constexpr int OFF_B = 64 * KA, OFF_T = 128 * KA;
+    static_assert(NBP % 2 == 0 && 64 % NBP == 0, "whole streams per wavefront, a double's two dwords in one copy");
     static_assert(TW % 64 == 0, "table copies fill whole wavefronts");
-    static_assert(2 * SNBUF * CW * 4 >= (3 * LB + 1) * 64 * 8, "join buffer fits the pair's chunk buffers");
-    static_assert(2 * NI <= 63, "two chunks of copies fit the vmcnt counter");
+    static_assert(2 * NBUF * CW * 4 >= (3 * LB + 1) * 64 * 8, "join buffer fits the pair's chunk buffers");
+    static_assert(NBUF == 2 || NBUF == 3, "one or two chunks in flight");
+    static_assert((NBUF - 1) * NI <= 63, "the copies in flight fit the vmcnt counter");
 };
 
 // double index of point u of a stream's chunk, past the stream's own offset
@@ -889,11 +892,11 @@ __device__ __forceinline__ void stage_dma(const void* g, float* l) {
 // (the zero array for lanes without points, with s0l = 0); la / lbn: its
 // alpha and beta lengths; clean_*: every stream's copy lies inside its rows
 // (wave-uniform; otherwise each dword past them is fetched from the zeros).
-template <int NBP>
+template <class S>
 __device__ __forceinline__ void stage_copy(float* buf, const double* al, const double* be, const double* T, int s0,
                                            int s0l, int la, int lbn, int t_rows, bool clean_a, bool clean_b,
                                            int lane) {
-    using S = StageShape<NBP>;
+    constexpr int NBP = S::NBP;
     const int b = lane % NBP;
     const char* sa = reinterpret_cast<const char*>(al + s0l) + 4 * b;
     const char* sb = reinterpret_cast<const char*>(be + s0l) + 4 * b;
@@ -931,67 +934,65 @@ __device__ __forceinline__ void stage_copy(float* buf, const double* al, const d
 }
 
 // One chunk of points with alpha (the arithmetic of split_point_a).
-template <int NBP, bool WC>
+template <class S, bool WC>
 __device__ __forceinline__ void stage_rows(const double* pa, const double* pb, const double* pt, const double* pq,
                                            double& diag, double* Ap, double* Zp, double* acc, double* Ca, double* Cb,
                                            double* P) {
-    constexpr int LB = StageShape<NBP>::LB, LS = StageShape<NBP>::LS;
     double a_u = pa[0], b_u = pb[0];
 #pragma unroll
     for (int u = 0; u < SSC; ++u) {
-        const double a_n = pa[stage_at(u + 1, NBP)];
-        const double b_n = pb[stage_at(u + 1, NBP)];
-        split_point_a<LB, WC>(pt[LS * u], pt[LS * u + 1], pq[LS * u], a_u, b_u, a_n, b_n, diag, Ap, Zp, acc, Ca, Cb,
-                              P);
+        const double a_n = pa[stage_at(u + 1, S::NBP)];
+        const double b_n = pb[stage_at(u + 1, S::NBP)];
+        split_point_a<S::LB, WC>(pt[S::LS * u], pt[S::LS * u + 1], pq[S::LS * u], a_u, b_u, a_n, b_n, diag, Ap, Zp,
+                                 acc, Ca, Cb, P);
         a_u = a_n;
         b_u = b_n;
     }
 }
 
 // One chunk of beta-only points (lane_point_bt; with carries split_point_b).
-template <int NBP, bool WC>
+template <class S, bool WC>
 __device__ __forceinline__ void stage_tail(const double* pb, const double* pt, const double* pq, double* Zp,
                                            double* acc, double* Cb, double* P) {
-    constexpr int LB = StageShape<NBP>::LB, LS = StageShape<NBP>::LS;
 #pragma unroll
     for (int u = 0; u < SSC; ++u) {
-        const double b_u = pb[stage_at(u, NBP)];
-        if (WC) split_point_b<LB>(pt[LS * u], pt[LS * u + 1], pq[LS * u], b_u, Zp, acc, Cb, P);
-        else lane_point_bt<LB>(pt[LS * u], pt[LS * u + 1], pq[LS * u], b_u, Zp, acc);
+        const double b_u = pb[stage_at(u, S::NBP)];
+        if (WC) split_point_b<S::LB>(pt[S::LS * u], pt[S::LS * u + 1], pq[S::LS * u], b_u, Zp, acc, Cb, P);
+        else lane_point_bt<S::LB>(pt[S::LS * u], pt[S::LS * u + 1], pq[S::LS * u], b_u, Zp, acc);
     }
 }
 
 // Chunks [c0, c1) of one wavefront, alpha steps below chunk ca.
-template <int NBP, bool WC>
+template <class S, bool WC>
 __device__ __forceinline__ void stage_run(float* mine, int c0, int c1, int ca, const double* al, const double* be,
                                           const double* T, bool has, int la, int lbn, int la_min, int lb_min,
                                           int t_rows, int lane, double& diag, double* Ap, double* Zp, double* acc,
                                           double* Ca, double* Cb, double* P) {
-    using S = StageShape<NBP>;
+    constexpr int NBP = S::NBP, NBUF = S::NBUF;
     auto copy = [&](int c) {
         const int s0 = c * SSC;
-        stage_copy<NBP>(mine + (c % SNBUF) * S::CW, al, be, T, s0, has ? s0 : 0, la, lbn, t_rows,
-                        s0 + S::PA <= la_min, s0 + S::PA <= lb_min, lane);
+        stage_copy<S>(mine + (c % NBUF) * S::CW, al, be, T, s0, has ? s0 : 0, la, lbn, t_rows,
+                      s0 + S::PA <= la_min, s0 + S::PA <= lb_min, lane);
     };
     if (c0 < c1) copy(c0);
-    if (c0 + 1 < c1) copy(c0 + 1);
+    if (NBUF == 3 && c0 + 1 < c1) copy(c0 + 1);
 #pragma unroll 1
     for (int c = c0; c < c1; ++c) {
-        if (c + 2 < c1) {
-            copy(c + 2);
-            stage_wait<2 * S::NI>();
-        } else if (c + 1 < c1) {
+        if (c + NBUF - 1 < c1) {
+            copy(c + NBUF - 1);
+            stage_wait<(NBUF - 1) * S::NI>();
+        } else if (NBUF == 3 && c + 1 < c1) {
             stage_wait<S::NI>();
         } else {
             stage_wait<0>();
         }
-        const float* buf = mine + (c % SNBUF) * S::CW;
+        const float* buf = mine + (c % NBUF) * S::CW;
         const double* pa = reinterpret_cast<const double*>(buf) + (lane / NBP) * (NBP / 2);
         const double* pb = pa + S::OFF_B / 2;
         const double* pt = reinterpret_cast<const double*>(buf + S::OFF_T);
         const double* pq = pt + 2 + lane % NBP;
-        if (c < ca) stage_rows<NBP, WC>(pa, pb, pt, pq, diag, Ap, Zp, acc, Ca, Cb, P);
-        else stage_tail<NBP, WC>(pb, pt, pq, Zp, acc, Cb, P);
+        if (c < ca) stage_rows<S, WC>(pa, pb, pt, pq, diag, Ap, Zp, acc, Ca, Cb, P);
+        else stage_tail<S, WC>(pb, pt, pq, Zp, acc, Cb, P);
     }
     stage_wait<0>();
 }
@@ -1007,13 +1008,12 @@ __device__ __forceinline__ int wave_min(int x) {
     return x;
 }
 
-template <int NBP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+template <int LB, int NBP, int NBUF, int OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
 void semisep_lane_stage_kernel(const dwba_plain_job* __restrict__ jobs, const double* const* __restrict__ tabs,
                                const int* __restrict__ out_index, int n_jobs, int L_max, double* __restrict__ out) {
-    using S = StageShape<NBP>;
-    constexpr int LB = S::LB;
-    __shared__ __attribute__((aligned(16))) float s_stage[4][SNBUF][S::CW];
+    using S = StageShape<LB, NBP, NBUF>;
+    __shared__ __attribute__((aligned(16))) float s_stage[4][NBUF][S::CW];
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int grp = w >> 1, h = w & 1;              // wave pair grp, part h of the range
     const long gt = (long)blockIdx.x * 128 + grp * 64 + lane;
@@ -1077,11 +1077,11 @@ void semisep_lane_stage_kernel(const dwba_plain_job* __restrict__ jobs, const do
     const int la = has ? nr : INT_MAX, lbn = has ? n : INT_MAX;
     float* mine = &s_stage[w][0][0];
     if (h == 0)
-        stage_run<NBP, false>(mine, 0, m_w / SSC, c_alpha, al, be, T0, has, la, lbn, la_min, lb_min, e_w, lane, diag,
-                              Ap, Zp, acc, Ca, Cb, P);
+        stage_run<S, false>(mine, 0, m_w / SSC, c_alpha, al, be, T0, has, la, lbn, la_min, lb_min, e_w, lane, diag,
+                            Ap, Zp, acc, Ca, Cb, P);
     else
-        stage_run<NBP, true>(mine, m_w / SSC, c_end, c_alpha, al, be, T0, has, la, lbn, la_min, lb_min, e_w, lane,
-                             diag, Ap, Zp, acc, Ca, Cb, P);
+        stage_run<S, true>(mine, m_w / SSC, c_end, c_alpha, al, be, T0, has, la, lbn, la_min, lb_min, e_w, lane,
+                           diag, Ap, Zp, acc, Ca, Cb, P);
     __syncthreads();                                    // every copy landed and read: the buffers are free
     double* s_x = reinterpret_cast<double*>(&s_stage[2 * grp][0][0]);   // [3 LB + 1][64]
     if (h == 0) {
@@ -1477,27 +1477,36 @@ int32_t dwba_il_semisep_lanes_split_f64(int32_t n_jobs, const dwba_plain_job* d_
     return DWBA_OK;
 }
 
-int32_t dwba_semisep_lane_stage_group(int32_t L_max) {
-    if (L_max < 16 || L_max > 63) return -1;
-    return 64 / lane_nbp(L_max, 8);
+static int stage_lb(int32_t lb) { return lb == 0 ? 4 : lb; }
+
+int32_t dwba_semisep_lane_stage_group(int32_t L_max, int32_t lb) {
+    lb = stage_lb(lb);
+    if ((lb != 4 && lb != 8) || L_max < 0 || L_max > 63) return -1;
+    const int nbp = lane_nbp(L_max, lb);
+    return nbp < (lb == 8 ? 4 : 2) ? -1 : 64 / nbp;   // the copies of 32 streams at lb = 8 overflow vmcnt
 }
 
 int32_t dwba_il_semisep_lanes_stage_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const double* const* d_tabs,
-                                        const int32_t* d_out_index, int32_t L_max, double* d_out, void* stream) {
+                                        const int32_t* d_out_index, int32_t L_max, int32_t lb, double* d_out,
+                                        void* stream) {
     DWBA_REQUIRE(initialised(), "dwba_il_semisep_lanes_stage_f64: call dwba_init first");
-    DWBA_REQUIRE(n_jobs > 0 && d_jobs && d_tabs && d_out && L_max >= 16 && L_max <= 63,
-                 "dwba_il_semisep_lanes_stage_f64: bad arguments (n_jobs=%d, L_max=%d; needs 16 <= L_max <= 63)",
-                 n_jobs, L_max);
-    const int nbp = lane_nbp(L_max, 8);
+    lb = stage_lb(lb);
+    DWBA_REQUIRE(n_jobs > 0 && d_jobs && d_tabs && d_out && dwba_semisep_lane_stage_group(L_max, lb) > 0,
+                 "dwba_il_semisep_lanes_stage_f64: bad arguments (n_jobs=%d, L_max=%d, lb=%d; needs lb 0, 4 or 8 "
+                 "and at least 2 multipole blocks)", n_jobs, L_max, lb);
+    const int nbp = lane_nbp(L_max, lb);
     const long blocks = (2L * n_jobs * nbp + 127) / 128;
     DWBA_REQUIRE(blocks <= INT32_MAX, "dwba_il_semisep_lanes_stage_f64: too many jobs (%d)", n_jobs);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    if (nbp == 4)
-        hipLaunchKernelGGL(semisep_lane_stage_kernel<4>, dim3((unsigned)blocks), dim3(256), 0, s, d_jobs, d_tabs,
-                           d_out_index, n_jobs, L_max, d_out);
-    else
-        hipLaunchKernelGGL(semisep_lane_stage_kernel<8>, dim3((unsigned)blocks), dim3(256), 0, s, d_jobs, d_tabs,
-                           d_out_index, n_jobs, L_max, d_out);
+    switch (lb * 100 + nbp) {
+#define C(B, P, NB, O) case B * 100 + P: \
+        hipLaunchKernelGGL((semisep_lane_stage_kernel<B, P, NB, O>), dim3((unsigned)blocks), dim3(256), 0, s, d_jobs, \
+                           d_tabs, d_out_index, n_jobs, L_max, d_out); break;
+        // lb 4: 128-136 VGPRs, 4 wavefronts per SIMD, one chunk in flight; lb 8: 214 VGPRs, 2, two
+        C(4, 2, 2, 4) C(4, 4, 2, 4) C(4, 8, 2, 4) C(4, 16, 2, 4) C(8, 4, 3, 2) C(8, 8, 3, 2)
+#undef C
+        default: return set_error(DWBA_ERR_ARG, "dwba_il_semisep_lanes_stage_f64: L_max %d, lb %d", L_max, lb);
+    }
     DWBA_LAUNCH_CHECK("semisep_lane_stage_kernel");
     return DWBA_OK;
 }

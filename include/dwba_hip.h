@@ -180,24 +180,26 @@ int32_t dwba_il_semisep_lanes_split_f64(int32_t n_jobs, const dwba_plain_job* d_
                                         const int32_t* d_out_index, int32_t L_max, int32_t lb,
                                         int32_t waves_per_simd, double* d_out, void* stream);
 
-/* LDS-staged form of the split kernel (16 <= L_max <= 63, lb = 8, tables of
- * stride dwba_semisep_lane_table_stride(L_max, 8)): the G =
- * dwba_semisep_lane_stage_group(L_max) streams of a group (16 for
- * L_max <= 31, 8 above) run at the same grid positions, split at one point
- * for the group, and every wavefront copies its streams' alpha / beta and the
- * group's grid-table rows into LDS two chunks of 16 points ahead of use.
- * Streams are as in the split kernel (the direct family of jobs
- * [0, n_jobs), then the exchange family); the streams of group g are
+/* LDS-staged form of the split kernel (lb = multipoles per lane, 4 or 8,
+ * 0 = 4; tables of stride dwba_semisep_lane_table_stride(L_max, lb)): the
+ * G = dwba_semisep_lane_stage_group(L_max, lb) streams of a group (64 / the
+ * lanes per stream: 8 for lb 4 and L_max 16..31) run at the same grid
+ * positions, split at one point for the group, and every wavefront copies its
+ * streams' alpha / beta and the group's grid-table rows into LDS chunks of 16
+ * points ahead of use.  Streams are as in the split kernel (the direct family
+ * of jobs [0, n_jobs), then the exchange family); the streams of group g are
  * [g G, g G + G).  Precondition: within a group, every stream with
  * n_cols > 0 has the same d_tabs pointer (pad each grid's jobs to a multiple
  * of G, and n_jobs with them, by jobs with n_rows = n_cols = 0); a group that
  * breaks it writes NaN for all of its streams.  d_out_index (may be NULL =
  * identity): job k writes output row d_out_index[k], none if negative.
- * Same values up to rounding order.  Replaces the same reference code as
- * dwba_il_semisep_f64. */
-int32_t dwba_semisep_lane_stage_group(int32_t L_max);
+ * dwba_semisep_lane_stage_group returns -1 where the kernel does not apply
+ * (lb 4 needs L_max >= 4, lb 8 L_max >= 16).  Same values up to rounding
+ * order.  Replaces the same reference code as dwba_il_semisep_f64. */
+int32_t dwba_semisep_lane_stage_group(int32_t L_max, int32_t lb);
 int32_t dwba_il_semisep_lanes_stage_f64(int32_t n_jobs, const dwba_plain_job* d_jobs, const double* const* d_tabs,
-                                        const int32_t* d_out_index, int32_t L_max, double* d_out, void* stream);
+                                        const int32_t* d_out_index, int32_t L_max, int32_t lb, double* d_out,
+                                        void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Full-Split preparation: alpha / beta vectors of a channel pair.       */

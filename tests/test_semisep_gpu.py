@@ -138,16 +138,17 @@ STAGE_GRIDS = [
 ]
 
 
-@pytest.mark.parametrize("L_max", [16, 20, 30, 31, 32, 63])
-def test_semisep_lane_stage_random_jobs(L_max):
+@pytest.mark.parametrize("lb,L_max", [(8, 16), (8, 20), (8, 30), (8, 31), (8, 32), (8, 63),
+                                       (4, 4), (4, 7), (4, 8), (4, 15), (4, 30), (4, 31), (4, 32), (4, 63)])
+def test_semisep_lane_stage_random_jobs(lb, L_max):
     """LDS-staged split lane kernel (dwba_il_semisep_lanes_stage_f64) against
     the oracle: jobs of several grids padded to whole groups per grid (in
     reverse order through the output index), ragged extents inside a group,
-    multipole blocks past L_max (L_max 16, 20), 8 pairs per group (L_max > 31)."""
+    multipole blocks past L_max, 32 to 4 pairs per group (lb 4, L_max 4 .. 63)."""
     from dwba_amd import _hip
     host, dev = _shared_jobs(STAGE_GRIDS, seed=300 + L_max)
-    print(f"\n[semisep lane stage] {len(host)} jobs L_max={L_max}", flush=True)
-    got = _hip.il_semisep_stage_f64_batch(dev, L_max).cpu().numpy()
+    print(f"\n[semisep lane stage] {len(host)} jobs L_max={L_max} lb={lb}", flush=True)
+    got = _hip.il_semisep_stage_f64_batch(dev, L_max, lb=lb).cpu().numpy()
     w = _check(host, got, L_max)
     print(f"  worst |d|/sum|aKb| {w:.2e}", flush=True)
 
@@ -162,7 +163,7 @@ def test_semisep_lane_stage_mixed_tables_read_nan():
     host, dev = _shared_jobs([(400, "exp", [(400, 400 - 9 * k) for k in range(16)])], seed=3)
     h2, d2 = _jobs([(200, 150, "exp")], seed=4)
     print("\n[semisep lane stage] mixed tables in one group", flush=True)
-    got = _hip.il_semisep_stage_f64_batch(dev + d2, 30, pad=False).cpu().numpy()
+    got = _hip.il_semisep_stage_f64_batch(dev + d2, 30, pad=False, lb=8).cpu().numpy()
     for k, (r, vecs, n_rows, n) in enumerate(host):
         ref = semisep.il_all_L(r, vecs[0], vecs[1], 30)
         mag = semisep.abs_bound_all_L(r, vecs[0], vecs[1], 30)
