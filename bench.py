@@ -351,7 +351,13 @@ def scan_kernel_ms(b, steps, stream):
         gd, ng, max_n, _ = b._lane_table_set(lbv)
         sj, ptrs, oidx, _ = b._lane_job_set(lbv, b.direct_gemv)
         hip.check(lib.dwba_semisep_lane_tables_f64(ng, hip.ptr(gd), max_n, Lm, lbv, sh), "tables")
-        if b.lane_split:
+        if int(b.lane_split) == 4:
+            sg, pg, og, ng2 = b._lane_job_set(lbv, b.direct_gemv, group=b._stage_group(lbv))
+            fn = lambda: hip.check(lib.dwba_il_semisep_lanes_stage_f64(ng2, hip.ptr(sg), hip.ptr(pg), hip.ptr(og), Lm,
+                                                                       lbv, hip.ptr(b.out), sh), "lanes_stage")
+            nbp = 64 // b._stage_group(lbv)
+            name = f"semisep_lane_stage_kernel<{lbv}, {nbp}, {2 if lbv == 4 else 3}, {4 if lbv == 4 else 2}>"
+        elif b.lane_split:
             occ = 3 if int(b.lane_split) == 3 else 2
             fn = lambda: hip.check(lib.dwba_il_semisep_lanes_split_f64(b.plain_count, hip.ptr(sj), hip.ptr(ptrs),
                                                                        hip.ptr(oidx), Lm, lbv, occ, hip.ptr(b.out),

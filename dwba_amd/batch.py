@@ -800,7 +800,8 @@ class ScanBatch:
         hit one table.  group > 0 (the LDS-staged kernel): energy order with
         each energy's jobs padded to whole groups of that many by empty jobs
         whose output index is -1, so a group never spans two grid tables."""
-        key = (lb, bool(split), half, "energy" if group else self.lane_order, group)
+        key = (lb, bool(split), half, "energy" if group else self.lane_order, group,
+               getattr(self, "stage_layout", "xcd") if group else None)
         hit = self._lane_sets.get(key)
         if hit is not None:
             return hit
@@ -829,10 +830,37 @@ class ScanBatch:
                     while len(padded) % group:
                         padded.append(-1)
                         pe.append(energy[p])
-            # whole groups dispatched longest first across energies (a group's first job is its longest)
-            starts = sorted(range(0, len(padded), group), key=lambda s: -int(src[padded[s]].n_cols))
-            perm = [padded[s + q] for s in starts for q in range(group)]
-            energy = [pe[s + q] for s in starts for q in range(group)]
+            glen = lambda s: int(src[padded[s]].n_cols)   # a group's first job is its longest
+            starts = list(range(0, len(padded), group))
+            if getattr(self, "stage_layout", "xcd") == "xcd":
+                # Workgroup w (two groups) runs on XCD w mod 8: each energy's groups go to one XCD, so its grid
+                # table is read from that XCD's L2.  Energies are dealt to eight queues longest first (LPT on
+                # points), each queue's groups longest first, two per workgroup, and the queues interleaved
+                # (empty groups fill the short queues); the exchange family starts at a multiple of eight
+                # workgroups since n_jobs becomes a multiple of 16 groups.
+                by_e = {}
+                for st in starts:
+                    by_e.setdefault(pe[st], []).append(st)
+                cost = {e: sum(glen(st) for st in v) for e, v in by_e.items()}
+                qs, load = [[] for _ in range(8)], [0] * 8
+                for e in sorted(by_e, key=lambda e: -cost[e]):
+                    x = int(np.argmin(load))
+                    load[x] += cost[e]
+                    qs[x] += by_e[e]
+                for q in qs:
+                    q.sort(key=lambda st: -glen(st))
+                    if len(q) % 2:
+                        q.append(None)
+                depth = max(len(q) for q in qs) // 2
+                order = []
+                for i in range(depth):
+                    for q in qs:
+                        order += q[2 * i:2 * i + 2] if 2 * i < len(q) else [None, None]
+            else:
+                order = sorted(starts, key=lambda st: -glen(st))      # longest first across energies
+            perm = [padded[st + q] if st is not None else -1 for st in order for q in range(group)]
+            e0 = pe[0]
+            energy = [pe[st + q] if st is not None else e0 for st in order for q in range(group)]
         n = len(perm)
         arr = (hip.PlainJob * max(n, 1))()
         for p, g in enumerate(perm):
