@@ -21,8 +21,9 @@ namespace scan_detail {
 constexpr int LD_PAD = 32;          // vectors of an energy are padded to a multiple of 32 points
 constexpr int GROUP_SLICE = 4;      // members per workgroup of the fused Filon pass
 constexpr int COUL_NODES = 10002;   // Coulomb phase-node buffer (generate_phase_nodes caps at 10001)
-constexpr int LANE_LB = 8;          // multipole block of the lane contraction
+constexpr int LANE_LB = 8;          // multipole block of the split lane contraction (L_max < 4)
 constexpr int LANE_WAVES = 2;       // its occupancy (waves per SIMD)
+constexpr int STAGE_LB = 4;         // multipole block of the LDS-staged lane contraction (L_max >= 4)
 constexpr double PI = 3.141592653589793;
 
 // ------------------------------------------------------------------ tables
@@ -426,6 +427,8 @@ struct dwba_scan {
     double levin_D[64], levin_x[8];
     // contraction
     int32_t n_jobs = 0;
+    int lane_lb = 8;                   // 4: the LDS-staged kernel on jobs padded to whole groups
+    bool stage = false;
     const dwba_plain_job* jobs = nullptr;
     const double* const* tabs = nullptr;
     const int32_t* out_index = nullptr;
@@ -486,8 +489,10 @@ int enqueue(dwba_scan* s, hipStream_t st) {
         rc = dwba_filon_fused_f64(s->n_slices, s->slices, s->prep, s->members, st);
         if (rc) return rc;
     }
-    rc = dwba_il_semisep_lanes_split_f64(s->n_jobs, s->jobs, s->tabs, s->out_index, s->L_max, LANE_LB, LANE_WAVES,
-                                         s->out, st);
+    rc = s->stage ? dwba_il_semisep_lanes_stage_f64(s->n_jobs, s->jobs, s->tabs, s->out_index, s->L_max, STAGE_LB,
+                                                    s->out, st)
+                  : dwba_il_semisep_lanes_split_f64(s->n_jobs, s->jobs, s->tabs, s->out_index, s->L_max, LANE_LB,
+                                                    LANE_WAVES, s->out, st);
     if (rc) return rc;
     if (s->n_gemv) {
         rc = dwba_semisep_gemv_f64(s->n_gemv, s->gemv, s->L_max, st);
@@ -600,8 +605,13 @@ int plan_scan(dwba_scan* s, int32_t n_energies, const dwba_scan_energy* E, int32
     s->L_max = Lm;
     DWBA_REQUIRE(cc_reference(opt->CC_nodes, s->cc_x, s->cc_w), "dwba_scan_plan: CC_nodes %d outside [2, %d]",
                  opt->CC_nodes, DWBA_MAX_CC);
-    const int stride = dwba_semisep_lane_table_stride(Lm, LANE_LB);
+    // the LDS-staged lane kernel where it applies: groups of G jobs of one energy
+    const int stage_group = dwba_semisep_lane_stage_group(Lm, STAGE_LB);
+    s->stage = stage_group > 0;
+    s->lane_lb = s->stage ? STAGE_LB : LANE_LB;
+    const int stride = dwba_semisep_lane_table_stride(Lm, s->lane_lb);
     DWBA_REQUIRE(stride > 0, "dwba_scan_plan: L_max %d outside [0, 63]", Lm);
+    const int n_slots = s->stage ? n_pairs + n_energies * (stage_group - 1) : n_pairs;   // job slots with padding
     std::vector<GridStats> gs;
     std::vector<int> wave_energy;
     std::vector<Plan> plan;
@@ -796,9 +806,9 @@ int plan_scan(dwba_scan* s, int32_t n_energies, const dwba_scan_energy* E, int32
     const size_t o_tj = nt ? A.take(sizeof(dwba_tail_job) * nt) : 0;
     const size_t o_tws = nt ? A.take(tail_ws_bytes) : 0;
     const size_t o_lgrid = A.take(sizeof(dwba_lane_grid) * n_energies);
-    const size_t o_jobs = A.take(sizeof(dwba_plain_job) * n_pairs);
-    const size_t o_tabs = A.take(sizeof(int64_t) * n_pairs);
-    const size_t o_oidx = A.take(sizeof(int32_t) * n_pairs);
+    const size_t o_jobs = A.take(sizeof(dwba_plain_job) * n_slots);
+    const size_t o_tabs = A.take(sizeof(int64_t) * n_slots);
+    const size_t o_oidx = A.take(sizeof(int32_t) * n_slots);
     const int ng = (int)gemv_energies.size();
     std::vector<size_t> o_t(ng);
     for (int g = 0; g < ng; ++g) o_t[g] = A.take(sizeof(double) * nl * ld[gemv_energies[g]]);
@@ -1014,21 +1024,35 @@ int plan_scan(dwba_scan* s, int32_t n_energies, const dwba_scan_energy* E, int32
         return jobs[a].n_cols > jobs[b].n_cols;
     });
     {
-        std::vector<dwba_plain_job> sj(n_pairs);
-        std::vector<int64_t> tp(n_pairs);
-        std::vector<int32_t> oi(n_pairs);
+        // energy order (energies with the longest jobs first, longest first within each); for the staged
+        // kernel each energy's jobs are padded to whole groups by empty jobs (output index -1)
+        std::vector<dwba_plain_job> sj;
+        std::vector<int64_t> tp;
+        std::vector<int32_t> oi;
+        sj.reserve(n_slots);
         for (int p = 0; p < n_pairs; ++p) {
-            sj[p] = jobs[perm[p]];
-            tp[p] = (int64_t)(uintptr_t)dptr(o_tab[plan[perm[p]].energy]);
-            oi[p] = perm[p];
+            const int e = plan[perm[p]].energy;
+            sj.push_back(jobs[perm[p]]);
+            tp.push_back((int64_t)(uintptr_t)dptr(o_tab[e]));
+            oi.push_back(perm[p]);
+            const bool last = p + 1 == n_pairs || plan[perm[p + 1]].energy != e;
+            while (s->stage && last && sj.size() % stage_group) {
+                dwba_plain_job z = jobs[perm[p]];
+                z.n_rows = 0;
+                z.n_cols = 0;
+                sj.push_back(z);
+                tp.push_back(tp.back());
+                oi.push_back(-1);
+            }
         }
-        DWBA_HIP_CHECK(up(o_jobs, sj.data(), sizeof(dwba_plain_job) * n_pairs));
-        DWBA_HIP_CHECK(up(o_tabs, tp.data(), sizeof(int64_t) * n_pairs));
-        DWBA_HIP_CHECK(up(o_oidx, oi.data(), sizeof(int32_t) * n_pairs));
+        DWBA_REQUIRE((int)sj.size() <= n_slots, "dwba_scan_plan: %zu job slots > %d", sj.size(), n_slots);
+        DWBA_HIP_CHECK(up(o_jobs, sj.data(), sizeof(dwba_plain_job) * sj.size()));
+        DWBA_HIP_CHECK(up(o_tabs, tp.data(), sizeof(int64_t) * tp.size()));
+        DWBA_HIP_CHECK(up(o_oidx, oi.data(), sizeof(int32_t) * oi.size()));
         s->jobs = at<dwba_plain_job>(s->base, o_jobs);
         s->tabs = at<const double* const>(s->base, o_tabs);
         s->out_index = at<int32_t>(s->base, o_oidx);
-        s->n_jobs = n_pairs;
+        s->n_jobs = (int32_t)sj.size();
     }
     if (ng) {
         std::vector<dwba_dgemv_energy> dg(ng);
@@ -1102,7 +1126,7 @@ int plan_scan(dwba_scan* s, int32_t n_energies, const dwba_scan_energy* E, int32
         }
         DWBA_HIP_CHECK(up(o_lgrid, lg.data(), sizeof(dwba_lane_grid) * n_energies));
         const int rc = dwba_semisep_lane_tables_f64(n_energies, at<dwba_lane_grid>(s->base, o_lgrid), max_n, Lm,
-                                                    LANE_LB, s->side);
+                                                    s->lane_lb, s->side);
         if (rc) return rc;
         DWBA_HIP_CHECK(hipStreamSynchronize(s->side));
     }

@@ -275,19 +275,21 @@ def test_ddot_mfma_against_numpy(count, L_max, nmax):
     assert bad.size == 0, ("wrong (pair, L) cells", bad.tolist()[:40], len(bad))
 
 
+@pytest.mark.parametrize("split,lb", [(2, 8), (4, 8), (4, 4)])
 @pytest.mark.parametrize("gemv", [False, True])
-def test_pipelined_step_bit_identical(gemv):
+def test_pipelined_step_bit_identical(gemv, split, lb):
     """The two-half pipelined step (prep of the second half of the energies
     beside the contraction of the first, on two streams, tails on a third)
-    gives the single-stream result bit for bit, eagerly and as graph replays."""
+    gives the single-stream result bit for bit, eagerly and as graph replays;
+    split 4: the LDS-staged lane kernel, whose groups never span energies."""
     import torch
     from dwba_amd.batch import ScanBatch
     scan = _small_scan(9, n_e=4, n_p=24)
     b = ScanBatch(scan, 20, precision="fp64", oscillatory_method="full_split")
     assert b._pipe is not None and 0 < b._pipe["kA"] < b.n_pairs
-    b.semisep_config = (0, 8)
+    b.semisep_config = (0, lb)
     b.semisep_lanes = True
-    b.lane_split = 2
+    b.lane_split = split
     b.direct_gemv = gemv and b.ss_gemv is not None
     b.overlap = None
     b.pipeline = False
@@ -398,6 +400,11 @@ def test_ionic_target_scan_matches_dropin(ionization):
     b.out.zero_()
     b.run()
     check("pipelined")
+    b.lane_split = 4                     # the LDS-staged lane kernel, 4 multipoles per lane
+    b.semisep_config = (0, 4)
+    b.out.zero_()
+    b.run()
+    check("pipelined, LDS-staged")
     b.capture()
     for rep in range(2):
         b.out.zero_()

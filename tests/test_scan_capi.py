@@ -290,8 +290,9 @@ def test_c_scan_matches_oracle_and_dropin(case):
 def test_cscan_binding_matches_scanbatch_step(z_ion):
     """The Python binding of the C engine (dwba_amd.cscan) on a
     multi-energy scan agrees with ScanBatch set to the same FP64 step (tail
-    fork, shared direct beta with matrix-core dots, split-lane contraction in
-    energy order, grouped Filon prep) to FP64 rounding."""
+    fork, shared direct beta with matrix-core dots, LDS-staged lane
+    contraction at 4 multipoles per lane, grouped Filon prep) to FP64
+    rounding."""
     from dwba_amd.batch import ScanBatch
     from dwba_amd.cscan import CScan
     scan = S.excitation_scan(n_energies=3, n_pairs=24, n_range=(1500, 3000), l_max=14, k_range=(0.5, 4.0), seed=31,
@@ -303,8 +304,8 @@ def test_cscan_binding_matches_scanbatch_step(z_ion):
     b.dot_kind = "mfma"
     b.semisep_lanes = True
     b.lane_tables = True
-    b.lane_split = 2
-    b.semisep_config = (0, 8)
+    b.lane_split = 4
+    b.semisep_config = (0, 4)
     b.lane_order = "energy"
     b.prep_grouped = True
     b.run()
