@@ -3,7 +3,8 @@
 * Excitation sweep: one energy at N = 15000 with k_i, k_f near the top of
   U[0.5, 8] and one at N = 4000 with low k, 400 channel pairs each
   (l_i, l_f ~ U{0..40}), L = 0..30, Full-Split quadrature, through ScanBatch
-  in fp64 (library defaults and autotuned), fp32 and bf16x3.
+  in fp64 (library defaults, autotuned, and the LDS-staged lane kernel at 4
+  and 8 multipoles per lane), fp32 and bf16x3.
 * Ionization batch: one ejected-energy point at N = 15000, 400 (l_i, l_f,
   l_ej) triples with l <= 30, L = 0..30, fp64 and fp32.
 
@@ -91,9 +92,14 @@ def _check(res, oracle, mfma_families, label):
     return worst
 
 
-def _batch(scan, precision, tune=False, tmp_path=None, monkeypatch=None, contraction=None):
+def _batch(scan, precision, tune=False, tmp_path=None, monkeypatch=None, contraction=None, stage_lb=None):
     from dwba_amd.batch import ScanBatch
     b = ScanBatch(scan, L_MAX, precision=precision, oscillatory_method="full_split", contraction=contraction)
+    if stage_lb:                            # the LDS-staged lane kernel at stage_lb multipoles per lane
+        b.semisep_lanes = True
+        b.lane_tables = True
+        b.lane_split = 4
+        b.semisep_config = (0, stage_lb)
     if tune:
         from dwba_amd import autotune as AT
         monkeypatch.setenv("DWBA_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
@@ -114,6 +120,12 @@ def test_excitation_fullsize_fp64_tuned(excitation, tmp_path, monkeypatch):
     _check(_batch(scan, "fp64", True, tmp_path, monkeypatch), ora, (), "excitation fp64 tuned")
 
 
+@pytest.mark.parametrize("lb", [4, 8])
+def test_excitation_fullsize_fp64_staged(excitation, lb):
+    scan, ora = excitation
+    _check(_batch(scan, "fp64", stage_lb=lb), ora, (), f"excitation fp64 LDS-staged lb {lb}")
+
+
 @pytest.mark.parametrize("contraction", ["block", "dense"])
 def test_excitation_fullsize_fp32(excitation, contraction):
     scan, ora = excitation
@@ -132,6 +144,11 @@ def test_excitation_fullsize_bf16x3(excitation, contraction):
 def test_ionization_fullsize_fp64(ionization):
     scan, ora = ionization
     _check(_batch(scan, "fp64"), ora, (), "ionization fp64")
+
+
+def test_ionization_fullsize_fp64_staged(ionization):
+    scan, ora = ionization
+    _check(_batch(scan, "fp64", stage_lb=4), ora, (), "ionization fp64 LDS-staged lb 4")
 
 
 @pytest.mark.parametrize("contraction", ["block", "dense"])
