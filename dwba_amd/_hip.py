@@ -152,6 +152,7 @@ _SIGS = {
     "dwba_ddot_f64": (c_i32, [c_i32, c_vp, c_i32, c_vp, c_vp]),
     "dwba_ddot_tiled_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_vp, c_vp]),
     "dwba_ddot_mfma_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_vp, c_vp]),
+    "dwba_ddot_stage_f64": (c_i32, [c_i32, c_vp, c_vp, c_i32, c_vp, c_vp]),
     "dwba_semisep_lane_table_stride": (c_i32, [c_i32, c_i32]),
     "dwba_filon_cells_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp]),
     "dwba_filon_fused_f64": (c_i32, [c_i32, c_vp, c_vp, c_vp, c_vp]),

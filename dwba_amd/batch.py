@@ -495,7 +495,8 @@ class ScanBatch:
         self._lane_sets = {}                                      # (lb, split) -> jobs sorted longest first
         self.plain_n_split = (nra, nca_split)
         self.direct_gemv = False          # chosen by tune() when the shared-beta path exists
-        self.dot_kind = "mfma"            # per-pair dots of that path: "mfma" (FP64 matrix cores) or "tiled"
+        self.dot_kind = "stage"           # per-pair dots of that path on the FP64 matrix cores: "stage" (operands
+                                          # staged through LDS) or "mfma" (direct loads); "tiled": FP64 VALU
         self.plain_n = (nra, nca)
         if self.contraction == "semiseparable":
             self.semisep_config = (0, 0)      # (seg, lb): library defaults until tuned
@@ -919,10 +920,10 @@ class ScanBatch:
         if direct_gemv:
             gd, ng, dd, nd, items_d, ni = self.ss_gemv
             hip.check(lib.dwba_semisep_gemv_f64(ng, hip.ptr(gd), Lm, sh), "dwba_semisep_gemv_f64")
-            if Lm <= 31 and self.dot_kind == "mfma":
+            if Lm <= 31 and self.dot_kind in ("mfma", "stage"):
                 it16, n16 = self.items16
-                hip.check(lib.dwba_ddot_mfma_f64(n16, hip.ptr(it16), hip.ptr(dd), Lm, hip.ptr(out), sh),
-                          "dwba_ddot_mfma_f64")
+                fn = lib.dwba_ddot_stage_f64 if self.dot_kind == "stage" else lib.dwba_ddot_mfma_f64
+                hip.check(fn(n16, hip.ptr(it16), hip.ptr(dd), Lm, hip.ptr(out), sh), "dwba_ddot_" + self.dot_kind + "_f64")
             elif Lm <= 31:
                 hip.check(lib.dwba_ddot_tiled_f64(ni, hip.ptr(items_d), hip.ptr(dd), Lm, hip.ptr(out), sh),
                           "dwba_ddot_tiled_f64")
@@ -935,7 +936,7 @@ class ScanBatch:
         ranges exist for those)."""
         return (self._pipe is not None and self.precision == "fp64" and self.contraction == "semiseparable"
                 and not self.prep_grouped and self.semisep_lanes and self.lane_tables and self.L_max <= 31
-                and (not self.direct_gemv or self.dot_kind == "mfma"))
+                and (not self.direct_gemv or self.dot_kind in ("mfma", "stage")))
 
     def _contract_half(self, half, out, sh):
         """The contraction of one half of the energies (pipelined step): the
@@ -954,8 +955,9 @@ class ScanBatch:
                 hip.check(lib.dwba_semisep_gemv_f64(gn, hip.ptr(gd) + g0 * ctypes.sizeof(DgemvEnergy), Lm, sh),
                           "dwba_semisep_gemv_f64")
             if i_n > 0:
-                hip.check(lib.dwba_ddot_mfma_f64(i_n, hip.ptr(it16) + i0 * 8, hip.ptr(dd), Lm, hip.ptr(out), sh),
-                          "dwba_ddot_mfma_f64")
+                fn = lib.dwba_ddot_stage_f64 if self.dot_kind == "stage" else lib.dwba_ddot_mfma_f64
+                hip.check(fn(i_n, hip.ptr(it16) + i0 * 8, hip.ptr(dd), Lm, hip.ptr(out), sh),
+                          "dwba_ddot_" + self.dot_kind + "_f64")
         if int(self.lane_split) == 4:
             sj, ptrs, oidx, n = self._lane_job_set(lbv, self.direct_gemv, half, group=self._stage_group(lbv))
             if n:
@@ -1101,8 +1103,8 @@ class ScanBatch:
                 d, n = self.ddot
                 if self.contraction == "block" and getattr(self, "items16", None) is not None:
                     it16, n16 = self.items16
-                    hip.check(lib.dwba_ddot_mfma_f64(n16, hip.ptr(it16), hip.ptr(d), Lm, hip.ptr(self.out), sh),
-                              "dwba_ddot_mfma_f64")
+                    hip.check(lib.dwba_ddot_stage_f64(n16, hip.ptr(it16), hip.ptr(d), Lm, hip.ptr(self.out), sh),
+                              "dwba_ddot_stage_f64")
                 else:
                     hip.check(lib.dwba_ddot_f64(n, hip.ptr(d), Lm, hip.ptr(self.out), sh), "dwba_ddot_f64")
             mark("direct", "end")

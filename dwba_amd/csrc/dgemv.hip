@@ -330,6 +330,153 @@ __global__ __launch_bounds__(64 * MW) void ddot_mfma_kernel(const dwba_ddot_pair
     }
 }
 
+// The same dots with the operands staged through LDS.  Loads in the MFMA
+// operand layout (lane m + 16k holds row m) send 16 rows through every load
+// instruction, one 16-byte access per lane: the vector L1 then spends its
+// time on tag lookups (about 64 per instruction, its address path stalled
+// more than half the time) rather than on bytes.  Here the workgroup and
+// item are those of ddot_mfma_kernel (<= 16 pairs sharing t, four
+// wavefronts taking interleaved chunks of DC points), but each wavefront
+// copies its chunk's 16 alpha rows and 32 t rows straight into its own LDS
+// buffers with global_load_lds_dwordx4 (a copy moves eight 128-byte rows,
+// 64 lanes x 16 bytes), one chunk ahead of the one it multiplies, and reads
+// the MFMA operands from there.  A copy cannot leave padding between its
+// rows, so row R is stored rotated by (R >> 1) & 7 of its 16-byte columns,
+// which puts the 16 rows of an operand read on distinct banks.  The last
+// chunk is copied dword by dword, points at and past the item's largest n
+// fetched from a zero array: nothing there is read.
+constexpr int DC = 16;        // points per chunk (one 128-byte row chunk)
+constexpr int DROWS = 48;     // rows per chunk buffer: 16 alpha, 32 t (multipoles 0..31)
+constexpr int DCOPIES = DROWS / 2;
+constexpr int DBUF = DROWS * 32;   // dwords per chunk buffer
+constexpr int DNBUF = 2;           // chunk buffers per wavefront (one chunk in flight)
+
+__device__ double g_ddot_zeros[2];
+
+typedef __attribute__((address_space(3))) void* ddot_lds_ptr;
+
+__device__ __forceinline__ void ddot_dma(const void* g, float* l) {
+    __builtin_amdgcn_global_load_lds(g, (ddot_lds_ptr)l, 4, 0, 0);
+}
+
+template <int N>
+__device__ __forceinline__ void ddot_wait() { asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory"); }
+
+// A chunk buffer holds its rows of 128 bytes (eight 16-byte columns) with
+// row R rotated by (R >> 1) & 7 columns: the 16 rows of an operand read then
+// fall on distinct banks.  Double index of element e of row R:
+__device__ __forceinline__ int ddot_at(int R, int e) {
+    return R * 16 + ((((e >> 1) - (R >> 1)) & 7) << 1) + (e & 1);
+}
+
+template <int NT>
+__global__ __launch_bounds__(64 * MW) void ddot_stage_kernel(const dwba_ddot_pair* __restrict__ pairs,
+                                                             const int2* __restrict__ items, int n_items,
+                                                             int L_max, double* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) float s_buf[MW][DNBUF][DBUF];    // also the final reduction
+    // consecutive items (one energy's pairs, sharing t) on one XCD: blocks are dealt to the 8 XCDs round-robin
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = nb / 8, rem = nb % 8, xcd = b % 8;
+    const int item = (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + b / 8;
+    if (item >= n_items) return;                           // uniform over the workgroup
+    const int2 it = items[item];
+    const int first = it.x, count = min(it.y, MQ);
+    if (count <= 0) return;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int m = lane & 15, kq = lane >> 4;
+    const int nl = L_max + 1;
+    const double* t = pairs[first].t;
+    const size_t ld = (size_t)pairs[first].ld;
+    int nmax = 0;
+    for (int q = 0; q < count; ++q) nmax = max(nmax, pairs[first + q].n);
+    const int n_chunks = (nmax + DC - 1) / DC;
+    // rows of the chunk buffer: 0..15 alpha of the item's pairs (pair 0's past the count: its results are
+    // never stored), 16..47 t rows 0..31 (row 0 past L_max).  A 16-byte copy moves rows 8i .. 8i + 7 (lane
+    // l: row 8i + l / 8, stored column l % 8); the last chunk, which may end inside a 16-byte column, is
+    // copied dword by dword (lane l: row 2j + l / 32, stored dword l % 32), each point tested.
+    const double* rows16[DROWS / 8];
+    const double* rows4[DCOPIES];
+    auto row_of = [&](int R) -> const double* {
+        return R < 16 ? pairs[first + (R < count ? R : 0)].alpha : t + (size_t)(R - 16 < nl ? R - 16 : 0) * ld;
+    };
+#pragma unroll
+    for (int i = 0; i < DROWS / 8; ++i) rows16[i] = row_of(8 * i + (lane >> 3));
+    const void* zeros = g_ddot_zeros;
+    float* mine = &s_buf[w][0][0];
+    auto copy = [&](int c) {
+        float* buf = mine + ((c / MW) % DNBUF) * DBUF;    // this wavefront's chunks take the buffers in turn
+        if ((c + 1) * DC <= nmax) {
+#pragma unroll
+            for (int i = 0; i < DROWS / 8; ++i) {
+                const int R = 8 * i + (lane >> 3);
+                const int col = ((lane & 7) + (R >> 1)) & 7;  // the 16-byte column stored at lane & 7
+                const char* src = reinterpret_cast<const char*>(rows16[i] + (size_t)c * DC) + 16 * col;
+                __builtin_amdgcn_global_load_lds(src, (ddot_lds_ptr)(buf + 256 * i), 16, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < DCOPIES; ++j) {
+                const int R = 2 * j + (lane >> 5);
+                const int dl = lane & 31;                       // stored dword of row R
+                const int e2 = (((dl >> 2) + (R >> 1)) & 7);    // its 16-byte column
+                const int d = 4 * e2 + (dl & 3);                // the row's dword stored there
+                const char* src = reinterpret_cast<const char*>(row_of(R) + (size_t)c * DC) + 4 * d;
+                ddot_dma(c * DC + (d >> 1) < nmax ? (const void*)src : zeros, buf + 64 * j);
+            }
+        }
+    };
+    v4d acc[NT];
+#pragma unroll
+    for (int c = 0; c < NT; ++c) acc[c] = v4d{0.0, 0.0, 0.0, 0.0};
+    if (w < n_chunks) copy(w);
+    for (int c = w; c < n_chunks; c += MW) {
+        if (c + MW < n_chunks) {
+            copy(c + MW);
+            // chunk c landed, chunk c + MW in flight (the 16-byte copies: whole chunks only, never the last)
+            ddot_wait<DROWS / 8>();
+        } else {
+            ddot_wait<0>();
+        }
+        const double* buf = reinterpret_cast<const double*>(mine + ((c / MW) % DNBUF) * DBUF);
+#pragma unroll
+        for (int u = 0; u < DC / 4; ++u) {
+            const int e = 4 * u + kq;
+            const double a = buf[ddot_at(m, e)];
+#pragma unroll
+            for (int tc = 0; tc < NT; ++tc)
+                acc[tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, buf[ddot_at(16 + 16 * tc + m, e)], acc[tc], 0, 0, 0);
+        }
+    }
+    // the wavefronts' partial sums through LDS (every copy has landed and been read)
+    __syncthreads();
+    double* s_red = reinterpret_cast<double*>(&s_buf[0][0][0]);   // [MW - 1][4 NT][64]
+    if (w > 0) {
+#pragma unroll
+        for (int tc = 0; tc < NT; ++tc)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) s_red[((w - 1) * 4 * NT + 4 * tc + v) * 64 + lane] = acc[tc][v];
+    }
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+        for (int q = 0; q < MW - 1; ++q)
+#pragma unroll
+            for (int tc = 0; tc < NT; ++tc)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[tc][v] += s_red[(q * 4 * NT + 4 * tc + v) * 64 + lane];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = kq + 4 * v;                    // pair in the item
+            if (row < count) {
+                double* o = out + (size_t)pairs[first + row].out_index * 2 * nl;
+#pragma unroll
+                for (int tc = 0; tc < NT; ++tc)
+                    if (m + 16 * tc < nl) o[m + 16 * tc] = acc[tc][v];
+            }
+        }
+    }
+}
+
 int nl_bucket_g(int L_max) {
     static const int buckets[] = {1, 2, 4, 8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64};
     for (int b : buckets)
@@ -400,6 +547,22 @@ int32_t dwba_ddot_mfma_f64(int32_t n_items, const int32_t* d_items, const dwba_d
                            reinterpret_cast<hipStream_t>(stream), d_pairs, reinterpret_cast<const int2*>(d_items),
                            n_items, L_max, d_out);
     DWBA_LAUNCH_CHECK("ddot_mfma_kernel");
+    return DWBA_OK;
+}
+
+int32_t dwba_ddot_stage_f64(int32_t n_items, const int32_t* d_items, const dwba_ddot_pair* d_pairs, int32_t L_max,
+                            double* d_out, void* stream) {
+    DWBA_REQUIRE(initialised(), "dwba_ddot_stage_f64: call dwba_init first");
+    DWBA_REQUIRE(n_items > 0 && d_items && d_pairs && d_out && L_max >= 0 && L_max <= 31,
+                 "dwba_ddot_stage_f64: bad arguments (L_max <= 31)");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (L_max < 16)
+        hipLaunchKernelGGL(ddot_stage_kernel<1>, dim3(n_items), dim3(64 * MW), 0, s, d_pairs,
+                           reinterpret_cast<const int2*>(d_items), n_items, L_max, d_out);
+    else
+        hipLaunchKernelGGL(ddot_stage_kernel<2>, dim3(n_items), dim3(64 * MW), 0, s, d_pairs,
+                           reinterpret_cast<const int2*>(d_items), n_items, L_max, d_out);
+    DWBA_LAUNCH_CHECK("ddot_stage_kernel");
     return DWBA_OK;
 }
 

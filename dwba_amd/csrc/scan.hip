@@ -497,7 +497,7 @@ int enqueue(dwba_scan* s, hipStream_t st) {
     if (s->n_gemv) {
         rc = dwba_semisep_gemv_f64(s->n_gemv, s->gemv, s->L_max, st);
         if (rc) return rc;
-        rc = s->L_max <= 31 ? dwba_ddot_mfma_f64(s->n_items16, s->items16, s->dots, s->L_max, s->out, st)
+        rc = s->L_max <= 31 ? dwba_ddot_stage_f64(s->n_items16, s->items16, s->dots, s->L_max, s->out, st)
                             : dwba_ddot_f64(s->n_dots, s->dots, s->L_max, s->out, st);
         if (rc) return rc;
     }

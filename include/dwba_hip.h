@@ -512,6 +512,16 @@ int32_t dwba_ddot_tiled_f64(int32_t n_items, const int32_t* d_items, const dwba_
 int32_t dwba_ddot_mfma_f64(int32_t n_items, const int32_t* d_items, const dwba_ddot_pair* d_pairs, int32_t L_max,
                            double* d_out, void* stream);
 
+/* The same FP64 matrix-core dots with the operands staged through LDS: the
+ * items of dwba_ddot_mfma_f64 (count <= 16 pairs sharing t and ld), whose
+ * four wavefronts copy their chunks of 16 points as whole row chunks into
+ * LDS one chunk ahead of use; the same contract on alpha (zero from n up to
+ * the item's largest n, nothing at or beyond that read) and t (rows of ld
+ * doubles, ld >= that n).  L_max <= 31.  Replaces the same reference code as
+ * dwba_ddot_mfma_f64. */
+int32_t dwba_ddot_stage_f64(int32_t n_items, const int32_t* d_items, const dwba_ddot_pair* d_pairs, int32_t L_max,
+                            double* d_out, void* stream);
+
 /* Per-pair assembly of I_L in place in d_out (layout [pair][2][L_max+1]):
  * contractions zeroed when non-finite in the oscillatory branch, tail added
  * (DWBA_TAIL_OUTER before, DWBA_TAIL_LEGACY after the L = 0 correction),

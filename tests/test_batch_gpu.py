@@ -148,7 +148,7 @@ def test_direct_gemv_path_matches_full_recurrence(L_max):
     b.run()
     r0 = b.results()
     mag = b.magnitudes()
-    for kind in ("mfma", "tiled"):
+    for kind in ("stage", "mfma", "tiled"):
         b.direct_gemv = True
         b.dot_kind = kind
         b.out.zero_()
@@ -273,6 +273,64 @@ def test_ddot_mfma_against_numpy(count, L_max, nmax):
     scale = np.abs(A[:, :nmax]) @ np.abs(T[:, :nmax]).T
     bad = np.argwhere(~(np.abs(got - ref) <= 1e-14 * scale + 1e-300))
     assert bad.size == 0, ("wrong (pair, L) cells", bad.tolist()[:40], len(bad))
+
+
+@pytest.mark.parametrize("counts,L_max,nmax", [((16,), 30, 1000), ((16,), 31, 37), ((11,), 12, 515), ((1,), 20, 16),
+                                                ((16,), 30, 3), ((16,), 15, 32), ((5,), 8, 0),
+                                                ((16, 13, 7, 16, 1, 10, 16, 12, 3), 30, 700)])
+def test_ddot_stage_against_numpy(counts, L_max, nmax):
+    """dwba_ddot_stage_f64 through the C ABI: items of up to 16 pairs, each
+    with its own t, ragged n per pair (alpha zero from n_p to the item's
+    largest n), chunk-boundary lengths, an empty item, nine items over the
+    eight XCD runs: out[p][L] = sum_{i < n_p} alpha_p[i] t[L][i]."""
+    import torch
+    from dwba_amd import _hip
+    from dwba_amd.batch import DdotPair
+    rng = np.random.default_rng(sum(counts) * 1000 + L_max + nmax)
+    nl = L_max + 1
+    ld = max(32, ((nmax + 31) // 32) * 32)
+    dev = torch.device("cuda", _hip.ensure_init(0))
+    P = sum(counts)
+    A = np.zeros((P, ld))
+    Ts, pairs, items, refs, scales = [], [], [], [], []
+    out = torch.full((P, 2, nl), np.nan, dtype=torch.float64, device=dev)
+    A_rows = []
+    first = 0
+    for it, cnt in enumerate(counts):
+        ns = rng.integers(0, nmax + 1, size=cnt)
+        ns[0] = nmax
+        T = np.zeros((nl, ld))
+        T[:, :nmax] = rng.standard_normal((nl, nmax))
+        for q in range(cnt):
+            A[first + q, :ns[q]] = rng.standard_normal(ns[q])
+        Ts.append((T, ns))
+        items.append((first, cnt))
+        first += cnt
+    A_d = torch.as_tensor(A, device=dev)
+    T_d = [torch.as_tensor(T, device=dev) for T, _ in Ts]
+    first = 0
+    for it, cnt in enumerate(counts):
+        for q in range(cnt):
+            pairs.append(DdotPair(_hip.ptr(A_d[first + q]), _hip.ptr(T_d[it]), int(Ts[it][1][q]), ld, first + q, 0))
+        first += cnt
+    pairs_d = _hip.struct_array_to_device((DdotPair * P)(*pairs), dev)
+    items_d = torch.as_tensor(np.asarray(items, dtype=np.int32), device=dev)
+    lib = _hip.load_library()
+    print(f"\n[ddot stage] items {counts} L_max={L_max} nmax={nmax}", flush=True)
+    _hip.check(lib.dwba_ddot_stage_f64(len(counts), _hip.ptr(items_d), _hip.ptr(pairs_d), L_max, _hip.ptr(out),
+                                       _hip.stream_handle()), "dwba_ddot_stage_f64")
+    got = out.cpu().numpy()[:, 0, :]
+    first = 0
+    for it, cnt in enumerate(counts):
+        T = Ts[it][0]
+        a = A[first:first + cnt, :nmax]
+        ref = a @ T[:, :nmax].T
+        scale = np.abs(a) @ np.abs(T[:, :nmax]).T
+        g = got[first:first + cnt]
+        bad = np.argwhere(~(np.abs(g - ref) <= 1e-14 * scale + 1e-300))
+        assert bad.size == 0, ("wrong (pair, L) cells", it, bad.tolist()[:40], len(bad))
+        first += cnt
+    assert np.all(np.isnan(out.cpu().numpy()[:, 1, :]))       # the exchange slots are not touched
 
 
 @pytest.mark.parametrize("split,lb", [(2, 8), (4, 8), (4, 4)])

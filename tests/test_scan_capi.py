@@ -301,7 +301,7 @@ def test_cscan_binding_matches_scanbatch_step(z_ion):
     b = ScanBatch(scan, L_max, precision="fp64", oscillatory_method="full_split")
     b.overlap = "start"
     b.direct_gemv = True
-    b.dot_kind = "mfma"
+    b.dot_kind = "stage"
     b.semisep_lanes = True
     b.lane_tables = True
     b.lane_split = 4
