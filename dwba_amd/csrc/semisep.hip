@@ -1122,8 +1122,8 @@ void semisep_lane_stage_kernel(const dwba_plain_job* __restrict__ jobs, const do
 //     B_{i-1} = q_{i-1}^L (B_i + beta_i / r_i),    B_{N-1} = 0      (j > i)
 constexpr int GT = 256;
 
-// A workgroup per (energy, L) sweeps the grid in chunks of GT points, one
-// point per thread.  Point i's step of the
+// A workgroup per (energy, L) sweeps the grid in chunks of GT * GPT points
+// (GPT consecutive points per thread, below).  Point i's step of the
 // forward recurrence is the affine map x -> q_i^L x + q_i^L beta_i (of the
 // backward one x -> q_{i-1}^L x + q_{i-1}^L beta_i / r_i); a chunk composes
 // them with an in-wave shuffle scan and a scan of the four wave totals in
@@ -1173,6 +1173,12 @@ __device__ __forceinline__ Aff block_scan_excl(Aff x, bool rev, double* s_m, dou
     return aff_then(pre, ex);
 }
 
+// Each thread takes GPT consecutive points of a chunk of GT * GPT: it
+// composes their affine steps serially, the workgroup scans the GT
+// compositions, and the thread replays its points from its exclusive prefix
+// -- one block scan (two barriers) per GT * GPT points instead of per GT.
+constexpr int GPT = 4;
+
 __global__ __launch_bounds__(GT) void semisep_gemv_scan_kernel(const dwba_dgemv_energy* __restrict__ en) {
     __shared__ double s_m[GT / 64], s_v[GT / 64];
     const dwba_dgemv_energy E = en[blockIdx.x];
@@ -1184,42 +1190,66 @@ __global__ __launch_bounds__(GT) void semisep_gemv_scan_kernel(const dwba_dgemv_
     double* __restrict__ t = E.t + (size_t)L * E.ld;
     const double dL = L == 0 ? 1.0 : exp((double)L * log(1.0 - 1e-12));
     const int top = min(rows, n);                             // t is needed for i < rows only
+    constexpr int CH = GT * GPT;
     // forward: A_{i+1} = q_i^L (A_i + beta_i), A_0 = 0; t_i = (A_i + d_L beta_i) / r_i
     double carry = 0.0;
-    for (int c0 = 0; c0 < top; c0 += GT) {
-        const int i = c0 + tid;
-        Aff f{1.0, 0.0};
-        double ri = 1.0, bi = 0.0;
-        if (i < n) {
-            ri = r[i];
-            bi = be[i];
-            const double q = i + 1 < n ? ri / r[i + 1] : 1.0;
-            const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
-            f = Aff{qL, qL * bi};
+    for (int c0 = 0; c0 < top; c0 += CH) {
+        const int i0 = c0 + tid * GPT;
+        double rr[GPT], bb[GPT], qq[GPT];
+        Aff f{1.0, 0.0};                                      // the thread's points, first to last
+#pragma unroll
+        for (int k = 0; k < GPT; ++k) {
+            const int i = i0 + k;
+            rr[k] = 1.0; bb[k] = 0.0; qq[k] = 1.0;
+            if (i < n) {
+                rr[k] = r[i];
+                bb[k] = be[i];
+                const double q = i + 1 < n ? rr[k] / r[i + 1] : 1.0;
+                qq[k] = L == 0 ? 1.0 : pow_int_rt(q, L);
+                f = aff_then(f, Aff{qq[k], qq[k] * bb[k]});
+            }
         }
         Aff tot;
         const Aff ex = block_scan_excl(f, false, s_m, s_v, &tot);
-        const double A = fma(ex.m, carry, ex.v);
-        if (i < top) t[i] = (A + dL * bi) / ri;
+        double A = fma(ex.m, carry, ex.v);
+#pragma unroll
+        for (int k = 0; k < GPT; ++k) {
+            const int i = i0 + k;
+            if (i < top) t[i] = (A + dL * bb[k]) / rr[k];
+            A = qq[k] * (A + bb[k]);
+        }
         carry = fma(tot.m, carry, tot.v);
     }
     // backward: B_{i-1} = q_{i-1}^L (B_i + beta_i / r_i), B_{n-1} = 0; t_i += B_i.
-    // Chunks from the top of the grid down; points >= top only feed the carry.
+    // Chunks from the top of the grid down, each thread's points last to first;
+    // points >= top only feed the carry.
     carry = 0.0;
-    const int nch = (n + GT - 1) / GT;
+    const int nch = (n + CH - 1) / CH;
     for (int c = nch - 1; c >= 0; --c) {
-        const int i = c * GT + tid;
-        Aff g{1.0, 0.0};
-        if (i < n) {
-            const double ri = r[i];
-            const double q = i > 0 ? r[i - 1] / ri : 1.0;
-            const double qL = L == 0 ? 1.0 : pow_int_rt(q, L);
-            g = Aff{qL, qL * (be[i] / ri)};
+        const int i0 = c * CH + tid * GPT;
+        double qq[GPT], ww[GPT];
+        Aff g{1.0, 0.0};                                      // the thread's points, last to first
+#pragma unroll
+        for (int k = GPT - 1; k >= 0; --k) {
+            const int i = i0 + k;
+            qq[k] = 1.0; ww[k] = 0.0;
+            if (i < n) {
+                const double ri = r[i];
+                const double q = i > 0 ? r[i - 1] / ri : 1.0;
+                qq[k] = L == 0 ? 1.0 : pow_int_rt(q, L);
+                ww[k] = be[i] / ri;
+                g = aff_then(g, Aff{qq[k], qq[k] * ww[k]});
+            }
         }
         Aff tot;
         const Aff ex = block_scan_excl(g, true, s_m, s_v, &tot);
-        const double B = fma(ex.m, carry, ex.v);
-        if (i < top) t[i] += B;
+        double B = fma(ex.m, carry, ex.v);
+#pragma unroll
+        for (int k = GPT - 1; k >= 0; --k) {
+            const int i = i0 + k;
+            if (i < top) t[i] += B;
+            B = qq[k] * (B + ww[k]);
+        }
         carry = fma(tot.m, carry, tot.v);
     }
 }
