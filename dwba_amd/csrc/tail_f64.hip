@@ -381,6 +381,45 @@ __device__ __forceinline__ void sweep4(double* acc, double w0, double ix0, doubl
     }
 }
 
+// The same for two nodes in the block form of sweep4: sweep2b on nodes 0, 1 and
+// then on nodes 2, 3 makes exactly sweep4's operations in its order, with half
+// its products live at a time.
+template <int LE>
+__device__ __forceinline__ void sweep2b(double* acc, double w0, double ix0, double w1, double ix1) {
+    const double ws[2] = {w0, w1}, ys[2] = {ix0, ix1};
+    double u[2][4], z[2], za[2];
+#pragma unroll
+    for (int n = 0; n < 2; ++n) {
+        const double y2 = ys[n] * ys[n];
+        u[n][0] = ws[n] * y2;
+        u[n][1] = u[n][0] * ys[n];
+        u[n][2] = u[n][1] * ys[n];
+        u[n][3] = u[n][2] * ys[n];
+        z[n] = y2 * y2;
+        za[n] = z[n];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int L = 1 + c;
+        if (L < LE) acc[L] = (acc[L] + u[0][c]) + u[1][c];
+    }
+#pragma unroll
+    for (int a = 1; 1 + 4 * a < LE; ++a) {
+        if (a > 1) {
+#pragma unroll
+            for (int n = 0; n < 2; ++n) za[n] *= z[n];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int L = 1 + 4 * a + c;
+            if (L < LE) {
+                acc[L] = fma(u[0][c], za[0], acc[L]);
+                acc[L] = fma(u[1][c], za[1], acc[L]);
+            }
+        }
+    }
+}
+
 // sin, cos of theta and the Filon coefficients alpha, beta, gamma at theta
 // (_filon_segment_complex's formulas, Taylor series below |theta| = 0.3).
 __device__ __forceinline__ void filon_coeffs(double th, double& rs, double& rc, double& al, double& be, double& ga) {
@@ -439,11 +478,19 @@ __device__ __forceinline__ void triple_weights(double x0, double x1, double x2, 
 // adds to per-L FP64 partial sums (a few hundred terms of similar size); the
 // lanes are combined in double-double.  The reference sums segments with
 // math.fsum; the two agree to ~1e-15 of the segment size.
+// A wavefront-uniform value moved to scalar registers (every lane holds the
+// same value; the vector registers it frees let three wavefronts share a SIMD).
+__device__ __forceinline__ double uni(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 template <int LE>
 __device__ __forceinline__ void filon_sweep(int neff, double om, double p0, double r_m, double r_max, double bstep,
                                             long nseg, long s_begin, long s_end, int lane, double* acc) {
+    om = uni(om); p0 = uni(p0); r_m = uni(r_m); r_max = uni(r_max); bstep = uni(bstep);
     const int nt = (neff - 1) / 2;
-    const double inv_om = -1.0 / om;
+    const double inv_om = uni(-1.0 / om);
     const long per = (s_end - s_begin + 63) / 64;
     const long s_lo = s_begin + (long)lane * per;
     const long s_hi = s_lo + per < s_end ? s_lo + per : s_end;
@@ -453,6 +500,7 @@ __device__ __forceinline__ void filon_sweep(int neff, double om, double p0, doub
     // formed once; only the last segment (ending at r_max) gets its own.
     double rs_u, rc_u, al_u, be_u, ga_u;
     filon_coeffs(om * (bstep / (neff - 1)), rs_u, rc_u, al_u, be_u, ga_u);
+    rs_u = uni(rs_u); rc_u = uni(rc_u); al_u = uni(al_u); be_u = uni(be_u); ga_u = uni(ga_u);
     // The phase at a segment's start is the one the previous segment's
     // rotations reached at its end (the same node); it is evaluated directly
     // only every ANCHOR segments (drift of a rotation chain: a few ulp per
@@ -505,36 +553,22 @@ __device__ __forceinline__ void filon_sweep(int neff, double om, double p0, doub
         const auto node = [&](int k) { return (k == neff - 1) ? b : inner(k); };
         const auto inv = [](double x) { return x > 1e-6 ? recip(x) : 0.0; };
         int t = 0;
-        // two triples (x0 x1 x2), (x2 x3 x4) per pass: four nodes swept together
+        // one triple (x0 x1 x2) per pass, x2 left pending; the triples of a
+        // pair (2u, 2u + 1) sweep their nodes x0, x1 in the block form (as one
+        // four-node block), the last triple of an odd count with running powers
+        const int nt_blk = nt & ~1;
 #pragma unroll 1
-        for (; t + 1 < nt; t += 2) {
-            // only the pass's last node can be the segment's end b
-            const double x1 = inner(2 * t + 1), x2 = inner(2 * t + 2), x3 = inner(2 * t + 3), x4 = node(2 * t + 4);
-            const double ix1 = inv(x1), ix2 = inv(x2), ix3 = inv(x3), ix4 = inv(x4);
-            // phases at x1..x4 by rotation from x0
-            const double s1 = fma(s0, rc, c0 * rs), c1 = fma(c0, rc, -(s0 * rs));
-            const double s2 = fma(s1, rc, c1 * rs), c2 = fma(c1, rc, -(s1 * rs));
-            const double s3 = fma(s2, rc, c2 * rs), c3 = fma(c2, rc, -(s2 * rs));
-            const double s4 = fma(s3, rc, c3 * rs), c4 = fma(c3, rc, -(s3 * rs));
-            double wa0, wm0, wb0, wa1, wm1, wb1;
-            triple_weights(x0, x1, x2, s0, c0, c1, s2, c2, small, cmid, om, p0, inv_om, al, be, ga, wa0, wm0, wb0);
-            triple_weights(x2, x3, x4, s2, c2, c3, s4, c4, small, cmid, om, p0, inv_om, al, be, ga, wa1, wm1, wb1);
-            sweep4<LE>(acc, w_pend + wa0, ix0, wm0, ix1, wb0 + wa1, ix2, wm1, ix3);
-            w_pend = wb1;
-            ix_pend = ix4;
-            x0 = x4;
-            ix0 = ix4;
-            s0 = s4;
-            c0 = c4;
-        }
-        if (t < nt) {                             // an odd number of triples: the last one alone
+        for (; t < nt; ++t) {
             const double x1 = inner(2 * t + 1), x2 = node(2 * t + 2);
             const double ix1 = inv(x1), ix2 = inv(x2);
             const double s1 = fma(s0, rc, c0 * rs), c1 = fma(c0, rc, -(s0 * rs));
             const double s2 = fma(s1, rc, c1 * rs), c2 = fma(c1, rc, -(s1 * rs));
             double wa, wm, wb;
             triple_weights(x0, x1, x2, s0, c0, c1, s2, c2, small, cmid, om, p0, inv_om, al, be, ga, wa, wm, wb);
-            sweep2<LE>(acc, w_pend + wa, ix0, wm, ix1);
+            if (t < nt_blk)
+                sweep2b<LE>(acc, w_pend + wa, ix0, wm, ix1);
+            else
+                sweep2<LE>(acc, w_pend + wa, ix0, wm, ix1);
             w_pend = wb;
             ix_pend = ix2;
             x0 = x2;
@@ -595,7 +629,7 @@ __device__ inline TailSetup tail_setup(const dwba_tail_job& J, int sgn, int NL) 
 // does not sweep and for signs that are not Filon (the Levin kernel then
 // writes part 0's row of its signs).
 template <int NL>
-__global__ __launch_bounds__(64) void tail_filon_kernel(const dwba_tail_job* __restrict__ jobs, TailParams TP,
+__global__ __launch_bounds__(64, NL <= 32 ? 3 : 2) void tail_filon_kernel(const dwba_tail_job* __restrict__ jobs, TailParams TP,
                                                         double* __restrict__ jre) {
     __shared__ double s_acc[64][SUM_COLS + 1];
     // blockIdx.y = 0: the Phi_+ sign (k_a + k_b, usually many more segments) is dispatched first
