@@ -552,30 +552,41 @@ __device__ __forceinline__ void filon_sweep(int neff, double om, double p0, doub
         const auto inner = [&](int k) { return __dadd_rn(__dmul_rn((double)k, step), a); };
         const auto node = [&](int k) { return (k == neff - 1) ? b : inner(k); };
         const auto inv = [](double x) { return x > 1e-6 ? recip(x) : 0.0; };
-        int t = 0;
-        // one triple (x0 x1 x2) per pass, x2 left pending; the triples of a
-        // pair (2u, 2u + 1) sweep their nodes x0, x1 in the block form (as one
-        // four-node block), the last triple of an odd count with running powers
-        const int nt_blk = nt & ~1;
+        // one triple (x0 x1 x2) per pass, x2 left pending, its nodes x0, x1 swept
+        // in the block form; in a segment that starts past the envelope cut-off
+        // (a > 1e-6, so every node is) 1/x1 and 1/x2 come from one reciprocal
+        // of x1 x2 (~2 ulp instead of ~1, at half the reciprocals)
+        const auto triples = [&](auto live) {
 #pragma unroll 1
-        for (; t < nt; ++t) {
-            const double x1 = inner(2 * t + 1), x2 = node(2 * t + 2);
-            const double ix1 = inv(x1), ix2 = inv(x2);
-            const double s1 = fma(s0, rc, c0 * rs), c1 = fma(c0, rc, -(s0 * rs));
-            const double s2 = fma(s1, rc, c1 * rs), c2 = fma(c1, rc, -(s1 * rs));
-            double wa, wm, wb;
-            triple_weights(x0, x1, x2, s0, c0, c1, s2, c2, small, cmid, om, p0, inv_om, al, be, ga, wa, wm, wb);
-            if (t < nt_blk)
+            for (int t = 0; t < nt; ++t) {
+                const double x1 = inner(2 * t + 1), x2 = node(2 * t + 2);
+                double ix1, ix2;
+                if constexpr (decltype(live)::value) {
+                    const double r12 = recip(x1 * x2);
+                    ix1 = x2 * r12;
+                    ix2 = x1 * r12;
+                } else {
+                    ix1 = inv(x1);
+                    ix2 = inv(x2);
+                }
+                const double s1 = fma(s0, rc, c0 * rs), c1 = fma(c0, rc, -(s0 * rs));
+                const double s2 = fma(s1, rc, c1 * rs), c2 = fma(c1, rc, -(s1 * rs));
+                double wa, wm, wb;
+                triple_weights(x0, x1, x2, s0, c0, c1, s2, c2, small, cmid, om, p0, inv_om, al, be, ga, wa, wm,
+                               wb);
                 sweep2b<LE>(acc, w_pend + wa, ix0, wm, ix1);
-            else
-                sweep2<LE>(acc, w_pend + wa, ix0, wm, ix1);
-            w_pend = wb;
-            ix_pend = ix2;
-            x0 = x2;
-            ix0 = ix2;
-            s0 = s2;
-            c0 = c2;
-        }
+                w_pend = wb;
+                ix_pend = ix2;
+                x0 = x2;
+                ix0 = ix2;
+                s0 = s2;
+                c0 = c2;
+            }
+        };
+        if (a > 1e-6)
+            triples(std::true_type{});
+        else
+            triples(std::false_type{});
         s_c = s0;
         c_c = c0;
     }
