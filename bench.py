@@ -539,12 +539,15 @@ def bench_batched(args, ionization=False):
             "note": "prep stage alone on one stream; algorithmic bytes: chi_i, chi_f read once, alpha_dir, "
                     "alpha_ex, beta_ex written once per pair, shared grid data once per energy"}
     if tail_ms:
-        tr, prov = _committed_traffic(["tail_filon_kernel<32>", "tail_levin_kernel<32>", "tail_combine_kernel"], wl)
+        # the tail kernels' multipole bucket (dwba_outer_tail_f64): smallest of these >= L_max + 1
+        nl = next(v for v in (2, 4, 8, 16, 24, 31, 32, 48, 64) if L_max + 1 <= v)
+        tk = [f"tail_filon_kernel<{nl}>", f"tail_levin_kernel<{nl}>", "tail_combine_kernel"]
+        tr, prov = _committed_traffic(tk, wl)
         cands["tails"] = {
             "bound": "fp64_valu", "achieved": W_tail / (tail_ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
             "unit": "TFLOP/s", "frac": W_tail / (tail_ms * 1e-3) / PEAK_FP64, "traffic": tr,
             "traffic_provenance": prov,
-            "kernel": "tail_filon_kernel<32> + tail_levin_kernel<32> + tail_combine_kernel (dwba_outer_tail_f64)",
+            "kernel": " + ".join(tk) + " (dwba_outer_tail_f64)",
             "kernel_ms": tail_ms, "work_flop_per_launch": W_tail, "nodes_per_launch": tail_nodes,
             "note": "3 flop per swept node and multipole L = 1..L_max (one FMA into the sum, one multiply of "
                     "the x^-(L+1) chain); phases, weights and reciprocals are overhead"}

@@ -750,7 +750,8 @@ __global__ void tail_combine_kernel(const dwba_tail_job* __restrict__ jobs, int 
 }
 
 int tail_nl_bucket(int L_max) {
-    static const int buckets[] = {2, 4, 8, 16, 24, 32, 48, 64};
+    // 31: L_max = 30 (the common multipole range) sweeps no unused L = 31
+    static const int buckets[] = {2, 4, 8, 16, 24, 31, 32, 48, 64};
     for (int b : buckets)
         if (L_max + 1 <= b) return b;
     return -1;
@@ -789,14 +790,14 @@ extern "C" int32_t dwba_outer_tail_f64(int32_t n_jobs, const dwba_tail_job* d_jo
     double* jre = reinterpret_cast<double*>(d_workspace);
     switch (nl) {
 #define C(n) case n: hipLaunchKernelGGL(dwba::tail_filon_kernel<n>, dim3(n_jobs, 2), dim3(64), 0, s, d_jobs, tp, jre); break;
-        C(2) C(4) C(8) C(16) C(24) C(32) C(48) C(64)
+        C(2) C(4) C(8) C(16) C(24) C(31) C(32) C(48) C(64)
 #undef C
         default: return dwba::set_error(DWBA_ERR_ARG, "dwba_outer_tail_f64: L_max %d", L_max);
     }
     DWBA_LAUNCH_CHECK("tail_filon_kernel");
     switch (nl) {
 #define C(n) case n: hipLaunchKernelGGL(dwba::tail_levin_kernel<n>, dim3(n_jobs, 2), dim3(64), 0, s, d_jobs, tp, jre); break;
-        C(2) C(4) C(8) C(16) C(24) C(32) C(48) C(64)
+        C(2) C(4) C(8) C(16) C(24) C(31) C(32) C(48) C(64)
 #undef C
         default: break;
     }
